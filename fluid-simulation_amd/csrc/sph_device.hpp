@@ -1,0 +1,225 @@
+// Device-side math of the SPH hot path (gfx950, IEEE binary64).
+//
+// Every expression keeps the evaluation order of the reference's Rust code
+// (left-associative products, powi(3) == (x*x)*x, nalgebra's
+// dot == (ax*bx + ay*by) + az*bz) and the library is built with
+// -ffp-contract=off, so each particle's sums are bit-identical to an
+// un-fused binary64 evaluation in the reference's neighbor order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+constexpr int kMaxBodies = 8;
+constexpr int kBlock = 256;
+
+// Precomputed kernel constants: wfac = alpha / h^3, gfac = alpha / h^4 with
+// alpha = 16 / pi (sph/kernel/cubic_spline.rs:9, kernel/mod.rs:29-45).
+struct KernelConst {
+    double h;
+    double wfac;
+    double gfac;
+};
+
+// Dense cubic-serendipity lattice on the device (utilities/discretization.rs:84-92).
+// cell_ok[c] != 0 when all 32 nodes of cell c are present.
+struct LatticeDev {
+    double x_min[3];
+    double x_max[3];
+    double dx;
+    int n[3];
+    const double *values;
+    const uint8_t *cell_ok;
+};
+
+// Per-boundary view passed to every pass (boundary_handling/mod.rs:100-119).
+struct BodyDev {
+    int kind;     // 0 StaticSampleBoundary, 1 VolumeMapBoundary
+    int dynamic;  // RigidBodyMotion attached
+    double com[3];
+    double q[4];  // w, i, j, k
+    double lin_vel[3];
+    double ang_vel[3];
+    LatticeDev sdf;
+    LatticeDev vm;
+    // static samples, sorted by cell; per-fluid-particle CSR into them
+    const double *sx, *sy, *sz, *svol;
+    const uint32_t *soff, *sidx;
+    // volume map: per fluid particle, the normal-mode sample (if flag)
+    const uint8_t *flag;
+    const double *bx, *by, *bz, *bvx, *bvy, *bvz, *bvol;
+};
+
+__device__ __forceinline__ double powi2(double x) { return x * x; }
+__device__ __forceinline__ double powi3(double x) { return (x * x) * x; }
+
+__device__ __forceinline__ double dot3(double ax, double ay, double az, double bx, double by,
+                                       double bz) {
+    return (ax * bx + ay * by) + az * bz;
+}
+
+// w(q) and dw/dq (cubic_spline.rs:12-17)
+__device__ __forceinline__ double kernel_w(double q) {
+    return powi3(fmax(1. - q, 0.)) - 4. * powi3(fmax(0.5 - q, 0.));
+}
+__device__ __forceinline__ double kernel_dqw(double q) {
+    return -3. * powi2(fmax(1. - q, 0.)) + 12. * powi2(fmax(0.5 - q, 0.));
+}
+
+// W(r, h) (kernel/mod.rs:29-31)
+__device__ __forceinline__ double kernel_fn(const KernelConst &K, double rx, double ry,
+                                            double rz) {
+    double rn = sqrt(dot3(rx, ry, rz, rx, ry, rz));
+    return K.wfac * kernel_w(rn / K.h);
+}
+
+// grad W(r, h) (kernel/mod.rs:37-45)
+__device__ __forceinline__ void kernel_grad(const KernelConst &K, double rx, double ry,
+                                            double rz, double &gx, double &gy, double &gz) {
+    double rn = sqrt(dot3(rx, ry, rz, rx, ry, rz));
+    if (rn == 0.0) {
+        gx = gy = gz = 0.0;
+        return;
+    }
+    double d = kernel_dqw(rn / K.h);
+    gx = ((K.gfac * rx) / rn) * d;
+    gy = ((K.gfac * ry) / rn) * d;
+    gz = ((K.gfac * rz) / rn) * d;
+}
+
+__device__ __forceinline__ void cross3(double ax, double ay, double az, double bx, double by,
+                                       double bz, double &ox, double &oy, double &oz) {
+    double x = ay * bz - az * by;
+    double y = az * bx - ax * bz;
+    double z = ax * by - ay * bx;
+    ox = x;
+    oy = y;
+    oz = z;
+}
+
+// UnitQuaternion * Vector3 as nalgebra evaluates it: t = 2 (q_v x v);
+// v' = t w + q_v x t + v
+__device__ __forceinline__ void quat_rotate(double w, double qi, double qj, double qk, double vx,
+                                            double vy, double vz, double &ox, double &oy,
+                                            double &oz) {
+    double tx, ty, tz, cx, cy, cz;
+    cross3(qi, qj, qk, vx, vy, vz, tx, ty, tz);
+    tx *= 2.0;
+    ty *= 2.0;
+    tz *= 2.0;
+    cross3(qi, qj, qk, tx, ty, tz, cx, cy, cz);
+    ox = tx * w + cx + vx;
+    oy = ty * w + cy + vy;
+    oz = tz * w + cz + vz;
+}
+
+__device__ __forceinline__ void world_to_local(const BodyDev &b, double px, double py, double pz,
+                                               double &lx, double &ly, double &lz) {
+    quat_rotate(b.q[0], -b.q[1], -b.q[2], -b.q[3], px - b.com[0], py - b.com[1], pz - b.com[2],
+                lx, ly, lz);
+}
+
+// RigidBodyMotion::velocity_at_point (rigid_body_motion.rs:132-137); zero for static
+__device__ __forceinline__ void body_velocity(const BodyDev &b, double px, double py, double pz,
+                                              double &vx, double &vy, double &vz) {
+    if (!b.dynamic) {
+        vx = vy = vz = 0.0;
+        return;
+    }
+    double cx, cy, cz;
+    cross3(b.ang_vel[0], b.ang_vel[1], b.ang_vel[2], px - b.com[0], py - b.com[1],
+           pz - b.com[2], cx, cy, cz);
+    vx = b.lin_vel[0] + cx;
+    vy = b.lin_vel[1] + cy;
+    vz = b.lin_vel[2] + cz;
+}
+
+// Per-node lattice offsets in reference_nodes() order (discretization.rs:262-300)
+__constant__ const int8_t kNodeOff[32][3] = {
+    {0, 0, 0}, {3, 0, 0}, {0, 3, 0}, {3, 3, 0}, {0, 0, 3}, {3, 0, 3}, {0, 3, 3}, {3, 3, 3},
+    {1, 0, 0}, {2, 0, 0}, {1, 3, 0}, {2, 3, 0}, {1, 0, 3}, {2, 0, 3}, {1, 3, 3}, {2, 3, 3},
+    {0, 1, 0}, {0, 2, 0}, {3, 1, 0}, {3, 2, 0}, {0, 1, 3}, {0, 2, 3}, {3, 1, 3}, {3, 2, 3},
+    {0, 0, 1}, {0, 0, 2}, {3, 0, 1}, {3, 0, 2}, {0, 3, 1}, {0, 3, 2}, {3, 3, 1}, {3, 3, 2}};
+
+// Shape function of node i at (xi, eta, zeta) (discretization.rs:306-348).
+// Reference coordinate of offset o: -1, -1/3, 1/3, 1 for o = 0..3.
+__device__ __forceinline__ double ref_coord(int o) {
+    return o == 0 ? -1.0 : (o == 1 ? -1. / 3. : (o == 2 ? 1. / 3. : 1.0));
+}
+
+// CubicSerendipityDiscretization::function (discretization.rs:437-483).
+// Returns 0 ok, 1 OutOfBounds, 2 PrunedCell.
+__device__ inline int serendipity(const LatticeDev &L, double px, double py, double pz,
+                                  double &out) {
+    double p[3] = {px, py, pz};
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double fi = floor((p[d] - L.x_min[d]) / L.dx);
+        if (p[d] < L.x_min[d] || p[d] > L.x_max[d]) return 1;
+        int idx = (int)fi;
+        if (idx > L.n[d] - 1) idx = L.n[d] - 1;
+        c[d] = idx;
+    }
+    if (!L.cell_ok[((size_t)c[2] * L.n[1] + c[1]) * L.n[0] + c[0]]) return 2;
+    double o0 = L.x_min[0] + (double)c[0] * L.dx;
+    double o1 = L.x_min[1] + (double)c[1] * L.dx;
+    double o2 = L.x_min[2] + (double)c[2] * L.dx;
+    double xi = 2.0 * (px - o0) / L.dx - 1.0;
+    double eta = 2.0 * (py - o1) / L.dx - 1.0;
+    double zeta = 2.0 * (pz - o2) / L.dx - 1.0;
+    double r2 = (xi * xi + eta * eta) + zeta * zeta;
+    const size_t nx = 3 * (size_t)L.n[0] + 1, ny = 3 * (size_t)L.n[1] + 1;
+    const size_t base = ((size_t)(3 * c[2]) * ny + (size_t)(3 * c[1])) * nx + (size_t)(3 * c[0]);
+    double s = -0.0;
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        int ox = kNodeOff[i][0], oy = kNodeOff[i][1], oz = kNodeOff[i][2];
+        double rx = ref_coord(ox), ry = ref_coord(oy), rz = ref_coord(oz);
+        double v;
+        if (i < 8) {
+            v = (1.0 / 64.0) * (1.0 + xi * rx) * (1.0 + eta * ry) * (1.0 + zeta * rz) *
+                (9.0 * r2 - 19.0);
+        } else if (i < 16) {
+            v = (9.0 / 64.0) * (1.0 - xi * xi) * (1.0 + 9.0 * xi * rx) * (1.0 + eta * ry) *
+                (1.0 + zeta * rz);
+        } else if (i < 24) {
+            v = (9.0 / 64.0) * (1.0 - eta * eta) * (1.0 + 9.0 * eta * ry) * (1.0 + xi * rx) *
+                (1.0 + zeta * rz);
+        } else {
+            v = (9.0 / 64.0) * (1.0 - zeta * zeta) * (1.0 + 9.0 * zeta * rz) * (1.0 + xi * rx) *
+                (1.0 + eta * ry);
+        }
+        s += L.values[base + ((size_t)oz * ny + (size_t)oy) * nx + (size_t)ox] * v;
+    }
+    out = s;
+    return 0;
+}
+
+// gradient by central differences with step dx/6 (discretization.rs:527-557)
+__device__ inline int serendipity_grad(const LatticeDev &L, double px, double py, double pz,
+                                       double &gx, double &gy, double &gz) {
+    double h = L.dx / 6.0;
+    double g[3];
+    double p[3] = {px, py, pz};
+#pragma unroll 1
+    for (int a = 0; a < 3; a++) {
+        double pp[3] = {p[0], p[1], p[2]}, pm[3] = {p[0], p[1], p[2]};
+        pp[a] += h;
+        pm[a] -= h;
+        double fp, fm;
+        int ep = serendipity(L, pp[0], pp[1], pp[2], fp);
+        int em = serendipity(L, pm[0], pm[1], pm[2], fm);
+        if (ep) return ep;
+        if (em) return em;
+        g[a] = (fp - fm) / (2.0 * h);
+    }
+    gx = g[0];
+    gy = g[1];
+    gz = g[2];
+    return 0;
+}
+
+}  // namespace sph

@@ -1,0 +1,424 @@
+"""Seeded scene construction for the SPH hot path.
+
+The reference builds its scenes from triangle meshes (parry3d SDF queries,
+mesh sampling, mass properties; ``simulation_lib/src/sph/setup/mod.rs:118-294``).
+That setup path is outside the hot path, so scenes here are generated directly:
+
+* fluid particles on a regular lattice at spacing ``dx`` with seeded uniform
+  jitter (the reference's ``sample_volume_shifted`` also yields a lattice,
+  ``utilities/sampling.rs:8-33``), mass ``rho0 * dx**3``
+  (``sph/fluid.rs:87``);
+* ``StaticSampleBoundary`` samples on the same lattice, as layers outside the
+  container box (the reference samples mesh surfaces at ``0.7 dx``,
+  ``static_sample_boundary.rs:63``; the boundary positions are therefore an
+  input of the hot path here);
+* ``VolumeMapBoundary`` fields on the reference's cubic-serendipity lattice rule
+  (cell size ``4 dx``, SDF padding ``3.1 h`` pruned at +-padding, volume-map
+  padding ``2 h + cell/6``; ``volume_map_boundary.rs:1120-1137, 1547-1636``;
+  ``utilities/discretization.rs:99-245``), with nodal values taken from the
+  analytic signed distance of a box or sphere and a volume map integrated
+  exactly for the local half-space (the reference integrates the discretized
+  SDF over the kernel ball with degree-30 Gauss-Legendre,
+  ``volume_map_boundary.rs:1684-1714``);
+* rigid-body mass properties in closed form for boxes and spheres.
+
+Both the HIP path and the CPU oracle consume the identical arrays, so per-step
+parity is judged on the hot path alone.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+SOLVERS = {"SESPH": 0, "IISPH": 1, "IISPHwOST": 2}
+INTEGRATORS = {"EulerCromer": 0, "TakePredicted": 1}
+
+# to_offset() of the 32 reference nodes (utilities/discretization.rs:262-300):
+# 8 corners, then the x-, y- and z-edge nodes
+NODE_OFFSETS = np.array(
+    [[0, 0, 0], [3, 0, 0], [0, 3, 0], [3, 3, 0], [0, 0, 3], [3, 0, 3], [0, 3, 3], [3, 3, 3],
+     [1, 0, 0], [2, 0, 0], [1, 3, 0], [2, 3, 0], [1, 0, 3], [2, 0, 3], [1, 3, 3], [2, 3, 3],
+     [0, 1, 0], [0, 2, 0], [3, 1, 0], [3, 2, 0], [0, 1, 3], [0, 2, 3], [3, 1, 3], [3, 2, 3],
+     [0, 0, 1], [0, 0, 2], [3, 0, 1], [3, 0, 2], [0, 3, 1], [0, 3, 2], [3, 3, 1], [3, 3, 2]],
+    dtype=np.int64)
+
+
+@dataclasses.dataclass
+class Params:
+    """Mirror of the reference's ``[parameters]`` + ``[procedures]``
+    (``sph/setup/input.rs:103-260``) restricted to what the hot path reads."""
+    kernel_support_radius: float
+    rest_density_grid_spacing: float
+    time_increment: float = 1e-3
+    cfl: bool = False
+    max_time_increment: float = 1e-3
+    cfl_number: float = 0.4
+    disable_particles_below: float = -1e30
+    fluid_viscosity: float = 0.01
+    boundary_viscosity: float = 0.01
+    boundary_pressure_acceleration_weighting: float = 1.0
+    boundary_rest_volume_weighting: float = 1.0
+    gravity_mode: int = 0          # 0 uniform -z, 1 radial
+    gravity_center: tuple = (0.0, 0.0, 0.0)
+    solver: str = "IISPH"
+    integrator: str = "EulerCromer"
+    stiffness: float = 1000.0
+    target_density_error: float = 0.1   # percent
+    relaxation_factor: float = 0.5
+    min_diagonal_element: float = 1e-10
+
+
+@dataclasses.dataclass
+class Lattice:
+    """Dense cubic-serendipity node lattice (``CubicSerendipityDiscretization``).
+
+    ``values``/``present`` have shape ``(3*n[2]+1, 3*n[1]+1, 3*n[0]+1)`` (x
+    fastest); ``present`` marks nodes that survive the reference's cell pruning.
+    """
+    x_min: np.ndarray
+    x_max: np.ndarray
+    dx: float
+    n: np.ndarray
+    values: np.ndarray
+    present: np.ndarray
+
+
+@dataclasses.dataclass
+class Rigid:
+    """Initial state of a ``RigidBodyMotion`` (``rigid_body_motion.rs:38-62``)."""
+    mass: float
+    inertia_body: np.ndarray      # 3x3
+    com: np.ndarray
+    q: np.ndarray = dataclasses.field(default_factory=lambda: np.array([1.0, 0.0, 0.0, 0.0]))
+    lin_vel: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+    ang_vel: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(3))
+
+    def packed(self) -> np.ndarray:
+        """mass, I_body[9], I_inv_body[9], com[3], q[4], v[3], omega[3]."""
+        ib = np.asarray(self.inertia_body, dtype=np.float64)
+        return np.concatenate([[self.mass], ib.ravel(), np.linalg.inv(ib).ravel(),
+                               self.com, self.q, self.lin_vel, self.ang_vel]).astype(np.float64)
+
+
+@dataclasses.dataclass
+class StaticSamples:
+    positions: np.ndarray                # (m,3) world frame, or body frame if rigid
+    rigid: Optional[Rigid] = None
+
+
+@dataclasses.dataclass
+class VolumeMap:
+    sdf: Lattice
+    vm: Lattice
+    rigid: Optional[Rigid] = None
+    kind: str = "box"                    # description only
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    params: Params
+    x: np.ndarray
+    v: np.ndarray
+    m: np.ndarray
+    fluid_id: np.ndarray
+    boundaries: List[object]
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+
+# --------------------------------------------------------------------------
+# signed distance functions (positive on the fluid side)
+# --------------------------------------------------------------------------
+def sd_box_solid(p, lo, hi):
+    """Exact signed distance of a solid box, positive outside."""
+    c = (np.asarray(lo) + np.asarray(hi)) * 0.5
+    half = (np.asarray(hi) - np.asarray(lo)) * 0.5
+    q = np.abs(p - c) - half
+    outside = np.linalg.norm(np.maximum(q, 0.0), axis=-1)
+    inside = np.minimum(np.max(q, axis=-1), 0.0)
+    return outside + inside
+
+
+def sd_box_container(p, lo, hi):
+    return -sd_box_solid(p, lo, hi)
+
+
+def sd_sphere_solid(p, c, r):
+    return np.linalg.norm(p - np.asarray(c), axis=-1) - r
+
+
+def sd_sphere_container(p, c, r):
+    return -sd_sphere_solid(p, c, r)
+
+
+# --------------------------------------------------------------------------
+# lattice construction (reference rule, analytic node values)
+# --------------------------------------------------------------------------
+def _node_positions(x_min, n, dx):
+    axes = [x_min[d] + (np.arange(3 * n[d] + 1, dtype=np.float64) / 3.0) * dx for d in range(3)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return np.stack([x, y, z], axis=-1)
+
+
+def _cell_node_values(values, n):
+    """(32, nz, ny, nx) view of each cell's node values."""
+    nx, ny, nz = (int(v) for v in n)
+    out = np.empty((32, nz, ny, nx), dtype=values.dtype)
+    for k, (ox, oy, oz) in enumerate(NODE_OFFSETS):
+        out[k] = values[oz:oz + 3 * nz:3, oy:oy + 3 * ny:3, ox:ox + 3 * nx:3]
+    return out
+
+
+def _present_from_kept(keep, n):
+    nx, ny, nz = (int(v) for v in n)
+    present = np.zeros((3 * nz + 1, 3 * ny + 1, 3 * nx + 1), dtype=np.uint8)
+    for ox, oy, oz in NODE_OFFSETS:
+        sl = present[oz:oz + 3 * nz:3, oy:oy + 3 * ny:3, ox:ox + 3 * nx:3]
+        sl |= keep.astype(np.uint8)
+    return present
+
+
+def _lattice(aabb_min, aabb_max, pad, dx, sdf, prune_lo, prune_hi, value_fn):
+    x_min = np.asarray(aabb_min, dtype=np.float64) - pad
+    x_max = np.asarray(aabb_max, dtype=np.float64) + pad
+    n = np.array([int(round((x_max[d] - x_min[d]) / dx)) for d in range(3)], dtype=np.int64)
+    pts = _node_positions(x_min, n, dx)
+    vals = value_fn(pts)
+    cells = _cell_node_values(vals, n)
+    keep = np.ones(cells.shape[1:], dtype=bool)
+    if prune_lo is not None:
+        keep &= ~np.all(cells < prune_lo, axis=0)
+    if prune_hi is not None:
+        keep &= ~np.all(cells > prune_hi, axis=0)
+    present = _present_from_kept(keep, n)
+    vals = np.where(present.astype(bool), vals, 0.0)
+    return Lattice(x_min, x_max, float(dx), n, np.ascontiguousarray(vals, dtype=np.float64),
+                   np.ascontiguousarray(present))
+
+
+def kernel_w(q):
+    """Cubic B-spline shape w(q) (``sph/kernel/cubic_spline.rs:12-13``)."""
+    return np.maximum(1.0 - q, 0.0) ** 3 - 4.0 * np.maximum(0.5 - q, 0.0) ** 3
+
+
+def _gamma(d, h):
+    """Cubic extension of the SDF (``volume_map_boundary.rs:1716-1726``)."""
+    g = kernel_w(np.abs(d) / h) / kernel_w(0.0)
+    return np.where(d <= 0.0, 1.0, np.where(d >= h, 0.0, g))
+
+
+_GL4_X, _GL4_W = np.polynomial.legendre.leggauss(4)
+
+
+def halfspace_volume(sd, h):
+    """Integral of gamma(sd(x)) over the kernel ball when sd varies linearly
+    across it: V(sd) = int_{-h}^{h} gamma(sd + s) * pi (h^2 - s^2) ds.
+
+    The integrand is a piecewise polynomial of degree <= 5 with breaks where
+    sd + s in {0, h/2, h}; 4-point Gauss-Legendre per piece is exact."""
+    sd = np.asarray(sd, dtype=np.float64)
+    flat = sd.ravel()
+    total = np.zeros_like(flat)
+    brk = np.stack([np.full_like(flat, -h), -flat, 0.5 * h - flat, h - flat, np.full_like(flat, h)], -1)
+    brk = np.clip(brk, -h, h)
+    brk.sort(axis=-1)
+    for k in range(4):
+        a, b = brk[:, k], brk[:, k + 1]
+        half = 0.5 * (b - a)
+        mid = 0.5 * (a + b)
+        for xg, wg in zip(_GL4_X, _GL4_W):
+            s = mid + half * xg
+            total += wg * half * _gamma(flat + s, h) * math.pi * (h * h - s * s)
+    ball = 4.0 / 3.0 * math.pi * h ** 3
+    out = np.where(flat >= 2.0 * h, 0.0, np.where(flat <= -h, ball, total))
+    return np.clip(out, 0.0, ball).reshape(sd.shape)
+
+
+def volume_map_fields(sd_fn, aabb_min, aabb_max, h, dx_particle):
+    """SDF + volume-map lattices on the reference's grid rule."""
+    cell = 4.0 * dx_particle                        # volume_map_boundary.rs:1120
+    pad_sd = 3.1 * h                                 # :1562
+    sdf = _lattice(aabb_min, aabb_max, pad_sd, cell, sd_fn, -pad_sd, pad_sd, sd_fn)
+    pad_vm = 2.0 * h + cell / 6.0                    # :1595
+    vm = _lattice(aabb_min, aabb_max, pad_vm, cell, sd_fn, 0.0, None,
+                  lambda pts: halfspace_volume(sd_fn(pts), h))
+    return sdf, vm
+
+
+# --------------------------------------------------------------------------
+# fluid blocks
+# --------------------------------------------------------------------------
+def lattice_block(lo, counts, dx, rng, jitter):
+    counts = [int(c) for c in counts]
+    ix, iy, iz = np.meshgrid(*(np.arange(c) for c in counts), indexing="ij")
+    idx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=-1).astype(np.float64)
+    pos = np.asarray(lo, dtype=np.float64) + (idx + 0.5) * dx
+    if jitter > 0:
+        pos = pos + rng.uniform(-jitter, jitter, size=pos.shape)
+    return pos
+
+
+def _fluid(blocks, dx):
+    xs, ms, fids = [], [], []
+    for pos, rho0, fid in blocks:
+        xs.append(pos)
+        ms.append(np.full(len(pos), rho0 * dx ** 3))
+        fids.append(np.full(len(pos), fid, dtype=np.uint32))
+    x = np.ascontiguousarray(np.concatenate(xs), dtype=np.float64)
+    return (x, np.zeros_like(x), np.concatenate(ms).astype(np.float64),
+            np.concatenate(fids).astype(np.uint32))
+
+
+def box_shell_samples(lo, hi, dx, layers):
+    """Lattice samples of ``layers`` layers just outside the box [lo, hi]."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    n = np.round((hi - lo) / dx).astype(int)
+    rngs = [np.arange(-layers, n[d] + layers) for d in range(3)]
+    ix, iy, iz = np.meshgrid(*rngs, indexing="ij")
+    idx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=-1)
+    inside = np.all((idx >= 0) & (idx < n), axis=-1)
+    idx = idx[~inside]
+    return np.ascontiguousarray(lo + (idx + 0.5) * dx, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# the five workloads
+# --------------------------------------------------------------------------
+def dam_break_sesph(side=20, seed=0, f32_state=True) -> Scene:
+    """SESPH dam break: side^3 particles at 0.02 m in a 1 x 0.6 x 0.6 m box of
+    two layers of static boundary samples; h = 2 dx; rho0 1000, k 1000,
+    nu 0.01; dt 5e-4; Euler-Cromer."""
+    dx = 0.02
+    rng = np.random.default_rng(seed)
+    pos = lattice_block([0.0, 0.0, 0.0], [side] * 3, dx, rng, 1e-4)
+    if f32_state:
+        pos = pos.astype(np.float32).astype(np.float64)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    p = Params(kernel_support_radius=2 * dx, rest_density_grid_spacing=dx, time_increment=5e-4,
+               fluid_viscosity=0.01, boundary_viscosity=0.01, solver="SESPH",
+               integrator="EulerCromer", stiffness=1000.0)
+    samples = box_shell_samples([0, 0, 0], [1.0, 0.6, 0.6], dx, 2)
+    if f32_state:
+        samples = samples.astype(np.float32).astype(np.float64)
+    return Scene("sesph_dam_break", p, x, v, m, fid, [StaticSamples(samples)])
+
+
+def block_drop_iisph(side=50, seed=0, dx=0.01, box=1.0) -> Scene:
+    """IISPH block drop: side^3 particles at dx in a box-shaped VolumeMapBoundary;
+    relaxed Jacobi omega 0.5, target 0.1 %; CFL dt (0.4, max 1e-3)."""
+    rng = np.random.default_rng(seed)
+    edge = side * dx
+    lo = [0.5 * (box - edge), 0.5 * (box - edge), 0.3 * box]
+    pos = lattice_block(lo, [side] * 3, dx, rng, 1e-4)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver="IISPH", integrator="EulerCromer",
+               target_density_error=0.1, relaxation_factor=0.5)
+    lo_b, hi_b = np.zeros(3), np.full(3, box)
+    sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
+    return Scene("iisph_block_drop", p, x, v, m, fid, [VolumeMap(sdf, vm, None, "box")])
+
+
+def two_fluid_ball(side=100, seed=0, dx=0.01, radius=1.0, solver="IISPHwOST") -> Scene:
+    """Two-fluid large ball: a side^3 cube (1M particles at side=100) centred in
+    a radius-1 m spherical VolumeMapBoundary, split at x = 0 into rest
+    densities 1000 and 1500; IISPHwOST + TakePredicted (or IISPH + EulerCromer);
+    CFL dt (0.4, max 1e-3); uniform gravity."""
+    rng = np.random.default_rng(seed)
+    edge = side * dx
+    lo = np.array([-0.5 * edge, -0.5 * edge, -0.5 * edge])
+    pos = lattice_block(lo, [side] * 3, dx, rng, 1e-4)
+    left = pos[:, 0] < 0.0
+    x, v, m, fid = _fluid([(pos[left], 1000.0, 0), (pos[~left], 1500.0, 1)], dx)
+    h = 2 * dx
+    integ = "TakePredicted" if solver == "IISPHwOST" else "EulerCromer"
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver=solver, integrator=integ,
+               target_density_error=0.1, relaxation_factor=0.5)
+    c = np.zeros(3)
+    sdf, vm = volume_map_fields(lambda q: sd_sphere_container(q, c, radius), c - radius,
+                                c + radius, h, dx)
+    return Scene(f"two_fluid_ball_{solver}", p, x, v, m, fid, [VolumeMap(sdf, vm, None, "sphere")])
+
+
+def rigid_coupling(counts=(158, 78, 40), seed=0, dx=0.0125) -> Scene:
+    """Rigid-body coupling: a fluid layer in a 2 x 1 x 1 m box VolumeMapBoundary
+    plus three two-way coupled dynamic bodies (cubes of edge 0.2 m with
+    densities 500 and 800, a sphere of radius 0.15 m with density 1200) placed
+    above the surface at seeded positions; IISPH; CFL dt."""
+    rng = np.random.default_rng(seed)
+    pos = lattice_block([dx, dx, dx], counts, dx, rng, 1e-4)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver="IISPH", integrator="EulerCromer")
+    lo_b, hi_b = np.zeros(3), np.array([2.0, 1.0, 1.0])
+    sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
+    bounds = [VolumeMap(sdf, vm, None, "box")]
+    surface = (counts[2] + 1) * dx
+    bodies = [("cube", 0.2, 500.0), ("cube", 0.2, 800.0), ("sphere", 0.15, 1200.0)]
+    for k, (kind, size, rho) in enumerate(bodies):
+        cx = 0.4 + 0.6 * k + rng.uniform(-0.05, 0.05)
+        cy = 0.5 + rng.uniform(-0.1, 0.1)
+        cz = surface + 0.2 + rng.uniform(0.0, 0.1)
+        if kind == "cube":
+            half = 0.5 * size
+            lo_l, hi_l = np.full(3, -half), np.full(3, half)
+            mass = rho * size ** 3
+            inertia = np.eye(3) * (mass * size * size / 6.0)
+            fn = (lambda lo_l, hi_l: (lambda q: sd_box_solid(q, lo_l, hi_l)))(lo_l, hi_l)
+        else:
+            lo_l, hi_l = np.full(3, -size), np.full(3, size)
+            mass = rho * 4.0 / 3.0 * math.pi * size ** 3
+            inertia = np.eye(3) * (0.4 * mass * size * size)
+            fn = (lambda r: (lambda q: sd_sphere_solid(q, np.zeros(3), r)))(size)
+        s_l, v_l = volume_map_fields(fn, lo_l, hi_l, h, dx)
+        rb = Rigid(mass=mass, inertia_body=inertia, com=np.array([cx, cy, cz]))
+        bounds.append(VolumeMap(s_l, v_l, rb, kind))
+    return Scene("rigid_coupling", p, x, v, m, fid, bounds)
+
+
+def small_iisph_box(side=12, seed=1, dx=0.02, box=0.6, solver="IISPH", jitter=1e-4) -> Scene:
+    """A small VolumeMap box scene for parity tests (CPU-oracle sized)."""
+    rng = np.random.default_rng(seed)
+    pos = lattice_block([0.5 * (box - side * dx), 0.5 * (box - side * dx), 0.02], [side] * 3,
+                        dx, rng, jitter)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    integ = "TakePredicted" if solver == "IISPHwOST" else "EulerCromer"
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver=solver, integrator=integ)
+    lo_b, hi_b = np.zeros(3), np.full(3, box)
+    sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
+    return Scene(f"small_box_{solver}", p, x, v, m, fid, [VolumeMap(sdf, vm, None, "box")])
+
+
+def small_rigid_scene(side=(14, 14, 8), seed=2, dx=0.02, solver="IISPH") -> Scene:
+    """Small two-way coupling case: fluid layer + one dynamic cube."""
+    rng = np.random.default_rng(seed)
+    pos = lattice_block([dx, dx, dx], side, dx, rng, 1e-4)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver=solver, integrator="EulerCromer")
+    lo_b, hi_b = np.zeros(3), np.array([(side[0] + 2) * dx, (side[1] + 2) * dx, 0.5])
+    sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
+    size = 0.1
+    lo_l, hi_l = np.full(3, -0.5 * size), np.full(3, 0.5 * size)
+    mass = 600.0 * size ** 3
+    rb = Rigid(mass=mass, inertia_body=np.eye(3) * (mass * size * size / 6.0),
+               com=np.array([0.5 * side[0] * dx + 0.013, 0.5 * side[1] * dx - 0.007,
+                             (side[2] + 1) * dx + 0.5 * size + 2 * dx]),
+               q=np.array([math.cos(0.1), math.sin(0.1) * 0.6, 0.0, math.sin(0.1) * 0.8]),
+               ang_vel=np.array([0.3, -0.2, 0.5]))
+    s_l, v_l = volume_map_fields(lambda q: sd_box_solid(q, lo_l, hi_l), lo_l, hi_l, h, dx)
+    return Scene("small_rigid", p, x, v, m, fid,
+                 [VolumeMap(sdf, vm, None, "box"), VolumeMap(s_l, v_l, rb, "cube")])

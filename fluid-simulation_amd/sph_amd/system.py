@@ -1,0 +1,269 @@
+"""Host-side mirror of the reference's plug-in API over the HIP library.
+
+Names follow simulation_lib: ``SpatialHashing.find_samples`` returns a
+``NeighborList`` (neighbor_search/mod.rs:18-161); ``System`` mirrors
+``System<K,I,P,N,B>`` with ``step_forward_in_time`` / ``take_measurement``
+(sph/mod.rs:40-184); the per-trait functions ``get_volume``,
+``find_boundary_samples``, ``add_non_pressure_accelerations`` and
+``solve_and_add_acceleration`` mirror the three traits a ``System`` is generic
+over. Every call runs on the GPU through libsphhip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .scenes import INTEGRATORS, SOLVERS, Params, Scene
+
+
+def _params_struct(P: Params) -> N.SphParams:
+    s = N.SphParams()
+    s.time_increment = P.time_increment
+    s.cfl = int(bool(P.cfl))
+    s.max_time_increment = P.max_time_increment
+    s.cfl_number = P.cfl_number
+    s.kernel_support_radius = P.kernel_support_radius
+    s.rest_density_grid_spacing = P.rest_density_grid_spacing
+    s.disable_particles_below = P.disable_particles_below
+    s.fluid_viscosity = P.fluid_viscosity
+    s.boundary_viscosity = P.boundary_viscosity
+    s.boundary_pressure_acceleration_weighting = P.boundary_pressure_acceleration_weighting
+    s.boundary_rest_volume_weighting = P.boundary_rest_volume_weighting
+    s.gravity_mode = int(P.gravity_mode)
+    for d in range(3):
+        s.gravity_center[d] = float(P.gravity_center[d])
+    s.solver = SOLVERS[P.solver]
+    s.integrator = INTEGRATORS[P.integrator]
+    s.stiffness = P.stiffness
+    s.target_density_error = P.target_density_error
+    s.relaxation_factor = P.relaxation_factor
+    s.min_diagonal_element = P.min_diagonal_element
+    return s
+
+
+def _lattice_struct(lat):
+    s = N.SphLattice()
+    for d in range(3):
+        s.x_min[d] = float(lat.x_min[d])
+        s.x_max[d] = float(lat.x_max[d])
+        s.n[d] = int(lat.n[d])
+    s.dx = float(lat.dx)
+    vals = np.ascontiguousarray(lat.values, dtype=np.float64)
+    pres = np.ascontiguousarray(lat.present, dtype=np.uint8)
+    s.values = N.dptr(vals)
+    s.present = pres.ctypes.data_as(N._u8p)
+    s._keep = (vals, pres)
+    return s
+
+
+def _rigid_struct(rb):
+    if rb is None:
+        return None
+    s = N.SphRigidInit()
+    ib = np.asarray(rb.inertia_body, dtype=np.float64)
+    s.mass = float(rb.mass)
+    for k, v in enumerate(ib.ravel()):
+        s.inertia_body[k] = float(v)
+    for k, v in enumerate(np.linalg.inv(ib).ravel()):
+        s.inertia_inv_body[k] = float(v)
+    for k in range(3):
+        s.com[k] = float(rb.com[k])
+        s.lin_vel[k] = float(rb.lin_vel[k])
+        s.ang_vel[k] = float(rb.ang_vel[k])
+    for k in range(4):
+        s.q[k] = float(rb.q[k])
+    return s
+
+
+class NeighborList:
+    """CSR neighbor list (neighbor_search/mod.rs:49-161)."""
+
+    def __init__(self, offsets, indices):
+        self.offsets = offsets
+        self.indices = indices
+
+    def num_samples(self):
+        return len(self.offsets) - 1
+
+    def get_neighbors(self, i):
+        return self.indices[self.offsets[i]:self.offsets[i + 1]]
+
+
+class _Ctx:
+    def __init__(self, params: Params):
+        N.lib()
+        self._p = _params_struct(params)
+        h = C.c_void_p()
+        N.check(N.lib().sph_ctx_create(C.byref(self._p), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().sph_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SpatialHashing:
+    """``SpatialHashing::new(search_range)`` + ``find_samples`` on the GPU."""
+
+    def __init__(self, search_range: float):
+        self.cell_size = float(search_range)
+        self._ctx = _Ctx(Params(kernel_support_radius=self.cell_size,
+                                rest_density_grid_spacing=self.cell_size))
+
+    def find_samples(self, search_range, positions, sample_positions) -> NeighborList:
+        pos = N.f64(positions).reshape(-1, 3)
+        smp = N.f64(sample_positions).reshape(-1, 3)
+        n, m = len(pos), len(smp)
+        off = np.zeros(n + 1, dtype=np.uint64)
+        total = C.c_size_t(0)
+        L = N.lib()
+        cap = max(64 * n, 1)
+        idx = np.zeros(cap, dtype=np.uint64)
+        st = L.sph_neighbors_find(self._ctx.h, self.cell_size, float(search_range), N.dptr(pos),
+                                  n, N.dptr(smp), m, off.ctypes.data_as(N._u64p),
+                                  idx.ctypes.data_as(N._u64p), cap, C.byref(total))
+        if st == -4:  # SPH_ERR_CAPACITY: retry with the exact size
+            cap = int(total.value)
+            idx = np.zeros(max(cap, 1), dtype=np.uint64)
+            st = L.sph_neighbors_find(self._ctx.h, self.cell_size, float(search_range),
+                                      N.dptr(pos), n, N.dptr(smp), m,
+                                      off.ctypes.data_as(N._u64p), idx.ctypes.data_as(N._u64p),
+                                      cap, C.byref(total))
+        N.check(st)
+        return NeighborList(off, idx[:int(total.value)].copy())
+
+
+class System:
+    """Device-resident ``System<K,I,P,N,B>`` for one scene (sph/mod.rs:75-97)."""
+
+    def __init__(self, scene: Scene, init: bool = True):
+        self.scene = scene
+        self._ctx = _Ctx(scene.params)
+        L = N.lib()
+        self._keep = []
+        for b in scene.boundaries:
+            rig = _rigid_struct(getattr(b, "rigid", None))
+            rp = C.byref(rig) if rig is not None else None
+            if hasattr(b, "positions"):
+                pos = N.f64(b.positions)
+                self._keep.append(pos)
+                N.check(L.sph_boundary_add_samples(self._ctx.h, N.dptr(pos), len(pos), rp))
+            else:
+                sdf, vm = _lattice_struct(b.sdf), _lattice_struct(b.vm)
+                N.check(L.sph_boundary_add_volume_map(self._ctx.h, C.byref(sdf), C.byref(vm),
+                                                      rp))
+        x, v, m = N.f64(scene.x), N.f64(scene.v), N.f64(scene.m)
+        fid = np.ascontiguousarray(scene.fluid_id, dtype=np.uint32)
+        N.check(L.sph_fluid_upload(self._ctx.h, len(x), N.dptr(x), N.dptr(v), N.dptr(m),
+                                   fid.ctypes.data_as(N._u32p)))
+        if init:
+            N.check(L.sph_system_init(self._ctx.h))
+
+    @property
+    def handle(self):
+        return self._ctx.h
+
+    def step_forward_in_time(self, n: int = 1):
+        N.check(N.lib().sph_step(self._ctx.h, int(n), None))
+
+    def __len__(self):
+        n = C.c_size_t()
+        N.check(N.lib().sph_fluid_len(self._ctx.h, C.byref(n)))
+        return int(n.value)
+
+    def take_measurement(self) -> dict:
+        m = N.SphMeasurement()
+        N.check(N.lib().sph_measure(self._ctx.h, C.byref(m)))
+        return m.as_dict()
+
+    def state(self) -> dict:
+        n = len(self)
+        out = {k: np.zeros((n, 3)) for k in ("x", "v", "a")}
+        for k in ("V", "p", "m"):
+            out[k] = np.zeros(n)
+        out["fluid_id"] = np.zeros(n, dtype=np.uint32)
+        N.check(N.lib().sph_fluid_download(self._ctx.h, N.dptr(out["x"]), N.dptr(out["v"]),
+                                           N.dptr(out["a"]), N.dptr(out["V"]), N.dptr(out["p"]),
+                                           N.dptr(out["m"]),
+                                           out["fluid_id"].ctypes.data_as(N._u32p)))
+        return out
+
+    def error_history(self):
+        cnt = C.c_size_t()
+        buf = np.zeros(100000)
+        N.check(N.lib().sph_error_history(self._ctx.h, N.dptr(buf), len(buf), C.byref(cnt)))
+        return buf[:min(int(cnt.value), len(buf))].copy()
+
+    def body_state(self, b: int) -> dict:
+        out = np.zeros(22)
+        N.check(N.lib().sph_body_state(self._ctx.h, int(b), N.dptr(out)))
+        return {"com": out[0:3], "q": out[3:7], "lin_vel": out[7:10], "ang_mom": out[10:13],
+                "force": out[13:16], "torque": out[16:19], "ang_vel": out[19:22]}
+
+    def body_forces(self, b: int):
+        out = np.zeros(6)
+        N.check(N.lib().sph_body_forces(self._ctx.h, int(b), N.dptr(out)))
+        return out
+
+    def clone(self) -> "System":
+        """Deep copy into an independent device context (DynClone)."""
+        h = C.c_void_p()
+        N.check(N.lib().sph_ctx_clone(self._ctx.h, C.byref(h)))
+        other = System.__new__(System)
+        other.scene = self.scene
+        other._keep = self._keep
+        ctx = _Ctx.__new__(_Ctx)
+        ctx._p = self._ctx._p
+        ctx.h = h
+        other._ctx = ctx
+        return other
+
+    # ---- per-trait entry points (host arrays in and out) -------------------
+    def get_volume(self, x):
+        x = N.f64(x).reshape(-1, 3)
+        V = np.zeros(len(x))
+        N.check(N.lib().sph_volume(self._ctx.h, len(x), N.dptr(x), N.dptr(V)))
+        return V
+
+    def find_boundary_samples(self, b: int, x):
+        x = N.f64(x).reshape(-1, 3)
+        out = np.zeros((len(x), 7))
+        flags = np.zeros(len(x), dtype=np.uint8)
+        N.check(N.lib().sph_boundary_find_samples(self._ctx.h, int(b), len(x), N.dptr(x),
+                                                  N.dptr(out), flags.ctypes.data_as(N._u8p)))
+        return out, flags
+
+    def add_non_pressure_accelerations(self, x, v, m, V):
+        x = N.f64(x).reshape(-1, 3)
+        v = N.f64(v).reshape(-1, 3)
+        m, V = N.f64(m), N.f64(V)
+        a = np.zeros_like(x)
+        ft = np.zeros(6 * max(len(self.scene.boundaries), 1))
+        N.check(N.lib().sph_nonpressure_accel(self._ctx.h, len(x), N.dptr(x), N.dptr(v),
+                                              N.dptr(m), N.dptr(V), N.dptr(a), N.dptr(ft)))
+        return a, ft.reshape(-1, 6)
+
+    def solve_and_add_acceleration(self, x, v, m, V, a, dt):
+        x = N.f64(x).reshape(-1, 3)
+        v = N.f64(v).reshape(-1, 3)
+        m, V = N.f64(m), N.f64(V)
+        a = N.f64(a).reshape(-1, 3).copy()
+        p = np.zeros(len(x))
+        ft = np.zeros(6 * max(len(self.scene.boundaries), 1))
+        xp = np.zeros_like(x)
+        vp = np.zeros_like(x)
+        info = N.SphMeasurement()
+        N.check(N.lib().sph_pressure_solve(self._ctx.h, len(x), N.dptr(x), N.dptr(v), N.dptr(m),
+                                           N.dptr(V), float(dt), N.dptr(a), N.dptr(p),
+                                           N.dptr(ft), N.dptr(xp), N.dptr(vp), C.byref(info)))
+        return {"a": a, "p": p, "body_ft": ft.reshape(-1, 6), "x_pred": xp, "v_pred": vp,
+                "info": info.as_dict()}

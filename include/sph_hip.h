@@ -1,0 +1,194 @@
+/*
+ * sph_hip.h — C ABI of the MI355X (gfx950) SPH particle hot path.
+ *
+ * The entry points replace the three plug-in traits of the reference solver
+ * (SammyMcFly/fluid-simulation, simulation_lib) that System<K,I,P,N,B> is
+ * generic over, plus a device-resident stepping path:
+ *
+ *   NeighborSearch     neighbor_search/mod.rs:18-28
+ *   PressureSolver     sph/pressure_solver/mod.rs:37-78
+ *   BoundaryHandling   sph/boundary_handling/mod.rs:39-119
+ *   System::step_forward_in_time / update   sph/mod.rs:125-144, 674-731
+ *
+ * Conventions
+ *   - every function returns 0 on success or a negative sph_status; no C++
+ *     exception crosses the ABI; sph_last_error() returns a thread-local
+ *     message for the last failure on the calling thread;
+ *   - host arrays are caller-owned and only read/written during the call;
+ *     device buffers are owned by the context;
+ *   - positions/velocities are [n][3] doubles (the layout of
+ *     &[Point3<f64>] / &[Vector3<f64>] in the reference); neighbor indices are
+ *     uint64_t (usize in the reference);
+ *   - a context is used by one thread at a time (the reference's Send, not
+ *     Sync); independent contexts run on their own HIP streams;
+ *   - all arithmetic is IEEE binary64 with no FMA contraction.
+ */
+#ifndef SPH_HIP_H
+#define SPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sph_status {
+    SPH_OK = 0,
+    SPH_ERR_ARG = -1,          /* invalid argument */
+    SPH_ERR_HIP = -2,          /* HIP runtime error (device, launch, memory) */
+    SPH_ERR_STATE = -3,        /* call out of order (e.g. step before init) */
+    SPH_ERR_CAPACITY = -4,     /* caller buffer too small; *total tells the size */
+    SPH_ERR_UNSUPPORTED = -5,  /* input outside what this build handles */
+    SPH_ERR_NO_DEVICE = -6     /* no GPU visible */
+} sph_status;
+
+enum { SPH_SOLVER_SESPH = 0, SPH_SOLVER_IISPH = 1, SPH_SOLVER_IISPHWOST = 2 };
+enum { SPH_INTEGRATOR_EULER_CROMER = 0, SPH_INTEGRATOR_TAKE_PREDICTED = 1 };
+enum { SPH_GRAVITY_UNIFORM = 0, SPH_GRAVITY_RADIAL = 1 };
+
+/* [parameters] + [procedures] of the reference (sph/setup/input.rs:103-260,
+ * SystemParameters sph/mod.rs:755-830). */
+typedef struct sph_params {
+    double time_increment;           /* fixed dt when cfl == 0 */
+    int32_t cfl;                     /* 1: CFL adaptive dt (feature cfl_time_step) */
+    double max_time_increment;
+    double cfl_number;
+    double kernel_support_radius;    /* h */
+    double rest_density_grid_spacing;/* dx */
+    double disable_particles_below;
+    double fluid_viscosity;
+    double boundary_viscosity;
+    double boundary_pressure_acceleration_weighting;
+    double boundary_rest_volume_weighting;
+    int32_t gravity_mode;
+    double gravity_center[3];
+    int32_t solver;
+    int32_t integrator;
+    double stiffness;
+    double target_density_error;     /* percent */
+    double relaxation_factor;
+    double min_diagonal_element;
+} sph_params;
+
+/* CubicSerendipityDiscretization as a dense node lattice
+ * (utilities/discretization.rs:84-92): (3n+1)^3 nodes, x fastest;
+ * present[k] != 0 where the reference's HashMap holds node k. */
+typedef struct sph_lattice {
+    double x_min[3];
+    double x_max[3];
+    double dx;
+    int64_t n[3];
+    const double *values;
+    const uint8_t *present;
+} sph_lattice;
+
+/* RigidBodyMotion::new arguments (boundary_handling/rigid_body_motion.rs:38-62);
+ * matrices row-major, quaternion (w, i, j, k). */
+typedef struct sph_rigid_init {
+    double mass;
+    double inertia_body[9];
+    double inertia_inv_body[9];
+    double com[3];
+    double q[4];
+    double lin_vel[3];
+    double ang_vel[3];
+} sph_rigid_init;
+
+/* Measurement row (measurement.rs:52-75) plus step counters. */
+typedef struct sph_measurement {
+    double time;
+    double time_step_size;
+    double density;                  /* CurrentSystemProperties::average_density */
+    double density_error;            /* percent, sph/mod.rs:489-529 */
+    double kinetic_energy;
+    uint32_t solver_iterations;      /* last solve (EQS2 for IISPHwOST) */
+    uint32_t eqs1_iterations;        /* IISPHwOST EQS1, else 0 */
+    double predicted_density_error;  /* percent */
+    uint64_t steps;
+    uint64_t num_particles;
+} sph_measurement;
+
+const char *sph_last_error(void);
+
+/* ---- lifecycle ------------------------------------------------------- */
+typedef struct sph_ctx sph_ctx;
+int sph_ctx_create(const sph_params *params, sph_ctx **out);
+/* Deep copy of all device state into an independent context
+ * (SPHSystem: DynClone, sph/mod.rs:40,71). */
+int sph_ctx_clone(const sph_ctx *src, sph_ctx **out);
+int sph_ctx_destroy(sph_ctx *ctx);
+
+/* ---- NeighborSearch::find_samples (neighbor_search/mod.rs:18-28,
+ *      spatial_hashing.rs:130-158) ------------------------------------------
+ * SpatialHashing with cell size `cell_size`: for each of the n query
+ * positions, the indices of the m samples closer than `range` (strict),
+ * in the reference's order. offsets has n+1 entries; indices has room for
+ * `cap` entries; *total receives the entry count (SPH_ERR_CAPACITY if it
+ * exceeds cap). */
+int sph_neighbors_find(sph_ctx *ctx, double cell_size, double range, const double *pos,
+                       size_t n, const double *samples, size_t m, uint64_t *offsets,
+                       uint64_t *indices, size_t cap, size_t *total);
+
+/* ---- BoundaryHandling::add_static_boundary / add_dynamic_boundary ----------
+ * (boundary_handling/mod.rs:44-60). StaticSampleBoundary: sample positions
+ * (world frame when rigid == NULL, body frame otherwise).
+ * VolumeMapBoundary: signed-distance and volume-map lattices (body frame for
+ * dynamic bodies). Return the boundary index (>= 0) or a negative status. */
+int sph_boundary_add_samples(sph_ctx *ctx, const double *pos, size_t m,
+                             const sph_rigid_init *rigid);
+int sph_boundary_add_volume_map(sph_ctx *ctx, const sph_lattice *sdf, const sph_lattice *vm,
+                                const sph_rigid_init *rigid);
+
+/* ---- device-resident system ----------------------------------------------- */
+/* Fluid::add_samples equivalent: the fluid state (sph/fluid.rs:26-104). */
+int sph_fluid_upload(sph_ctx *ctx, size_t n, const double *x, const double *v, const double *m,
+                     const uint32_t *fluid_id);
+/* BoundaryHandling::initialize + System::new_boxed's update() (sph/mod.rs:389-421). */
+int sph_system_init(sph_ctx *ctx);
+/* nsteps x SPHSystem::step_forward_in_time (sph/mod.rs:125-144); *out may be NULL. */
+int sph_step(sph_ctx *ctx, int nsteps, sph_measurement *out);
+int sph_fluid_len(const sph_ctx *ctx, size_t *n);
+/* Fluid state in the reference's particle order (any pointer may be NULL). */
+int sph_fluid_download(sph_ctx *ctx, double *x, double *v, double *a, double *V, double *p,
+                       double *m, uint32_t *fluid_id);
+/* SPHSystem::take_measurement (sph/mod.rs:147-184). */
+int sph_measure(sph_ctx *ctx, sph_measurement *out);
+/* predicted density error (%) of every Jacobi iteration of the last solve */
+int sph_error_history(sph_ctx *ctx, double *out, size_t cap, size_t *count);
+/* rigid state: com[3], q[4], lin_vel[3], ang_mom[3], force[3], torque[3], ang_vel[3] */
+int sph_body_state(sph_ctx *ctx, int b, double out[22]);
+/* F[3], tau[3] applied to boundary b during the last update() (FP64) */
+int sph_body_forces(sph_ctx *ctx, int b, double out[6]);
+
+/* ---- per-trait entry points on host arrays --------------------------------
+ * Each call uploads the given fluid state, runs one phase of System::update
+ * on the device with the context's boundaries, and returns the result in the
+ * caller's particle order. */
+/* get_volume (sph/quantities.rs:14-61) */
+int sph_volume(sph_ctx *ctx, size_t n, const double *x, double *V_out);
+/* VolumeMapBoundary::find_boundary_samples for boundary b
+ * (volume_map_boundary.rs:1259-1424): per particle, 7 doubles (pos[3], vel[3],
+ * vol) of the normal-mode sample and flags[i] = 1 when it exists. */
+int sph_boundary_find_samples(sph_ctx *ctx, int b, size_t n, const double *x,
+                              double *samples_out, uint8_t *flags_out);
+/* reset_acceleration + add_gravity_acceleration + add_viscosity_acceleration
+ * (sph/non_pressure_accelerations.rs:19-185); V must hold get_volume's result. */
+int sph_nonpressure_accel(sph_ctx *ctx, size_t n, const double *x, const double *v,
+                          const double *m, const double *V, double *a_out,
+                          double *body_ft_out);
+/* PressureSolver::solve_and_add_acceleration (pressure_solver/mod.rs:65-72)
+ * for params.solver: a_inout holds the non-pressure acceleration on entry and
+ * the total on return; p_out the pressure; body_ft_out (6 per boundary, may be
+ * NULL) the force/torque mirrored onto dynamic bodies; for IISPHwOST,
+ * xpred_out/vpred_out (may be NULL) the predicted state committed by
+ * TakePredicted. */
+int sph_pressure_solve(sph_ctx *ctx, size_t n, const double *x, const double *v,
+                       const double *m, const double *V, double dt, double *a_inout,
+                       double *p_out, double *body_ft_out, double *xpred_out,
+                       double *vpred_out, sph_measurement *info_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,181 @@
+/*
+ * refcpu — CPU oracle for the SPH particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C, f64 restatement of the
+ * reference solver's per-step algorithm (SammyMcFly/fluid-simulation,
+ * simulation_lib). It exists to check the HIP implementation and to serve as
+ * the timed CPU baseline. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product path never does.
+ *
+ * Every function names the reference file:line it restates. Evaluation order
+ * of floating-point expressions follows the reference's Rust expressions
+ * (left-associative, powi(3) == (x*x)*x, nalgebra dot == (ax*bx+ay*by)+az*bz),
+ * and the library is compiled with -ffp-contract=off, because rustc never
+ * contracts a*b+c into an FMA.
+ *
+ * Parity pins (what checks each part) are listed in oracle/README.md.
+ */
+#ifndef REFCPU_H
+#define REFCPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- kernel (sph/kernel/mod.rs:17-46, cubic_spline.rs:8-17) */
+double ref_w(double q);
+double ref_dqw(double q);
+double ref_kernel_function(const double r[3], double h);
+void ref_kernel_gradient(const double r[3], double h, double out[3]);
+
+/* ---------------- spatial hashing (neighbor_search/spatial_hashing.rs) */
+void ref_get_cell(const double p[3], double cell_size, int32_t out[3]);
+uint64_t ref_map_int_to_uint(int32_t k);
+uint64_t ref_hash(int32_t k, int32_t l, int32_t m);
+
+/* CSR neighbor list (neighbor_search/mod.rs:49-161). */
+typedef struct ref_nlist {
+    size_t n;          /* number of query samples */
+    size_t total;      /* number of entries */
+    uint64_t *offsets; /* n + 1 */
+    uint64_t *indices; /* total */
+} ref_nlist;
+
+/* SpatialHashing::new(cell_size) + find_samples(range, positions, samples)
+ * (spatial_hashing.rs:27-36, 75-103, 118-158). Allocates *out. */
+int ref_find_samples(double cell_size, double range, const double *pos, size_t n,
+                     const double *samples, size_t m, ref_nlist *out);
+void ref_nlist_free(ref_nlist *nl);
+/* helpers for ctypes callers: one call allocates and returns a handle */
+ref_nlist *ref_find_samples_alloc(double cell_size, double range, const double *pos,
+                                  size_t n, const double *samples, size_t m);
+size_t ref_nlist_total(const ref_nlist *nl);
+void ref_nlist_copy(const ref_nlist *nl, uint64_t *offsets, uint64_t *indices);
+void ref_nlist_destroy(ref_nlist *nl);
+
+/* ---------------- cubic serendipity lattice (utilities/discretization.rs) */
+typedef struct ref_lattice {
+    double x_min[3];
+    double x_max[3];
+    double dx;
+    int64_t n[3];            /* cells per axis */
+    const double *values;    /* dense nodes (3n+1)^3, x fastest */
+    const uint8_t *present;  /* node present in the reference's HashMap */
+} ref_lattice;
+
+/* status codes of function()/gradient(): 0 ok, 1 OutOfBounds, 2 PrunedCell */
+int ref_serendipity_function(const ref_lattice *l, const double p[3], double *out);
+int ref_serendipity_gradient(const ref_lattice *l, const double p[3], double out[3]);
+void ref_shape_functions(double xi, double eta, double zeta, double out[32]);
+void ref_reference_nodes(double out[32 * 3]);
+int ref_node_offset(double r); /* to_offset (discretization.rs:262-264) */
+
+/* ---------------- rigid body (boundary_handling/rigid_body_motion.rs) */
+typedef struct ref_rigid {
+    double mass;
+    double inertia_inv_body[9];  /* row-major */
+    double com[3];
+    double q[4];                 /* w, i, j, k */
+    double lin_vel[3];
+    double ang_mom[3];
+    double force[3];
+    double torque[3];
+    double inertia_inv_world[9];
+    double ang_vel[3];
+} ref_rigid;
+
+void ref_rigid_init(ref_rigid *rb, double mass, const double inertia_body[9],
+                    const double inertia_inv_body[9], const double com[3],
+                    const double q[4], const double lin_vel[3], const double ang_vel[3]);
+void ref_rigid_add_force(ref_rigid *rb, const double f[3], const double loc[3]);
+void ref_rigid_step(ref_rigid *rb, double dt);
+void ref_rigid_world_to_local(const ref_rigid *rb, const double p[3], double out[3]);
+void ref_rigid_local_to_world(const ref_rigid *rb, const double p[3], double out[3]);
+void ref_rigid_rotate(const ref_rigid *rb, const double v[3], double out[3]);
+void ref_rigid_velocity_at_point(const ref_rigid *rb, const double p[3], double out[3]);
+void ref_quat_rotate(const double q[4], const double v[3], double out[3]);
+void ref_quat_to_matrix(const double q[4], double R[9]);
+
+/* ---------------- system (sph/mod.rs) */
+enum { REF_SESPH = 0, REF_IISPH = 1, REF_IISPHWOST = 2 };
+enum { REF_EULER_CROMER = 0, REF_TAKE_PREDICTED = 1 };
+enum { REF_GRAVITY_UNIFORM = 0, REF_GRAVITY_RADIAL = 1 };
+
+typedef struct ref_params {
+    double time_increment;      /* fixed dt (cfl == 0) */
+    int32_t cfl;                /* 1: CFL adaptive dt (feature cfl_time_step) */
+    double max_time_increment;
+    double cfl_number;
+    double kernel_support_radius;
+    double rest_density_grid_spacing;
+    double disable_particles_below;
+    double fluid_viscosity;
+    double boundary_viscosity;
+    double boundary_pressure_acceleration_weighting;
+    double boundary_rest_volume_weighting;
+    int32_t gravity_mode;
+    double gravity_center[3];
+    int32_t solver;
+    int32_t integrator;
+    double stiffness;
+    double target_density_error; /* percent */
+    double relaxation_factor;
+    double min_diagonal_element;
+} ref_params;
+
+typedef struct ref_measurement {
+    double time;
+    double time_step_size;
+    double density;
+    double density_error;
+    double kinetic_energy;
+    uint32_t solver_iterations;
+    uint32_t eqs1_iterations;   /* IISPHwOST first solve, 0 otherwise */
+    double predicted_density_error;
+    uint64_t steps;
+    uint64_t num_particles;
+} ref_measurement;
+
+typedef struct ref_system ref_system;
+
+ref_system *ref_system_create(const ref_params *p, size_t n, const double *x,
+                              const double *v, const double *m, const uint32_t *fluid_id);
+void ref_system_destroy(ref_system *s);
+/* StaticSampleBoundary entry: samples in world frame (static) or body frame
+ * (dynamic; rigid_init != NULL). Returns boundary index or -1. */
+int ref_system_add_static_samples(ref_system *s, const double *pos, size_t m,
+                                  const double *rigid_init);
+/* VolumeMapBoundary entry. rigid_init (or NULL) = mass, I_body[9], I_inv_body[9],
+ * com[3], q[4], v[3], omega[3]  (32 doubles). */
+int ref_system_add_volume_map(ref_system *s, const ref_lattice *sdf,
+                              const ref_lattice *vm, const double *rigid_init);
+/* BoundaryHandling::initialize + resize_slots + update() (System::new_boxed). */
+int ref_system_init(ref_system *s);
+int ref_system_step(ref_system *s, int nsteps);
+size_t ref_system_len(const ref_system *s);
+void ref_system_get(const ref_system *s, double *x, double *v, double *a, double *V,
+                    double *p, double *m, uint32_t *fluid_id, uint64_t *origin);
+void ref_system_measure(const ref_system *s, ref_measurement *out);
+/* per-iteration predicted density errors of the last solve (TDE solve) */
+size_t ref_system_error_history(const ref_system *s, double *out, size_t cap);
+/* body state: com[3], q[4], lin_vel[3], ang_mom[3], force[3], torque[3], ang_vel[3] */
+int ref_system_body_state(const ref_system *s, int b, double out[22]);
+/* forces/torques applied to dynamic bodies during the last update(), in FP64:
+ * F[3], tau[3] per boundary (zeros for static) */
+int ref_system_body_forces(const ref_system *s, int b, double out[6]);
+/* fluid neighbor list of the last update() */
+const ref_nlist *ref_system_fluid_nlist(const ref_system *s);
+/* boundary samples of boundary b for fluid particle i, mode 0 normal, 1 viscosity:
+ * writes up to cap entries of (pos[3], vel[3], vol) and returns the count */
+size_t ref_system_boundary_samples(const ref_system *s, int b, size_t i, int mode,
+                                   double *out, size_t cap);
+double ref_system_static_volume(const ref_system *s, int b, size_t k);
+void ref_system_set_threads(int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
