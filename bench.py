@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the SPH particle hot path on one MI355X.
+
+Workload: the two-fluid large ball, 1,000,000 particles (a 100^3 lattice at
+dx = 0.01 m split at x = 0 into rest densities 1000 and 1500) in a radius-1 m
+spherical volume-map boundary, IISPHwOST + TakePredicted, CFL dt
+(lambda 0.4, max 1e-3), target average density error 0.1 %.
+
+A step is one SPHSystem::step_forward_in_time on the device-resident state.
+W warm-up steps run untimed, then exactly K steps are timed between device
+synchronizations. value = particles x K / wall time (particle-steps/s).
+
+The JSON line also carries
+  roofline      the dominant kernel's algorithmic HBM bytes per launch divided
+                by its mean launch duration (HIP events on the library's own
+                stream, over the timed steps), against 8 TB/s;
+  cpu_baseline  the CPU oracle (binary64 restatement of the reference) timed
+                on this host over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fluid-simulation_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) HBM bytes per particle per launch, binary64 SoA:
+# the particle's own fields read once and its outputs written once; neighbor
+# data is gathered from cache after the cell sort and the neighbor-index list
+# is not counted. `per_body` is added per volume-map boundary (the particle's
+# boundary-sample slot).
+ALGO_BYTES = {
+    "neighbor_count": {"base": 28, "per_body": 0},    # x,y,z r; count w
+    "pairs_volume": {"base": 32, "per_body": 33},     # x,y,z r; V w
+    "boundary_sample": {"base": 81, "per_body": 0},   # x,y,z r; flag + 7 f64 w
+    "nonpressure": {"base": 88, "per_body": 33},      # x,v,m,V r; a w
+    "source_diag": {"base": 88, "per_body": 57},      # x,v*,m,V r; s,a_ii,p w
+    "pressure_accel": {"base": 72, "per_body": 33},   # x,m,V,p r; a w
+    "jacobi": {"base": 88, "per_body": 33},           # x,V,a_p,s,a_ii r; p r/w
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_scene(workload, side):
+    from sph_amd import scenes
+    if workload == "two_fluid_ball":
+        return scenes.two_fluid_ball(side=side, solver="IISPHwOST")
+    if workload == "two_fluid_ball_iisph":
+        return scenes.two_fluid_ball(side=side, solver="IISPH")
+    if workload == "block_drop":
+        return scenes.block_drop_iisph()
+    raise SystemExit(f"unknown workload {workload}")
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(scene, budget_s, threads):
+    """The oracle on this host's cores over a bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcpu
+    t0 = time.perf_counter()
+    o = refcpu.OracleSystem(scene, threads=threads)
+    setup = time.perf_counter() - t0
+    steps, elapsed = 0, 0.0
+    while steps < 50 and (steps == 0 or elapsed < budget_s):
+        t = time.perf_counter()
+        o.step(1)
+        elapsed += time.perf_counter() - t
+        steps += 1
+    m = o.measure()
+    return {
+        "value": scene.n * steps / elapsed,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle (binary64 C restatement of the reference CPU path, OpenMP) on the "
+                   f"same {scene.n}-particle initial state: the first {steps} step(s) after "
+                   f"System::new, {elapsed:.1f} s of stepping ({setup:.1f} s setup untimed); "
+                   f"IISPH iterations {m['solver_iterations']}, density error "
+                   f"{m['density_error']:.4f} %"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="two_fluid_ball")
+    ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M particles)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; the hot path's "
+                          "multi-GPU decomposition is described in DESIGN.md and not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from sph_amd import system
+    scene = build_scene(args.workload, args.side)
+    log(f"[bench] scene {scene.name}: {scene.n} particles")
+    sim = system.System(scene)
+    for _ in range(args.warmup):
+        sim.step_forward_in_time(1)
+    sim.take_measurement()  # drains the stream
+    sim.profile(True)
+    lat = []
+    iters = []
+    errs = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        sim.step_forward_in_time(1)   # returns after the step's last device read-back
+        lat.append(time.perf_counter() - t)
+        m = sim.take_measurement()
+        iters.append(m["solver_iterations"] + m["eqs1_iterations"])
+        errs.append(m["density_error"])
+    # each step call returns after the step's final device read-back, so the
+    # sum of the step latencies is the device-synchronized time of the K steps
+    total = float(np.sum(lat))
+    wall = time.perf_counter() - t_start
+    prof = sim.profile_read()
+    sim.profile(False)
+    n = len(sim)
+    value = n * args.steps / total
+
+    nb = len(scene.boundaries)
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    per_unit = ALGO_BYTES[dom]["base"] + ALGO_BYTES[dom]["per_body"] * nb
+    bytes_per_launch = per_unit * d["units"] / max(d["launches"], 1)
+    avg_s = d["ms"] / 1e3 / max(d["launches"], 1)
+    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+
+    out = {
+        "metric": "particle-steps/sec (IISPH, 1M particles) at <=0.1% density error; p50/p99 step latency",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "wall_s_incl_measurements": wall,
+        "p50_step_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_step_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lattice + jitter, analytic volume-map boundary)",
+        "config": {"workload": scene.name, "particles": n, "solver": scene.params.solver,
+                   "integrator": scene.params.integrator, "dt": "CFL 0.4, max 1e-3",
+                   "target_density_error_pct": scene.params.target_density_error,
+                   "parallelism": "single GPU"},
+        "density_error_pct_max": float(max(errs)),
+        "density_error_pct_last": float(errs[-1]),
+        "jacobi_iterations_per_step_mean": float(np.mean(iters)),
+        "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(dom),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": avg_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        try:
+            out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds, threads)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline unavailable: {e}")
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

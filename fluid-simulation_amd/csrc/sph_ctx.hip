@@ -51,6 +51,80 @@ struct Error : std::runtime_error {
 static unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------------------
+// optional per-kernel timing with HIP events on the context's stream
+// ------------------------------------------------------------------------
+struct Prof {
+    bool on = false;
+    struct Rec {
+        int id;
+        size_t units;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    uint64_t launches[SPH_PROF_KERNELS] = {0};
+    uint64_t units[SPH_PROF_KERNELS] = {0};
+    double ms[SPH_PROF_KERNELS] = {0};
+
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCK(hipEventCreate(&e));
+        return e;
+    }
+    void begin(hipStream_t s, Rec &r) {
+        r.a = get();
+        r.b = get();
+        HIPCK(hipEventRecord(r.a, s));
+    }
+    void end(hipStream_t s, Rec &r) {
+        HIPCK(hipEventRecord(r.b, s));
+        pending.push_back(r);
+    }
+    void drain() {  // caller synchronized the stream
+        for (auto &r : pending) {
+            float t = 0;
+            HIPCK(hipEventElapsedTime(&t, r.a, r.b));
+            launches[r.id]++;
+            units[r.id] += r.units;
+            ms[r.id] += t;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        for (int k = 0; k < SPH_PROF_KERNELS; k++) {
+            launches[k] = units[k] = 0;
+            ms[k] = 0;
+        }
+    }
+    ~Prof() {
+        for (auto &r : pending) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// pair-cache arguments of the neighbor passes
+#define ELL(c) (c)->nbr_cnt.p, (c)->slice_off.p, (c)->ell_j.p, (c)->ell_gx.p, (c)->ell_gy.p, (c)->ell_gz.p
+
+// LAUNCH with event timing under profile id `pid` (units = particles)
+#define PLAUNCH(ctx, pid, kernel, n, ...)                        \
+    do {                                                         \
+        Prof::Rec r_{pid, (size_t)(n), nullptr, nullptr};        \
+        if ((ctx)->prof.on) (ctx)->prof.begin((ctx)->s, r_);     \
+        LAUNCH(kernel, n, (ctx)->s, __VA_ARGS__);                \
+        if ((ctx)->prof.on) (ctx)->prof.end((ctx)->s, r_);       \
+    } while (0)
+
+// ------------------------------------------------------------------------
 // device buffer
 // ------------------------------------------------------------------------
 template <class T>
@@ -320,6 +394,9 @@ struct sph_ctx {
     DBuf<uint32_t> perm, cell_cnt, cell_start, cursor;
     GridDev grid{};
     DBuf<uint32_t> nbr_off, nbr_idx, nbr_cnt;
+    // wavefront-sliced ELL pair cache: neighbor index + grad W per slot
+    DBuf<uint32_t> slice_len, slice_off, ell_j;
+    DBuf<double> ell_gx, ell_gy, ell_gz;
     DBuf<uint32_t> scan_a, scan_b, scan_c;
     // reductions
     DBuf<double> partials, red;
@@ -339,6 +416,7 @@ struct sph_ctx {
     uint32_t last_iter = 0, eqs1_iter = 0;
     double last_err = 0, avg_density = 0;
     std::vector<double> err_hist;
+    Prof prof;
 };
 
 namespace sph {
@@ -560,15 +638,30 @@ static void fluid_neighbors(sph_ctx *c) {
     permute(c, c->m, c->tmp, n);
     permute(c, c->fid, c->tmpu, n);
     permute(c, c->ref, c->tmpu, n);
-    // neighbor list in sorted index space
-    LAUNCH(k_neighbors<false>, n, c->s, n, c->x.p, c->y.p, c->z.p, c->P.kernel_support_radius,
-           c->reach, c->dthr, c->grid, c->x.p, c->y.p, c->z.p, (const uint32_t *)nullptr,
-           c->nbr_cnt.p, (uint32_t *)nullptr);
-    uint32_t total = scan_u32(c, c->nbr_cnt.p, n, c->nbr_off.p, true);
-    c->nbr_idx.reserve(std::max<uint32_t>(total, 1));
-    LAUNCH(k_neighbors<true>, n, c->s, n, c->x.p, c->y.p, c->z.p, c->P.kernel_support_radius,
-           c->reach, c->dthr, c->grid, c->x.p, c->y.p, c->z.p, c->nbr_off.p,
-           (uint32_t *)nullptr, c->nbr_idx.p);
+    // neighbor counts and per-slice maxima; the pair cache is filled after the
+    // boundary samples exist (pairs_and_volume)
+    const size_t nsl = (n + 63) / 64;
+    c->slice_len.reserve(nsl + 1);
+    c->slice_off.reserve(nsl + 1);
+    PLAUNCH(c, SPH_PROF_NEIGHBOR_COUNT, k_count_ell, n, n, c->x.p, c->y.p, c->z.p,
+            c->P.kernel_support_radius, c->reach, c->dthr, c->grid, c->nbr_cnt.p,
+            c->slice_len.p);
+    uint32_t total = scan_u32(c, c->slice_len.p, nsl, c->slice_off.p, true);
+    uint64_t slots = (uint64_t)total * 64;
+    if (slots > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "neighbor cache exceeds 2^32 slots");
+    c->ell_j.reserve(std::max<uint64_t>(slots, 1));
+    c->ell_gx.reserve(std::max<uint64_t>(slots, 1));
+    c->ell_gy.reserve(std::max<uint64_t>(slots, 1));
+    c->ell_gz.reserve(std::max<uint64_t>(slots, 1));
+}
+
+// neighbor indices + grad W into the pair cache, and the volume (get_volume)
+static void pairs_and_volume(sph_ctx *c) {
+    const size_t n = c->n;
+    PLAUNCH(c, SPH_PROF_PAIRS_VOLUME, k_pairs_volume, n, n, c->K, c->V0,
+            c->P.kernel_support_radius, c->reach, c->dthr, c->grid, c->x.p, c->y.p, c->z.p,
+            c->slice_off.p, c->ell_j.p, c->ell_gx.p, c->ell_gy.p, c->ell_gz.p, c->d_bodies.p,
+            (int)c->bodies.size(), c->V.p);
 }
 
 // StaticSampleBoundary samples: sort once by (cell, index) and build their grid
@@ -654,7 +747,7 @@ static void boundary_samples(sph_ctx *c) {
             b.flag.reserve(n);
             DBuf<double> *f[] = {&b.bx, &b.by, &b.bz, &b.bvx, &b.bvy, &b.bvz, &b.bvol};
             for (auto *q : f) q->reserve(n);
-            LAUNCH(k_vm_sample, n, c->s, n, c->x.p, c->y.p, c->z.p, b.view(), range,
+            PLAUNCH(c, SPH_PROF_BOUNDARY_SAMPLE, k_vm_sample, n, n, c->x.p, c->y.p, c->z.p, b.view(), range,
                    c->P.rest_density_grid_spacing, vmax, b.flag.p, b.bx.p, b.by.p, b.bz.p,
                    b.bvx.p, b.bvy.p, b.bvz.p, b.bvol.p);
         }
@@ -691,8 +784,8 @@ static void apply_body_partials(sph_ctx *c, size_t n) {
 static void pressure_accel(sph_ctx *c, double *tx, double *ty, double *tz, int overwrite,
                            int mirror) {
     const size_t n = c->n;
-    LAUNCH(k_pressure_accel, n, c->s, n, c->K, c->P.boundary_pressure_acceleration_weighting,
-           c->x.p, c->y.p, c->z.p, c->m.p, c->V.p, c->p.p, c->nbr_off.p, c->nbr_idx.p,
+    PLAUNCH(c, SPH_PROF_PRESSURE_ACCEL, k_pressure_accel, n, n, c->K, c->P.boundary_pressure_acceleration_weighting,
+           c->x.p, c->y.p, c->z.p, c->m.p, c->V.p, c->p.p, ELL(c),
            c->d_bodies.p, (int)c->bodies.size(), overwrite, mirror, tx, ty, tz, c->partials.p);
     if (mirror) apply_body_partials(c, n);
 }
@@ -714,8 +807,8 @@ static uint32_t resolve_pressure(sph_ctx *c, int vp, int after) {
     if (n == 0) return 0;
     SolveConst S = solve_const(c);
     const int nb = (int)c->bodies.size();
-    LAUNCH(k_source_diag, n, c->s, n, c->K, S, vp, c->x.p, c->y.p, c->z.p, c->svx.p, c->svy.p,
-           c->svz.p, c->m.p, c->V.p, c->nbr_off.p, c->nbr_idx.p, c->d_bodies.p, nb, c->s_f.p,
+    PLAUNCH(c, SPH_PROF_SOURCE_DIAG, k_source_diag, n, n, c->K, S, vp, c->x.p, c->y.p, c->z.p, c->svx.p, c->svy.p,
+           c->svz.p, c->m.p, c->V.p, ELL(c), c->d_bodies.p, nb, c->s_f.p,
            c->a_ff.p, c->p.p);
     uint32_t iter = 0;
     double err = INFINITY;
@@ -725,14 +818,15 @@ static uint32_t resolve_pressure(sph_ctx *c, int vp, int after) {
         bool conv = after >= 0 ? (iter > (uint32_t)after) : (iter > 2 && err < c->P.target_density_error);
         if (conv) break;
         pressure_accel(c, c->pax.p, c->pay.p, c->paz.p, 1, 0);
-        LAUNCH(k_jacobi, n, c->s, n, c->K, S, c->x.p, c->y.p, c->z.p, c->V.p, c->pax.p,
-               c->pay.p, c->paz.p, c->s_f.p, c->a_ff.p, c->nbr_off.p, c->nbr_idx.p,
+        PLAUNCH(c, SPH_PROF_JACOBI, k_jacobi, n, n, c->K, S, c->x.p, c->y.p, c->z.p, c->V.p, c->pax.p,
+               c->pay.p, c->paz.p, c->s_f.p, c->a_ff.p, ELL(c),
                c->d_bodies.p, nb, c->p.p, c->partials.p);
         hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, c->s, c->partials.p,
                            (int)nbk, 1, c->red.p, 0);
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(c->pinned, c->red.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
         HIPCK(hipStreamSynchronize(c->s));
+        if (c->prof.on) c->prof.drain();
         err = c->pinned[0] / (double)n * 100.0;
         c->err_hist.push_back(err);
         iter++;
@@ -805,8 +899,8 @@ static void nonpressure(sph_ctx *c) {
         b.rb.add_force(f, b.rb.com);
         for (int k = 0; k < 3; k++) b.applied[k] += f[k];
     }
-    LAUNCH(k_nonpressure, n, c->s, n, c->K, NP, c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p,
-           c->vz.p, c->m.p, c->V.p, c->nbr_off.p, c->nbr_idx.p, c->d_bodies.p,
+    PLAUNCH(c, SPH_PROF_NONPRESSURE, k_nonpressure, n, n, c->K, NP, c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p,
+           c->vz.p, c->m.p, c->V.p, ELL(c), c->d_bodies.p,
            (int)c->bodies.size(), c->ax.p, c->ay.p, c->az.p, c->partials.p);
     apply_body_partials(c, n);
 }
@@ -902,13 +996,13 @@ static void update(sph_ctx *c) {
         for (int k = 0; k < 6; k++) b->applied[k] = 0;
     fluid_neighbors(c);
     boundary_samples(c);
-    LAUNCH(k_volume, n, c->s, n, c->K, c->V0, c->x.p, c->y.p, c->z.p, c->nbr_off.p,
-           c->nbr_idx.p, c->d_bodies.p, (int)c->bodies.size(), c->V.p);
+    pairs_and_volume(c);
     nonpressure(c);
     solve_pressure(c);
     double sums[3];
     measure_sums(c, sums);
     c->avg_density = n ? sums[0] / (double)n : 0.0;
+    if (c->prof.on) c->prof.drain();
 }
 
 static void integrate(sph_ctx *c) {
@@ -1019,6 +1113,7 @@ static void prepare_external(sph_ctx *c, size_t n, const double *x, const double
     set_host_fluid(c, n, x, v, m, nullptr);
     fluid_neighbors(c);
     boundary_samples(c);
+    pairs_and_volume(c);
 }
 
 static void split3(const std::vector<double> &a, const std::vector<double> &b,
@@ -1147,6 +1242,7 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
             std::vector<double> keep_dt{c->dt};
             fluid_neighbors(c);
             boundary_samples(c);
+            pairs_and_volume(c);
             c->dt = keep_dt[0];
         }
         c->initialized = s->initialized;
@@ -1426,12 +1522,31 @@ int sph_body_forces(sph_ctx *c, int bi, double out[6]) {
     });
 }
 
+int sph_profile(sph_ctx *c, int enable) {
+    return guarded([&] {
+        if (!c) throw Error(SPH_ERR_ARG, "null argument");
+        HIPCK(hipStreamSynchronize(c->s));
+        c->prof.drain();
+        c->prof.reset();
+        c->prof.on = enable != 0;
+    });
+}
+
+int sph_profile_read(sph_ctx *c, int kernel, uint64_t *launches, uint64_t *units, double *ms) {
+    return guarded([&] {
+        if (!c || kernel < 0 || kernel >= SPH_PROF_KERNELS) throw Error(SPH_ERR_ARG, "bad argument");
+        HIPCK(hipStreamSynchronize(c->s));
+        c->prof.drain();
+        if (launches) *launches = c->prof.launches[kernel];
+        if (units) *units = c->prof.units[kernel];
+        if (ms) *ms = c->prof.ms[kernel];
+    });
+}
+
 int sph_volume(sph_ctx *c, size_t n, const double *x, double *V_out) {
     return guarded([&] {
         if (!c || (!x && n) || (!V_out && n)) throw Error(SPH_ERR_ARG, "null argument");
         prepare_external(c, n, x, nullptr, nullptr);
-        LAUNCH(k_volume, n, c->s, n, c->K, c->V0, c->x.p, c->y.p, c->z.p, c->nbr_off.p,
-               c->nbr_idx.p, c->d_bodies.p, (int)c->bodies.size(), c->V.p);
         std::vector<double> A;
         download_field(c, c->V, A);
         if (n) memcpy(V_out, A.data(), n * sizeof(double));

@@ -289,6 +289,113 @@ template __global__ void k_neighbors<true>(size_t, const double *, const double 
                                            const uint32_t *, uint32_t *, uint32_t *);
 
 // ------------------------------------------------------------------------
+// wavefront-sliced ELL pair cache. For slice s (64 consecutive sorted
+// particles) the k-th neighbor of lane l lives at 64 (slice_off[s] + k) + l, so
+// every load of a neighbor pass is one contiguous 64-lane access. Each slot
+// holds the neighbor index and grad W_ij, computed once per update and reused
+// by every pass of the step (the reference recomputes grad W in every pass).
+// ------------------------------------------------------------------------
+__global__ void k_count_ell(size_t nq, const double *qx, const double *qy, const double *qz,
+                            double cs, int reach, double dthr, GridDev g, uint32_t *count,
+                            uint32_t *slice_len) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cnt = 0;
+    if (i < nq) {
+        const double px = qx[i], py = qy[i], pz = qz[i];
+        const int cx = sat_i32(floor(px / cs)), cy = sat_i32(floor(py / cs)),
+                  cz = sat_i32(floor(pz / cs));
+        int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
+        if (zlo < 0) zlo = 0;
+        if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
+        if (zlo <= zhi) {
+            for (int dx = -reach; dx <= reach; dx++) {
+                int gx = cx + dx - g.org[0];
+                if (gx < 0 || gx >= g.dim[0]) continue;
+                for (int dy = -reach; dy <= reach; dy++) {
+                    int gy = cy + dy - g.org[1];
+                    if (gy < 0 || gy >= g.dim[1]) continue;
+                    uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
+                    uint32_t a = g.start[row + zlo], b = g.start[row + zhi + 1];
+                    for (uint32_t j = a; j < b; j++) {
+                        double ddx = px - qx[j], ddy = py - qy[j], ddz = pz - qz[j];
+                        double d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                        cnt += d2 < dthr ? 1u : 0u;
+                    }
+                }
+            }
+        }
+        count[i] = cnt;
+    }
+    uint32_t mx = cnt;
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    if ((threadIdx.x & 63) == 0 && i < nq) slice_len[i >> 6] = mx;
+}
+
+// neighbor indices + grad W into the ELL cache, and get_volume
+// (quantities.rs:14-61) from the same distances
+__global__ void k_pairs_volume(size_t n, KernelConst K, double V0, double cs, int reach,
+                               double dthr, GridDev g, const double *x, const double *y,
+                               const double *z, const uint32_t *slice_off, uint32_t *nj,
+                               double *gxo, double *gyo, double *gzo, const BodyDev *bodies,
+                               int nb, double *V) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = x[i], py = y[i], pz = z[i];
+    const int cx = sat_i32(floor(px / cs)), cy = sat_i32(floor(py / cs)),
+              cz = sat_i32(floor(pz / cs));
+    size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+    double accu = 0.;
+    int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
+    if (zlo < 0) zlo = 0;
+    if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
+    if (zlo <= zhi) {
+        for (int dx = -reach; dx <= reach; dx++) {
+            int gx = cx + dx - g.org[0];
+            if (gx < 0 || gx >= g.dim[0]) continue;
+            for (int dy = -reach; dy <= reach; dy++) {
+                int gy = cy + dy - g.org[1];
+                if (gy < 0 || gy >= g.dim[1]) continue;
+                uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
+                uint32_t a = g.start[row + zlo], b = g.start[row + zhi + 1];
+                for (uint32_t j = a; j < b; j++) {
+                    double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
+                    double d2 = (rx * rx + ry * ry) + rz * rz;
+                    if (d2 < dthr) {
+                        double rn = sqrt(d2);
+                        double q = rn / K.h;
+                        accu += V0 * (K.wfac * kernel_w(q));
+                        double wx = 0.0, wy = 0.0, wz = 0.0;
+                        if (rn != 0.0) {
+                            double d = kernel_dqw(q);
+                            wx = ((K.gfac * rx) / rn) * d;
+                            wy = ((K.gfac * ry) / rn) * d;
+                            wz = ((K.gfac * rz) / rn) * d;
+                        }
+                        nj[e] = j;
+                        gxo[e] = wx;
+                        gyo[e] = wy;
+                        gzo[e] = wz;
+                        e += 64;
+                    }
+                }
+            }
+        }
+    }
+    for (int bi = 0; bi < nb; bi++) {
+        const BodyDev &b = bodies[bi];
+        if (b.kind == 0) {
+            for (uint32_t k = b.soff[i]; k < b.soff[i + 1]; k++) {
+                uint32_t s = b.sidx[k];
+                accu += b.svol[s] * kernel_fn(K, px - b.sx[s], py - b.sy[s], pz - b.sz[s]);
+            }
+        } else if (b.flag[i]) {
+            accu += b.bvol[i] * kernel_fn(K, px - b.bx[i], py - b.by[i], pz - b.bz[i]);
+        }
+    }
+    V[i] = V0 / accu;
+}
+
+// ------------------------------------------------------------------------
 // exclusive scan of uint32 (three-phase, 1024 elements per block)
 // ------------------------------------------------------------------------
 __global__ void k_scan_block(const uint32_t *in, size_t n, uint32_t *out, uint32_t *block_sums) {
@@ -417,43 +524,16 @@ __device__ inline void visc_samples(double px, double py, double pz, double ox, 
 }
 
 // ------------------------------------------------------------------------
-// get_volume (quantities.rs:14-61)
-// ------------------------------------------------------------------------
-__global__ void k_volume(size_t n, KernelConst K, double V0, const double *x, const double *y,
-                         const double *z, const uint32_t *off, const uint32_t *idx,
-                         const BodyDev *bodies, int nb, double *V) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double px = x[i], py = y[i], pz = z[i];
-    double accu = 0.;
-    for (uint32_t e = off[i]; e < off[i + 1]; e++) {
-        uint32_t j = idx[e];
-        accu += V0 * kernel_fn(K, px - x[j], py - y[j], pz - z[j]);
-    }
-    for (int bi = 0; bi < nb; bi++) {
-        const BodyDev &b = bodies[bi];
-        if (b.kind == 0) {
-            for (uint32_t e = b.soff[i]; e < b.soff[i + 1]; e++) {
-                uint32_t s = b.sidx[e];
-                accu += b.svol[s] * kernel_fn(K, px - b.sx[s], py - b.sy[s], pz - b.sz[s]);
-            }
-        } else if (b.flag[i]) {
-            accu += b.bvol[i] * kernel_fn(K, px - b.bx[i], py - b.by[i], pz - b.bz[i]);
-        }
-    }
-    V[i] = V0 / accu;
-}
-
-// ------------------------------------------------------------------------
 // reset + gravity + artificial viscosity (non_pressure_accelerations.rs:19-185)
 // with the reaction on dynamic bodies reduced per block in FP64.
 // ------------------------------------------------------------------------
 __global__ void k_nonpressure(size_t n, KernelConst K, NonPressureConst P, const double *x,
                               const double *y, const double *z, const double *vx,
                               const double *vy, const double *vz, const double *m,
-                              const double *V, const uint32_t *off, const uint32_t *idx,
-                              const BodyDev *bodies, int nb, double *ax, double *ay, double *az,
-                              double *body_partials) {
+                              const double *V, const uint32_t *cnt, const uint32_t *slice_off,
+                              const uint32_t *nj, const double *gxa, const double *gya,
+                              const double *gza, const BodyDev *bodies, int nb, double *ax,
+                              double *ay, double *az, double *body_partials) {
     __shared__ double lds[6 * kBlock];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < n;
@@ -475,17 +555,18 @@ __global__ void k_nonpressure(size_t n, KernelConst K, NonPressureConst P, const
             gy = 0.0 + (9.81 * dy) / dist;
             gz = 0.0 + (9.81 * dz) / dist;
         }
-        for (uint32_t e = off[i]; e < off[i + 1]; e++) {
-            uint32_t j = idx[e];
+        const uint32_t c = cnt[i];
+        size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+        for (uint32_t k = 0; k < c; k++, e += 64) {
+            uint32_t j = nj[e];
             double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
-            double wx, wy, wz;
-            kernel_grad(K, rx, ry, rz, wx, wy, wz);
-            double c = P.nu_f * 2. * (3. + 2.) * V[j] *
-                       dot3(ux - vx[j], uy - vy[j], uz - vz[j], rx, ry, rz) /
-                       (dot3(rx, ry, rz, rx, ry, rz) + P.eps2);
-            acx += c * wx;
-            acy += c * wy;
-            acz += c * wz;
+            double wx = gxa[e], wy = gya[e], wz = gza[e];
+            double cf = P.nu_f * 2. * (3. + 2.) * V[j] *
+                        dot3(ux - vx[j], uy - vy[j], uz - vz[j], rx, ry, rz) /
+                        (dot3(rx, ry, rz, rx, ry, rz) + P.eps2);
+            acx += cf * wx;
+            acy += cf * wy;
+            acz += cf * wz;
         }
     }
     for (int bi = 0; bi < nb; bi++) {
@@ -563,26 +644,30 @@ __global__ void k_sesph_pressure(size_t n, double k, double V0, const double *V,
 // ------------------------------------------------------------------------
 __global__ void k_pressure_accel(size_t n, KernelConst K, double gamma, const double *x,
                                  const double *y, const double *z, const double *m,
-                                 const double *V, const double *p, const uint32_t *off,
-                                 const uint32_t *idx, const BodyDev *bodies, int nb,
-                                 int overwrite, int mirror, double *tx_, double *ty_,
-                                 double *tz_, double *body_partials) {
+                                 const double *V, const double *p, const uint32_t *cnt,
+                                 const uint32_t *slice_off, const uint32_t *nj,
+                                 const double *gxa, const double *gya, const double *gza,
+                                 const BodyDev *bodies, int nb, int overwrite, int mirror,
+                                 double *tx_, double *ty_, double *tz_, double *body_partials) {
     __shared__ double lds[6 * kBlock];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < n;
     double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0;
     double acx = 0., acy = 0., acz = 0.;
     if (live) {
-        px = x[i]; py = y[i]; pz = z[i];
         Vi = V[i]; mi = m[i]; pi = p[i];
-        for (uint32_t e = off[i]; e < off[i + 1]; e++) {
-            uint32_t j = idx[e];
-            double wx, wy, wz;
-            kernel_grad(K, px - x[j], py - y[j], pz - z[j], wx, wy, wz);
-            double c = Vi / mi * V[j] * (pi + p[j]);
-            acx -= c * wx;
-            acy -= c * wy;
-            acz -= c * wz;
+        const uint32_t c = cnt[i];
+        size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+        const double vm = Vi / mi;
+        for (uint32_t k = 0; k < c; k++, e += 64) {
+            uint32_t j = nj[e];
+            double cf = vm * V[j] * (pi + p[j]);
+            acx -= cf * gxa[e];
+            acy -= cf * gya[e];
+            acz -= cf * gza[e];
+        }
+        if (nb) {
+            px = x[i]; py = y[i]; pz = z[i];
         }
     }
     for (int bi = 0; bi < nb; bi++) {
@@ -660,9 +745,10 @@ __global__ void k_pred_vel(size_t n, double dt, int to_pred, const double *vx, c
 __global__ void k_source_diag(size_t n, KernelConst K, SolveConst S, int vp, const double *x,
                               const double *y, const double *z, const double *svx,
                               const double *svy, const double *svz, const double *m,
-                              const double *V, const uint32_t *off, const uint32_t *idx,
-                              const BodyDev *bodies, int nb, double *s_f, double *a_ff,
-                              double *p) {
+                              const double *V, const uint32_t *cnt, const uint32_t *slice_off,
+                              const uint32_t *nj, const double *gxa, const double *gya,
+                              const double *gza, const BodyDev *bodies, int nb, double *s_f,
+                              double *a_ff, double *p) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double px = x[i], py = y[i], pz = z[i];
@@ -671,10 +757,11 @@ __global__ void k_source_diag(size_t n, KernelConst K, SolveConst S, int vp, con
     const double dt = S.dt, dt2 = S.dt * S.dt;
     double src = vp ? 1. - S.V0 / Vi : 0.;
     double sfx = 0., sfy = 0., sfz = 0., sf2 = 0., sbx = 0., sby = 0., sbz = 0.;
-    for (uint32_t e = off[i]; e < off[i + 1]; e++) {
-        uint32_t j = idx[e];
-        double wx, wy, wz;
-        kernel_grad(K, px - x[j], py - y[j], pz - z[j], wx, wy, wz);
+    const uint32_t c = cnt[i];
+    size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+    for (uint32_t k = 0; k < c; k++, e += 64) {
+        uint32_t j = nj[e];
+        double wx = gxa[e], wy = gya[e], wz = gza[e];
         double Vj = V[j];
         src -= dt * Vj * dot3(ux - svx[j], uy - svy[j], uz - svz[j], wx, wy, wz);
         sfx += Vj * wx;
@@ -719,21 +806,25 @@ __global__ void k_source_diag(size_t n, KernelConst K, SolveConst S, int vp, con
 __global__ void k_jacobi(size_t n, KernelConst K, SolveConst S, const double *x, const double *y,
                          const double *z, const double *V, const double *pax, const double *pay,
                          const double *paz, const double *s_f, const double *a_ff,
-                         const uint32_t *off, const uint32_t *idx, const BodyDev *bodies, int nb,
-                         double *p, double *err_partials) {
+                         const uint32_t *cnt, const uint32_t *slice_off, const uint32_t *nj,
+                         const double *gxa, const double *gya, const double *gza,
+                         const BodyDev *bodies, int nb, double *p, double *err_partials) {
     __shared__ double lds[kBlock];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double err = 0.;
     if (i < n) {
-        const double px = x[i], py = y[i], pz = z[i];
         const double ux = pax[i], uy = pay[i], uz = paz[i];
         const double dt2 = S.dt * S.dt;
         double adp = 0.;
-        for (uint32_t e = off[i]; e < off[i + 1]; e++) {
-            uint32_t j = idx[e];
-            double wx, wy, wz;
-            kernel_grad(K, px - x[j], py - y[j], pz - z[j], wx, wy, wz);
-            adp += dt2 * V[j] * dot3(ux - pax[j], uy - pay[j], uz - paz[j], wx, wy, wz);
+        const uint32_t c = cnt[i];
+        size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+        for (uint32_t k = 0; k < c; k++, e += 64) {
+            uint32_t j = nj[e];
+            adp += dt2 * V[j] * dot3(ux - pax[j], uy - pay[j], uz - paz[j], gxa[e], gya[e], gza[e]);
+        }
+        double px = 0, py = 0, pz = 0;
+        if (nb) {
+            px = x[i]; py = y[i]; pz = z[i];
         }
         for (int bi = 0; bi < nb; bi++) {
             const BodyDev &b = bodies[bi];

@@ -66,39 +66,48 @@ __global__ void k_vm_sample(size_t n, const double *x, const double *y, const do
                             BodyDev b, double range, double dxr, double vmax, uint8_t *flag,
                             double *bx, double *by, double *bz, double *bvx, double *bvy,
                             double *bvz, double *bvol);
-__global__ void k_volume(size_t n, KernelConst K, double V0, const double *x, const double *y,
-                         const double *z, const uint32_t *off, const uint32_t *idx,
-                         const BodyDev *bodies, int nb, double *V);
+__global__ void k_count_ell(size_t nq, const double *qx, const double *qy, const double *qz,
+                            double cs, int reach, double dthr, GridDev g, uint32_t *count,
+                            uint32_t *slice_len);
+__global__ void k_pairs_volume(size_t n, KernelConst K, double V0, double cs, int reach,
+                               double dthr, GridDev g, const double *x, const double *y,
+                               const double *z, const uint32_t *slice_off, uint32_t *nj,
+                               double *gxo, double *gyo, double *gzo, const BodyDev *bodies,
+                               int nb, double *V);
 __global__ void k_static_volume(size_t n, KernelConst K, double gamma_v, const double *x,
                                 const double *y, const double *z, const uint32_t *off,
                                 const uint32_t *idx, double *vol);
 __global__ void k_nonpressure(size_t n, KernelConst K, NonPressureConst P, const double *x,
                               const double *y, const double *z, const double *vx,
                               const double *vy, const double *vz, const double *m,
-                              const double *V, const uint32_t *off, const uint32_t *idx,
-                              const BodyDev *bodies, int nb, double *ax, double *ay, double *az,
-                              double *body_partials);
+                              const double *V, const uint32_t *cnt, const uint32_t *slice_off,
+                              const uint32_t *nj, const double *gxa, const double *gya,
+                              const double *gza, const BodyDev *bodies, int nb, double *ax,
+                              double *ay, double *az, double *body_partials);
 __global__ void k_sesph_pressure(size_t n, double k, double V0, const double *V, double *p);
 __global__ void k_pressure_accel(size_t n, KernelConst K, double gamma, const double *x,
                                  const double *y, const double *z, const double *m,
-                                 const double *V, const double *p, const uint32_t *off,
-                                 const uint32_t *idx, const BodyDev *bodies, int nb,
-                                 int overwrite, int mirror, double *tx_, double *ty_,
-                                 double *tz_, double *body_partials);
+                                 const double *V, const double *p, const uint32_t *cnt,
+                                 const uint32_t *slice_off, const uint32_t *nj,
+                                 const double *gxa, const double *gya, const double *gza,
+                                 const BodyDev *bodies, int nb, int overwrite, int mirror,
+                                 double *tx_, double *ty_, double *tz_, double *body_partials);
 __global__ void k_pred_vel(size_t n, double dt, int to_pred, const double *vx, const double *vy,
                            const double *vz, const double *ax, const double *ay,
                            const double *az, double *svx, double *svy, double *svz);
 __global__ void k_source_diag(size_t n, KernelConst K, SolveConst S, int vp, const double *x,
                               const double *y, const double *z, const double *svx,
                               const double *svy, const double *svz, const double *m,
-                              const double *V, const uint32_t *off, const uint32_t *idx,
-                              const BodyDev *bodies, int nb, double *s_f, double *a_ff,
-                              double *p);
+                              const double *V, const uint32_t *cnt, const uint32_t *slice_off,
+                              const uint32_t *nj, const double *gxa, const double *gya,
+                              const double *gza, const BodyDev *bodies, int nb, double *s_f,
+                              double *a_ff, double *p);
 __global__ void k_jacobi(size_t n, KernelConst K, SolveConst S, const double *x, const double *y,
                          const double *z, const double *V, const double *pax, const double *pay,
                          const double *paz, const double *s_f, const double *a_ff,
-                         const uint32_t *off, const uint32_t *idx, const BodyDev *bodies, int nb,
-                         double *p, double *err_partials);
+                         const uint32_t *cnt, const uint32_t *slice_off, const uint32_t *nj,
+                         const double *gxa, const double *gya, const double *gza,
+                         const BodyDev *bodies, int nb, double *p, double *err_partials);
 __global__ void k_pred_pos(size_t n, double dt, const double *x, const double *y,
                            const double *z, const double *svx, const double *svy,
                            const double *svz, double *sx, double *sy, double *sz);

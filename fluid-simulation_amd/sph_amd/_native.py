@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsphhip.so")
 
 SPH_OK = 0
+PROF_KERNELS = ["neighbor_count", "pairs_volume", "boundary_sample", "nonpressure",
+                "source_diag", "pressure_accel", "jacobi"]
 STATUS = {-1: "SPH_ERR_ARG", -2: "SPH_ERR_HIP", -3: "SPH_ERR_STATE", -4: "SPH_ERR_CAPACITY",
           -5: "SPH_ERR_UNSUPPORTED", -6: "SPH_ERR_NO_DEVICE"}
 
@@ -88,6 +90,8 @@ SIGNATURES = [
     ("sph_error_history", C.c_int, [C.c_void_p, _dp, C.c_size_t, _szp]),
     ("sph_body_state", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("sph_body_forces", C.c_int, [C.c_void_p, C.c_int, _dp]),
+    ("sph_profile", C.c_int, [C.c_void_p, C.c_int]),
+    ("sph_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
     ("sph_volume", C.c_int, [C.c_void_p, C.c_size_t, _dp, _dp]),
     ("sph_boundary_find_samples", C.c_int, [C.c_void_p, C.c_int, C.c_size_t, _dp, _dp, _u8p]),
     ("sph_nonpressure_accel", C.c_int, [C.c_void_p, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]),

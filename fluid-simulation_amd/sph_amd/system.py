@@ -214,6 +214,20 @@ class System:
         N.check(N.lib().sph_body_forces(self._ctx.h, int(b), N.dptr(out)))
         return out
 
+    def profile(self, enable: bool = True):
+        """Start (and clear) or stop HIP-event timing of the hot kernels."""
+        N.check(N.lib().sph_profile(self._ctx.h, int(bool(enable))))
+
+    def profile_read(self) -> dict:
+        out = {}
+        for k, name in enumerate(N.PROF_KERNELS):
+            launches, units, ms = C.c_uint64(), C.c_uint64(), C.c_double()
+            N.check(N.lib().sph_profile_read(self._ctx.h, k, C.byref(launches), C.byref(units),
+                                             C.byref(ms)))
+            out[name] = {"launches": int(launches.value), "units": int(units.value),
+                         "ms": float(ms.value)}
+        return out
+
     def clone(self) -> "System":
         """Deep copy into an independent device context (DynClone)."""
         h = C.c_void_p()

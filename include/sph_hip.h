@@ -161,6 +161,24 @@ int sph_body_state(sph_ctx *ctx, int b, double out[22]);
 /* F[3], tau[3] applied to boundary b during the last update() (FP64) */
 int sph_body_forces(sph_ctx *ctx, int b, double out[6]);
 
+/* ---- kernel timing ------------------------------------------------------------
+ * HIP-event timing of the hot kernels on the context's stream (no counterpart
+ * in the reference, which only times whole steps, sph/mod.rs:127,143).
+ * sph_profile(ctx, 1) clears and starts; sph_profile_read returns the launch
+ * count, the particles processed and the summed kernel milliseconds. */
+enum {
+    SPH_PROF_NEIGHBOR_COUNT = 0,   /* cell sweep: neighbor counts per particle */
+    SPH_PROF_PAIRS_VOLUME = 1,     /* cell sweep: pair cache (index, grad W) + volume */
+    SPH_PROF_BOUNDARY_SAMPLE = 2,  /* volume-map boundary samples */
+    SPH_PROF_NONPRESSURE = 3,      /* gravity + viscosity */
+    SPH_PROF_SOURCE_DIAG = 4,      /* IISPH source term + diagonal + initial pressure */
+    SPH_PROF_PRESSURE_ACCEL = 5,   /* pressure acceleration */
+    SPH_PROF_JACOBI = 6,           /* relaxed Jacobi update + error */
+    SPH_PROF_KERNELS = 7
+};
+int sph_profile(sph_ctx *ctx, int enable);
+int sph_profile_read(sph_ctx *ctx, int kernel, uint64_t *launches, uint64_t *units, double *ms);
+
 /* ---- per-trait entry points on host arrays --------------------------------
  * Each call uploads the given fluid state, runs one phase of System::update
  * on the device with the context's boundaries, and returns the result in the
