@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M particles)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain device copy of known size (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -189,6 +191,12 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.copy_probe:
+        # cloning the system copies each binary64 particle array device-to-device
+        # (one runtime copy kernel per array, 8 bytes x particles each); those
+        # dispatches calibrate the HBM counters against a known byte count
+        sim.clone()
+        out["copy_probe_bytes"] = 8 * n
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         try:

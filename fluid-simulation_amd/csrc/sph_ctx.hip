@@ -400,8 +400,7 @@ struct sph_ctx {
     DBuf<uint32_t> scan_a, scan_b, scan_c;
     // reductions
     DBuf<double> partials, red;
-    DBuf<int> bbox;
-    DBuf<unsigned long long> vmax_bits;
+    DBuf<int> bbox, ipart;
     DBuf<unsigned int> counters;
     DBuf<BodyDev> d_bodies;
     double *pinned = nullptr;  // small readbacks
@@ -598,6 +597,19 @@ static void reserve_fluid(sph_ctx *c, size_t n) {
     c->red.reserve(kMaxBodies * 6 + 64);
 }
 
+// cell coordinates of `cnt` points and their occupied-cell box into bbox (device)
+static void cell_box(sph_ctx *c, size_t cnt, double cs, const double *X, const double *Y,
+                     const double *Z, int *ci, int *bbox) {
+    hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, c->s, bbox);
+    HIPCK(hipGetLastError());
+    if (cnt == 0) return;
+    unsigned nbk = nblocks(cnt);
+    c->ipart.reserve((size_t)nbk * 6);
+    LAUNCH(k_cells, cnt, c->s, cnt, cs, X, Y, Z, ci, c->ipart.p);
+    hipLaunchKernelGGL(k_fold_bbox, dim3(1), dim3(kBlock), 0, c->s, c->ipart.p, (int)nbk, bbox);
+    HIPCK(hipGetLastError());
+}
+
 static void upload_bodies(sph_ctx *c) {
     std::vector<BodyDev> v;
     for (auto &b : c->bodies) v.push_back(b->view());
@@ -612,10 +624,7 @@ static void upload_bodies(sph_ctx *c) {
 static void fluid_neighbors(sph_ctx *c) {
     const size_t n = c->n;
     c->bbox.reserve(6);
-    hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, c->s, c->bbox.p);
-    HIPCK(hipGetLastError());
-    LAUNCH(k_cells, n, c->s, n, c->P.kernel_support_radius, c->x.p, c->y.p, c->z.p, c->ci.p,
-           c->bbox.p);
+    cell_box(c, n, c->P.kernel_support_radius, c->x.p, c->y.p, c->z.p, c->ci.p, c->bbox.p);
     int bb[6];
     HIPCK(hipMemcpyAsync(bb, c->bbox.p, sizeof bb, hipMemcpyDeviceToHost, c->s));
     HIPCK(hipStreamSynchronize(c->s));
@@ -655,13 +664,13 @@ static void fluid_neighbors(sph_ctx *c) {
     c->ell_gz.reserve(std::max<uint64_t>(slots, 1));
 }
 
-// neighbor indices + grad W into the pair cache, and the volume (get_volume)
+// neighbor indices, then grad W into the pair cache and the volume (get_volume)
 static void pairs_and_volume(sph_ctx *c) {
     const size_t n = c->n;
-    PLAUNCH(c, SPH_PROF_PAIRS_VOLUME, k_pairs_volume, n, n, c->K, c->V0,
-            c->P.kernel_support_radius, c->reach, c->dthr, c->grid, c->x.p, c->y.p, c->z.p,
-            c->slice_off.p, c->ell_j.p, c->ell_gx.p, c->ell_gy.p, c->ell_gz.p, c->d_bodies.p,
-            (int)c->bodies.size(), c->V.p);
+    PLAUNCH(c, SPH_PROF_PAIRS_VOLUME, k_fill_ell, n, n, c->P.kernel_support_radius, c->reach,
+            c->dthr, c->grid, c->x.p, c->y.p, c->z.p, c->slice_off.p, c->ell_j.p);
+    PLAUNCH(c, SPH_PROF_PAIRS_VOLUME, k_pairs_volume, n, n, c->K, c->V0, c->x.p, c->y.p, c->z.p,
+            ELL(c), c->d_bodies.p, (int)c->bodies.size(), c->V.p);
 }
 
 // StaticSampleBoundary samples: sort once by (cell, index) and build their grid
@@ -680,9 +689,7 @@ static void prepare_samples(sph_ctx *c, Body &b) {
     DBuf<int> sci;
     sci.reserve(3 * std::max<size_t>(m, 1));
     c->bbox.reserve(6);
-    hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, c->s, c->bbox.p);
-    HIPCK(hipGetLastError());
-    LAUNCH(k_cells, m, c->s, m, c->P.kernel_support_radius, hx.p, hy.p, hz.p, sci.p, c->bbox.p);
+    cell_box(c, m, c->P.kernel_support_radius, hx.p, hy.p, hz.p, sci.p, c->bbox.p);
     int bb[6];
     HIPCK(hipMemcpyAsync(bb, c->bbox.p, sizeof bb, hipMemcpyDeviceToHost, c->s));
     HIPCK(hipStreamSynchronize(c->s));
@@ -961,34 +968,36 @@ static void compact_below(sph_ctx *c) {
     c->n = nn;
 }
 
+// max |v| over the fluid and the number of particles below the disable threshold
+static double speed_below(sph_ctx *c, double *vmax) {
+    const size_t n = c->n;
+    if (n == 0) {
+        *vmax = 0.0;
+        return 0.0;
+    }
+    unsigned nbk = nblocks(n);
+    LAUNCH(k_speed_below, n, c->s, n, c->vx.p, c->vy.p, c->vz.p, c->z.p,
+           c->P.disable_particles_below, c->partials.p);
+    hipLaunchKernelGGL(k_fold_speed, dim3(1), dim3(kBlock), 0, c->s, c->partials.p, (int)nbk,
+                       c->red.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(c->pinned, c->red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCK(hipStreamSynchronize(c->s));
+    *vmax = c->pinned[0];
+    return c->pinned[1];
+}
+
 // System::update (sph/mod.rs:674-731)
 static void update(sph_ctx *c) {
     size_t n = c->n;
-    c->counters.reserve(4);
-    c->vmax_bits.reserve(1);
-    LAUNCH(k_fill_u32, 4, c->s, (uint32_t *)c->counters.p, (size_t)4, 0u);
-    HIPCK(hipMemsetAsync(c->vmax_bits.p, 0, sizeof(unsigned long long), c->s));
-    LAUNCH(k_speed_below, n, c->s, n, c->vx.p, c->vy.p, c->vz.p, c->z.p,
-           c->P.disable_particles_below, c->vmax_bits.p, c->counters.p);
-    unsigned long long vbits = 0;
-    unsigned int below = 0;
-    HIPCK(hipMemcpyAsync(&vbits, c->vmax_bits.p, sizeof vbits, hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipMemcpyAsync(&below, c->counters.p, sizeof below, hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
-    if (below) {
+    double vmax = 0.0;
+    double below = speed_below(c, &vmax);
+    if (below > 0.0) {
         compact_below(c);
         n = c->n;
-        // max speed over the remaining particles
-        LAUNCH(k_fill_u32, 4, c->s, (uint32_t *)c->counters.p, (size_t)4, 0u);
-        HIPCK(hipMemsetAsync(c->vmax_bits.p, 0, sizeof(unsigned long long), c->s));
-        LAUNCH(k_speed_below, n, c->s, n, c->vx.p, c->vy.p, c->vz.p, c->z.p,
-               c->P.disable_particles_below, c->vmax_bits.p, c->counters.p);
-        HIPCK(hipMemcpyAsync(&vbits, c->vmax_bits.p, sizeof vbits, hipMemcpyDeviceToHost, c->s));
-        HIPCK(hipStreamSynchronize(c->s));
+        speed_below(c, &vmax);  // max speed over the remaining particles
     }
     if (c->P.cfl) {  // set_cfl_time_step (sph/mod.rs:825-829)
-        double vmax;
-        memcpy(&vmax, &vbits, sizeof vmax);
         double cfl = c->P.cfl_number * c->P.rest_density_grid_spacing / vmax;
         c->dt = std::fmin(c->P.max_time_increment, cfl);
     }
@@ -1284,11 +1293,8 @@ int sph_neighbors_find(sph_ctx *c, double cell_size, double range, const double 
         DBuf<int> bbs, bbq;
         bbs.reserve(6);
         bbq.reserve(6);
-        hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, c->s, bbs.p);
-        hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, c->s, bbq.p);
-        HIPCK(hipGetLastError());
-        LAUNCH(k_cells, m, c->s, m, cell_size, sx0.p, sy0.p, sz0.p, sci.p, bbs.p);
-        LAUNCH(k_cells, n, c->s, n, cell_size, qx.p, qy.p, qz.p, qci.p, bbq.p);
+        cell_box(c, m, cell_size, sx0.p, sy0.p, sz0.p, sci.p, bbs.p);
+        cell_box(c, n, cell_size, qx.p, qy.p, qz.p, qci.p, bbq.p);
         int b1[6], b2[6];
         HIPCK(hipMemcpyAsync(b1, bbs.p, sizeof b1, hipMemcpyDeviceToHost, c->s));
         HIPCK(hipMemcpyAsync(b2, bbq.p, sizeof b2, hipMemcpyDeviceToHost, c->s));

@@ -52,6 +52,17 @@ struct BodyDev {
     const double *bx, *by, *bz, *bvx, *bvy, *bvz, *bvol;
 };
 
+// XCD-aware block order: the dispatcher hands workgroup b to XCD b % 8, so
+// remap such that each XCD walks one contiguous range of sorted particles and
+// the neighbor data its blocks gather stays in that XCD's L2.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned x = b % kXcds, k = b / kXcds;
+    const unsigned q = nb / kXcds, r = nb % kXcds;
+    return x * q + (x < r ? x : r) + k;
+}
+
 __device__ __forceinline__ double powi2(double x) { return x * x; }
 __device__ __forceinline__ double powi3(double x) { return (x * x) * x; }
 

@@ -81,38 +81,61 @@ __device__ __forceinline__ int sat_i32(double v) {
     return (int)v;
 }
 
+// per-block partials of the occupied-cell box (min x,y,z; max x,y,z); a
+// single-block pass folds them (no contended atomics on six words)
 __global__ void k_cells(size_t n, double cs, const double *x, const double *y, const double *z,
-                        int *ci, int *bbox /* minx,miny,minz,maxx,maxy,maxz */) {
+                        int *ci, int *bbox_partials) {
+    __shared__ int lds[6][kBlock / 64];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int c0 = 2147483647, c1 = 2147483647, c2 = 2147483647;
-    int d0 = -2147483647 - 1, d1 = d0, d2 = d0;
+    int v[6] = {2147483647, 2147483647, 2147483647, -2147483647 - 1, -2147483647 - 1,
+                -2147483647 - 1};
     if (i < n) {
         int a = sat_i32(floor(x[i] / cs)), b = sat_i32(floor(y[i] / cs)),
             c = sat_i32(floor(z[i] / cs));
         ci[3 * i] = a;
         ci[3 * i + 1] = b;
         ci[3 * i + 2] = c;
-        c0 = d0 = a;
-        c1 = d1 = b;
-        c2 = d2 = c;
+        v[0] = v[3] = a;
+        v[1] = v[4] = b;
+        v[2] = v[5] = c;
     }
-    // wavefront reduction, then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
-        c0 = min(c0, __shfl_xor(c0, off));
-        c1 = min(c1, __shfl_xor(c1, off));
-        c2 = min(c2, __shfl_xor(c2, off));
-        d0 = max(d0, __shfl_xor(d0, off));
-        d1 = max(d1, __shfl_xor(d1, off));
-        d2 = max(d2, __shfl_xor(d2, off));
+#pragma unroll
+        for (int k = 0; k < 3; k++) v[k] = min(v[k], __shfl_xor(v[k], off));
+#pragma unroll
+        for (int k = 3; k < 6; k++) v[k] = max(v[k], __shfl_xor(v[k], off));
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&bbox[0], c0);
-        atomicMin(&bbox[1], c1);
-        atomicMin(&bbox[2], c2);
-        atomicMax(&bbox[3], d0);
-        atomicMax(&bbox[4], d1);
-        atomicMax(&bbox[5], d2);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) lds[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / 64; q++) {
+            for (int k = 0; k < 3; k++) v[k] = min(v[k], lds[k][q]);
+            for (int k = 3; k < 6; k++) v[k] = max(v[k], lds[k][q]);
+        }
+        for (int k = 0; k < 6; k++) bbox_partials[(size_t)blockIdx.x * 6 + k] = v[k];
     }
+}
+
+__global__ void k_fold_bbox(const int *partials, int nparts, int *bbox) {
+    __shared__ int lds[6][kBlock];
+    int v[6] = {2147483647, 2147483647, 2147483647, -2147483647 - 1, -2147483647 - 1,
+                -2147483647 - 1};
+    for (int q = threadIdx.x; q < nparts; q += kBlock) {
+        for (int k = 0; k < 3; k++) v[k] = min(v[k], partials[(size_t)q * 6 + k]);
+        for (int k = 3; k < 6; k++) v[k] = max(v[k], partials[(size_t)q * 6 + k]);
+    }
+    for (int k = 0; k < 6; k++) lds[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int k = 0; k < 3; k++) lds[k][threadIdx.x] = min(lds[k][threadIdx.x], lds[k][threadIdx.x + s]);
+            for (int k = 3; k < 6; k++) lds[k][threadIdx.x] = max(lds[k][threadIdx.x], lds[k][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) bbox[threadIdx.x] = lds[threadIdx.x][0];
 }
 
 __global__ void k_init_bbox(int *bbox) {
@@ -120,24 +143,56 @@ __global__ void k_init_bbox(int *bbox) {
     else if (threadIdx.x < 6) bbox[threadIdx.x] = -2147483647 - 1;
 }
 
-// max |v| (calc_max_speed, sph/mod.rs:424-441) and count of z < threshold
+// per-block max |v| (calc_max_speed, sph/mod.rs:424-441) and count of z < threshold
 __global__ void k_speed_below(size_t n, const double *vx, const double *vy, const double *vz,
-                              const double *z, double thr, unsigned long long *vmax_bits,
-                              unsigned int *below) {
+                              const double *z, double thr, double *partials /* 2 per block */) {
+    __shared__ double lds[2][kBlock / 64];
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double s = 0.0;
-    unsigned int b = 0;
+    double s = 0.0, b = 0.0;
     if (i < n) {
         s = sqrt(dot3(vx[i], vy[i], vz[i], vx[i], vy[i], vz[i]));
-        b = z[i] < thr ? 1u : 0u;
+        b = z[i] < thr ? 1.0 : 0.0;
     }
     for (int off = 32; off > 0; off >>= 1) {
         s = fmax(s, __shfl_xor(s, off));
         b += __shfl_xor(b, off);
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(vmax_bits, (unsigned long long)__double_as_longlong(s));
-        if (b) atomicAdd(below, b);
+        lds[0][w] = s;
+        lds[1][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kBlock / 64; q++) {
+            s = fmax(s, lds[0][q]);
+            b += lds[1][q];
+        }
+        partials[(size_t)blockIdx.x * 2] = s;
+        partials[(size_t)blockIdx.x * 2 + 1] = b;
+    }
+}
+
+__global__ void k_fold_speed(const double *partials, int nparts, double *out) {
+    __shared__ double lds[2][kBlock];
+    double s = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < nparts; q += kBlock) {
+        s = fmax(s, partials[(size_t)q * 2]);
+        b += partials[(size_t)q * 2 + 1];
+    }
+    lds[0][threadIdx.x] = s;
+    lds[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int t = kBlock / 2; t > 0; t >>= 1) {
+        if (threadIdx.x < t) {
+            lds[0][threadIdx.x] = fmax(lds[0][threadIdx.x], lds[0][threadIdx.x + t]);
+            lds[1][threadIdx.x] += lds[1][threadIdx.x + t];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = lds[0][0];
+        out[1] = lds[1][0];
     }
 }
 
@@ -298,7 +353,8 @@ template __global__ void k_neighbors<true>(size_t, const double *, const double 
 __global__ void k_count_ell(size_t nq, const double *qx, const double *qy, const double *qz,
                             double cs, int reach, double dthr, GridDev g, uint32_t *count,
                             uint32_t *slice_len) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     uint32_t cnt = 0;
     if (i < nq) {
         const double px = qx[i], py = qy[i], pz = qz[i];
@@ -331,55 +387,73 @@ __global__ void k_count_ell(size_t nq, const double *qx, const double *qy, const
     if ((threadIdx.x & 63) == 0 && i < nq) slice_len[i >> 6] = mx;
 }
 
-// neighbor indices + grad W into the ELL cache, and get_volume
-// (quantities.rs:14-61) from the same distances
-__global__ void k_pairs_volume(size_t n, KernelConst K, double V0, double cs, int reach,
-                               double dthr, GridDev g, const double *x, const double *y,
-                               const double *z, const uint32_t *slice_off, uint32_t *nj,
-                               double *gxo, double *gyo, double *gzo, const BodyDev *bodies,
-                               int nb, double *V) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// neighbor indices into the ELL cache (reference order within each particle)
+__global__ void k_fill_ell(size_t n, double cs, int reach, double dthr, GridDev g,
+                           const double *x, const double *y, const double *z,
+                           const uint32_t *slice_off, uint32_t *nj) {
+    const unsigned lb = xcd_block();
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double px = x[i], py = y[i], pz = z[i];
     const int cx = sat_i32(floor(px / cs)), cy = sat_i32(floor(py / cs)),
               cz = sat_i32(floor(pz / cs));
     size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
-    double accu = 0.;
     int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
     if (zlo < 0) zlo = 0;
     if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
-    if (zlo <= zhi) {
-        for (int dx = -reach; dx <= reach; dx++) {
-            int gx = cx + dx - g.org[0];
-            if (gx < 0 || gx >= g.dim[0]) continue;
-            for (int dy = -reach; dy <= reach; dy++) {
-                int gy = cy + dy - g.org[1];
-                if (gy < 0 || gy >= g.dim[1]) continue;
-                uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
-                uint32_t a = g.start[row + zlo], b = g.start[row + zhi + 1];
-                for (uint32_t j = a; j < b; j++) {
-                    double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
-                    double d2 = (rx * rx + ry * ry) + rz * rz;
-                    if (d2 < dthr) {
-                        double rn = sqrt(d2);
-                        double q = rn / K.h;
-                        accu += V0 * (K.wfac * kernel_w(q));
-                        double wx = 0.0, wy = 0.0, wz = 0.0;
-                        if (rn != 0.0) {
-                            double d = kernel_dqw(q);
-                            wx = ((K.gfac * rx) / rn) * d;
-                            wy = ((K.gfac * ry) / rn) * d;
-                            wz = ((K.gfac * rz) / rn) * d;
-                        }
-                        nj[e] = j;
-                        gxo[e] = wx;
-                        gyo[e] = wy;
-                        gzo[e] = wz;
-                        e += 64;
-                    }
+    if (zlo > zhi) return;
+    for (int dx = -reach; dx <= reach; dx++) {
+        int gx = cx + dx - g.org[0];
+        if (gx < 0 || gx >= g.dim[0]) continue;
+        for (int dy = -reach; dy <= reach; dy++) {
+            int gy = cy + dy - g.org[1];
+            if (gy < 0 || gy >= g.dim[1]) continue;
+            uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
+            uint32_t a = g.start[row + zlo], b = g.start[row + zhi + 1];
+            for (uint32_t j = a; j < b; j++) {
+                double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
+                double d2 = (rx * rx + ry * ry) + rz * rz;
+                if (d2 < dthr) {
+                    nj[e] = j;
+                    e += 64;
                 }
             }
         }
+    }
+}
+
+// grad W per cached pair and get_volume (quantities.rs:14-61); the lanes of a
+// wavefront step through the slots in lockstep, so every cache write is one
+// contiguous 64-lane store
+__global__ void k_pairs_volume(size_t n, KernelConst K, double V0, const double *x,
+                               const double *y, const double *z, const uint32_t *cnt,
+                               const uint32_t *slice_off, const uint32_t *nj, double *gxo,
+                               double *gyo, double *gzo, const BodyDev *bodies, int nb,
+                               double *V) {
+    const unsigned lb = xcd_block();
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = x[i], py = y[i], pz = z[i];
+    const uint32_t c = cnt[i];
+    size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+    double accu = 0.;
+    #pragma unroll 4
+        for (uint32_t k = 0; k < c; k++, e += 64) {
+        uint32_t j = nj[e];
+        double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
+        double rn = sqrt((rx * rx + ry * ry) + rz * rz);
+        double q = rn / K.h;
+        accu += V0 * (K.wfac * kernel_w(q));
+        double wx = 0.0, wy = 0.0, wz = 0.0;
+        if (rn != 0.0) {
+            double d = kernel_dqw(q);
+            wx = ((K.gfac * rx) / rn) * d;
+            wy = ((K.gfac * ry) / rn) * d;
+            wz = ((K.gfac * rz) / rn) * d;
+        }
+        gxo[e] = wx;
+        gyo[e] = wy;
+        gzo[e] = wz;
     }
     for (int bi = 0; bi < nb; bi++) {
         const BodyDev &b = bodies[bi];
@@ -535,7 +609,8 @@ __global__ void k_nonpressure(size_t n, KernelConst K, NonPressureConst P, const
                               const double *gza, const BodyDev *bodies, int nb, double *ax,
                               double *ay, double *az, double *body_partials) {
     __shared__ double lds[6 * kBlock];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     const bool live = i < n;
     double px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, mi = 0;
     double gx = 0.0, gy = 0.0, gz = 0.0;
@@ -557,6 +632,7 @@ __global__ void k_nonpressure(size_t n, KernelConst K, NonPressureConst P, const
         }
         const uint32_t c = cnt[i];
         size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+        #pragma unroll 4
         for (uint32_t k = 0; k < c; k++, e += 64) {
             uint32_t j = nj[e];
             double rx = px - x[j], ry = py - y[j], rz = pz - z[j];
@@ -619,7 +695,7 @@ __global__ void k_nonpressure(size_t n, KernelConst K, NonPressureConst P, const
         if (b.dynamic) {  // uniform across the block
             block_sum<6>(f, lds);
             if (threadIdx.x == 0) {
-                double *dst = body_partials + ((size_t)blockIdx.x * kMaxBodies + bi) * 6;
+                double *dst = body_partials + ((size_t)lb * kMaxBodies + bi) * 6;
                 for (int k = 0; k < 6; k++) dst[k] = f[k];
             }
         }
@@ -650,7 +726,8 @@ __global__ void k_pressure_accel(size_t n, KernelConst K, double gamma, const do
                                  const BodyDev *bodies, int nb, int overwrite, int mirror,
                                  double *tx_, double *ty_, double *tz_, double *body_partials) {
     __shared__ double lds[6 * kBlock];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     const bool live = i < n;
     double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0;
     double acx = 0., acy = 0., acz = 0.;
@@ -659,6 +736,7 @@ __global__ void k_pressure_accel(size_t n, KernelConst K, double gamma, const do
         const uint32_t c = cnt[i];
         size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
         const double vm = Vi / mi;
+        #pragma unroll 4
         for (uint32_t k = 0; k < c; k++, e += 64) {
             uint32_t j = nj[e];
             double cf = vm * V[j] * (pi + p[j]);
@@ -705,7 +783,7 @@ __global__ void k_pressure_accel(size_t n, KernelConst K, double gamma, const do
         if (mirror && b.dynamic) {
             block_sum<6>(f, lds);
             if (threadIdx.x == 0) {
-                double *dst = body_partials + ((size_t)blockIdx.x * kMaxBodies + bi) * 6;
+                double *dst = body_partials + ((size_t)lb * kMaxBodies + bi) * 6;
                 for (int k = 0; k < 6; k++) dst[k] = f[k];
             }
         }
@@ -749,7 +827,8 @@ __global__ void k_source_diag(size_t n, KernelConst K, SolveConst S, int vp, con
                               const uint32_t *nj, const double *gxa, const double *gya,
                               const double *gza, const BodyDev *bodies, int nb, double *s_f,
                               double *a_ff, double *p) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double px = x[i], py = y[i], pz = z[i];
     const double ux = svx[i], uy = svy[i], uz = svz[i];
@@ -759,7 +838,8 @@ __global__ void k_source_diag(size_t n, KernelConst K, SolveConst S, int vp, con
     double sfx = 0., sfy = 0., sfz = 0., sf2 = 0., sbx = 0., sby = 0., sbz = 0.;
     const uint32_t c = cnt[i];
     size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
-    for (uint32_t k = 0; k < c; k++, e += 64) {
+    #pragma unroll 4
+        for (uint32_t k = 0; k < c; k++, e += 64) {
         uint32_t j = nj[e];
         double wx = gxa[e], wy = gya[e], wz = gza[e];
         double Vj = V[j];
@@ -810,7 +890,8 @@ __global__ void k_jacobi(size_t n, KernelConst K, SolveConst S, const double *x,
                          const double *gxa, const double *gya, const double *gza,
                          const BodyDev *bodies, int nb, double *p, double *err_partials) {
     __shared__ double lds[kBlock];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block();
+    size_t i = (size_t)lb * blockDim.x + threadIdx.x;
     double err = 0.;
     if (i < n) {
         const double ux = pax[i], uy = pay[i], uz = paz[i];
@@ -818,6 +899,7 @@ __global__ void k_jacobi(size_t n, KernelConst K, SolveConst S, const double *x,
         double adp = 0.;
         const uint32_t c = cnt[i];
         size_t e = (size_t)slice_off[i >> 6] * 64 + (i & 63);
+        #pragma unroll 4
         for (uint32_t k = 0; k < c; k++, e += 64) {
             uint32_t j = nj[e];
             adp += dt2 * V[j] * dot3(ux - pax[j], uy - pay[j], uz - paz[j], gxa[e], gya[e], gza[e]);
@@ -850,7 +932,7 @@ __global__ void k_jacobi(size_t n, KernelConst K, SolveConst S, const double *x,
     }
     double v[1] = {err};
     block_sum<1>(v, lds);
-    if (threadIdx.x == 0) err_partials[blockIdx.x] = v[0];
+    if (threadIdx.x == 0) err_partials[lb] = v[0];
 }
 
 // IISPHwOST: x* = x + dt v*   (iisph_optimized_source_term.rs:103-113)

@@ -40,11 +40,12 @@ __global__ void k_integrate_ec(size_t n, double dt, double *x, double *y, double
                                double *vy, double *vz, const double *ax, const double *ay,
                                const double *az);
 __global__ void k_cells(size_t n, double cs, const double *x, const double *y, const double *z,
-                        int *ci, int *bbox);
+                        int *ci, int *bbox_partials);
+__global__ void k_fold_bbox(const int *partials, int nparts, int *bbox);
 __global__ void k_init_bbox(int *bbox);
 __global__ void k_speed_below(size_t n, const double *vx, const double *vy, const double *vz,
-                              const double *z, double thr, unsigned long long *vmax_bits,
-                              unsigned int *below);
+                              const double *z, double thr, double *partials);
+__global__ void k_fold_speed(const double *partials, int nparts, double *out);
 __global__ void k_hash_check(GridDev g, int reach, unsigned int *flag);
 __global__ void k_cell_count(size_t n, const int *ci, GridDev g, uint32_t *count);
 __global__ void k_cell_scatter(size_t n, const int *ci, GridDev g, const uint32_t *start,
@@ -69,11 +70,14 @@ __global__ void k_vm_sample(size_t n, const double *x, const double *y, const do
 __global__ void k_count_ell(size_t nq, const double *qx, const double *qy, const double *qz,
                             double cs, int reach, double dthr, GridDev g, uint32_t *count,
                             uint32_t *slice_len);
-__global__ void k_pairs_volume(size_t n, KernelConst K, double V0, double cs, int reach,
-                               double dthr, GridDev g, const double *x, const double *y,
-                               const double *z, const uint32_t *slice_off, uint32_t *nj,
-                               double *gxo, double *gyo, double *gzo, const BodyDev *bodies,
-                               int nb, double *V);
+__global__ void k_fill_ell(size_t n, double cs, int reach, double dthr, GridDev g,
+                           const double *x, const double *y, const double *z,
+                           const uint32_t *slice_off, uint32_t *nj);
+__global__ void k_pairs_volume(size_t n, KernelConst K, double V0, const double *x,
+                               const double *y, const double *z, const uint32_t *cnt,
+                               const uint32_t *slice_off, const uint32_t *nj, double *gxo,
+                               double *gyo, double *gzo, const BodyDev *bodies, int nb,
+                               double *V);
 __global__ void k_static_volume(size_t n, KernelConst K, double gamma_v, const double *x,
                                 const double *y, const double *z, const uint32_t *off,
                                 const uint32_t *idx, double *vol);
