@@ -1,0 +1,96 @@
+// Multi-scene launches of the per-step passes.
+//
+// A batch holds one or more independent systems ("scenes"). Each pass is one
+// launch over the concatenation of all scenes' workgroups: workgroup lb maps to
+// scene s with prefix[s] <= lb < prefix[s+1] (a scene's workgroups never
+// straddle another scene, so per-scene reductions stay deterministic block
+// sums). Scenes keep their own particle arrays; the cell table, the pair
+// cache and the per-particle sort scratch are batch-global, addressed through
+// per-scene bases.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sph_hip.h"
+#include "sph_device.hpp"
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+constexpr int kMaxBodiesPerScene = kMaxBodies;
+
+struct SceneView {
+    uint32_t n;           // particles
+    uint32_t part_base;   // first index of this scene in batch-global per-particle arrays
+    uint32_t slice_base;  // first 64-particle slice in the batch-global slice arrays
+    uint32_t cell_base;   // first cell in the batch-global cell table
+    uint32_t body_base;   // first body in the batch body table
+    uint32_t nb;          // bodies of this scene
+    int active;           // pressure solve still iterating
+    int integrator;       // SPH_INTEGRATOR_*
+    int reach;
+    int gravity_mode;
+    KernelConst K;
+    SolveConst S;         // dt, V0, min diagonal, omega, gamma
+    double V0, cs, dthr, dxr, vmax_ball, disable_below, stiffness;
+    double nu_f, nu_b, eps2;
+    double gcenter[3];
+    GridDev grid;         // occupied-cell box; start = batch table + cell_base
+    double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *m, *V, *p;
+    double *s_f, *a_ff, *pax, *pay, *paz, *svx, *svy, *svz, *sxp, *syp, *szp;
+    double *tx, *ty, *tz, *tvx, *tvy, *tvz, *tm;  // permutation targets
+    uint32_t *fid, *ref, *tfid, *tref;
+};
+
+struct BatchDev {
+    const SceneView *views;
+    const uint32_t *pblk;  // workgroup prefix over particles (ns + 1)
+    const uint32_t *cblk;  // workgroup prefix over cells (ns + 1)
+    int ns;
+    int *ci;               // 3 per particle
+    uint32_t *perm;
+    uint32_t *cnt;
+    // wavefront-sliced ELL pair cache: slice s holds 64 consecutive sorted particles of
+    // one scene, and the k-th neighbor of lane l sits at slot 64 (slice_off[s] + k) + l,
+    // so the k-th accesses of a wavefront are one contiguous run (256 B of indices,
+    // 512 B per gradient component); slice_len[s] = the slice's largest neighbor count
+    uint32_t *slice_len;
+    const uint32_t *slice_off;
+    uint32_t *ell_j;                   // neighbor's scene-local sorted index
+    double *ell_gx, *ell_gy, *ell_gz;  // grad W(x_i - x_j)
+    uint32_t *cell_cnt;
+    uint32_t *cell_start;
+    uint32_t *cursor;
+    const BodyDev *bodies;
+    double *partials;      // per workgroup
+};
+
+__global__ void kb_integrate_ec(BatchDev B);
+__global__ void kb_speed_below(BatchDev B);
+__global__ void kb_cells(BatchDev B);
+__global__ void kb_cell_count(BatchDev B);
+__global__ void kb_cell_scatter(BatchDev B);
+__global__ void kb_cell_order(BatchDev B);
+__global__ void kb_permute(BatchDev B);
+__global__ void kb_count_ell(BatchDev B);
+__global__ void kb_fill_ell(BatchDev B);
+__global__ void kb_vm_sample(BatchDev B);
+__global__ void kb_pairs_volume(BatchDev B);
+__global__ void kb_nonpressure(BatchDev B);
+__global__ void kb_sesph_pressure(BatchDev B);
+__global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror);
+__global__ void kb_pred_vel(BatchDev B, int to_pred);
+__global__ void kb_source_diag(BatchDev B, int vp);
+__global__ void kb_jacobi(BatchDev B);
+__global__ void kb_pred_pos(BatchDev B);
+__global__ void kb_ost_commit(BatchDev B);
+__global__ void kb_measure(BatchDev B);
+// per-scene folds of per-workgroup partials: out[s*rows + r] = sum (or max) over the
+// scene's workgroups of partials[lb*rows + r]
+__global__ void kb_fold_sum(const double *partials, const uint32_t *blk, int ns, int rows,
+                            double *out);
+__global__ void kb_fold_speed(const double *partials, const uint32_t *blk, int ns, double *out);
+__global__ void kb_fold_bbox(const int *partials, const uint32_t *blk, int ns, int *out);
+
+}  // namespace sph

@@ -1,11 +1,15 @@
 // Host orchestration of the SPH hot path and its C ABI (include/sph_hip.h).
 //
-// A context owns one HIP stream and all device state of one system: fluid
-// SoA arrays (binary64), the cell grid, the CSR neighbor list, per-body
-// boundary data and reduction scratch. System::update (sph/mod.rs:674-731)
-// is one sequence of launches on that stream; the host only reads back the
-// few scalars the reference's control flow depends on (max speed for the CFL
-// step, the occupied-cell box, the Jacobi error per iteration).
+// A Scene is one system's device state (binary64 SoA particle fields, solver
+// scratch, boundaries, rigid bodies). A Batch owns a HIP stream and the
+// launch-time buffers (cell table, pair cache, sort scratch, reductions) and
+// advances any number of scenes together: System::update
+// (sph/mod.rs:674-731) is one sequence of multi-scene launches. A context
+// (sph_ctx) is one scene stepped by its own batch; a world (sph_world) steps
+// many contexts in shared launches. The host reads back only the scalars the
+// reference's control flow depends on, once per phase for all scenes (max
+// speed for the CFL step, the occupied-cell box, the Jacobi error per
+// iteration, rigid-body forces).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,13 +22,14 @@
 #include <vector>
 
 #include "../../include/sph_hip.h"
+#include "sph_batch.hpp"
 #include "sph_device.hpp"
 #include "sph_kernels.hpp"
 
 namespace sph {
 
 // ------------------------------------------------------------------------
-// errors
+// errors and launches
 // ------------------------------------------------------------------------
 struct Error : std::runtime_error {
     int code;
@@ -51,7 +56,47 @@ struct Error : std::runtime_error {
 static unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------------------
-// optional per-kernel timing with HIP events on the context's stream
+// device buffer
+// ------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        size_t c = std::max<size_t>(n, cap + cap / 4);
+        T *q = nullptr;
+        HIPCK(hipMalloc((void **)&q, std::max<size_t>(c, 1) * sizeof(T)));
+        if (p) (void)hipFree(p);
+        p = q;
+        cap = c;
+    }
+    // pageable host source: the caller synchronizes before the source dies
+    void upload(const T *h, size_t n, hipStream_t s) {
+        reserve(n);
+        if (n) HIPCK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *h, size_t n, hipStream_t s) const {
+        if (n) HIPCK(hipMemcpyAsync(h, p, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    void copy_from(const DBuf &o, size_t n, hipStream_t s) {
+        reserve(n);
+        if (n) HIPCK(hipMemcpyAsync(p, o.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    void swap(DBuf &o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
+};
+
+// ------------------------------------------------------------------------
+// per-kernel timing with HIP events on a batch's stream
 // ------------------------------------------------------------------------
 struct Prof {
     bool on = false;
@@ -85,7 +130,7 @@ struct Prof {
         HIPCK(hipEventRecord(r.b, s));
         pending.push_back(r);
     }
-    void drain() {  // caller synchronized the stream
+    void drain() {  // the caller synchronized the stream
         for (auto &r : pending) {
             float t = 0;
             HIPCK(hipEventElapsedTime(&t, r.a, r.b));
@@ -109,65 +154,6 @@ struct Prof {
             (void)hipEventDestroy(r.b);
         }
         for (auto e : pool) (void)hipEventDestroy(e);
-    }
-};
-
-// pair-cache arguments of the neighbor passes
-#define ELL(c) (c)->nbr_cnt.p, (c)->slice_off.p, (c)->ell_j.p, (c)->ell_gx.p, (c)->ell_gy.p, (c)->ell_gz.p
-
-// LAUNCH with event timing under profile id `pid` (units = particles)
-#define PLAUNCH(ctx, pid, kernel, n, ...)                        \
-    do {                                                         \
-        Prof::Rec r_{pid, (size_t)(n), nullptr, nullptr};        \
-        if ((ctx)->prof.on) (ctx)->prof.begin((ctx)->s, r_);     \
-        LAUNCH(kernel, n, (ctx)->s, __VA_ARGS__);                \
-        if ((ctx)->prof.on) (ctx)->prof.end((ctx)->s, r_);       \
-    } while (0)
-
-// ------------------------------------------------------------------------
-// device buffer
-// ------------------------------------------------------------------------
-template <class T>
-struct DBuf {
-    T *p = nullptr;
-    size_t cap = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() {
-        if (p) (void)hipFree(p);
-    }
-    void reserve(size_t n) {
-        if (n <= cap) return;
-        size_t c = std::max<size_t>(n, cap + cap / 4);
-        T *q = nullptr;
-        HIPCK(hipMalloc((void **)&q, std::max<size_t>(c, 1) * sizeof(T)));
-        if (p) (void)hipFree(p);
-        p = q;
-        cap = c;
-    }
-    // grow keeping the first `keep` elements
-    void grow_keep(size_t n, size_t keep, hipStream_t s) {
-        if (n <= cap) return;
-        size_t c = std::max<size_t>(n, cap + cap / 4);
-        T *q = nullptr;
-        HIPCK(hipMalloc((void **)&q, std::max<size_t>(c, 1) * sizeof(T)));
-        if (p && keep) HIPCK(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
-        HIPCK(hipStreamSynchronize(s));
-        if (p) (void)hipFree(p);
-        p = q;
-        cap = c;
-    }
-    void upload(const T *h, size_t n, hipStream_t s) {
-        reserve(n);
-        if (n) HIPCK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-    void download(T *h, size_t n, hipStream_t s) const {
-        if (n) HIPCK(hipMemcpyAsync(h, p, n * sizeof(T), hipMemcpyDeviceToHost, s));
-    }
-    void copy_from(const DBuf &o, size_t n, hipStream_t s) {
-        reserve(n);
-        if (n) HIPCK(hipMemcpyAsync(p, o.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     }
 };
 
@@ -277,6 +263,10 @@ struct LatticeHost {
     LatticeDev d{};
     DBuf<double> values;
     DBuf<uint8_t> cell_ok;
+    size_t nodes() const {
+        return (3 * (size_t)d.n[0] + 1) * (3 * (size_t)d.n[1] + 1) * (3 * (size_t)d.n[2] + 1);
+    }
+    size_t cells() const { return (size_t)d.n[0] * d.n[1] * d.n[2]; }
     void load(const sph_lattice &L, hipStream_t s) {
         for (int k = 0; k < 3; k++) {
             d.x_min[k] = L.x_min[k];
@@ -285,9 +275,7 @@ struct LatticeHost {
             d.n[k] = (int)L.n[k];
         }
         d.dx = L.dx;
-        const size_t nx = 3 * (size_t)d.n[0] + 1, ny = 3 * (size_t)d.n[1] + 1,
-                     nz = 3 * (size_t)d.n[2] + 1;
-        const size_t nodes = nx * ny * nz;
+        const size_t nx = 3 * (size_t)d.n[0] + 1, ny = 3 * (size_t)d.n[1] + 1;
         // a cell is evaluable iff all 32 of its nodes are present
         // (discretization.rs:455-463)
         static const int off[32][3] = {
@@ -295,10 +283,9 @@ struct LatticeHost {
             {1, 0, 0}, {2, 0, 0}, {1, 3, 0}, {2, 3, 0}, {1, 0, 3}, {2, 0, 3}, {1, 3, 3}, {2, 3, 3},
             {0, 1, 0}, {0, 2, 0}, {3, 1, 0}, {3, 2, 0}, {0, 1, 3}, {0, 2, 3}, {3, 1, 3}, {3, 2, 3},
             {0, 0, 1}, {0, 0, 2}, {3, 0, 1}, {3, 0, 2}, {0, 3, 1}, {0, 3, 2}, {3, 3, 1}, {3, 3, 2}};
-        const size_t ncell = (size_t)d.n[0] * d.n[1] * d.n[2];
-        std::vector<uint8_t> ok(ncell, 1);
-        std::vector<double> vals(nodes);
-        for (size_t k = 0; k < nodes; k++) vals[k] = L.present[k] ? L.values[k] : 0.0;
+        std::vector<uint8_t> ok(cells(), 1);
+        std::vector<double> vals(nodes());
+        for (size_t k = 0; k < vals.size(); k++) vals[k] = L.present[k] ? L.values[k] : 0.0;
         for (int cz = 0; cz < d.n[2]; cz++)
             for (int cy = 0; cy < d.n[1]; cy++)
                 for (int cx = 0; cx < d.n[0]; cx++) {
@@ -310,9 +297,16 @@ struct LatticeHost {
                     }
                     ok[((size_t)cz * d.n[1] + cy) * d.n[0] + cx] = good;
                 }
-        values.upload(vals.data(), nodes, s);
-        cell_ok.upload(ok.data(), ncell, s);
+        values.upload(vals.data(), vals.size(), s);
+        cell_ok.upload(ok.data(), ok.size(), s);
         HIPCK(hipStreamSynchronize(s));
+        d.values = values.p;
+        d.cell_ok = cell_ok.p;
+    }
+    void copy_from(const LatticeHost &o, hipStream_t s) {
+        d = o.d;
+        values.copy_from(o.values, o.nodes(), s);
+        cell_ok.copy_from(o.cell_ok, o.cells(), s);
         d.values = values.p;
         d.cell_ok = cell_ok.p;
     }
@@ -365,57 +359,181 @@ static double distance_threshold(double range) {
     return t;
 }
 
-}  // namespace sph
-
-using namespace sph;
-
 // ------------------------------------------------------------------------
-// context
+// one system
 // ------------------------------------------------------------------------
-struct sph_ctx {
+struct Scene {
     sph_params P{};
-    double V0 = 0;
+    double V0 = 0, dthr = 0;
     KernelConst K{};
-    double dthr = 0;
     int reach = 1;
-    hipStream_t s = nullptr;
-    int device = 0;
 
     size_t n = 0;
-    // fluid state (sorted order)
+    // persistent fields in (cell, reference index) order
     DBuf<double> x, y, z, vx, vy, vz, ax, ay, az, m, V, p;
     DBuf<uint32_t> fid, ref;
-    // solver scratch
+    // solver and integrator slots
     DBuf<double> s_f, a_ff, pax, pay, paz, svx, svy, svz, sxp, syp, szp, ix, iy, iz, ivx, ivy, ivz;
-    DBuf<double> tmp;   // gather scratch (f64)
-    DBuf<uint32_t> tmpu;
-    // grid + neighbors
-    DBuf<int> ci;
-    DBuf<uint32_t> perm, cell_cnt, cell_start, cursor;
+    // permutation targets
+    DBuf<double> tx, ty, tz, tvx, tvy, tvz, tm;
+    DBuf<uint32_t> tfid, tref;
+    std::vector<std::unique_ptr<Body>> bodies;
+
     GridDev grid{};
-    DBuf<uint32_t> nbr_off, nbr_idx, nbr_cnt;
-    // wavefront-sliced ELL pair cache: neighbor index + grad W per slot
-    DBuf<uint32_t> slice_len, slice_off, ell_j;
-    DBuf<double> ell_gx, ell_gy, ell_gz;
-    DBuf<uint32_t> scan_a, scan_b, scan_c;
-    // reductions
-    DBuf<double> partials, red;
-    DBuf<int> bbox, ipart;
-    DBuf<unsigned int> counters;
-    DBuf<BodyDev> d_bodies;
-    double *pinned = nullptr;  // small readbacks
+    int box[6] = {0, 0, 0, 0, 0, 0};
     int checked_box[6] = {1, 1, 1, 0, 0, 0};
     bool hash_box_valid = false;
+    // pressure-solve state of the running update
+    bool active = false;
+    uint32_t iter = 0;
+    double err = 0;
+    // placement in the batch that steps it
+    uint32_t part_base = 0, slice_base = 0, cell_base = 0, body_base = 0;
 
-    std::vector<std::unique_ptr<Body>> bodies;
     bool boundaries_initialized = false;
     bool initialized = false;
+    bool in_world = false;
     double dt = 0, time = 0;
     uint64_t steps = 0;
     uint32_t last_iter = 0, eqs1_iter = 0;
     double last_err = 0, avg_density = 0;
     std::vector<double> err_hist;
+
+    size_t ncells() const { return (size_t)grid.dim[0] * grid.dim[1] * grid.dim[2]; }
+
+    SceneView view(const uint32_t *cell_start) const {
+        SceneView v{};
+        v.n = (uint32_t)n;
+        v.part_base = part_base;
+        v.slice_base = slice_base;
+        v.cell_base = cell_base;
+        v.body_base = body_base;
+        v.nb = (uint32_t)bodies.size();
+        v.active = active ? 1 : 0;
+        v.integrator = P.integrator;
+        v.reach = reach;
+        v.gravity_mode = P.gravity_mode;
+        v.K = K;
+        v.S.dt = dt;
+        v.S.V0 = V0;
+        v.S.eps = P.min_diagonal_element;
+        v.S.omega = P.relaxation_factor;
+        v.S.gamma = P.boundary_pressure_acceleration_weighting;
+        v.V0 = V0;
+        v.cs = P.kernel_support_radius;
+        v.dthr = dthr;
+        v.dxr = P.rest_density_grid_spacing;
+        const double r = P.kernel_support_radius;
+        v.vmax_ball = 4.0 / 3.0 * 3.14159265358979323846 * ((r * r) * r);  // ball_volume
+        v.disable_below = P.disable_particles_below;
+        v.stiffness = P.stiffness;
+        v.nu_f = P.fluid_viscosity;
+        v.nu_b = P.boundary_viscosity;
+        v.eps2 = 0.01 * (P.rest_density_grid_spacing * P.rest_density_grid_spacing);
+        for (int k = 0; k < 3; k++) v.gcenter[k] = P.gravity_center[k];
+        v.grid = grid;
+        v.grid.start = cell_start ? cell_start + cell_base : nullptr;
+        v.x = x.p; v.y = y.p; v.z = z.p;
+        v.vx = vx.p; v.vy = vy.p; v.vz = vz.p;
+        v.ax = ax.p; v.ay = ay.p; v.az = az.p;
+        v.m = m.p; v.V = V.p; v.p = p.p;
+        v.s_f = s_f.p; v.a_ff = a_ff.p;
+        v.pax = pax.p; v.pay = pay.p; v.paz = paz.p;
+        v.svx = svx.p; v.svy = svy.p; v.svz = svz.p;
+        v.sxp = sxp.p; v.syp = syp.p; v.szp = szp.p;
+        v.tx = tx.p; v.ty = ty.p; v.tz = tz.p;
+        v.tvx = tvx.p; v.tvy = tvy.p; v.tvz = tvz.p; v.tm = tm.p;
+        v.fid = fid.p; v.ref = ref.p; v.tfid = tfid.p; v.tref = tref.p;
+        return v;
+    }
+};
+
+// ------------------------------------------------------------------------
+// launch-time buffers shared by the scenes of a batch
+// ------------------------------------------------------------------------
+struct Batch {
+    hipStream_t s = nullptr;
+    double *pinned = nullptr;  // readbacks
+    size_t pinned_cap = 0;
+    // small uploads are staged through pinned memory that is recycled after
+    // every stream synchronization
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
+    std::vector<Scene *> sc;
+
+    DBuf<int> ci, ipart;
+    DBuf<uint32_t> perm, cnt, slice_len, slice_off, ell_j, cell_cnt, cell_start, cursor;
+    DBuf<uint32_t> pblk, cblk, scan_a, scan_b, scan_c;
+    DBuf<double> ell_gx, ell_gy, ell_gz, partials, red;
+    DBuf<SceneView> views;
+    DBuf<BodyDev> bodies;
+    // utility scratch (standalone queries and static samples)
+    DBuf<int> bbox;
+    DBuf<uint32_t> ucnt, ucursor;
+    DBuf<unsigned int> flag;
     Prof prof;
+
+    std::vector<uint32_t> hp_blk, hc_blk;
+    size_t total_particles = 0, total_pblocks = 0, total_cells = 0, total_slices = 0;
+
+    void init() {
+        HIPCK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ensure_pinned(64);
+    }
+    void ensure_pinned(size_t n) {
+        if (n <= pinned_cap) return;
+        sync();
+        if (pinned) HIPCK(hipHostFree(pinned));
+        pinned = nullptr;
+        pinned_cap = 0;
+        size_t c = std::max<size_t>(n, 64);
+        HIPCK(hipHostMalloc((void **)&pinned, c * sizeof(double), hipHostMallocDefault));
+        pinned_cap = c;
+    }
+    void sync() {
+        HIPCK(hipStreamSynchronize(s));
+        arena_off = 0;
+    }
+    // asynchronous host->device copy of `bytes` from a host buffer the caller may reuse at once
+    void put(void *dst, const void *src, size_t bytes) {
+        if (!bytes) return;
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (arena_off + need > arena_cap) {
+            sync();
+            if (need > arena_cap) {
+                if (arena) HIPCK(hipHostFree(arena));
+                arena = nullptr;
+                arena_cap = 0;
+                size_t c = std::max<size_t>(need, (size_t)1 << 20);
+                HIPCK(hipHostMalloc((void **)&arena, c, hipHostMallocDefault));
+                arena_cap = c;
+            }
+        }
+        char *h = arena + arena_off;
+        arena_off += need;
+        memcpy(h, src, bytes);
+        HIPCK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s));
+    }
+    ~Batch() {
+        if (s) (void)hipStreamSynchronize(s);
+        if (pinned) (void)hipHostFree(pinned);
+        if (arena) (void)hipHostFree(arena);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+}  // namespace sph
+
+using namespace sph;
+
+struct sph_ctx {
+    Scene scene;
+    Batch own;
+};
+
+struct sph_world {
+    Batch batch;
+    std::vector<sph_ctx *> members;
 };
 
 namespace sph {
@@ -447,56 +565,61 @@ static void check_device() {
     if (e != hipSuccess || cnt <= 0) throw Error(SPH_ERR_NO_DEVICE, "no HIP device visible");
 }
 
-// exclusive scan of n uint32 into out (n+1 entries: out[n] = total); returns total
-static uint32_t scan_u32(sph_ctx *c, const uint32_t *in, size_t n, uint32_t *out, bool read_total) {
-    // out[0..n) = exclusive scan; total written at out[n]
-    const size_t B = 1024;
-    size_t nb1 = (n + B - 1) / B;
+// multi-scene launch of `nblk` workgroups; timed under `pid` when profiling
+// (pid == SPH_PROF_KERNELS: not timed)
+#define PLAUNCH(B, pid, units, kernel, nblk, ...)                                       \
+    do {                                                                                \
+        if ((nblk) > 0) {                                                               \
+            Prof::Rec r_{pid, (size_t)(units), nullptr, nullptr};                       \
+            const bool t_ = (B).prof.on && (pid) < SPH_PROF_KERNELS;                     \
+            if (t_) (B).prof.begin((B).s, r_);                                          \
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(nblk)), dim3(kBlock), 0, (B).s,  \
+                               __VA_ARGS__);                                            \
+            HIPCK(hipGetLastError());                                                   \
+            if (t_) (B).prof.end((B).s, r_);                                            \
+        }                                                                               \
+    } while (0)
+
+// exclusive scan of n uint32 into out[0..n]; out[n] = total (returned)
+static uint32_t scan_u32(Batch &B, const uint32_t *in, size_t n, uint32_t *out) {
+    const size_t BS = 1024;
     if (n == 0) {
         uint32_t z = 0;
-        HIPCK(hipMemcpyAsync(out, &z, sizeof z, hipMemcpyHostToDevice, c->s));
-        HIPCK(hipStreamSynchronize(c->s));
+        B.put(out, &z, sizeof z);
         return 0;
     }
-    c->scan_a.reserve(nb1 + 1);
-    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb1), dim3(512), 0, c->s, in, n, out,
-                       c->scan_a.p);
+    size_t nb1 = (n + BS - 1) / BS;
+    B.scan_a.reserve(nb1 + 1);
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb1), dim3(512), 0, B.s, in, n, out, B.scan_a.p);
     HIPCK(hipGetLastError());
     if (nb1 > 1) {
-        size_t nb2 = (nb1 + B - 1) / B;
-        c->scan_b.reserve(nb1 + 1);
-        c->scan_c.reserve(nb2 + 1);
-        hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb2), dim3(512), 0, c->s, c->scan_a.p,
-                           nb1, c->scan_b.p, c->scan_c.p);
+        size_t nb2 = (nb1 + BS - 1) / BS;
+        B.scan_b.reserve(nb1 + 1);
+        B.scan_c.reserve(nb2 + 2);
+        hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb2), dim3(512), 0, B.s, B.scan_a.p, nb1,
+                           B.scan_b.p, B.scan_c.p);
         HIPCK(hipGetLastError());
         if (nb2 > 1) {
-            if (nb2 > B) throw Error(SPH_ERR_UNSUPPORTED, "scan longer than 2^30 elements");
-            // third level: a single block
-            DBuf<uint32_t> t3;
-            t3.reserve(2);
-            hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(512), 0, c->s, c->scan_c.p, nb2,
-                               c->scan_c.p, t3.p);
+            if (nb2 > BS) throw Error(SPH_ERR_UNSUPPORTED, "scan longer than 2^30 elements");
+            // third level in a single workgroup; its block sum goes past the scanned entries
+            hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(512), 0, B.s, B.scan_c.p, nb2,
+                               B.scan_c.p, B.scan_c.p + nb2 + 1);
             HIPCK(hipGetLastError());
-            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb2), dim3(512), 0, c->s, c->scan_b.p,
-                               nb1, c->scan_c.p);
+            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb2), dim3(512), 0, B.s, B.scan_b.p,
+                               nb1, B.scan_c.p);
             HIPCK(hipGetLastError());
-            HIPCK(hipStreamSynchronize(c->s));
         }
-        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb1), dim3(512), 0, c->s, out, n,
-                           c->scan_b.p);
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb1), dim3(512), 0, B.s, out, n, B.scan_b.p);
         HIPCK(hipGetLastError());
     }
-    // total = out[n-1] + in[n-1]
-    uint32_t last[2];
-    HIPCK(hipMemcpyAsync(&last[0], out + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipMemcpyAsync(&last[1], in + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
+    uint32_t *last = reinterpret_cast<uint32_t *>(B.pinned);
+    HIPCK(hipMemcpyAsync(&last[0], out + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
+    HIPCK(hipMemcpyAsync(&last[1], in + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
+    B.sync();
     uint64_t total = (uint64_t)last[0] + last[1];
     if (total > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "more than 2^32 entries");
     uint32_t t32 = (uint32_t)total;
-    HIPCK(hipMemcpyAsync(out + n, &t32, sizeof t32, hipMemcpyHostToDevice, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
-    (void)read_total;
+    B.put(out + n, &t32, sizeof t32);
     return t32;
 }
 
@@ -517,164 +640,90 @@ static GridDev make_grid(const int lo[3], const int hi[3]) {
 }
 
 // refuse cell boxes where two cells that can share a stencil share a hash bucket
-static void hash_guard(sph_ctx *c, const int lo[3], const int hi[3], int reach) {
+static void hash_guard(Batch &B, int *checked, bool &valid, const int lo[3], const int hi[3],
+                       int reach) {
     int elo[3], ehi[3];
     for (int k = 0; k < 3; k++) {
         elo[k] = lo[k] - reach;
         ehi[k] = hi[k] + reach;
     }
-    if (c->hash_box_valid) {
+    if (valid) {
         bool inside = true;
         for (int k = 0; k < 3; k++)
-            inside = inside && elo[k] >= c->checked_box[k] && ehi[k] <= c->checked_box[3 + k];
+            inside = inside && elo[k] >= checked[k] && ehi[k] <= checked[3 + k];
         if (inside) return;
     }
-    // check a margin around the box so small drifts do not re-trigger the check
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 3; k++) {  // margin so that small drifts do not re-check
         elo[k] -= 4;
         ehi[k] += 4;
     }
     GridDev g = make_grid(elo, ehi);
-    c->counters.reserve(4);
-    LAUNCH(k_fill_u32, 4, c->s, (uint32_t *)c->counters.p, (size_t)4, 0u);
+    B.flag.reserve(4);
+    LAUNCH(k_fill_u32, 4, B.s, (uint32_t *)B.flag.p, (size_t)4, 0u);
     size_t nc = (size_t)g.dim[0] * g.dim[1] * g.dim[2];
-    LAUNCH(k_hash_check, nc, c->s, g, 2 * reach, c->counters.p);
-    unsigned int flag = 0;
-    HIPCK(hipMemcpyAsync(&flag, c->counters.p, sizeof flag, hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
-    if (flag)
+    LAUNCH(k_hash_check, nc, B.s, g, 2 * reach, B.flag.p);
+    unsigned int *f = reinterpret_cast<unsigned int *>(B.pinned);
+    HIPCK(hipMemcpyAsync(f, B.flag.p, sizeof *f, hipMemcpyDeviceToHost, B.s));
+    B.sync();
+    if (*f)
         throw Error(SPH_ERR_UNSUPPORTED,
                     "two cells of one neighbor stencil share a spatial-hash bucket; the "
                     "reference would report duplicate neighbors there");
     for (int k = 0; k < 3; k++) {
-        c->checked_box[k] = elo[k];
-        c->checked_box[3 + k] = ehi[k];
+        checked[k] = elo[k];
+        checked[3 + k] = ehi[k];
     }
-    c->hash_box_valid = true;
+    valid = true;
 }
 
-// counting sort of `cnt` points (with cell coords ci) into (cell, key) order;
-// fills perm (sorted -> current index) and cell_start over grid g
-static void cell_sort(sph_ctx *c, size_t cnt, const int *ci, const GridDev &g,
-                      DBuf<uint32_t> &cell_cnt, DBuf<uint32_t> &start, DBuf<uint32_t> &perm,
-                      const uint32_t *order_key) {
-    size_t ncell = (size_t)g.dim[0] * g.dim[1] * g.dim[2];
-    cell_cnt.reserve(ncell);
-    start.reserve(ncell + 1);
-    c->cursor.reserve(ncell);
-    perm.reserve(cnt);
-    LAUNCH(k_fill_u32, ncell, c->s, cell_cnt.p, ncell, 0u);
-    LAUNCH(k_fill_u32, ncell, c->s, c->cursor.p, ncell, 0u);
-    LAUNCH(k_cell_count, cnt, c->s, cnt, ci, g, cell_cnt.p);
-    scan_u32(c, cell_cnt.p, ncell, start.p, false);
-    LAUNCH(k_cell_scatter, cnt, c->s, cnt, ci, g, start.p, c->cursor.p, perm.p);
-    LAUNCH(k_cell_order, ncell, c->s, ncell, start.p, perm.p, order_key);
-}
-
-template <class T>
-static void permute(sph_ctx *c, DBuf<T> &field, DBuf<T> &scratch, size_t cnt) {
-    scratch.reserve(cnt);
-    LAUNCH(k_gather<T>, cnt, c->s, cnt, c->perm.p, field.p, scratch.p);
-    std::swap(field.p, scratch.p);
-    std::swap(field.cap, scratch.cap);
-}
-
-static void reserve_fluid(sph_ctx *c, size_t n) {
-    DBuf<double> *f64s[] = {&c->x,   &c->y,   &c->z,   &c->vx,  &c->vy,  &c->vz,  &c->ax,
-                            &c->ay,  &c->az,  &c->m,   &c->V,   &c->p,   &c->s_f, &c->a_ff,
-                            &c->pax, &c->pay, &c->paz, &c->svx, &c->svy, &c->svz, &c->sxp,
-                            &c->syp, &c->szp, &c->ix,  &c->iy,  &c->iz,  &c->ivx, &c->ivy,
-                            &c->ivz, &c->tmp};
-    for (auto *b : f64s) b->reserve(n);
-    c->fid.reserve(n);
-    c->ref.reserve(n);
-    c->tmpu.reserve(n);
-    c->ci.reserve(3 * n);
-    c->perm.reserve(n);
-    c->nbr_off.reserve(n + 1);
-    c->nbr_cnt.reserve(n);
-    c->partials.reserve((size_t)(nblocks(n) + 1) * kMaxBodies * 6 + 64);
-    c->red.reserve(kMaxBodies * 6 + 64);
-}
-
-// cell coordinates of `cnt` points and their occupied-cell box into bbox (device)
-static void cell_box(sph_ctx *c, size_t cnt, double cs, const double *X, const double *Y,
+// ---- single point-set helpers: standalone queries and static samples ----
+static void cell_box(Batch &B, size_t cnt, double cs, const double *X, const double *Y,
                      const double *Z, int *ci, int *bbox) {
-    hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, c->s, bbox);
+    hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, B.s, bbox);
     HIPCK(hipGetLastError());
     if (cnt == 0) return;
     unsigned nbk = nblocks(cnt);
-    c->ipart.reserve((size_t)nbk * 6);
-    LAUNCH(k_cells, cnt, c->s, cnt, cs, X, Y, Z, ci, c->ipart.p);
-    hipLaunchKernelGGL(k_fold_bbox, dim3(1), dim3(kBlock), 0, c->s, c->ipart.p, (int)nbk, bbox);
+    B.ipart.reserve((size_t)nbk * 6);
+    LAUNCH(k_cells, cnt, B.s, cnt, cs, X, Y, Z, ci, B.ipart.p);
+    hipLaunchKernelGGL(k_fold_bbox, dim3(1), dim3(kBlock), 0, B.s, B.ipart.p, (int)nbk, bbox);
     HIPCK(hipGetLastError());
 }
 
-static void upload_bodies(sph_ctx *c) {
-    std::vector<BodyDev> v;
-    for (auto &b : c->bodies) v.push_back(b->view());
-    c->d_bodies.reserve(std::max<size_t>(v.size(), 1));
-    if (!v.empty())
-        HIPCK(hipMemcpyAsync(c->d_bodies.p, v.data(), v.size() * sizeof(BodyDev),
-                             hipMemcpyHostToDevice, c->s));
+static void cell_sort(Batch &B, size_t cnt, const int *ci, const GridDev &g,
+                      DBuf<uint32_t> &start, DBuf<uint32_t> &perm) {
+    size_t ncell = (size_t)g.dim[0] * g.dim[1] * g.dim[2];
+    B.ucnt.reserve(ncell);
+    start.reserve(ncell + 1);
+    B.ucursor.reserve(ncell);
+    perm.reserve(std::max<size_t>(cnt, 1));
+    LAUNCH(k_fill_u32, ncell, B.s, B.ucnt.p, ncell, 0u);
+    LAUNCH(k_fill_u32, ncell, B.s, B.ucursor.p, ncell, 0u);
+    LAUNCH(k_cell_count, cnt, B.s, cnt, ci, g, B.ucnt.p);
+    scan_u32(B, B.ucnt.p, ncell, start.p);
+    LAUNCH(k_cell_scatter, cnt, B.s, cnt, ci, g, start.p, B.ucursor.p, perm.p);
+    LAUNCH(k_cell_order, ncell, B.s, ncell, start.p, perm.p, (const uint32_t *)nullptr);
 }
 
-// sort the fluid by (cell, reference index), build the cell grid and the
-// CSR neighbor list (System::update -> N::find_samples, sph/mod.rs:701-706)
-static void fluid_neighbors(sph_ctx *c) {
-    const size_t n = c->n;
-    c->bbox.reserve(6);
-    cell_box(c, n, c->P.kernel_support_radius, c->x.p, c->y.p, c->z.p, c->ci.p, c->bbox.p);
-    int bb[6];
-    HIPCK(hipMemcpyAsync(bb, c->bbox.p, sizeof bb, hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
-    if (n == 0) {
-        for (int k = 0; k < 3; k++) {
-            bb[k] = 0;
-            bb[3 + k] = 0;
-        }
-    }
-    hash_guard(c, bb, bb + 3, c->reach);
-    c->grid = make_grid(bb, bb + 3);
-    cell_sort(c, n, c->ci.p, c->grid, c->cell_cnt, c->cell_start, c->perm, c->ref.p);
-    c->grid.start = c->cell_start.p;
-    permute(c, c->x, c->tmp, n);
-    permute(c, c->y, c->tmp, n);
-    permute(c, c->z, c->tmp, n);
-    permute(c, c->vx, c->tmp, n);
-    permute(c, c->vy, c->tmp, n);
-    permute(c, c->vz, c->tmp, n);
-    permute(c, c->m, c->tmp, n);
-    permute(c, c->fid, c->tmpu, n);
-    permute(c, c->ref, c->tmpu, n);
-    // neighbor counts and per-slice maxima; the pair cache is filled after the
-    // boundary samples exist (pairs_and_volume)
-    const size_t nsl = (n + 63) / 64;
-    c->slice_len.reserve(nsl + 1);
-    c->slice_off.reserve(nsl + 1);
-    PLAUNCH(c, SPH_PROF_NEIGHBOR_COUNT, k_count_ell, n, n, c->x.p, c->y.p, c->z.p,
-            c->P.kernel_support_radius, c->reach, c->dthr, c->grid, c->nbr_cnt.p,
-            c->slice_len.p);
-    uint32_t total = scan_u32(c, c->slice_len.p, nsl, c->slice_off.p, true);
-    uint64_t slots = (uint64_t)total * 64;
-    if (slots > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "neighbor cache exceeds 2^32 slots");
-    c->ell_j.reserve(std::max<uint64_t>(slots, 1));
-    c->ell_gx.reserve(std::max<uint64_t>(slots, 1));
-    c->ell_gy.reserve(std::max<uint64_t>(slots, 1));
-    c->ell_gz.reserve(std::max<uint64_t>(slots, 1));
+// ------------------------------------------------------------------------
+// scene field storage
+// ------------------------------------------------------------------------
+static void reserve_scene(Scene &c, size_t n) {
+    const size_t k = std::max<size_t>(n, 1);
+    DBuf<double> *f64s[] = {&c.x,   &c.y,   &c.z,   &c.vx,  &c.vy,   &c.vz,  &c.ax,  &c.ay,
+                            &c.az,  &c.m,   &c.V,   &c.p,   &c.s_f,  &c.a_ff, &c.pax, &c.pay,
+                            &c.paz, &c.svx, &c.svy, &c.svz, &c.sxp,  &c.syp, &c.szp, &c.ix,
+                            &c.iy,  &c.iz,  &c.ivx, &c.ivy, &c.ivz,  &c.tx,  &c.ty,  &c.tz,
+                            &c.tvx, &c.tvy, &c.tvz, &c.tm};
+    for (auto *b : f64s) b->reserve(k);
+    c.fid.reserve(k);
+    c.ref.reserve(k);
+    c.tfid.reserve(k);
+    c.tref.reserve(k);
 }
 
-// neighbor indices, then grad W into the pair cache and the volume (get_volume)
-static void pairs_and_volume(sph_ctx *c) {
-    const size_t n = c->n;
-    PLAUNCH(c, SPH_PROF_PAIRS_VOLUME, k_fill_ell, n, n, c->P.kernel_support_radius, c->reach,
-            c->dthr, c->grid, c->x.p, c->y.p, c->z.p, c->slice_off.p, c->ell_j.p);
-    PLAUNCH(c, SPH_PROF_PAIRS_VOLUME, k_pairs_volume, n, n, c->K, c->V0, c->x.p, c->y.p, c->z.p,
-            ELL(c), c->d_bodies.p, (int)c->bodies.size(), c->V.p);
-}
-
-// StaticSampleBoundary samples: sort once by (cell, index) and build their grid
-static void prepare_samples(sph_ctx *c, Body &b) {
+// StaticSampleBoundary samples: sort once by (cell, index), build their grid and
+// pseudo volumes (BoundaryType::initialize, static_sample_boundary.rs:354-390)
+static void prepare_samples(Batch &B, Scene &c, Body &b) {
     size_t m = b.ns;
     DBuf<double> hx, hy, hz;
     std::vector<double> X(m), Y(m), Z(m);
@@ -683,265 +732,165 @@ static void prepare_samples(sph_ctx *c, Body &b) {
         Y[k] = b.spos[3 * k + 1];
         Z[k] = b.spos[3 * k + 2];
     }
-    hx.upload(X.data(), m, c->s);
-    hy.upload(Y.data(), m, c->s);
-    hz.upload(Z.data(), m, c->s);
+    hx.upload(X.data(), m, B.s);
+    hy.upload(Y.data(), m, B.s);
+    hz.upload(Z.data(), m, B.s);
+    B.sync();
     DBuf<int> sci;
     sci.reserve(3 * std::max<size_t>(m, 1));
-    c->bbox.reserve(6);
-    cell_box(c, m, c->P.kernel_support_radius, hx.p, hy.p, hz.p, sci.p, c->bbox.p);
+    B.bbox.reserve(6);
+    cell_box(B, m, c.P.kernel_support_radius, hx.p, hy.p, hz.p, sci.p, B.bbox.p);
     int bb[6];
-    HIPCK(hipMemcpyAsync(bb, c->bbox.p, sizeof bb, hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
+    HIPCK(hipMemcpyAsync(bb, B.bbox.p, sizeof bb, hipMemcpyDeviceToHost, B.s));
+    B.sync();
     if (m == 0)
         for (int k = 0; k < 6; k++) bb[k] = 0;
     b.sg = make_grid(bb, bb + 3);
-    DBuf<uint32_t> cnt;
-    cell_sort(c, m, sci.p, b.sg, cnt, b.sgrid, b.sperm, nullptr);
+    cell_sort(B, m, sci.p, b.sg, b.sgrid, b.sperm);
     b.sg.start = b.sgrid.p;
-    b.sx.reserve(m);
-    b.sy.reserve(m);
-    b.sz.reserve(m);
-    b.svol.reserve(m);
-    LAUNCH(k_gather<double>, m, c->s, m, b.sperm.p, hx.p, b.sx.p);
-    LAUNCH(k_gather<double>, m, c->s, m, b.sperm.p, hy.p, b.sy.p);
-    LAUNCH(k_gather<double>, m, c->s, m, b.sperm.p, hz.p, b.sz.p);
-    // pseudo volumes (BoundaryType::initialize)
+    b.sx.reserve(std::max<size_t>(m, 1));
+    b.sy.reserve(std::max<size_t>(m, 1));
+    b.sz.reserve(std::max<size_t>(m, 1));
+    b.svol.reserve(std::max<size_t>(m, 1));
+    LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, hx.p, b.sx.p);
+    LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, hy.p, b.sy.p);
+    LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, hz.p, b.sz.p);
     DBuf<uint32_t> off, idx, scnt;
     off.reserve(m + 1);
     scnt.reserve(std::max<size_t>(m, 1));
-    LAUNCH(k_neighbors<false>, m, c->s, m, b.sx.p, b.sy.p, b.sz.p, c->P.kernel_support_radius,
-           c->reach, c->dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, (const uint32_t *)nullptr, scnt.p,
+    LAUNCH(k_neighbors<false>, m, B.s, m, b.sx.p, b.sy.p, b.sz.p, c.P.kernel_support_radius,
+           c.reach, c.dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, (const uint32_t *)nullptr, scnt.p,
            (uint32_t *)nullptr);
-    uint32_t total = scan_u32(c, scnt.p, m, off.p, true);
+    uint32_t total = scan_u32(B, scnt.p, m, off.p);
     idx.reserve(std::max<uint32_t>(total, 1));
-    LAUNCH(k_neighbors<true>, m, c->s, m, b.sx.p, b.sy.p, b.sz.p, c->P.kernel_support_radius,
-           c->reach, c->dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, off.p, (uint32_t *)nullptr, idx.p);
-    LAUNCH(k_static_volume, m, c->s, m, c->K, c->P.boundary_rest_volume_weighting, b.sx.p,
-           b.sy.p, b.sz.p, off.p, idx.p, b.svol.p);
-    HIPCK(hipStreamSynchronize(c->s));
+    LAUNCH(k_neighbors<true>, m, B.s, m, b.sx.p, b.sy.p, b.sz.p, c.P.kernel_support_radius,
+           c.reach, c.dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, off.p, (uint32_t *)nullptr, idx.p);
+    LAUNCH(k_static_volume, m, B.s, m, c.K, c.P.boundary_rest_volume_weighting, b.sx.p, b.sy.p,
+           b.sz.p, off.p, idx.p, b.svol.p);
+    B.sync();
 }
 
-// B::find_boundary_samples (sph/mod.rs:707-712)
-static void boundary_samples(sph_ctx *c) {
-    const size_t n = c->n;
-    const double range = c->P.kernel_support_radius;
-    const double vmax = 4.0 / 3.0 * 3.14159265358979323846 * ((range * range) * range);
-    for (auto &bp : c->bodies) {
-        Body &b = *bp;
-        if (b.kind == 0) {
-            b.soff.reserve(n + 1);
-            b.scnt.reserve(std::max<size_t>(n, 1));
-            int bb[6];
-            // the query cells may lie outside the sample grid: guard the union box
-            HIPCK(hipMemcpyAsync(bb, c->bbox.p, sizeof bb, hipMemcpyDeviceToHost, c->s));
-            HIPCK(hipStreamSynchronize(c->s));
-            int lo[3], hi[3];
-            for (int k = 0; k < 3; k++) {
-                lo[k] = std::min(bb[k], b.sg.org[k]);
-                hi[k] = std::max(bb[3 + k], b.sg.org[k] + b.sg.dim[k] - 1);
-            }
-            if (n) hash_guard(c, lo, hi, c->reach);
-            LAUNCH(k_neighbors<false>, n, c->s, n, c->x.p, c->y.p, c->z.p, range, c->reach,
-                   c->dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, (const uint32_t *)nullptr, b.scnt.p,
-                   (uint32_t *)nullptr);
-            uint32_t total = scan_u32(c, b.scnt.p, n, b.soff.p, true);
-            b.sidx.reserve(std::max<uint32_t>(total, 1));
-            LAUNCH(k_neighbors<true>, n, c->s, n, c->x.p, c->y.p, c->z.p, range, c->reach,
-                   c->dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, b.soff.p, (uint32_t *)nullptr,
-                   b.sidx.p);
-        } else {
-            b.flag.reserve(n);
-            DBuf<double> *f[] = {&b.bx, &b.by, &b.bz, &b.bvx, &b.bvy, &b.bvz, &b.bvol};
-            for (auto *q : f) q->reserve(n);
-            PLAUNCH(c, SPH_PROF_BOUNDARY_SAMPLE, k_vm_sample, n, n, c->x.p, c->y.p, c->z.p, b.view(), range,
-                   c->P.rest_density_grid_spacing, vmax, b.flag.p, b.bx.p, b.by.p, b.bz.p,
-                   b.bvx.p, b.bvy.p, b.bvz.p, b.bvol.p);
-        }
+static void init_boundaries(Batch &B, Scene &c) {
+    if (c.boundaries_initialized) return;
+    for (auto &b : c.bodies)
+        if (b->kind == 0) prepare_samples(B, c, *b);
+    c.boundaries_initialized = true;
+}
+
+// ------------------------------------------------------------------------
+// batch layout
+// ------------------------------------------------------------------------
+static void upload_views(Batch &B) {
+    std::vector<SceneView> v;
+    v.reserve(B.sc.size());
+    for (Scene *c : B.sc) v.push_back(c->view(B.cell_start.p));
+    B.views.reserve(std::max<size_t>(v.size(), 1));
+    B.put(B.views.p, v.data(), v.size() * sizeof(SceneView));
+}
+
+static void upload_bodies(Batch &B) {
+    std::vector<BodyDev> v;
+    for (Scene *c : B.sc)
+        for (auto &b : c->bodies) v.push_back(b->view());
+    B.bodies.reserve(std::max<size_t>(v.size(), 1));
+    B.put(B.bodies.p, v.data(), v.size() * sizeof(BodyDev));
+}
+
+static BatchDev batch_dev(Batch &B) {
+    BatchDev d{};
+    d.views = B.views.p;
+    d.pblk = B.pblk.p;
+    d.cblk = B.cblk.p;
+    d.ns = (int)B.sc.size();
+    d.ci = B.ci.p;
+    d.perm = B.perm.p;
+    d.cnt = B.cnt.p;
+    d.slice_len = B.slice_len.p;
+    d.slice_off = B.slice_off.p;
+    d.ell_j = B.ell_j.p;
+    d.ell_gx = B.ell_gx.p;
+    d.ell_gy = B.ell_gy.p;
+    d.ell_gz = B.ell_gz.p;
+    d.cell_cnt = B.cell_cnt.p;
+    d.cell_start = B.cell_start.p;
+    d.cursor = B.cursor.p;
+    d.bodies = B.bodies.p;
+    d.partials = B.partials.p;
+    return d;
+}
+
+// particle-side placement of every scene; per-particle batch buffers sized
+static void layout(Batch &B) {
+    const size_t ns = B.sc.size();
+    B.hp_blk.assign(ns + 1, 0);
+    size_t parts = 0, slices = 0, bodies = 0, blocks = 0;
+    for (size_t s = 0; s < ns; s++) {
+        Scene &c = *B.sc[s];
+        if (c.bodies.size() > (size_t)kMaxBodiesPerScene)
+            throw Error(SPH_ERR_UNSUPPORTED, "too many boundaries in one system");
+        c.part_base = (uint32_t)parts;
+        c.slice_base = (uint32_t)slices;
+        c.body_base = (uint32_t)bodies;
+        B.hp_blk[s] = (uint32_t)blocks;
+        parts += c.n;
+        slices += (c.n + 63) / 64;
+        bodies += c.bodies.size();
+        blocks += nblocks(c.n);
     }
-    upload_bodies(c);
+    B.hp_blk[ns] = (uint32_t)blocks;
+    if (parts >= 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "more than 2^32-1 particles");
+    B.total_particles = parts;
+    B.total_pblocks = blocks;
+    B.total_slices = slices;
+    B.pblk.reserve(ns + 1);
+    B.put(B.pblk.p, B.hp_blk.data(), (ns + 1) * sizeof(uint32_t));
+    B.ci.reserve(3 * std::max<size_t>(parts, 1));
+    B.perm.reserve(std::max<size_t>(parts, 1));
+    B.cnt.reserve(std::max<size_t>(parts, 1));
+    B.slice_len.reserve(slices + 1);
+    B.slice_off.reserve(slices + 1);
+    B.partials.reserve((blocks + 1) * kMaxBodiesPerScene * 6 + 64);
+    B.red.reserve(std::max<size_t>(ns, 1) * kMaxBodiesPerScene * 6 + 64);
+    B.ensure_pinned(std::max<size_t>(ns, 1) * kMaxBodiesPerScene * 6 + 64);
+    upload_bodies(B);
+    upload_views(B);
 }
 
-static bool any_dynamic(const sph_ctx *c) {
-    for (auto &b : c->bodies)
-        if (b->dynamic) return true;
-    return false;
-}
-
-// reduce per-block body partials and add them to the rigid bodies
-static void apply_body_partials(sph_ctx *c, size_t n) {
-    if (!any_dynamic(c)) return;
-    unsigned nbk = nblocks(n);
-    if (nbk == 0) return;
-    const int rows = kMaxBodies * 6;
-    hipLaunchKernelGGL(k_reduce_rows, dim3(rows), dim3(kBlock), 0, c->s, c->partials.p,
-                       (int)nbk, rows, c->red.p, 0);
+// ------------------------------------------------------------------------
+// phases of System::update over all scenes of a batch
+// ------------------------------------------------------------------------
+// max |v| and the below-threshold count of every scene
+static void speed_below(Batch &B, std::vector<double> &vmax, std::vector<double> &below) {
+    const size_t ns = B.sc.size();
+    vmax.assign(ns, 0.0);
+    below.assign(ns, 0.0);
+    if (B.total_pblocks == 0) return;
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_speed_below, B.total_pblocks, d);
+    hipLaunchKernelGGL(kb_fold_speed, dim3((unsigned)ns), dim3(kBlock), 0, B.s, B.partials.p,
+                       B.pblk.p, (int)ns, B.red.p);
     HIPCK(hipGetLastError());
-    double out[kMaxBodies * 6];
-    HIPCK(hipMemcpyAsync(out, c->red.p, sizeof out, hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
-    for (size_t bi = 0; bi < c->bodies.size(); bi++) {
-        Body &b = *c->bodies[bi];
-        if (!b.dynamic) continue;
-        b.rb.add_force_torque(out + bi * 6);
-        for (int k = 0; k < 6; k++) b.applied[k] += out[bi * 6 + k];
+    HIPCK(hipMemcpyAsync(B.pinned, B.red.p, 2 * ns * sizeof(double), hipMemcpyDeviceToHost, B.s));
+    B.sync();
+    for (size_t s = 0; s < ns; s++) {
+        vmax[s] = B.pinned[2 * s];
+        below[s] = B.pinned[2 * s + 1];
     }
-}
-
-static void pressure_accel(sph_ctx *c, double *tx, double *ty, double *tz, int overwrite,
-                           int mirror) {
-    const size_t n = c->n;
-    PLAUNCH(c, SPH_PROF_PRESSURE_ACCEL, k_pressure_accel, n, n, c->K, c->P.boundary_pressure_acceleration_weighting,
-           c->x.p, c->y.p, c->z.p, c->m.p, c->V.p, c->p.p, ELL(c),
-           c->d_bodies.p, (int)c->bodies.size(), overwrite, mirror, tx, ty, tz, c->partials.p);
-    if (mirror) apply_body_partials(c, n);
-}
-
-static SolveConst solve_const(const sph_ctx *c) {
-    SolveConst S;
-    S.dt = c->dt;
-    S.V0 = c->V0;
-    S.eps = c->P.min_diagonal_element;
-    S.omega = c->P.relaxation_factor;
-    S.gamma = c->P.boundary_pressure_acceleration_weighting;
-    return S;
-}
-
-// IISPH::resolve_pressure_globally (iisph.rs:378-560); after < 0: target
-// density error, else AfterIteration(after)
-static uint32_t resolve_pressure(sph_ctx *c, int vp, int after) {
-    const size_t n = c->n;
-    if (n == 0) return 0;
-    SolveConst S = solve_const(c);
-    const int nb = (int)c->bodies.size();
-    PLAUNCH(c, SPH_PROF_SOURCE_DIAG, k_source_diag, n, n, c->K, S, vp, c->x.p, c->y.p, c->z.p, c->svx.p, c->svy.p,
-           c->svz.p, c->m.p, c->V.p, ELL(c), c->d_bodies.p, nb, c->s_f.p,
-           c->a_ff.p, c->p.p);
-    uint32_t iter = 0;
-    double err = INFINITY;
-    c->err_hist.clear();
-    const unsigned nbk = nblocks(n);
-    for (;;) {
-        bool conv = after >= 0 ? (iter > (uint32_t)after) : (iter > 2 && err < c->P.target_density_error);
-        if (conv) break;
-        pressure_accel(c, c->pax.p, c->pay.p, c->paz.p, 1, 0);
-        PLAUNCH(c, SPH_PROF_JACOBI, k_jacobi, n, n, c->K, S, c->x.p, c->y.p, c->z.p, c->V.p, c->pax.p,
-               c->pay.p, c->paz.p, c->s_f.p, c->a_ff.p, ELL(c),
-               c->d_bodies.p, nb, c->p.p, c->partials.p);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(kBlock), 0, c->s, c->partials.p,
-                           (int)nbk, 1, c->red.p, 0);
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(c->pinned, c->red.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
-        HIPCK(hipStreamSynchronize(c->s));
-        if (c->prof.on) c->prof.drain();
-        err = c->pinned[0] / (double)n * 100.0;
-        c->err_hist.push_back(err);
-        iter++;
-        if (iter >= 100000u) throw Error(SPH_ERR_UNSUPPORTED, "pressure solve did not converge");
-    }
-    c->last_iter = iter;
-    c->last_err = err;
-    return iter;
-}
-
-static void solve_pressure(sph_ctx *c) {
-    const size_t n = c->n;
-    const double dt = c->dt;
-    if (c->P.solver == SPH_SOLVER_SESPH) {  // sesph.rs:31-67
-        LAUNCH(k_sesph_pressure, n, c->s, n, c->P.stiffness, c->V0, c->V.p, c->p.p);
-        pressure_accel(c, c->ax.p, c->ay.p, c->az.p, 0, 1);
-    } else if (c->P.solver == SPH_SOLVER_IISPH) {  // iisph.rs:55-96
-        LAUNCH(k_pred_vel, n, c->s, n, dt, 0, c->vx.p, c->vy.p, c->vz.p, c->ax.p, c->ay.p,
-               c->az.p, c->svx.p, c->svy.p, c->svz.p);
-        resolve_pressure(c, 1, -1);
-        pressure_accel(c, c->ax.p, c->ay.p, c->az.p, 0, 1);
-    } else {  // iisph_optimized_source_term.rs:53-217
-        LAUNCH(k_pred_vel, n, c->s, n, dt, 0, c->vx.p, c->vy.p, c->vz.p, c->ax.p, c->ay.p,
-               c->az.p, c->svx.p, c->svy.p, c->svz.p);
-        c->eqs1_iter = resolve_pressure(c, 0, 3);
-        pressure_accel(c, c->ax.p, c->ay.p, c->az.p, 1, 1);
-        LAUNCH(k_pred_vel, n, c->s, n, dt, 1, c->vx.p, c->vy.p, c->vz.p, c->ax.p, c->ay.p,
-               c->az.p, c->svx.p, c->svy.p, c->svz.p);
-        LAUNCH(k_pred_pos, n, c->s, n, dt, c->x.p, c->y.p, c->z.p, c->svx.p, c->svy.p, c->svz.p,
-               c->sxp.p, c->syp.p, c->szp.p);
-        resolve_pressure(c, 1, -1);
-        pressure_accel(c, c->ax.p, c->ay.p, c->az.p, 1, 1);
-        LAUNCH(k_ost_commit, n, c->s, n, dt, c->ax.p, c->ay.p, c->az.p, c->sxp.p, c->syp.p,
-               c->szp.p, c->svx.p, c->svy.p, c->svz.p);
-        // solver slots become the integrator slots (TakePredicted commits them)
-        std::swap(c->ix.p, c->sxp.p); std::swap(c->ix.cap, c->sxp.cap);
-        std::swap(c->iy.p, c->syp.p); std::swap(c->iy.cap, c->syp.cap);
-        std::swap(c->iz.p, c->szp.p); std::swap(c->iz.cap, c->szp.cap);
-        std::swap(c->ivx.p, c->svx.p); std::swap(c->ivx.cap, c->svx.cap);
-        std::swap(c->ivy.p, c->svy.p); std::swap(c->ivy.cap, c->svy.cap);
-        std::swap(c->ivz.p, c->svz.p); std::swap(c->ivz.cap, c->svz.cap);
-    }
-}
-
-// reset_acceleration + gravity + viscosity (sph/mod.rs:640-657)
-static void nonpressure(sph_ctx *c) {
-    const size_t n = c->n;
-    NonPressureConst NP;
-    NP.gravity_mode = c->P.gravity_mode;
-    for (int k = 0; k < 3; k++) NP.gcenter[k] = c->P.gravity_center[k];
-    NP.nu_f = c->P.fluid_viscosity;
-    NP.nu_b = c->P.boundary_viscosity;
-    NP.eps2 = 0.01 * (c->P.rest_density_grid_spacing * c->P.rest_density_grid_spacing);
-    // gravity onto dynamic bodies first (non_pressure_accelerations.rs:64-103)
-    for (auto &bp : c->bodies) {
-        Body &b = *bp;
-        if (!b.dynamic) continue;
-        double acc[3];
-        if (c->P.gravity_mode == SPH_GRAVITY_UNIFORM) {
-            acc[0] = 0.0;
-            acc[1] = 0.0;
-            acc[2] = -9.81;
-        } else {
-            double d[3] = {c->P.gravity_center[0] - b.rb.com[0], c->P.gravity_center[1] - b.rb.com[1],
-                           c->P.gravity_center[2] - b.rb.com[2]};
-            double dist = std::sqrt(((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]) + 1e-4);
-            for (int k = 0; k < 3; k++) acc[k] = (9.81 * d[k]) / dist;
-        }
-        double f[3] = {b.rb.mass * acc[0], b.rb.mass * acc[1], b.rb.mass * acc[2]};
-        b.rb.add_force(f, b.rb.com);
-        for (int k = 0; k < 3; k++) b.applied[k] += f[k];
-    }
-    PLAUNCH(c, SPH_PROF_NONPRESSURE, k_nonpressure, n, n, c->K, NP, c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p,
-           c->vz.p, c->m.p, c->V.p, ELL(c), c->d_bodies.p,
-           (int)c->bodies.size(), c->ax.p, c->ay.p, c->az.p, c->partials.p);
-    apply_body_partials(c, n);
-}
-
-static void measure_sums(sph_ctx *c, double out[3]) {
-    const size_t n = c->n;
-    if (n == 0) {
-        out[0] = out[1] = out[2] = 0;
-        return;
-    }
-    unsigned nbk = nblocks(n);
-    LAUNCH(k_measure, n, c->s, n, c->V0, c->V.p, c->m.p, c->vx.p, c->vy.p, c->vz.p,
-           c->partials.p);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(3), dim3(kBlock), 0, c->s, c->partials.p, (int)nbk, 3,
-                       c->red.p, 0);
-    HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(c->pinned, c->red.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
-    for (int k = 0; k < 3; k++) out[k] = c->pinned[k];
 }
 
 // Fluid::disable/swap/drop_inactive on the reference index order (fluid.rs:159-210)
-static void compact_below(sph_ctx *c) {
-    const size_t n = c->n;
+static void compact_below(Batch &B, Scene &c) {
+    const size_t n = c.n;
     std::vector<double> zh(n);
     std::vector<uint32_t> rh(n);
-    c->z.download(zh.data(), n, c->s);
-    c->ref.download(rh.data(), n, c->s);
-    HIPCK(hipStreamSynchronize(c->s));
-    std::vector<uint32_t> order(n);   // order[r] = storage slot of reference index r
+    c.z.download(zh.data(), n, B.s);
+    c.ref.download(rh.data(), n, B.s);
+    B.sync();
+    std::vector<uint32_t> order(n);  // order[r] = storage slot of reference index r
     for (size_t k = 0; k < n; k++) order[rh[k]] = (uint32_t)k;
     size_t id = 0, nn = n;
     while (id < nn) {
-        if (zh[order[id]] < c->P.disable_particles_below) {
+        if (zh[order[id]] < c.P.disable_particles_below) {
             nn -= 1;
             std::swap(order[id], order[nn]);
         } else {
@@ -950,99 +899,436 @@ static void compact_below(sph_ctx *c) {
     }
     std::vector<uint32_t> newref(n, 0xffffffffu);
     for (size_t r = 0; r < nn; r++) newref[order[r]] = (uint32_t)r;
-    std::vector<uint32_t> keep;
+    std::vector<uint32_t> keep, kref;
     keep.reserve(nn);
-    std::vector<uint32_t> kref;
     kref.reserve(nn);
     for (size_t k = 0; k < n; k++)
         if (newref[k] != 0xffffffffu) {
             keep.push_back((uint32_t)k);
             kref.push_back(newref[k]);
         }
-    c->perm.upload(keep.data(), nn, c->s);
-    DBuf<double> *f64s[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->m};
-    for (auto *f : f64s) permute(c, *f, c->tmp, nn);
-    permute(c, c->fid, c->tmpu, nn);
-    c->ref.upload(kref.data(), nn, c->s);
-    HIPCK(hipStreamSynchronize(c->s));
-    c->n = nn;
+    DBuf<uint32_t> sel;
+    sel.upload(keep.data(), nn, B.s);
+    LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.x.p, c.tx.p);
+    LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.y.p, c.ty.p);
+    LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.z.p, c.tz.p);
+    LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.vx.p, c.tvx.p);
+    LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.vy.p, c.tvy.p);
+    LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.vz.p, c.tvz.p);
+    LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.m.p, c.tm.p);
+    LAUNCH(k_gather<uint32_t>, nn, B.s, nn, sel.p, c.fid.p, c.tfid.p);
+    c.tref.upload(kref.data(), nn, B.s);
+    B.sync();
+    c.x.swap(c.tx);
+    c.y.swap(c.ty);
+    c.z.swap(c.tz);
+    c.vx.swap(c.tvx);
+    c.vy.swap(c.tvy);
+    c.vz.swap(c.tvz);
+    c.m.swap(c.tm);
+    c.fid.swap(c.tfid);
+    c.ref.swap(c.tref);
+    c.n = nn;
 }
 
-// max |v| over the fluid and the number of particles below the disable threshold
-static double speed_below(sph_ctx *c, double *vmax) {
-    const size_t n = c->n;
-    if (n == 0) {
-        *vmax = 0.0;
-        return 0.0;
+// cell boxes, hash guard, grids and the batch cell table
+static void cells_and_grids(Batch &B) {
+    const size_t ns = B.sc.size();
+    std::vector<int> boxes(6 * ns, 0);
+    if (B.total_pblocks) {
+        BatchDev d = batch_dev(B);
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cells, B.total_pblocks, d);
+        B.ipart.reserve(6 * ns + 6);
+        hipLaunchKernelGGL(kb_fold_bbox, dim3((unsigned)ns), dim3(kBlock), 0, B.s,
+                           reinterpret_cast<const int *>(B.partials.p), B.pblk.p, (int)ns,
+                           B.ipart.p);
+        HIPCK(hipGetLastError());
+        int *hb = reinterpret_cast<int *>(B.pinned);
+        HIPCK(hipMemcpyAsync(hb, B.ipart.p, 6 * ns * sizeof(int), hipMemcpyDeviceToHost, B.s));
+        B.sync();
+        memcpy(boxes.data(), hb, 6 * ns * sizeof(int));
     }
-    unsigned nbk = nblocks(n);
-    LAUNCH(k_speed_below, n, c->s, n, c->vx.p, c->vy.p, c->vz.p, c->z.p,
-           c->P.disable_particles_below, c->partials.p);
-    hipLaunchKernelGGL(k_fold_speed, dim3(1), dim3(kBlock), 0, c->s, c->partials.p, (int)nbk,
-                       c->red.p);
+    B.hc_blk.assign(ns + 1, 0);
+    size_t cells = 0, cblocks = 0;
+    for (size_t s = 0; s < ns; s++) {
+        Scene &c = *B.sc[s];
+        int *bb = &boxes[6 * s];
+        if (c.n == 0)
+            for (int k = 0; k < 6; k++) bb[k] = 0;
+        memcpy(c.box, bb, sizeof c.box);
+        if (c.n) hash_guard(B, c.checked_box, c.hash_box_valid, bb, bb + 3, c.reach);
+        c.grid = make_grid(bb, bb + 3);
+        c.cell_base = (uint32_t)cells;
+        B.hc_blk[s] = (uint32_t)cblocks;
+        cells += c.ncells();
+        cblocks += nblocks(c.ncells());
+    }
+    B.hc_blk[ns] = (uint32_t)cblocks;
+    if (cells >= 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "cell table too large");
+    B.total_cells = cells;
+    B.cblk.reserve(ns + 1);
+    B.put(B.cblk.p, B.hc_blk.data(), (ns + 1) * sizeof(uint32_t));
+    B.cell_cnt.reserve(cells + 1);
+    B.cell_start.reserve(cells + 1);
+    B.cursor.reserve(cells + 1);
+    upload_views(B);
+}
+
+// counting sort of every scene by (cell, reference index) and the permutation
+// of the persistent fields (System::update -> N::find_samples, sph/mod.rs:701-706)
+static void sort_scenes(Batch &B) {
+    BatchDev d = batch_dev(B);
+    LAUNCH(k_fill_u32, B.total_cells, B.s, B.cell_cnt.p, B.total_cells, 0u);
+    LAUNCH(k_fill_u32, B.total_cells, B.s, B.cursor.p, B.total_cells, 0u);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cell_count, B.total_pblocks, d);
+    scan_u32(B, B.cell_cnt.p, B.total_cells, B.cell_start.p);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cell_scatter, B.total_pblocks, d);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cell_order, B.hc_blk.back(), d);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_permute, B.total_pblocks, d);
+    for (Scene *c : B.sc) {
+        c->x.swap(c->tx);
+        c->y.swap(c->ty);
+        c->z.swap(c->tz);
+        c->vx.swap(c->tvx);
+        c->vy.swap(c->tvy);
+        c->vz.swap(c->tvz);
+        c->m.swap(c->tm);
+        c->fid.swap(c->tfid);
+        c->ref.swap(c->tref);
+    }
+    upload_views(B);
+}
+
+// neighbor counts, slice widths and the pair cache's size
+static void neighbor_counts(Batch &B) {
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_NEIGHBOR_COUNT, B.total_particles, kb_count_ell, B.total_pblocks, d);
+    uint32_t rows = scan_u32(B, B.slice_len.p, B.total_slices, B.slice_off.p);
+    uint64_t slots = (uint64_t)rows * 64;
+    if (slots > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "neighbor cache exceeds 2^32 slots");
+    B.ell_j.reserve(std::max<uint64_t>(slots, 1));
+    B.ell_gx.reserve(std::max<uint64_t>(slots, 1));
+    B.ell_gy.reserve(std::max<uint64_t>(slots, 1));
+    B.ell_gz.reserve(std::max<uint64_t>(slots, 1));
+}
+
+// B::find_boundary_samples (sph/mod.rs:707-712)
+static void boundary_samples(Batch &B) {
+    bool any_vm = false;
+    for (Scene *c : B.sc) {
+        const size_t n = c->n;
+        for (auto &bp : c->bodies) {
+            Body &b = *bp;
+            if (b.kind == 0) {
+                // per-scene CSR over the static samples
+                b.soff.reserve(n + 1);
+                b.scnt.reserve(std::max<size_t>(n, 1));
+                int lo[3], hi[3];  // the query cells may lie outside the sample grid
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = std::min(c->box[k], b.sg.org[k]);
+                    hi[k] = std::max(c->box[3 + k], b.sg.org[k] + b.sg.dim[k] - 1);
+                }
+                if (n) hash_guard(B, c->checked_box, c->hash_box_valid, lo, hi, c->reach);
+                const double range = c->P.kernel_support_radius;
+                LAUNCH(k_neighbors<false>, n, B.s, n, c->x.p, c->y.p, c->z.p, range, c->reach,
+                       c->dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, (const uint32_t *)nullptr, b.scnt.p,
+                       (uint32_t *)nullptr);
+                uint32_t total = scan_u32(B, b.scnt.p, n, b.soff.p);
+                b.sidx.reserve(std::max<uint32_t>(total, 1));
+                LAUNCH(k_neighbors<true>, n, B.s, n, c->x.p, c->y.p, c->z.p, range, c->reach,
+                       c->dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, b.soff.p, (uint32_t *)nullptr,
+                       b.sidx.p);
+            } else {
+                any_vm = true;
+                b.flag.reserve(std::max<size_t>(n, 1));
+                DBuf<double> *f[] = {&b.bx, &b.by, &b.bz, &b.bvx, &b.bvy, &b.bvz, &b.bvol};
+                for (auto *q : f) q->reserve(std::max<size_t>(n, 1));
+            }
+        }
+    }
+    upload_bodies(B);
+    if (any_vm) {
+        BatchDev d = batch_dev(B);
+        PLAUNCH(B, SPH_PROF_BOUNDARY_SAMPLE, B.total_particles, kb_vm_sample, B.total_pblocks, d);
+    }
+}
+
+// neighbor indices, then grad W into the pair cache and the volume (get_volume)
+static void pairs_and_volume(Batch &B) {
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_PAIRS_VOLUME, B.total_particles, kb_fill_ell, B.total_pblocks, d);
+    PLAUNCH(B, SPH_PROF_PAIRS_VOLUME, B.total_particles, kb_pairs_volume, B.total_pblocks, d);
+}
+
+static bool any_dynamic(const Batch &B) {
+    for (const Scene *c : B.sc)
+        for (auto &b : c->bodies)
+            if (b->dynamic && c->n) return true;
+    return false;
+}
+
+// reduce per-workgroup body partials per scene and add them to the rigid bodies
+static void apply_body_partials(Batch &B) {
+    if (!any_dynamic(B) || B.total_pblocks == 0) return;
+    const size_t ns = B.sc.size();
+    const int rows = kMaxBodiesPerScene * 6;
+    hipLaunchKernelGGL(kb_fold_sum, dim3((unsigned)(ns * rows)), dim3(kBlock), 0, B.s,
+                       B.partials.p, B.pblk.p, (int)ns, rows, B.red.p);
     HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(c->pinned, c->red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCK(hipStreamSynchronize(c->s));
-    *vmax = c->pinned[0];
-    return c->pinned[1];
+    HIPCK(hipMemcpyAsync(B.pinned, B.red.p, ns * rows * sizeof(double), hipMemcpyDeviceToHost, B.s));
+    B.sync();
+    for (size_t s = 0; s < ns; s++) {
+        Scene &c = *B.sc[s];
+        if (c.n == 0) continue;
+        for (size_t bi = 0; bi < c.bodies.size(); bi++) {
+            Body &b = *c.bodies[bi];
+            if (!b.dynamic) continue;
+            const double *f = B.pinned + s * rows + bi * 6;
+            b.rb.add_force_torque(f);
+            for (int k = 0; k < 6; k++) b.applied[k] += f[k];
+        }
+    }
 }
 
-// System::update (sph/mod.rs:674-731)
-static void update(sph_ctx *c) {
-    size_t n = c->n;
-    double vmax = 0.0;
-    double below = speed_below(c, &vmax);
-    if (below > 0.0) {
-        compact_below(c);
-        n = c->n;
-        speed_below(c, &vmax);  // max speed over the remaining particles
-    }
-    if (c->P.cfl) {  // set_cfl_time_step (sph/mod.rs:825-829)
-        double cfl = c->P.cfl_number * c->P.rest_density_grid_spacing / vmax;
-        c->dt = std::fmin(c->P.max_time_increment, cfl);
-    }
-    for (auto &b : c->bodies)
-        for (int k = 0; k < 6; k++) b->applied[k] = 0;
-    fluid_neighbors(c);
-    boundary_samples(c);
-    pairs_and_volume(c);
-    nonpressure(c);
-    solve_pressure(c);
-    double sums[3];
-    measure_sums(c, sums);
-    c->avg_density = n ? sums[0] / (double)n : 0.0;
-    if (c->prof.on) c->prof.drain();
+static void pressure_accel(Batch &B, int overwrite) {
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_PRESSURE_ACCEL, B.total_particles, kb_pressure_accel, B.total_pblocks, d,
+            0, overwrite, 1);
+    apply_body_partials(B);
 }
 
-static void integrate(sph_ctx *c) {
-    const size_t n = c->n;
-    if (c->P.integrator == SPH_INTEGRATOR_EULER_CROMER) {
-        LAUNCH(k_integrate_ec, n, c->s, n, c->dt, c->x.p, c->y.p, c->z.p, c->vx.p, c->vy.p,
-               c->vz.p, c->ax.p, c->ay.p, c->az.p);
-    } else {  // take_predicted.rs:14-27
-        std::swap(c->x.p, c->ix.p); std::swap(c->x.cap, c->ix.cap);
-        std::swap(c->y.p, c->iy.p); std::swap(c->y.cap, c->iy.cap);
-        std::swap(c->z.p, c->iz.p); std::swap(c->z.cap, c->iz.cap);
-        std::swap(c->vx.p, c->ivx.p); std::swap(c->vx.cap, c->ivx.cap);
-        std::swap(c->vy.p, c->ivy.p); std::swap(c->vy.cap, c->ivy.cap);
-        std::swap(c->vz.p, c->ivz.p); std::swap(c->vz.cap, c->ivz.cap);
+// IISPH::resolve_pressure_globally (iisph.rs:378-560) for every scene at once;
+// after < 0: target density error, else AfterIteration(after). Each scene
+// leaves the loop on its own termination test; finished scenes' workgroups
+// return at once.
+static void resolve_pressure(Batch &B, int vp, int after) {
+    const size_t ns = B.sc.size();
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_SOURCE_DIAG, B.total_particles, kb_source_diag, B.total_pblocks, d, vp);
+    for (Scene *c : B.sc) {
+        c->iter = 0;
+        c->err = INFINITY;
+        c->err_hist.clear();
+        c->active = c->n > 0;
     }
+    bool dirty = true;
+    for (;;) {
+        bool any = false;
+        for (Scene *c : B.sc) {
+            if (!c->active) continue;
+            bool conv = after >= 0 ? (c->iter > (uint32_t)after)
+                                   : (c->iter > 2 && c->err < c->P.target_density_error);
+            if (conv) {
+                c->active = false;
+                dirty = true;
+                continue;
+            }
+            if (c->iter >= 100000u)
+                throw Error(SPH_ERR_UNSUPPORTED, "pressure solve did not converge");
+            any = true;
+        }
+        if (!any) break;
+        if (dirty) {
+            upload_views(B);
+            dirty = false;
+        }
+        PLAUNCH(B, SPH_PROF_PRESSURE_ACCEL, B.total_particles, kb_pressure_accel, B.total_pblocks,
+                d, 1, 1, 0);
+        PLAUNCH(B, SPH_PROF_JACOBI, B.total_particles, kb_jacobi, B.total_pblocks, d);
+        hipLaunchKernelGGL(kb_fold_sum, dim3((unsigned)ns), dim3(kBlock), 0, B.s, B.partials.p,
+                           B.pblk.p, (int)ns, 1, B.red.p);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(B.pinned, B.red.p, ns * sizeof(double), hipMemcpyDeviceToHost, B.s));
+        B.sync();
+        if (B.prof.on) B.prof.drain();
+        for (size_t s = 0; s < ns; s++) {
+            Scene &c = *B.sc[s];
+            if (!c.active) continue;
+            c.err = B.pinned[s] / (double)c.n * 100.0;
+            c.err_hist.push_back(c.err);
+            c.iter++;
+        }
+    }
+    for (Scene *c : B.sc) {
+        c->last_iter = c->iter;
+        c->last_err = c->n ? c->err : 0.0;
+        c->active = false;
+    }
+    upload_views(B);
+}
+
+// PressureSolver::solve_and_add_acceleration for every scene (one solver per batch)
+static void solve_pressure(Batch &B) {
+    if (B.sc.empty()) return;
+    const int solver = B.sc[0]->P.solver;
+    BatchDev d = batch_dev(B);
+    if (solver == SPH_SOLVER_SESPH) {  // sesph.rs:31-67
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_sesph_pressure, B.total_pblocks, d);
+        pressure_accel(B, 0);
+    } else if (solver == SPH_SOLVER_IISPH) {  // iisph.rs:55-96
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_pred_vel, B.total_pblocks, d, 0);
+        resolve_pressure(B, 1, -1);
+        pressure_accel(B, 0);
+    } else {  // iisph_optimized_source_term.rs:53-217
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_pred_vel, B.total_pblocks, d, 0);
+        resolve_pressure(B, 0, 3);
+        for (Scene *c : B.sc) c->eqs1_iter = c->last_iter;
+        pressure_accel(B, 1);
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_pred_vel, B.total_pblocks, d, 1);
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_pred_pos, B.total_pblocks, d);
+        resolve_pressure(B, 1, -1);
+        pressure_accel(B, 1);
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_ost_commit, B.total_pblocks, d);
+        // solver slots become the integrator slots (TakePredicted commits them)
+        for (Scene *c : B.sc) {
+            c->ix.swap(c->sxp);
+            c->iy.swap(c->syp);
+            c->iz.swap(c->szp);
+            c->ivx.swap(c->svx);
+            c->ivy.swap(c->svy);
+            c->ivz.swap(c->svz);
+        }
+        upload_views(B);
+    }
+}
+
+// reset_acceleration + gravity + viscosity (sph/mod.rs:640-657)
+static void nonpressure(Batch &B) {
+    // gravity onto dynamic bodies first (non_pressure_accelerations.rs:64-103)
+    for (Scene *c : B.sc) {
+        for (auto &bp : c->bodies) {
+            Body &b = *bp;
+            if (!b.dynamic) continue;
+            double acc[3];
+            if (c->P.gravity_mode == SPH_GRAVITY_UNIFORM) {
+                acc[0] = 0.0;
+                acc[1] = 0.0;
+                acc[2] = -9.81;
+            } else {
+                double dd[3] = {c->P.gravity_center[0] - b.rb.com[0],
+                                c->P.gravity_center[1] - b.rb.com[1],
+                                c->P.gravity_center[2] - b.rb.com[2]};
+                double dist = std::sqrt(((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]) + 1e-4);
+                for (int k = 0; k < 3; k++) acc[k] = (9.81 * dd[k]) / dist;
+            }
+            double f[3] = {b.rb.mass * acc[0], b.rb.mass * acc[1], b.rb.mass * acc[2]};
+            b.rb.add_force(f, b.rb.com);
+            for (int k = 0; k < 3; k++) b.applied[k] += f[k];
+        }
+    }
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_NONPRESSURE, B.total_particles, kb_nonpressure, B.total_pblocks, d);
+    apply_body_partials(B);
+}
+
+static void measure_sums(Batch &B, std::vector<double> &out) {
+    const size_t ns = B.sc.size();
+    out.assign(3 * ns, 0.0);
+    if (B.total_pblocks == 0) return;
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_measure, B.total_pblocks, d);
+    hipLaunchKernelGGL(kb_fold_sum, dim3((unsigned)(3 * ns)), dim3(kBlock), 0, B.s, B.partials.p,
+                       B.pblk.p, (int)ns, 3, B.red.p);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(B.pinned, B.red.p, 3 * ns * sizeof(double), hipMemcpyDeviceToHost, B.s));
+    B.sync();
+    for (size_t k = 0; k < 3 * ns; k++) out[k] = B.pinned[k];
+}
+
+// neighbors + boundary samples + pair cache + volume for the batch's scenes
+static void prepare(Batch &B) {
+    cells_and_grids(B);
+    sort_scenes(B);
+    neighbor_counts(B);
+    boundary_samples(B);
+    pairs_and_volume(B);
+}
+
+// System::update (sph/mod.rs:674-731) for every scene of the batch
+static void update(Batch &B) {
+    layout(B);
+    std::vector<double> vmax, below;
+    speed_below(B, vmax, below);
+    bool compacted = false;
+    for (size_t s = 0; s < B.sc.size(); s++)
+        if (below[s] > 0.0) {
+            compact_below(B, *B.sc[s]);
+            compacted = true;
+        }
+    if (compacted) {
+        layout(B);
+        speed_below(B, vmax, below);  // max speed over the remaining particles
+    }
+    for (size_t s = 0; s < B.sc.size(); s++) {
+        Scene &c = *B.sc[s];
+        if (c.P.cfl) {  // set_cfl_time_step (sph/mod.rs:825-829)
+            double cfl = c.P.cfl_number * c.P.rest_density_grid_spacing / vmax[s];
+            c.dt = std::fmin(c.P.max_time_increment, cfl);
+        }
+        for (auto &b : c.bodies)
+            for (int k = 0; k < 6; k++) b->applied[k] = 0;
+    }
+    prepare(B);
+    nonpressure(B);
+    solve_pressure(B);
+    std::vector<double> sums;
+    measure_sums(B, sums);
+    for (size_t s = 0; s < B.sc.size(); s++) {
+        Scene &c = *B.sc[s];
+        c.avg_density = c.n ? sums[3 * s] / (double)c.n : 0.0;
+    }
+    if (B.prof.on) {
+        B.sync();
+        B.prof.drain();
+    }
+}
+
+// integrator + rigid bodies of every scene, then update (sph/mod.rs:125-144)
+static void step_batch(Batch &B) {
+    layout(B);
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_integrate_ec, B.total_pblocks, d);
+    for (Scene *c : B.sc) {
+        const double dt = c->dt;
+        if (c->P.integrator == SPH_INTEGRATOR_TAKE_PREDICTED) {  // take_predicted.rs:14-27
+            c->x.swap(c->ix);
+            c->y.swap(c->iy);
+            c->z.swap(c->iz);
+            c->vx.swap(c->ivx);
+            c->vy.swap(c->ivy);
+            c->vz.swap(c->ivz);
+        }
+        for (auto &b : c->bodies)
+            if (b->dynamic) b->rb.step(dt);  // B::step_forward_in_time
+        c->steps++;
+        c->time += dt;
+    }
+    update(B);
+}
+
+// a context's scene alone in the context's own batch
+static Batch &solo(sph_ctx *c) {
+    c->own.sc.assign(1, &c->scene);
+    return c->own;
 }
 
 static void fill_measurement(sph_ctx *c, sph_measurement *o) {
-    double sums[3];
-    measure_sums(c, sums);
+    Scene &S = c->scene;
+    Batch &B = solo(c);
+    layout(B);
+    std::vector<double> sums;
+    measure_sums(B, sums);
     memset(o, 0, sizeof *o);
-    const size_t n = c->n;
-    o->time = c->time;
-    o->time_step_size = c->dt;
-    o->density = c->avg_density;
+    const size_t n = S.n;
+    o->time = S.time;
+    o->time_step_size = S.dt;
+    o->density = S.avg_density;
     o->density_error = n ? 100. * (sums[1] / (double)n - 1.) : 0.0;
     o->kinetic_energy = n ? sums[2] / (double)n : 0.0;
-    o->solver_iterations = c->P.solver == SPH_SOLVER_SESPH ? 0 : c->last_iter;
-    o->eqs1_iterations = c->P.solver == SPH_SOLVER_IISPHWOST ? c->eqs1_iter : 0;
-    o->predicted_density_error = c->P.solver == SPH_SOLVER_SESPH ? 0.0 : c->last_err;
-    o->steps = c->steps;
+    o->solver_iterations = S.P.solver == SPH_SOLVER_SESPH ? 0 : S.last_iter;
+    o->eqs1_iterations = S.P.solver == SPH_SOLVER_IISPHWOST ? S.eqs1_iter : 0;
+    o->predicted_density_error = S.P.solver == SPH_SOLVER_SESPH ? 0.0 : S.last_err;
+    o->steps = S.steps;
     o->num_particles = n;
 }
 
@@ -1058,71 +1344,68 @@ static void validate_params(const sph_params &P) {
 
 static void set_host_fluid(sph_ctx *c, size_t n, const double *xyz, const double *v,
                            const double *m, const uint32_t *fid) {
+    Scene &S = c->scene;
+    Batch &B = c->own;
     if (n >= 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "more than 2^32-1 particles");
-    reserve_fluid(c, n);
-    std::vector<double> X(n), Y(n), Z(n);
+    reserve_scene(S, n);
+    std::vector<double> X(n), Y(n), Z(n), U(n), W(n), T(n);
     for (size_t i = 0; i < n; i++) {
         X[i] = xyz[3 * i];
         Y[i] = xyz[3 * i + 1];
         Z[i] = xyz[3 * i + 2];
+        U[i] = v ? v[3 * i] : 0.0;
+        W[i] = v ? v[3 * i + 1] : 0.0;
+        T[i] = v ? v[3 * i + 2] : 0.0;
     }
-    c->x.upload(X.data(), n, c->s);
-    c->y.upload(Y.data(), n, c->s);
-    c->z.upload(Z.data(), n, c->s);
-    for (size_t i = 0; i < n; i++) {
-        X[i] = v ? v[3 * i] : 0.0;
-        Y[i] = v ? v[3 * i + 1] : 0.0;
-        Z[i] = v ? v[3 * i + 2] : 0.0;
-    }
-    c->vx.upload(X.data(), n, c->s);
-    c->vy.upload(Y.data(), n, c->s);
-    c->vz.upload(Z.data(), n, c->s);
-    if (m) c->m.upload(m, n, c->s);
-    else LAUNCH(k_fill_f64, n, c->s, c->m.p, n, 1.0);
+    S.x.upload(X.data(), n, B.s);
+    S.y.upload(Y.data(), n, B.s);
+    S.z.upload(Z.data(), n, B.s);
+    S.vx.upload(U.data(), n, B.s);
+    S.vy.upload(W.data(), n, B.s);
+    S.vz.upload(T.data(), n, B.s);
+    if (m) S.m.upload(m, n, B.s);
+    else LAUNCH(k_fill_f64, n, B.s, S.m.p, n, 1.0);
     std::vector<uint32_t> ids(n), r(n);
     for (size_t i = 0; i < n; i++) {
         ids[i] = fid ? fid[i] : 0u;
         r[i] = (uint32_t)i;
     }
-    c->fid.upload(ids.data(), n, c->s);
-    c->ref.upload(r.data(), n, c->s);
-    LAUNCH(k_fill_f64, n, c->s, c->ax.p, n, 0.0);
-    LAUNCH(k_fill_f64, n, c->s, c->ay.p, n, 0.0);
-    LAUNCH(k_fill_f64, n, c->s, c->az.p, n, 0.0);
-    DBuf<double> *zero[] = {&c->ix, &c->iy, &c->iz, &c->ivx, &c->ivy, &c->ivz, &c->p, &c->V};
-    for (auto *q : zero) LAUNCH(k_fill_f64, n, c->s, q->p, n, 0.0);
-    HIPCK(hipStreamSynchronize(c->s));
-    c->n = n;
+    S.fid.upload(ids.data(), n, B.s);
+    S.ref.upload(r.data(), n, B.s);
+    DBuf<double> *zero[] = {&S.ax, &S.ay, &S.az, &S.ix,  &S.iy, &S.iz,
+                            &S.ivx, &S.ivy, &S.ivz, &S.p, &S.V};
+    for (auto *q : zero) LAUNCH(k_fill_f64, n, B.s, q->p, n, 0.0);
+    B.sync();
+    S.n = n;
 }
 
 // download a sorted-order f64 field into caller order (by reference index)
 static void download_field(sph_ctx *c, const DBuf<double> &f, std::vector<double> &out) {
-    const size_t n = c->n;
+    Scene &S = c->scene;
+    const size_t n = S.n;
     std::vector<double> h(n);
     std::vector<uint32_t> r(n);
-    f.download(h.data(), n, c->s);
-    c->ref.download(r.data(), n, c->s);
-    HIPCK(hipStreamSynchronize(c->s));
+    f.download(h.data(), n, c->own.s);
+    S.ref.download(r.data(), n, c->own.s);
+    c->own.sync();
     out.assign(n, 0.0);
     for (size_t k = 0; k < n; k++) out[r[k]] = h[k];
 }
 
-static void init_boundaries(sph_ctx *c) {
-    if (c->boundaries_initialized) return;
-    for (auto &b : c->bodies)
-        if (b->kind == 0) prepare_samples(c, *b);
-    c->boundaries_initialized = true;
+static void check_solo(const sph_ctx *c) {
+    if (c->scene.in_world) throw Error(SPH_ERR_STATE, "system is attached to a world");
 }
 
-// prepare neighbors + boundary samples for a caller-provided state (per-trait
-// entry points); the caller's index is the reference index
+// neighbors + boundary samples + pair cache + volume for a caller-provided
+// state (per-trait entry points); the caller's index is the reference index
 static void prepare_external(sph_ctx *c, size_t n, const double *x, const double *v,
                              const double *m) {
-    init_boundaries(c);
+    check_solo(c);
+    init_boundaries(c->own, c->scene);
     set_host_fluid(c, n, x, v, m, nullptr);
-    fluid_neighbors(c);
-    boundary_samples(c);
-    pairs_and_volume(c);
+    Batch &B = solo(c);
+    layout(B);
+    prepare(B);
 }
 
 static void split3(const std::vector<double> &a, const std::vector<double> &b,
@@ -1137,14 +1420,15 @@ static void split3(const std::vector<double> &a, const std::vector<double> &b,
 // upload a caller-order f64 field into sorted order
 static void upload_sorted(sph_ctx *c, const double *src, size_t stride, size_t comp,
                           DBuf<double> &dst) {
-    const size_t n = c->n;
+    Scene &S = c->scene;
+    const size_t n = S.n;
     std::vector<uint32_t> r(n);
-    c->ref.download(r.data(), n, c->s);
-    HIPCK(hipStreamSynchronize(c->s));
+    S.ref.download(r.data(), n, c->own.s);
+    c->own.sync();
     std::vector<double> h(n);
     for (size_t k = 0; k < n; k++) h[k] = src[stride * r[k] + comp];
-    dst.upload(h.data(), n, c->s);
-    HIPCK(hipStreamSynchronize(c->s));
+    dst.upload(h.data(), n, c->own.s);
+    c->own.sync();
 }
 
 }  // namespace sph
@@ -1162,20 +1446,19 @@ int sph_ctx_create(const sph_params *params, sph_ctx **out) {
         validate_params(*params);
         check_device();
         std::unique_ptr<sph_ctx> c(new sph_ctx());
-        c->P = *params;
+        Scene &S = c->scene;
+        S.P = *params;
         const double h = params->kernel_support_radius, dxr = params->rest_density_grid_spacing;
-        c->V0 = (dxr * dxr) * dxr;  // rest_volume = dx.powi(3) (sph/mod.rs:816)
+        S.V0 = (dxr * dxr) * dxr;  // rest_volume = dx.powi(3) (sph/mod.rs:816)
         const double alpha = 16. / 3.14159265358979323846;
-        c->K.h = h;
-        c->K.wfac = alpha / ((h * h) * h);
-        c->K.gfac = alpha / ((h * h) * (h * h));
-        c->dthr = distance_threshold(h);
-        c->reach = (int)std::ceil(h / h);
-        c->dt = params->cfl ? 0.0 : params->time_increment;
-        HIPCK(hipGetDevice(&c->device));
-        HIPCK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-        HIPCK(hipHostMalloc((void **)&c->pinned, 64 * sizeof(double), hipHostMallocDefault));
-        reserve_fluid(c.get(), 1);
+        S.K.h = h;
+        S.K.wfac = alpha / ((h * h) * h);
+        S.K.gfac = alpha / ((h * h) * (h * h));
+        S.dthr = distance_threshold(h);
+        S.reach = (int)std::ceil(h / h);
+        S.dt = params->cfl ? 0.0 : params->time_increment;
+        c->own.init();
+        reserve_scene(S, 1);
         *out = c.release();
     });
 }
@@ -1183,11 +1466,8 @@ int sph_ctx_create(const sph_params *params, sph_ctx **out) {
 int sph_ctx_destroy(sph_ctx *c) {
     if (!c) return SPH_OK;
     return guarded([&] {
-        if (c->s) (void)hipStreamSynchronize(c->s);
-        if (c->pinned) (void)hipHostFree(c->pinned);
-        hipStream_t s = c->s;
+        if (c->scene.in_world) throw Error(SPH_ERR_STATE, "detach the system from its world first");
         delete c;
-        if (s) (void)hipStreamDestroy(s);
     });
 }
 
@@ -1195,23 +1475,25 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
     return guarded([&] {
         if (!src || !out) throw Error(SPH_ERR_ARG, "null argument");
         sph_ctx *c = nullptr;
-        int st = sph_ctx_create(&src->P, &c);
+        int st = sph_ctx_create(&src->scene.P, &c);
         if (st) throw Error(st, sph_last_error());
         std::unique_ptr<sph_ctx, int (*)(sph_ctx *)> guard(c, sph_ctx_destroy);
-        sph_ctx *s = const_cast<sph_ctx *>(src);
-        HIPCK(hipStreamSynchronize(s->s));
-        const size_t n = s->n;
-        reserve_fluid(c, n);
-        DBuf<double> sph_ctx::*f64s[] = {
-            &sph_ctx::x,   &sph_ctx::y,   &sph_ctx::z,   &sph_ctx::vx,  &sph_ctx::vy,
-            &sph_ctx::vz,  &sph_ctx::ax,  &sph_ctx::ay,  &sph_ctx::az,  &sph_ctx::m,
-            &sph_ctx::V,   &sph_ctx::p,   &sph_ctx::ix,  &sph_ctx::iy,  &sph_ctx::iz,
-            &sph_ctx::ivx, &sph_ctx::ivy, &sph_ctx::ivz};
-        for (auto f : f64s) (c->*f).copy_from(s->*f, n, c->s);
-        c->fid.copy_from(s->fid, n, c->s);
-        c->ref.copy_from(s->ref, n, c->s);
-        c->n = n;
-        for (auto &bp : s->bodies) {
+        const Scene &s = src->scene;
+        Scene &d = c->scene;
+        hipStream_t cs = c->own.s;
+        HIPCK(hipStreamSynchronize(src->own.s));
+        const size_t n = s.n;
+        reserve_scene(d, n);
+        DBuf<double> Scene::*f64s[] = {&Scene::x,  &Scene::y,   &Scene::z,   &Scene::vx,
+                                       &Scene::vy, &Scene::vz,  &Scene::ax,  &Scene::ay,
+                                       &Scene::az, &Scene::m,   &Scene::V,   &Scene::p,
+                                       &Scene::ix, &Scene::iy,  &Scene::iz,  &Scene::ivx,
+                                       &Scene::ivy, &Scene::ivz};
+        for (auto f : f64s) (d.*f).copy_from(s.*f, n, cs);
+        d.fid.copy_from(s.fid, n, cs);
+        d.ref.copy_from(s.ref, n, cs);
+        d.n = n;
+        for (auto &bp : s.bodies) {
             std::unique_ptr<Body> b(new Body());
             b->kind = bp->kind;
             b->dynamic = bp->dynamic;
@@ -1220,41 +1502,29 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
             b->spos = bp->spos;
             memcpy(b->applied, bp->applied, sizeof b->applied);
             if (bp->kind == 1) {
-                auto copy_lat = [&](LatticeHost &d, const LatticeHost &o) {
-                    d.d = o.d;
-                    size_t nodes = (3 * (size_t)o.d.n[0] + 1) * (3 * (size_t)o.d.n[1] + 1) *
-                                   (3 * (size_t)o.d.n[2] + 1);
-                    size_t cells = (size_t)o.d.n[0] * o.d.n[1] * o.d.n[2];
-                    d.values.copy_from(o.values, nodes, c->s);
-                    d.cell_ok.copy_from(o.cell_ok, cells, c->s);
-                    d.d.values = d.values.p;
-                    d.d.cell_ok = d.cell_ok.p;
-                };
-                copy_lat(b->sdf, bp->sdf);
-                copy_lat(b->vm, bp->vm);
+                b->sdf.copy_from(bp->sdf, cs);
+                b->vm.copy_from(bp->vm, cs);
             }
-            c->bodies.push_back(std::move(b));
+            d.bodies.push_back(std::move(b));
         }
-        c->boundaries_initialized = false;
-        init_boundaries(c);
-        c->dt = s->dt;
-        c->time = s->time;
-        c->steps = s->steps;
-        c->last_iter = s->last_iter;
-        c->eqs1_iter = s->eqs1_iter;
-        c->last_err = s->last_err;
-        c->avg_density = s->avg_density;
-        c->err_hist = s->err_hist;
-        HIPCK(hipStreamSynchronize(c->s));
-        if (s->initialized) {
-            // rebuild neighbor/boundary state so the clone is self-contained
-            std::vector<double> keep_dt{c->dt};
-            fluid_neighbors(c);
-            boundary_samples(c);
-            pairs_and_volume(c);
-            c->dt = keep_dt[0];
+        c->own.sync();
+        d.boundaries_initialized = false;
+        init_boundaries(c->own, d);
+        d.dt = s.dt;
+        d.time = s.time;
+        d.steps = s.steps;
+        d.last_iter = s.last_iter;
+        d.eqs1_iter = s.eqs1_iter;
+        d.last_err = s.last_err;
+        d.avg_density = s.avg_density;
+        d.err_hist = s.err_hist;
+        if (s.initialized) {  // rebuild neighbor/boundary state so the clone is self-contained
+            Batch &B = solo(c);
+            layout(B);
+            prepare(B);
+            B.sync();
         }
-        c->initialized = s->initialized;
+        d.initialized = s.initialized;
         *out = guard.release();
     });
 }
@@ -1268,6 +1538,8 @@ int sph_neighbors_find(sph_ctx *c, double cell_size, double range, const double 
         if (!(cell_size > 0)) throw Error(SPH_ERR_ARG, "cell_size must be > 0");
         if (n >= 0xffffffffull || m >= 0xffffffffull)
             throw Error(SPH_ERR_UNSUPPORTED, "more than 2^32-1 points");
+        check_solo(c);
+        Batch &B = c->own;
         const int reach = (int)std::ceil(range / cell_size);
         if (reach < 0 || reach > 64) throw Error(SPH_ERR_UNSUPPORTED, "range / cell_size out of range");
         const double dthr = distance_threshold(range);
@@ -1279,26 +1551,25 @@ int sph_neighbors_find(sph_ctx *c, double cell_size, double range, const double 
                 hy[i] = a[3 * i + 1];
                 hz[i] = a[3 * i + 2];
             }
-            X.upload(hx.data(), cnt, c->s);
-            Y.upload(hy.data(), cnt, c->s);
-            Z.upload(hz.data(), cnt, c->s);
-            HIPCK(hipStreamSynchronize(c->s));
+            X.upload(hx.data(), cnt, B.s);
+            Y.upload(hy.data(), cnt, B.s);
+            Z.upload(hz.data(), cnt, B.s);
+            B.sync();
         };
         DBuf<double> qx, qy, qz, sx0, sy0, sz0, sx, sy, sz;
         soa(pos, n, qx, qy, qz);
         soa(samples, m, sx0, sy0, sz0);
-        DBuf<int> sci, qci;
+        DBuf<int> sci, qci, bbs, bbq;
         sci.reserve(3 * std::max<size_t>(m, 1));
         qci.reserve(3 * std::max<size_t>(n, 1));
-        DBuf<int> bbs, bbq;
         bbs.reserve(6);
         bbq.reserve(6);
-        cell_box(c, m, cell_size, sx0.p, sy0.p, sz0.p, sci.p, bbs.p);
-        cell_box(c, n, cell_size, qx.p, qy.p, qz.p, qci.p, bbq.p);
+        cell_box(B, m, cell_size, sx0.p, sy0.p, sz0.p, sci.p, bbs.p);
+        cell_box(B, n, cell_size, qx.p, qy.p, qz.p, qci.p, bbq.p);
         int b1[6], b2[6];
-        HIPCK(hipMemcpyAsync(b1, bbs.p, sizeof b1, hipMemcpyDeviceToHost, c->s));
-        HIPCK(hipMemcpyAsync(b2, bbq.p, sizeof b2, hipMemcpyDeviceToHost, c->s));
-        HIPCK(hipStreamSynchronize(c->s));
+        HIPCK(hipMemcpyAsync(b1, bbs.p, sizeof b1, hipMemcpyDeviceToHost, B.s));
+        HIPCK(hipMemcpyAsync(b2, bbq.p, sizeof b2, hipMemcpyDeviceToHost, B.s));
+        B.sync();
         offsets[0] = 0;
         if (n == 0) {
             *total = 0;
@@ -1314,42 +1585,42 @@ int sph_neighbors_find(sph_ctx *c, double cell_size, double range, const double 
             lo[k] = std::min(b1[k], b2[k]);
             hi[k] = std::max(b1[3 + k], b2[3 + k]);
         }
-        c->hash_box_valid = false;
-        hash_guard(c, lo, hi, reach);
-        c->hash_box_valid = false;
+        int checked[6] = {1, 1, 1, 0, 0, 0};
+        bool valid = false;
+        hash_guard(B, checked, valid, lo, hi, reach);
         GridDev g = make_grid(b1, b1 + 3);
-        DBuf<uint32_t> cnt, start, perm;
-        cell_sort(c, m, sci.p, g, cnt, start, perm, nullptr);
+        DBuf<uint32_t> start, perm;
+        cell_sort(B, m, sci.p, g, start, perm);
         g.start = start.p;
         sx.reserve(m);
         sy.reserve(m);
         sz.reserve(m);
-        LAUNCH(k_gather<double>, m, c->s, m, perm.p, sx0.p, sx.p);
-        LAUNCH(k_gather<double>, m, c->s, m, perm.p, sy0.p, sy.p);
-        LAUNCH(k_gather<double>, m, c->s, m, perm.p, sz0.p, sz.p);
+        LAUNCH(k_gather<double>, m, B.s, m, perm.p, sx0.p, sx.p);
+        LAUNCH(k_gather<double>, m, B.s, m, perm.p, sy0.p, sy.p);
+        LAUNCH(k_gather<double>, m, B.s, m, perm.p, sz0.p, sz.p);
         DBuf<uint32_t> qcnt, off, idx;
         qcnt.reserve(n);
         off.reserve(n + 1);
-        LAUNCH(k_neighbors<false>, n, c->s, n, qx.p, qy.p, qz.p, cell_size, reach, dthr, g, sx.p,
+        LAUNCH(k_neighbors<false>, n, B.s, n, qx.p, qy.p, qz.p, cell_size, reach, dthr, g, sx.p,
                sy.p, sz.p, (const uint32_t *)nullptr, qcnt.p, (uint32_t *)nullptr);
-        uint32_t tot = scan_u32(c, qcnt.p, n, off.p, true);
+        uint32_t tot = scan_u32(B, qcnt.p, n, off.p);
         *total = tot;
         DBuf<uint64_t> off64;
         off64.reserve(n + 1);
-        LAUNCH(k_widen_u32, n + 1, c->s, n + 1, off.p, off64.p);
-        off64.download(offsets, n + 1, c->s);
-        HIPCK(hipStreamSynchronize(c->s));
+        LAUNCH(k_widen_u32, n + 1, B.s, n + 1, off.p, off64.p);
+        off64.download(offsets, n + 1, B.s);
+        B.sync();
         if (tot > cap) throw Error(SPH_ERR_CAPACITY, "indices buffer too small");
         if (tot == 0) return;
         if (!indices) throw Error(SPH_ERR_ARG, "null indices");
         idx.reserve(tot);
-        LAUNCH(k_neighbors<true>, n, c->s, n, qx.p, qy.p, qz.p, cell_size, reach, dthr, g, sx.p,
+        LAUNCH(k_neighbors<true>, n, B.s, n, qx.p, qy.p, qz.p, cell_size, reach, dthr, g, sx.p,
                sy.p, sz.p, off.p, (uint32_t *)nullptr, idx.p);
         DBuf<uint64_t> idx64;
         idx64.reserve(tot);
-        LAUNCH(k_map_indices, tot, c->s, (size_t)tot, perm.p, idx.p, idx64.p);
-        idx64.download(indices, tot, c->s);
-        HIPCK(hipStreamSynchronize(c->s));
+        LAUNCH(k_map_indices, tot, B.s, (size_t)tot, perm.p, idx.p, idx64.p);
+        idx64.download(indices, tot, B.s);
+        B.sync();
     });
 }
 
@@ -1357,7 +1628,9 @@ int sph_boundary_add_samples(sph_ctx *c, const double *pos, size_t m, const sph_
     int idx = -1;
     int st = guarded([&] {
         if (!c || (!pos && m)) throw Error(SPH_ERR_ARG, "null argument");
-        if (c->bodies.size() >= (size_t)kMaxBodies) throw Error(SPH_ERR_UNSUPPORTED, "too many boundaries");
+        check_solo(c);
+        if (c->scene.bodies.size() >= (size_t)kMaxBodiesPerScene)
+            throw Error(SPH_ERR_UNSUPPORTED, "too many boundaries");
         if (rigid)
             throw Error(SPH_ERR_UNSUPPORTED,
                         "dynamic StaticSampleBoundary bodies are not built; use a volume map");
@@ -1365,9 +1638,9 @@ int sph_boundary_add_samples(sph_ctx *c, const double *pos, size_t m, const sph_
         b->kind = 0;
         b->ns = m;
         b->spos.assign(pos, pos + 3 * m);
-        c->bodies.push_back(std::move(b));
-        c->boundaries_initialized = false;
-        idx = (int)c->bodies.size() - 1;
+        c->scene.bodies.push_back(std::move(b));
+        c->scene.boundaries_initialized = false;
+        idx = (int)c->scene.bodies.size() - 1;
     });
     return st ? st : idx;
 }
@@ -1378,20 +1651,22 @@ int sph_boundary_add_volume_map(sph_ctx *c, const sph_lattice *sdf, const sph_la
     int st = guarded([&] {
         if (!c || !sdf || !vm || !sdf->values || !sdf->present || !vm->values || !vm->present)
             throw Error(SPH_ERR_ARG, "null argument");
-        if (c->bodies.size() >= (size_t)kMaxBodies) throw Error(SPH_ERR_UNSUPPORTED, "too many boundaries");
-        if (rigid && c->P.solver == SPH_SOLVER_IISPHWOST)  // SUPPORTS_DYNAMIC_BOUNDARIES (setup/mod.rs:242-244)
+        check_solo(c);
+        if (c->scene.bodies.size() >= (size_t)kMaxBodiesPerScene)
+            throw Error(SPH_ERR_UNSUPPORTED, "too many boundaries");
+        if (rigid && c->scene.P.solver == SPH_SOLVER_IISPHWOST)  // SUPPORTS_DYNAMIC_BOUNDARIES (setup/mod.rs:242-244)
             throw Error(SPH_ERR_ARG, "IISPHwOST does not support dynamic boundaries");
         if (rigid && !(rigid->mass != 0.0)) throw Error(SPH_ERR_ARG, "rigid body mass cannot be zero");
         std::unique_ptr<Body> b(new Body());
         b->kind = 1;
-        b->sdf.load(*sdf, c->s);
-        b->vm.load(*vm, c->s);
+        b->sdf.load(*sdf, c->own.s);
+        b->vm.load(*vm, c->own.s);
         if (rigid) {
             b->dynamic = true;
             b->rb.init(*rigid);
         }
-        c->bodies.push_back(std::move(b));
-        idx = (int)c->bodies.size() - 1;
+        c->scene.bodies.push_back(std::move(b));
+        idx = (int)c->scene.bodies.size() - 1;
     });
     return st ? st : idx;
 }
@@ -1400,36 +1675,33 @@ int sph_fluid_upload(sph_ctx *c, size_t n, const double *x, const double *v, con
                      const uint32_t *fluid_id) {
     return guarded([&] {
         if (!c || (!x && n) || (!m && n)) throw Error(SPH_ERR_ARG, "null argument");
+        check_solo(c);
         set_host_fluid(c, n, x, v, m, fluid_id);
-        c->initialized = false;
+        c->scene.initialized = false;
     });
 }
 
 int sph_system_init(sph_ctx *c) {
     return guarded([&] {
         if (!c) throw Error(SPH_ERR_ARG, "null argument");
-        init_boundaries(c);
-        c->dt = c->P.cfl ? 0.0 : c->P.time_increment;
-        c->time = 0;
-        c->steps = 0;
-        update(c);
-        c->initialized = true;
+        check_solo(c);
+        Scene &S = c->scene;
+        init_boundaries(c->own, S);
+        S.dt = S.P.cfl ? 0.0 : S.P.time_increment;
+        S.time = 0;
+        S.steps = 0;
+        update(solo(c));
+        S.initialized = true;
     });
 }
 
 int sph_step(sph_ctx *c, int nsteps, sph_measurement *out) {
     return guarded([&] {
         if (!c) throw Error(SPH_ERR_ARG, "null argument");
-        if (!c->initialized) throw Error(SPH_ERR_STATE, "sph_system_init has not run");
-        for (int k = 0; k < nsteps; k++) {
-            const double dt = c->dt;
-            integrate(c);
-            for (auto &b : c->bodies)
-                if (b->dynamic) b->rb.step(dt);  // B::step_forward_in_time
-            c->steps++;
-            c->time += dt;
-            update(c);
-        }
+        check_solo(c);
+        if (!c->scene.initialized) throw Error(SPH_ERR_STATE, "sph_system_init has not run");
+        Batch &B = solo(c);
+        for (int k = 0; k < nsteps; k++) step_batch(B);
         if (out) fill_measurement(c, out);
     });
 }
@@ -1439,7 +1711,7 @@ int sph_fluid_len(const sph_ctx *c, size_t *n) {
         set_error("null argument");
         return SPH_ERR_ARG;
     }
-    *n = c->n;
+    *n = c->scene.n;
     return SPH_OK;
 }
 
@@ -1447,43 +1719,44 @@ int sph_fluid_download(sph_ctx *c, double *x, double *v, double *a, double *V, d
                        double *m, uint32_t *fluid_id) {
     return guarded([&] {
         if (!c) throw Error(SPH_ERR_ARG, "null argument");
-        std::vector<double> A, B, C;
+        Scene &S = c->scene;
+        std::vector<double> A, B_, C_;
         if (x) {
-            download_field(c, c->x, A);
-            download_field(c, c->y, B);
-            download_field(c, c->z, C);
-            split3(A, B, C, x);
+            download_field(c, S.x, A);
+            download_field(c, S.y, B_);
+            download_field(c, S.z, C_);
+            split3(A, B_, C_, x);
         }
         if (v) {
-            download_field(c, c->vx, A);
-            download_field(c, c->vy, B);
-            download_field(c, c->vz, C);
-            split3(A, B, C, v);
+            download_field(c, S.vx, A);
+            download_field(c, S.vy, B_);
+            download_field(c, S.vz, C_);
+            split3(A, B_, C_, v);
         }
         if (a) {
-            download_field(c, c->ax, A);
-            download_field(c, c->ay, B);
-            download_field(c, c->az, C);
-            split3(A, B, C, a);
+            download_field(c, S.ax, A);
+            download_field(c, S.ay, B_);
+            download_field(c, S.az, C_);
+            split3(A, B_, C_, a);
         }
         if (V) {
-            download_field(c, c->V, A);
+            download_field(c, S.V, A);
             memcpy(V, A.data(), A.size() * sizeof(double));
         }
         if (p) {
-            download_field(c, c->p, A);
+            download_field(c, S.p, A);
             memcpy(p, A.data(), A.size() * sizeof(double));
         }
         if (m) {
-            download_field(c, c->m, A);
+            download_field(c, S.m, A);
             memcpy(m, A.data(), A.size() * sizeof(double));
         }
         if (fluid_id) {
-            const size_t n = c->n;
+            const size_t n = S.n;
             std::vector<uint32_t> h(n), r(n);
-            c->fid.download(h.data(), n, c->s);
-            c->ref.download(r.data(), n, c->s);
-            HIPCK(hipStreamSynchronize(c->s));
+            S.fid.download(h.data(), n, c->own.s);
+            S.ref.download(r.data(), n, c->own.s);
+            c->own.sync();
             for (size_t k = 0; k < n; k++) fluid_id[r[k]] = h[k];
         }
     });
@@ -1499,17 +1772,18 @@ int sph_measure(sph_ctx *c, sph_measurement *out) {
 int sph_error_history(sph_ctx *c, double *out, size_t cap, size_t *count) {
     return guarded([&] {
         if (!c || !count) throw Error(SPH_ERR_ARG, "null argument");
-        *count = c->err_hist.size();
+        const auto &h = c->scene.err_hist;
+        *count = h.size();
         if (out)
-            for (size_t k = 0; k < std::min(cap, c->err_hist.size()); k++) out[k] = c->err_hist[k];
+            for (size_t k = 0; k < std::min(cap, h.size()); k++) out[k] = h[k];
     });
 }
 
 int sph_body_state(sph_ctx *c, int bi, double out[22]) {
     return guarded([&] {
-        if (!c || !out || bi < 0 || (size_t)bi >= c->bodies.size())
+        if (!c || !out || bi < 0 || (size_t)bi >= c->scene.bodies.size())
             throw Error(SPH_ERR_ARG, "bad boundary index");
-        const Rigid &r = c->bodies[bi]->rb;
+        const Rigid &r = c->scene.bodies[bi]->rb;
         memcpy(out, r.com, 3 * sizeof(double));
         memcpy(out + 3, r.q, 4 * sizeof(double));
         memcpy(out + 7, r.v, 3 * sizeof(double));
@@ -1522,30 +1796,30 @@ int sph_body_state(sph_ctx *c, int bi, double out[22]) {
 
 int sph_body_forces(sph_ctx *c, int bi, double out[6]) {
     return guarded([&] {
-        if (!c || !out || bi < 0 || (size_t)bi >= c->bodies.size())
+        if (!c || !out || bi < 0 || (size_t)bi >= c->scene.bodies.size())
             throw Error(SPH_ERR_ARG, "bad boundary index");
-        memcpy(out, c->bodies[bi]->applied, 6 * sizeof(double));
+        memcpy(out, c->scene.bodies[bi]->applied, 6 * sizeof(double));
     });
 }
 
 int sph_profile(sph_ctx *c, int enable) {
     return guarded([&] {
         if (!c) throw Error(SPH_ERR_ARG, "null argument");
-        HIPCK(hipStreamSynchronize(c->s));
-        c->prof.drain();
-        c->prof.reset();
-        c->prof.on = enable != 0;
+        c->own.sync();
+        c->own.prof.drain();
+        c->own.prof.reset();
+        c->own.prof.on = enable != 0;
     });
 }
 
 int sph_profile_read(sph_ctx *c, int kernel, uint64_t *launches, uint64_t *units, double *ms) {
     return guarded([&] {
         if (!c || kernel < 0 || kernel >= SPH_PROF_KERNELS) throw Error(SPH_ERR_ARG, "bad argument");
-        HIPCK(hipStreamSynchronize(c->s));
-        c->prof.drain();
-        if (launches) *launches = c->prof.launches[kernel];
-        if (units) *units = c->prof.units[kernel];
-        if (ms) *ms = c->prof.ms[kernel];
+        c->own.sync();
+        c->own.prof.drain();
+        if (launches) *launches = c->own.prof.launches[kernel];
+        if (units) *units = c->own.prof.units[kernel];
+        if (ms) *ms = c->own.prof.ms[kernel];
     });
 }
 
@@ -1554,18 +1828,18 @@ int sph_volume(sph_ctx *c, size_t n, const double *x, double *V_out) {
         if (!c || (!x && n) || (!V_out && n)) throw Error(SPH_ERR_ARG, "null argument");
         prepare_external(c, n, x, nullptr, nullptr);
         std::vector<double> A;
-        download_field(c, c->V, A);
+        download_field(c, c->scene.V, A);
         if (n) memcpy(V_out, A.data(), n * sizeof(double));
-        c->initialized = false;
+        c->scene.initialized = false;
     });
 }
 
 int sph_boundary_find_samples(sph_ctx *c, int bi, size_t n, const double *x,
                               double *samples_out, uint8_t *flags_out) {
     return guarded([&] {
-        if (!c || (!x && n) || bi < 0 || (size_t)bi >= c->bodies.size())
+        if (!c || (!x && n) || bi < 0 || (size_t)bi >= c->scene.bodies.size())
             throw Error(SPH_ERR_ARG, "bad argument");
-        Body &b = *c->bodies[bi];
+        Body &b = *c->scene.bodies[bi];
         if (b.kind != 1) throw Error(SPH_ERR_ARG, "boundary is not a volume map");
         prepare_external(c, n, x, nullptr, nullptr);
         std::vector<double> f[7];
@@ -1573,16 +1847,16 @@ int sph_boundary_find_samples(sph_ctx *c, int bi, size_t n, const double *x,
         for (int k = 0; k < 7; k++) download_field(c, *src[k], f[k]);
         std::vector<uint8_t> fl(n), flr(n);
         std::vector<uint32_t> r(n);
-        b.flag.download(fl.data(), n, c->s);
-        c->ref.download(r.data(), n, c->s);
-        HIPCK(hipStreamSynchronize(c->s));
+        b.flag.download(fl.data(), n, c->own.s);
+        c->scene.ref.download(r.data(), n, c->own.s);
+        c->own.sync();
         for (size_t k = 0; k < n; k++) flr[r[k]] = fl[k];
         for (size_t i = 0; i < n; i++) {
             if (flags_out) flags_out[i] = flr[i];
             if (samples_out)
                 for (int k = 0; k < 7; k++) samples_out[7 * i + k] = flr[i] ? f[k][i] : 0.0;
         }
-        c->initialized = false;
+        c->scene.initialized = false;
     });
 }
 
@@ -1591,22 +1865,23 @@ int sph_nonpressure_accel(sph_ctx *c, size_t n, const double *x, const double *v
     return guarded([&] {
         if (!c || (n && (!x || !v || !m || !V || !a_out))) throw Error(SPH_ERR_ARG, "null argument");
         prepare_external(c, n, x, v, m);
-        upload_sorted(c, V, 1, 0, c->V);
-        for (auto &b : c->bodies)
+        Scene &S = c->scene;
+        upload_sorted(c, V, 1, 0, S.V);
+        for (auto &b : S.bodies)
             for (int k = 0; k < 6; k++) b->applied[k] = 0;
         std::vector<Rigid> saved;
-        for (auto &b : c->bodies) saved.push_back(b->rb);
-        nonpressure(c);
-        std::vector<double> A, B, C;
-        download_field(c, c->ax, A);
-        download_field(c, c->ay, B);
-        download_field(c, c->az, C);
-        split3(A, B, C, a_out);
-        for (size_t bi = 0; bi < c->bodies.size(); bi++) {
-            if (body_ft_out) memcpy(body_ft_out + 6 * bi, c->bodies[bi]->applied, 6 * sizeof(double));
-            c->bodies[bi]->rb = saved[bi];  // stateless with respect to the bodies
+        for (auto &b : S.bodies) saved.push_back(b->rb);
+        nonpressure(c->own);
+        std::vector<double> A, B_, C_;
+        download_field(c, S.ax, A);
+        download_field(c, S.ay, B_);
+        download_field(c, S.az, C_);
+        split3(A, B_, C_, a_out);
+        for (size_t bi = 0; bi < S.bodies.size(); bi++) {
+            if (body_ft_out) memcpy(body_ft_out + 6 * bi, S.bodies[bi]->applied, 6 * sizeof(double));
+            S.bodies[bi]->rb = saved[bi];  // stateless with respect to the bodies
         }
-        c->initialized = false;
+        S.initialized = false;
     });
 }
 
@@ -1617,54 +1892,153 @@ int sph_pressure_solve(sph_ctx *c, size_t n, const double *x, const double *v, c
     return guarded([&] {
         if (!c || (n && (!x || !v || !m || !V || !a_inout))) throw Error(SPH_ERR_ARG, "null argument");
         prepare_external(c, n, x, v, m);
-        upload_sorted(c, V, 1, 0, c->V);
-        upload_sorted(c, a_inout, 3, 0, c->ax);
-        upload_sorted(c, a_inout, 3, 1, c->ay);
-        upload_sorted(c, a_inout, 3, 2, c->az);
-        for (auto &b : c->bodies)
+        Scene &S = c->scene;
+        upload_sorted(c, V, 1, 0, S.V);
+        upload_sorted(c, a_inout, 3, 0, S.ax);
+        upload_sorted(c, a_inout, 3, 1, S.ay);
+        upload_sorted(c, a_inout, 3, 2, S.az);
+        for (auto &b : S.bodies)
             for (int k = 0; k < 6; k++) b->applied[k] = 0;
         std::vector<Rigid> saved;
-        for (auto &b : c->bodies) saved.push_back(b->rb);
-        const double keep_dt = c->dt;
-        c->dt = dt;
-        solve_pressure(c);
-        c->dt = keep_dt;
-        std::vector<double> A, B, C;
-        download_field(c, c->ax, A);
-        download_field(c, c->ay, B);
-        download_field(c, c->az, C);
-        split3(A, B, C, a_inout);
+        for (auto &b : S.bodies) saved.push_back(b->rb);
+        const double keep_dt = S.dt;
+        S.dt = dt;
+        upload_views(c->own);
+        solve_pressure(c->own);
+        S.dt = keep_dt;
+        std::vector<double> A, B_, C_;
+        download_field(c, S.ax, A);
+        download_field(c, S.ay, B_);
+        download_field(c, S.az, C_);
+        split3(A, B_, C_, a_inout);
         if (p_out) {
-            download_field(c, c->p, A);
+            download_field(c, S.p, A);
             memcpy(p_out, A.data(), n * sizeof(double));
         }
-        if (c->P.solver == SPH_SOLVER_IISPHWOST) {
+        if (S.P.solver == SPH_SOLVER_IISPHWOST) {
             if (xpred_out) {
-                download_field(c, c->ix, A);
-                download_field(c, c->iy, B);
-                download_field(c, c->iz, C);
-                split3(A, B, C, xpred_out);
+                download_field(c, S.ix, A);
+                download_field(c, S.iy, B_);
+                download_field(c, S.iz, C_);
+                split3(A, B_, C_, xpred_out);
             }
             if (vpred_out) {
-                download_field(c, c->ivx, A);
-                download_field(c, c->ivy, B);
-                download_field(c, c->ivz, C);
-                split3(A, B, C, vpred_out);
+                download_field(c, S.ivx, A);
+                download_field(c, S.ivy, B_);
+                download_field(c, S.ivz, C_);
+                split3(A, B_, C_, vpred_out);
             }
         }
-        for (size_t bi = 0; bi < c->bodies.size(); bi++) {
-            if (body_ft_out) memcpy(body_ft_out + 6 * bi, c->bodies[bi]->applied, 6 * sizeof(double));
-            c->bodies[bi]->rb = saved[bi];
+        for (size_t bi = 0; bi < S.bodies.size(); bi++) {
+            if (body_ft_out) memcpy(body_ft_out + 6 * bi, S.bodies[bi]->applied, 6 * sizeof(double));
+            S.bodies[bi]->rb = saved[bi];
         }
         if (info_out) {
             memset(info_out, 0, sizeof *info_out);
-            info_out->solver_iterations = c->P.solver == SPH_SOLVER_SESPH ? 0 : c->last_iter;
-            info_out->eqs1_iterations = c->P.solver == SPH_SOLVER_IISPHWOST ? c->eqs1_iter : 0;
-            info_out->predicted_density_error = c->P.solver == SPH_SOLVER_SESPH ? 0.0 : c->last_err;
+            info_out->solver_iterations = S.P.solver == SPH_SOLVER_SESPH ? 0 : S.last_iter;
+            info_out->eqs1_iterations = S.P.solver == SPH_SOLVER_IISPHWOST ? S.eqs1_iter : 0;
+            info_out->predicted_density_error = S.P.solver == SPH_SOLVER_SESPH ? 0.0 : S.last_err;
             info_out->time_step_size = dt;
             info_out->num_particles = n;
         }
-        c->initialized = false;
+        S.initialized = false;
+    });
+}
+
+// ---- worlds: many systems stepped together in shared launches ---------------
+int sph_world_create(sph_world **out) {
+    return guarded([&] {
+        if (!out) throw Error(SPH_ERR_ARG, "null argument");
+        check_device();
+        std::unique_ptr<sph_world> w(new sph_world());
+        w->batch.init();
+        *out = w.release();
+    });
+}
+
+int sph_world_destroy(sph_world *w) {
+    if (!w) return SPH_OK;
+    return guarded([&] {
+        for (sph_ctx *c : w->members) c->scene.in_world = false;
+        delete w;
+    });
+}
+
+int sph_world_attach(sph_world *w, sph_ctx *c) {
+    return guarded([&] {
+        if (!w || !c) throw Error(SPH_ERR_ARG, "null argument");
+        if (c->scene.in_world) throw Error(SPH_ERR_STATE, "system already attached to a world");
+        if (!c->scene.initialized) throw Error(SPH_ERR_STATE, "sph_system_init has not run");
+        if (!w->members.empty() && w->members[0]->scene.P.solver != c->scene.P.solver)
+            throw Error(SPH_ERR_UNSUPPORTED, "all systems of a world use one pressure solver");
+        c->own.sync();
+        c->scene.in_world = true;
+        w->members.push_back(c);
+    });
+}
+
+int sph_world_detach(sph_world *w, sph_ctx *c) {
+    return guarded([&] {
+        if (!w || !c) throw Error(SPH_ERR_ARG, "null argument");
+        auto it = std::find(w->members.begin(), w->members.end(), c);
+        if (it == w->members.end()) throw Error(SPH_ERR_ARG, "system is not attached to this world");
+        w->batch.sync();
+        w->members.erase(it);
+        c->scene.in_world = false;
+    });
+}
+
+int sph_world_step(sph_world *w, int nsteps) {
+    return guarded([&] {
+        if (!w) throw Error(SPH_ERR_ARG, "null argument");
+        Batch &B = w->batch;
+        B.sc.clear();
+        for (sph_ctx *c : w->members) B.sc.push_back(&c->scene);
+        if (B.sc.empty()) return;
+        for (int k = 0; k < nsteps; k++) step_batch(B);
+        B.sync();
+    });
+}
+
+int sph_world_size(const sph_world *w, size_t *systems, size_t *particles) {
+    return guarded([&] {
+        if (!w) throw Error(SPH_ERR_ARG, "null argument");
+        size_t p = 0;
+        for (sph_ctx *c : w->members) p += c->scene.n;
+        if (systems) *systems = w->members.size();
+        if (particles) *particles = p;
+    });
+}
+
+int sph_world_profile(sph_world *w, int enable) {
+    return guarded([&] {
+        if (!w) throw Error(SPH_ERR_ARG, "null argument");
+        w->batch.sync();
+        w->batch.prof.drain();
+        w->batch.prof.reset();
+        w->batch.prof.on = enable != 0;
+    });
+}
+
+int sph_world_profile_read(sph_world *w, int kernel, uint64_t *launches, uint64_t *units,
+                           double *ms) {
+    return guarded([&] {
+        if (!w || kernel < 0 || kernel >= SPH_PROF_KERNELS) throw Error(SPH_ERR_ARG, "bad argument");
+        w->batch.sync();
+        w->batch.prof.drain();
+        if (launches) *launches = w->batch.prof.launches[kernel];
+        if (units) *units = w->batch.prof.units[kernel];
+        if (ms) *ms = w->batch.prof.ms[kernel];
+    });
+}
+
+int sph_device_memory(size_t *free_bytes, size_t *total_bytes) {
+    return guarded([&] {
+        check_device();
+        size_t f = 0, t = 0;
+        HIPCK(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
     });
 }
 

@@ -233,4 +233,83 @@ __device__ inline int serendipity_grad(const LatticeDev &L, double px, double py
     return 0;
 }
 
+
+// ---- helpers shared by the kernel files ----
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double *lds /* N*kBlock */) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < N; k++) lds[k * kBlock + t] = v[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int k = 0; k < N; k++) lds[k * kBlock + t] += lds[k * kBlock + t + s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = lds[k * kBlock];
+    __syncthreads();
+}
+
+__device__ __forceinline__ int sat_i32(double v) {
+    if (v != v) return 0;
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return (int)(-2147483647 - 1);
+    return (int)v;
+}
+
+__device__ inline int body_sd(const BodyDev &b, double px, double py, double pz, double &sd) {
+    if (!b.dynamic) return serendipity(b.sdf, px, py, pz, sd);
+    double lx, ly, lz;
+    world_to_local(b, px, py, pz, lx, ly, lz);
+    return serendipity(b.sdf, lx, ly, lz, sd);
+}
+__device__ inline int body_sdg(const BodyDev &b, double px, double py, double pz, double &gx,
+                               double &gy, double &gz) {
+    if (!b.dynamic) return serendipity_grad(b.sdf, px, py, pz, gx, gy, gz);
+    double lx, ly, lz, ux, uy, uz;
+    world_to_local(b, px, py, pz, lx, ly, lz);
+    int e = serendipity_grad(b.sdf, lx, ly, lz, ux, uy, uz);
+    if (e) return e;
+    quat_rotate(b.q[0], b.q[1], b.q[2], b.q[3], ux, uy, uz, gx, gy, gz);
+    return 0;
+}
+__device__ inline int body_vm(const BodyDev &b, double px, double py, double pz, double &v) {
+    if (!b.dynamic) return serendipity(b.vm, px, py, pz, v);
+    double lx, ly, lz;
+    world_to_local(b, px, py, pz, lx, ly, lz);
+    return serendipity(b.vm, lx, ly, lz, v);
+}
+
+// the four viscosity samples around the normal sample (volume_map_boundary.rs:1347-1376)
+__device__ inline void visc_samples(double px, double py, double pz, double ox, double oy,
+                                    double oz, double range, double (&vp)[4][3]) {
+    double vx = ox - px, vy = oy - py, vz = oz - pz;
+    double vn = sqrt(dot3(vx, vy, vz, vx, vy, vz));
+    double ux = vx / vn, uy = vy / vn, uz = vz / vn;
+    double tx = 1., ty = 0., tz = 0.;
+    if (fabs(dot3(ux, uy, uz, tx, ty, tz)) > 0.9) {
+        tx = 0.;
+        ty = 1.;
+    }
+    double t1x, t1y, t1z, t2x, t2y, t2z;
+    cross3(vx, vy, vz, tx, ty, tz, t1x, t1y, t1z);
+    cross3(vx, vy, vz, t1x, t1y, t1z, t2x, t2y, t2z);
+    double n1 = sqrt(dot3(t1x, t1y, t1z, t1x, t1y, t1z));
+    double n2 = sqrt(dot3(t2x, t2y, t2z, t2x, t2y, t2z));
+    t1x = t1x / n1;
+    t1y = t1y / n1;
+    t1z = t1z / n1;
+    t2x = t2x / n2;
+    t2y = t2y / n2;
+    t2z = t2z / n2;
+    double d = 0.5 * range;
+    vp[0][0] = ox + d * t1x; vp[0][1] = oy + d * t1y; vp[0][2] = oz + d * t1z;
+    vp[1][0] = ox - d * t1x; vp[1][1] = oy - d * t1y; vp[1][2] = oz - d * t1z;
+    vp[2][0] = ox + d * t2x; vp[2][1] = oy + d * t2y; vp[2][2] = oz + d * t2z;
+    vp[3][0] = ox - d * t2x; vp[3][1] = oy - d * t2y; vp[3][2] = oz - d * t2z;
+}
+
 }  // namespace sph

@@ -97,6 +97,15 @@ SIGNATURES = [
     ("sph_nonpressure_accel", C.c_int, [C.c_void_p, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("sph_pressure_solve", C.c_int, [C.c_void_p, C.c_size_t, _dp, _dp, _dp, _dp, C.c_double,
                                      _dp, _dp, _dp, _dp, _dp, C.POINTER(SphMeasurement)]),
+    ("sph_world_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("sph_world_destroy", C.c_int, [C.c_void_p]),
+    ("sph_world_attach", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("sph_world_detach", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("sph_world_step", C.c_int, [C.c_void_p, C.c_int]),
+    ("sph_world_size", C.c_int, [C.c_void_p, _szp, _szp]),
+    ("sph_world_profile", C.c_int, [C.c_void_p, C.c_int]),
+    ("sph_world_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
+    ("sph_device_memory", C.c_int, [_szp, _szp]),
 ]
 
 _lib = None

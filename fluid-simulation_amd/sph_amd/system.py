@@ -219,14 +219,7 @@ class System:
         N.check(N.lib().sph_profile(self._ctx.h, int(bool(enable))))
 
     def profile_read(self) -> dict:
-        out = {}
-        for k, name in enumerate(N.PROF_KERNELS):
-            launches, units, ms = C.c_uint64(), C.c_uint64(), C.c_double()
-            N.check(N.lib().sph_profile_read(self._ctx.h, k, C.byref(launches), C.byref(units),
-                                             C.byref(ms)))
-            out[name] = {"launches": int(launches.value), "units": int(units.value),
-                         "ms": float(ms.value)}
-        return out
+        return _read_profile(N.lib().sph_profile_read, self._ctx.h)
 
     def clone(self) -> "System":
         """Deep copy into an independent device context (DynClone)."""
@@ -281,3 +274,74 @@ class System:
                                            N.dptr(ft), N.dptr(xp), N.dptr(vp), C.byref(info)))
         return {"a": a, "p": p, "body_ft": ft.reshape(-1, 6), "x_pred": xp, "v_pred": vp,
                 "info": info.as_dict()}
+
+
+def _read_profile(fn, handle) -> dict:
+    out = {}
+    for k, name in enumerate(N.PROF_KERNELS):
+        launches, units, ms = C.c_uint64(), C.c_uint64(), C.c_double()
+        N.check(fn(handle, k, C.byref(launches), C.byref(units), C.byref(ms)))
+        out[name] = {"launches": int(launches.value), "units": int(units.value),
+                     "ms": float(ms.value)}
+    return out
+
+
+class World:
+    """Many systems advanced together by shared multi-system launches.
+
+    The reference runs one ``System`` per simulation job
+    (simulation_worker/src/lib.rs). A world steps every attached system by
+    whole time steps in one sequence of launches; each system's results are
+    identical to stepping it alone. Attached systems are stepped only through
+    the world (their own ``step_forward_in_time`` raises until detached).
+    """
+
+    def __init__(self, systems=()):
+        N.lib()
+        h = C.c_void_p()
+        N.check(N.lib().sph_world_create(C.byref(h)))
+        self.h = h
+        self.members = []
+        for s in systems:
+            self.attach(s)
+
+    def attach(self, sim: "System"):
+        N.check(N.lib().sph_world_attach(self.h, sim.handle))
+        self.members.append(sim)
+
+    def detach(self, sim: "System"):
+        N.check(N.lib().sph_world_detach(self.h, sim.handle))
+        self.members.remove(sim)
+
+    def step(self, n: int = 1):
+        N.check(N.lib().sph_world_step(self.h, int(n)))
+
+    def size(self):
+        ns, npart = C.c_size_t(), C.c_size_t()
+        N.check(N.lib().sph_world_size(self.h, C.byref(ns), C.byref(npart)))
+        return int(ns.value), int(npart.value)
+
+    def profile(self, enable: bool = True):
+        N.check(N.lib().sph_world_profile(self.h, int(bool(enable))))
+
+    def profile_read(self) -> dict:
+        return _read_profile(N.lib().sph_world_profile_read, self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().sph_world_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_memory():
+    """(free, total) bytes of device memory."""
+    f, t = C.c_size_t(), C.c_size_t()
+    N.check(N.lib().sph_device_memory(C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)

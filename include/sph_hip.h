@@ -206,6 +206,37 @@ int sph_pressure_solve(sph_ctx *ctx, size_t n, const double *x, const double *v,
                        double *p_out, double *body_ft_out, double *xpred_out,
                        double *vpred_out, sph_measurement *info_out);
 
+/* ---- worlds: many systems advanced in shared launches ------------------------
+ * The reference runs one System per simulation worker
+ * (simulation_worker/src/lib.rs: one Box<dyn SPHSystem> per job). A world
+ * advances every attached context by whole time steps in one sequence of
+ * multi-system launches: each launch covers all systems' particles, and every
+ * per-system result (state, step size, iteration counts, error history,
+ * measurements) is bit-identical to stepping that context alone with
+ * sph_step. Each system keeps its own time step size and pressure-solve
+ * termination; the host waits once per Jacobi iteration for all systems.
+ * All systems of a world use one pressure solver (params.solver).
+ * An attached context is stepped only through its world; sph_measure,
+ * sph_fluid_download, sph_error_history and sph_body_* still read it, and
+ * sph_step / the per-trait calls fail with SPH_ERR_STATE until it is detached.
+ * A world and its contexts are used by one thread at a time. */
+typedef struct sph_world sph_world;
+int sph_world_create(sph_world **out);
+/* detaches every context (they stay valid) and frees the world */
+int sph_world_destroy(sph_world *world);
+/* ctx must have run sph_system_init */
+int sph_world_attach(sph_world *world, sph_ctx *ctx);
+int sph_world_detach(sph_world *world, sph_ctx *ctx);
+/* SPHSystem::step_forward_in_time for every attached system, nsteps times */
+int sph_world_step(sph_world *world, int nsteps);
+int sph_world_size(const sph_world *world, size_t *systems, size_t *particles);
+/* kernel timing of the world's launches (as sph_profile / sph_profile_read) */
+int sph_world_profile(sph_world *world, int enable);
+int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
+                           uint64_t *units, double *ms);
+/* free and total device memory (hipMemGetInfo), for admission control */
+int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

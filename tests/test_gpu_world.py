@@ -1,0 +1,112 @@
+"""Worlds: many systems advanced in shared multi-system launches.
+
+Every per-particle expression is evaluated in the same order whether a system
+runs alone or inside a world, and every per-system reduction folds only that
+system's workgroups in a fixed shape, so a system stepped inside a world must
+be bit-identical to the same system stepped alone: state, step size,
+iteration counts, error history, measurements and rigid-body state.
+"""
+import numpy as np
+import pytest
+
+from sph_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def native():
+    from sph_amd import system
+    return system
+
+
+def assert_same(a, b):
+    sa, sb = a.state(), b.state()
+    for k in ("x", "v", "a", "V", "p", "m", "fluid_id"):
+        assert np.array_equal(sa[k], sb[k]), k
+    ma, mb = a.take_measurement(), b.take_measurement()
+    assert ma == mb
+    assert np.array_equal(a.error_history(), b.error_history())
+    for bi in range(len(a.scene.boundaries)):
+        assert np.array_equal(a.body_forces(bi), b.body_forces(bi)), bi
+        if a.scene.boundaries[bi].rigid is not None:
+            ba, bb = a.body_state(bi), b.body_state(bi)
+            for k in ba:
+                assert np.array_equal(ba[k], bb[k]), (bi, k)
+
+
+def run_world_vs_solo(native, scs, steps):
+    solo = [native.System(sc) for sc in scs]
+    mates = [native.System(sc) for sc in scs]
+    w = native.World(mates)
+    assert w.size() == (len(scs), sum(len(s) for s in mates))
+    for _ in range(steps):
+        for s in solo:
+            s.step_forward_in_time(1)
+        w.step(1)
+        for a, b in zip(solo, mates):
+            assert_same(a, b)
+    return solo, mates, w
+
+
+def test_world_iisph_mixed_sizes_and_bodies(native):
+    scs = [scenes.small_iisph_box(side=10, seed=1, jitter=2e-3),
+           scenes.small_rigid_scene(),
+           scenes.small_iisph_box(side=13, seed=4, dx=0.025, box=0.7, jitter=2e-3),
+           scenes.small_iisph_box(side=7, seed=5, jitter=2e-3)]
+    scs[2].params.max_time_increment = 4e-4
+    scs[3].params.target_density_error = 0.05
+    solo, mates, _ = run_world_vs_solo(native, scs, 8)
+    # the systems run with different step sizes and iteration counts
+    dts = {s.take_measurement()["time_step_size"] for s in solo}
+    assert len(dts) > 1
+
+
+def test_world_iisphwost(native):
+    scs = [scenes.small_iisph_box(side=9, seed=2, solver="IISPHwOST", jitter=2e-3),
+           scenes.small_iisph_box(side=12, seed=3, solver="IISPHwOST", jitter=2e-3)]
+    run_world_vs_solo(native, scs, 6)
+
+
+def test_world_sesph_static_samples(native):
+    scs = [scenes.dam_break_sesph(side=8, seed=0), scenes.dam_break_sesph(side=11, seed=1)]
+    run_world_vs_solo(native, scs, 6)
+
+
+def test_world_compaction(native):
+    a = scenes.small_iisph_box(side=8, seed=1, jitter=2e-3)
+    a.params.disable_particles_below = float(np.sort(a.x[:, 2])[40]) - 1e-6
+    b = scenes.small_iisph_box(side=9, seed=2, jitter=2e-3)
+    solo, mates, _ = run_world_vs_solo(native, [a, b], 6)
+    assert len(mates[0]) < a.n and len(mates[1]) == b.n
+
+
+def test_world_attach_detach(native):
+    sa = scenes.small_iisph_box(side=9, seed=1, jitter=2e-3)
+    sb = scenes.small_iisph_box(side=10, seed=2, jitter=2e-3)
+    ref = native.System(sa)
+    a, b = native.System(sa), native.System(sb)
+    w = native.World([a, b])
+    with pytest.raises(native.N.SphError):
+        a.step_forward_in_time(1)  # attached systems step through their world
+    w.step(2)
+    ref.step_forward_in_time(2)
+    w.detach(a)
+    assert w.size()[0] == 1
+    a.step_forward_in_time(2)  # continues alone
+    ref.step_forward_in_time(2)
+    assert_same(ref, a)
+    w.step(1)
+    c = native.System(scenes.small_iisph_box(side=6, seed=3, jitter=2e-3))
+    w.attach(c)  # joins mid-run
+    w.step(2)
+    assert w.size() == (2, len(b) + len(c))
+
+
+def test_world_rejects_mixed_solvers(native):
+    a = native.System(scenes.small_iisph_box(side=6, seed=1, jitter=2e-3))
+    b = native.System(scenes.small_iisph_box(side=6, seed=1, solver="IISPHwOST", jitter=2e-3))
+    w = native.World([a])
+    with pytest.raises(native.N.SphError):
+        w.attach(b)
+    w.step(1)
