@@ -16,6 +16,12 @@ The JSON line also carries
                 stream, over the timed steps), against 8 TB/s;
   cpu_baseline  the CPU oracle (binary64 restatement of the reference) timed
                 on this host over a bounded sample of the same workload.
+
+--workload serving runs the serving stream instead: seeded Poisson arrivals
+of IISPH scene requests of mixed sizes served by continuous batching (one
+world, HBM-budget admission) and, as the baseline, one request at a time with
+the plain per-scene path; it reports throughput and p50/p99 step latency of
+both.
 """
 import argparse
 import json
@@ -99,6 +105,59 @@ def cpu_baseline(scene, budget_s, threads):
     }
 
 
+def serving_bench(args):
+    """The serving stream, batched and sequential, on the same requests."""
+    from sph_amd import serving, system
+    t = time.perf_counter()
+    stream = serving.make_stream(n_requests=args.requests, rate=args.rate,
+                                 n_min=args.n_min, n_max=args.n_max, steps=args.serve_steps,
+                                 seed=0)
+    log(f"[bench] serving stream: {len(stream)} requests, "
+        f"{sum(r.scene.n for r in stream)} particles, built in {time.perf_counter() - t:.1f} s")
+    free, total = system.device_memory()
+    budget = int(args.budget_gb * 2**30) if args.budget_gb else int(0.8 * free)
+    # warm the library and the device (module loads, first allocations)
+    serving.BatchedServer(budget, max_slots=2).run(serving.make_stream(
+        n_requests=2, rate=1000.0, n_min=2000, n_max=3000, steps=3, seed=99))
+    log("[bench] batched run")
+    bat = serving.BatchedServer(budget, max_slots=args.slots).run(stream).summary()
+    log(f"[bench] batched: {bat['throughput_particle_steps_per_s']:.3e} particle-steps/s, "
+        f"makespan {bat['makespan_s']:.1f} s")
+    seq = None
+    if not args.no_sequential:
+        log("[bench] sequential run")
+        seq = serving.SequentialServer().run(stream).summary()
+        log(f"[bench] sequential: {seq['throughput_particle_steps_per_s']:.3e} "
+            f"particle-steps/s, makespan {seq['makespan_s']:.1f} s")
+    out = {
+        "metric": "serving stream throughput (particle-steps/s, continuous batching); "
+                  "p50/p99 step latency",
+        "value": bat["throughput_particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded serving stream)",
+        "config": {"workload": "serving_stream", "requests": args.requests,
+                   "arrival_rate_per_s": args.rate, "particles": [args.n_min, args.n_max],
+                   "steps_per_request": args.serve_steps, "solver": "IISPH",
+                   "hbm_budget_bytes": budget, "max_slots": args.slots,
+                   "device_memory_bytes": total},
+        "p50_step_ms": bat["p50_step_ms"],
+        "p99_step_ms": bat["p99_step_ms"],
+        "batched": bat,
+        "sequential": seq,
+        "speedup_vs_sequential": (bat["throughput_particle_steps_per_s"] /
+                                  seq["throughput_particle_steps_per_s"]) if seq else None,
+        "busy_speedup_vs_sequential": (bat["busy_throughput_particle_steps_per_s"] /
+                                       seq["busy_throughput_particle_steps_per_s"])
+        if seq else None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +169,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
+    ap.add_argument("--requests", type=int, default=256, help="serving: requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: arrivals per second")
+    ap.add_argument("--n-min", type=int, default=4096, help="serving: smallest scene")
+    ap.add_argument("--n-max", type=int, default=262144, help="serving: largest scene")
+    ap.add_argument("--serve-steps", type=int, default=200, help="serving: steps per request")
+    ap.add_argument("--slots", type=int, default=64, help="serving: resident scenes at most")
+    ap.add_argument("--budget-gb", type=float, default=0.0,
+                    help="serving: HBM budget (default 80%% of free device memory)")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="serving: skip the sequential baseline")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -117,6 +186,10 @@ def main():
                           "multi-GPU decomposition is described in DESIGN.md and not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload == "serving":
+        serving_bench(args)
+        return
 
     from sph_amd import system
     scene = build_scene(args.workload, args.side)

@@ -9,6 +9,13 @@
 
 namespace sph {
 
+// launch bounds of the pair passes: at least WAVES waves per SIMD, with
+// 256-thread workgroups; WAVES = 0 keeps the compiler's default bounds
+#define PAIR_PASS_BOUNDS                                                       \
+    __attribute__((amdgpu_flat_work_group_size(WAVES > 0 ? kBlock : 1,         \
+                                               WAVES > 0 ? kBlock : 1024),     \
+                   amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1)))
+
 // scene of workgroup lb: the largest s with blk[s] <= lb (block-uniform)
 __device__ __forceinline__ int scene_of(const uint32_t *blk, int ns, uint32_t lb) {
     int lo = 0, hi = ns;
@@ -33,6 +40,44 @@ __device__ __forceinline__ int scene_of(const uint32_t *blk, int ns, uint32_t lb
 // pair-cache slot of particle i's first neighbor
 #define ELL_BASE(B, v, i) ((size_t)(B).slice_off[(v).slice_base + ((i) >> 6)] * 64 + ((i)&63))
 
+// The field pointers of the scene and body tables are read from device
+// memory, so the compiler cannot tell their address space and would emit FLAT
+// accesses; gm() marks them as global memory (GLOBAL loads and stores, SGPR
+// base addressing, no LDS-counter waits).
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GLOBAL_AS T *gm(T *p) {
+    return (GLOBAL_AS T *)p;
+}
+
+// packed per-particle records gathered by the neighbor loops (one 32-byte
+// record instead of separate 8-byte loads): q4[i] = {x, y, z, V},
+// pa4[i] = {pressure acceleration x, y, z, 0}
+typedef double rec4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ rec4 ld4(const double *a, size_t i) {
+    return ((const GLOBAL_AS rec4 *)a)[i];
+}
+__device__ __forceinline__ void st4(double *a, size_t i, double x, double y, double z,
+                                    double w) {
+    rec4 r = {x, y, z, w};
+    ((GLOBAL_AS rec4 *)a)[i] = r;
+}
+
+// grad W of pair slot e: read from the pair cache (CACHED) or recomputed from
+// r = x_i - x_j with the same expression (bit-identical either way)
+template <bool CACHED>
+__device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &K, size_t e,
+                                          double rx, double ry, double rz, double &wx,
+                                          double &wy, double &wz) {
+    if (CACHED) {
+        wx = gm(B.ell_gx)[e];
+        wy = gm(B.ell_gy)[e];
+        wz = gm(B.ell_gz)[e];
+    } else {
+        kernel_grad(K, rx, ry, rz, wx, wy, wz);
+    }
+}
+
 // ------------------------------------------------------------------------
 // integration (integration_schemes/euler_cromer.rs:12-25)
 // ------------------------------------------------------------------------
@@ -40,14 +85,14 @@ __global__ void kb_integrate_ec(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live || v.integrator != SPH_INTEGRATOR_EULER_CROMER) return;
     const double dt = v.S.dt;
-    double nvx = v.vx[i] + dt * v.ax[i], nvy = v.vy[i] + dt * v.ay[i],
-           nvz = v.vz[i] + dt * v.az[i];
-    v.vx[i] = nvx;
-    v.vy[i] = nvy;
-    v.vz[i] = nvz;
-    v.x[i] = v.x[i] + dt * nvx;
-    v.y[i] = v.y[i] + dt * nvy;
-    v.z[i] = v.z[i] + dt * nvz;
+    double nvx = gm(v.vx)[i] + dt * gm(v.ax)[i], nvy = gm(v.vy)[i] + dt * gm(v.ay)[i],
+           nvz = gm(v.vz)[i] + dt * gm(v.az)[i];
+    gm(v.vx)[i] = nvx;
+    gm(v.vy)[i] = nvy;
+    gm(v.vz)[i] = nvz;
+    gm(v.x)[i] = gm(v.x)[i] + dt * nvx;
+    gm(v.y)[i] = gm(v.y)[i] + dt * nvy;
+    gm(v.z)[i] = gm(v.z)[i] + dt * nvz;
 }
 
 // ------------------------------------------------------------------------
@@ -59,8 +104,8 @@ __global__ void kb_speed_below(BatchDev B) {
     PARTICLE_THREAD(B);
     double sp = 0.0, b = 0.0;
     if (live) {
-        sp = sqrt(dot3(v.vx[i], v.vy[i], v.vz[i], v.vx[i], v.vy[i], v.vz[i]));
-        b = v.z[i] < v.disable_below ? 1.0 : 0.0;
+        sp = sqrt(dot3(gm(v.vx)[i], gm(v.vy)[i], gm(v.vz)[i], gm(v.vx)[i], gm(v.vy)[i], gm(v.vz)[i]));
+        b = gm(v.z)[i] < v.disable_below ? 1.0 : 0.0;
     }
     for (int off = 32; off > 0; off >>= 1) {
         sp = fmax(sp, __shfl_xor(sp, off));
@@ -77,8 +122,8 @@ __global__ void kb_speed_below(BatchDev B) {
             sp = fmax(sp, lds[0][q]);
             b += lds[1][q];
         }
-        B.partials[(size_t)lb * 2] = sp;
-        B.partials[(size_t)lb * 2 + 1] = b;
+        gm(B.partials)[(size_t)lb * 2] = sp;
+        gm(B.partials)[(size_t)lb * 2 + 1] = b;
     }
 }
 
@@ -91,11 +136,11 @@ __global__ void kb_cells(BatchDev B) {
     int c[6] = {2147483647, 2147483647, 2147483647, -2147483647 - 1, -2147483647 - 1,
                 -2147483647 - 1};
     if (live) {
-        int a = sat_i32(floor(v.x[i] / v.cs)), b = sat_i32(floor(v.y[i] / v.cs)),
-            d = sat_i32(floor(v.z[i] / v.cs));
-        B.ci[3 * gi] = a;
-        B.ci[3 * gi + 1] = b;
-        B.ci[3 * gi + 2] = d;
+        int a = sat_i32(floor(gm(v.x)[i] / v.cs)), b = sat_i32(floor(gm(v.y)[i] / v.cs)),
+            d = sat_i32(floor(gm(v.z)[i] / v.cs));
+        gm(B.ci)[3 * gi] = a;
+        gm(B.ci)[3 * gi + 1] = b;
+        gm(B.ci)[3 * gi + 2] = d;
         c[0] = c[3] = a;
         c[1] = c[4] = b;
         c[2] = c[5] = d;
@@ -132,16 +177,16 @@ __device__ __forceinline__ uint32_t local_key(const GridDev &g, int cx, int cy, 
 __global__ void kb_cell_count(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
-    uint32_t k = v.cell_base + local_key(v.grid, B.ci[3 * gi], B.ci[3 * gi + 1], B.ci[3 * gi + 2]);
+    uint32_t k = v.cell_base + local_key(v.grid, gm(B.ci)[3 * gi], gm(B.ci)[3 * gi + 1], gm(B.ci)[3 * gi + 2]);
     atomicAdd(&B.cell_cnt[k], 1u);
 }
 
 __global__ void kb_cell_scatter(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
-    uint32_t k = v.cell_base + local_key(v.grid, B.ci[3 * gi], B.ci[3 * gi + 1], B.ci[3 * gi + 2]);
-    uint32_t slot = B.cell_start[k] + atomicAdd(&B.cursor[k], 1u);  // batch position
-    B.perm[slot] = (uint32_t)i;                                      // scene-local source
+    uint32_t k = v.cell_base + local_key(v.grid, gm(B.ci)[3 * gi], gm(B.ci)[3 * gi + 1], gm(B.ci)[3 * gi + 2]);
+    uint32_t slot = gm(B.cell_start)[k] + atomicAdd(&B.cursor[k], 1u);  // batch position
+    gm(B.perm)[slot] = (uint32_t)i;                                      // scene-local source
 }
 
 // members of each cell ordered by reference index (the reference's bucket order)
@@ -149,19 +194,19 @@ __global__ void kb_cell_order(BatchDev B) {
     const unsigned lb = blockIdx.x;
     const int sid = scene_of(B.cblk, B.ns, lb);
     const SceneView &v = B.views[sid];
-    const size_t c = (size_t)(lb - B.cblk[sid]) * kBlock + threadIdx.x;
+    const size_t c = (size_t)(lb - gm(B.cblk)[sid]) * kBlock + threadIdx.x;
     const size_t ncell = (size_t)v.grid.dim[0] * v.grid.dim[1] * v.grid.dim[2];
     if (c >= ncell) return;
     const size_t gc = v.cell_base + c;
-    uint32_t lo = B.cell_start[gc], hi = B.cell_start[gc + 1];
+    uint32_t lo = gm(B.cell_start)[gc], hi = gm(B.cell_start)[gc + 1];
     uint32_t *perm = B.perm;
     for (uint32_t a = lo + 1; a < hi; a++) {
         uint32_t val = perm[a];
-        uint32_t key = v.ref[val];
+        uint32_t key = gm(v.ref)[val];
         uint32_t b = a;
         while (b > lo) {
             uint32_t u = perm[b - 1];
-            if (v.ref[u] <= key) break;
+            if (gm(v.ref)[u] <= key) break;
             perm[b] = u;
             b--;
         }
@@ -173,16 +218,18 @@ __global__ void kb_cell_order(BatchDev B) {
 __global__ void kb_permute(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
-    const uint32_t src = B.perm[gi];
-    v.tx[i] = v.x[src];
-    v.ty[i] = v.y[src];
-    v.tz[i] = v.z[src];
-    v.tvx[i] = v.vx[src];
-    v.tvy[i] = v.vy[src];
-    v.tvz[i] = v.vz[src];
-    v.tm[i] = v.m[src];
-    v.tfid[i] = v.fid[src];
-    v.tref[i] = v.ref[src];
+    const uint32_t src = gm(B.perm)[gi];
+    const double x = gm(v.x)[src], y = gm(v.y)[src], z = gm(v.z)[src];
+    gm(v.tx)[i] = x;
+    gm(v.ty)[i] = y;
+    gm(v.tz)[i] = z;
+    st4(v.q4, i, x, y, z, 0.0);
+    gm(v.tvx)[i] = gm(v.vx)[src];
+    gm(v.tvy)[i] = gm(v.vy)[src];
+    gm(v.tvz)[i] = gm(v.vz)[src];
+    gm(v.tm)[i] = gm(v.m)[src];
+    gm(v.tfid)[i] = gm(v.fid)[src];
+    gm(v.tref)[i] = gm(v.ref)[src];
 }
 
 // ------------------------------------------------------------------------
@@ -194,7 +241,8 @@ __global__ void kb_count_ell(BatchDev B) {
     PARTICLE_THREAD(B);
     uint32_t cnt = 0;
     if (live) {
-        const double px = v.x[i], py = v.y[i], pz = v.z[i];
+        const rec4 qi = ld4(v.q4, i);
+        const double px = qi.x, py = qi.y, pz = qi.z;
         const GridDev &g = v.grid;
         const int cx = sat_i32(floor(px / v.cs)), cy = sat_i32(floor(py / v.cs)),
                   cz = sat_i32(floor(pz / v.cs));
@@ -209,27 +257,29 @@ __global__ void kb_count_ell(BatchDev B) {
                     int gy = cy + dy - g.org[1];
                     if (gy < 0 || gy >= g.dim[1]) continue;
                     uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
-                    uint32_t a = g.start[row + zlo] - v.part_base,
-                             b = g.start[row + zhi + 1] - v.part_base;
+                    uint32_t a = gm(g.start)[row + zlo] - v.part_base,
+                             b = gm(g.start)[row + zhi + 1] - v.part_base;
                     for (uint32_t j = a; j < b; j++) {
-                        double ddx = px - v.x[j], ddy = py - v.y[j], ddz = pz - v.z[j];
+                        const rec4 qj = ld4(v.q4, j);
+                        double ddx = px - qj.x, ddy = py - qj.y, ddz = pz - qj.z;
                         double d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
                         cnt += d2 < v.dthr ? 1u : 0u;
                     }
                 }
             }
         }
-        B.cnt[gi] = cnt;
+        gm(B.cnt)[gi] = cnt;
     }
     uint32_t mx = cnt;
     for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-    if ((threadIdx.x & 63) == 0 && live) B.slice_len[v.slice_base + (i >> 6)] = mx;
+    if ((threadIdx.x & 63) == 0 && live) gm(B.slice_len)[v.slice_base + (i >> 6)] = mx;
 }
 
 __global__ void kb_fill_ell(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
-    const double px = v.x[i], py = v.y[i], pz = v.z[i];
+    const rec4 qi = ld4(v.q4, i);
+    const double px = qi.x, py = qi.y, pz = qi.z;
     const GridDev &g = v.grid;
     const int cx = sat_i32(floor(px / v.cs)), cy = sat_i32(floor(py / v.cs)),
               cz = sat_i32(floor(pz / v.cs));
@@ -245,12 +295,13 @@ __global__ void kb_fill_ell(BatchDev B) {
             int gy = cy + dy - g.org[1];
             if (gy < 0 || gy >= g.dim[1]) continue;
             uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
-            uint32_t a = g.start[row + zlo] - v.part_base, b = g.start[row + zhi + 1] - v.part_base;
+            uint32_t a = gm(g.start)[row + zlo] - v.part_base, b = gm(g.start)[row + zhi + 1] - v.part_base;
             for (uint32_t j = a; j < b; j++) {
-                double rx = px - v.x[j], ry = py - v.y[j], rz = pz - v.z[j];
+                const rec4 qj = ld4(v.q4, j);
+                double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
                 double d2 = (rx * rx + ry * ry) + rz * rz;
                 if (d2 < v.dthr) {
-                    B.ell_j[e] = j;
+                    gm(B.ell_j)[e] = j;
                     e += 64;
                 }
             }
@@ -265,7 +316,7 @@ __global__ void kb_fill_ell(BatchDev B) {
 __global__ void kb_vm_sample(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
-    const double px = v.x[i], py = v.y[i], pz = v.z[i];
+    const double px = gm(v.x)[i], py = gm(v.y)[i], pz = gm(v.z)[i];
     const double range = v.K.h;
     for (uint32_t bi = 0; bi < v.nb; bi++) {
         const BodyDev &b = B.bodies[v.body_base + bi];
@@ -280,17 +331,17 @@ __global__ void kb_vm_sample(BatchDev B) {
                 double ox = px - (gx / gn) * d, oy = py - (gy / gn) * d, oz = pz - (gz / gn) * d;
                 double vx, vy, vz;
                 body_velocity(b, ox, oy, oz, vx, vy, vz);
-                const_cast<double *>(b.bx)[i] = ox;
-                const_cast<double *>(b.by)[i] = oy;
-                const_cast<double *>(b.bz)[i] = oz;
-                const_cast<double *>(b.bvx)[i] = vx;
-                const_cast<double *>(b.bvy)[i] = vy;
-                const_cast<double *>(b.bvz)[i] = vz;
-                const_cast<double *>(b.bvol)[i] = vol;
+                gm(const_cast<double *>(b.bx))[i] = ox;
+                gm(const_cast<double *>(b.by))[i] = oy;
+                gm(const_cast<double *>(b.bz))[i] = oz;
+                gm(const_cast<double *>(b.bvx))[i] = vx;
+                gm(const_cast<double *>(b.bvy))[i] = vy;
+                gm(const_cast<double *>(b.bvz))[i] = vz;
+                gm(const_cast<double *>(b.bvol))[i] = vol;
                 ok = 1;
             }
         }
-        const_cast<uint8_t *>(b.flag)[i] = ok;
+        gm(const_cast<uint8_t *>(b.flag))[i] = ok;
     }
 }
 
@@ -298,51 +349,63 @@ __global__ void kb_vm_sample(BatchDev B) {
 // grad W per cached pair (lockstep, coalesced stores) and get_volume
 // (quantities.rs:14-61)
 // ------------------------------------------------------------------------
-__global__ void kb_pairs_volume(BatchDev B) {
+template <bool CACHED, int UNROLL, int WAVES>
+__global__ PAIR_PASS_BOUNDS void kb_pairs_volume(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
     const KernelConst K = v.K;
-    const double px = v.x[i], py = v.y[i], pz = v.z[i];
-    const uint32_t c = B.cnt[gi];
+    const rec4 qi = ld4(v.q4, i);
+    const double px = qi.x, py = qi.y, pz = qi.z;
+    const uint32_t c = gm(B.cnt)[gi];
     size_t e = ELL_BASE(B, v, i);
     double accu = 0.;
-#pragma unroll 4
+#pragma unroll UNROLL
     for (uint32_t k = 0; k < c; k++, e += 64) {
-        uint32_t j = B.ell_j[e];
-        double rx = px - v.x[j], ry = py - v.y[j], rz = pz - v.z[j];
+        uint32_t j = gm(B.ell_j)[e];
+        const rec4 qj = ld4(v.q4, j);  // .w is being written by this pass; unused here
+        double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
         double rn = sqrt((rx * rx + ry * ry) + rz * rz);
         double q = rn / K.h;
         accu += v.V0 * (K.wfac * kernel_w(q));
-        double wx = 0.0, wy = 0.0, wz = 0.0;
-        if (rn != 0.0) {
-            double d = kernel_dqw(q);
-            wx = ((K.gfac * rx) / rn) * d;
-            wy = ((K.gfac * ry) / rn) * d;
-            wz = ((K.gfac * rz) / rn) * d;
+        if (CACHED) {  // the expression of kernel_grad
+            const double d = kernel_dqw(q);
+            const double rs = rn != 0.0 ? rn : 1.0;
+            const double wx = ((K.gfac * rx) / rs) * d;
+            const double wy = ((K.gfac * ry) / rs) * d;
+            const double wz = ((K.gfac * rz) / rs) * d;
+            gm(B.ell_gx)[e] = wx;
+            gm(B.ell_gy)[e] = wy;
+            gm(B.ell_gz)[e] = wz;
         }
-        B.ell_gx[e] = wx;
-        B.ell_gy[e] = wy;
-        B.ell_gz[e] = wz;
     }
     for (uint32_t bi = 0; bi < v.nb; bi++) {
         const BodyDev &b = B.bodies[v.body_base + bi];
         if (b.kind == 0) {
-            for (uint32_t k = b.soff[i]; k < b.soff[i + 1]; k++) {
-                uint32_t s = b.sidx[k];
-                accu += b.svol[s] * kernel_fn(K, px - b.sx[s], py - b.sy[s], pz - b.sz[s]);
+            for (uint32_t k = gm(b.soff)[i]; k < gm(b.soff)[i + 1]; k++) {
+                uint32_t s = gm(b.sidx)[k];
+                accu += gm(b.svol)[s] * kernel_fn(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s]);
             }
-        } else if (b.flag[i]) {
-            accu += b.bvol[i] * kernel_fn(K, px - b.bx[i], py - b.by[i], pz - b.bz[i]);
+        } else if (gm(b.flag)[i]) {
+            accu += gm(b.bvol)[i] * kernel_fn(K, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i]);
         }
     }
-    v.V[i] = v.V0 / accu;
+    const double Vi = v.V0 / accu;
+    gm(v.V)[i] = Vi;
+    gm(v.q4)[4 * i + 3] = Vi;
+}
+
+// q4[i].w = V[i] after V was set from outside the volume pass
+__global__ void kb_pack_volume(BatchDev B) {
+    PARTICLE_THREAD(B);
+    if (live) gm(v.q4)[4 * i + 3] = gm(v.V)[i];
 }
 
 // ------------------------------------------------------------------------
 // reset + gravity + artificial viscosity (non_pressure_accelerations.rs:19-185);
 // reactions onto dynamic bodies summed per workgroup in FP64
 // ------------------------------------------------------------------------
-__global__ void kb_nonpressure(BatchDev B) {
+template <bool CACHED, int UNROLL, int WAVES>
+__global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
     __shared__ double lds[6 * kBlock];
     PARTICLE_THREAD(B);
     const KernelConst K = v.K;
@@ -350,32 +413,36 @@ __global__ void kb_nonpressure(BatchDev B) {
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double acx = 0., acy = 0., acz = 0.;
     if (live) {
-        px = v.x[i]; py = v.y[i]; pz = v.z[i];
-        ux = v.vx[i]; uy = v.vy[i]; uz = v.vz[i];
-        mi = v.m[i];
+        const rec4 qi = ld4(v.q4, i);
+        px = qi.x; py = qi.y; pz = qi.z;
+        ux = gm(v.vx)[i]; uy = gm(v.vy)[i]; uz = gm(v.vz)[i];
+        mi = gm(v.m)[i];
         if (v.gravity_mode == 0) {
             gx = 0.0 + 0.0;
             gy = 0.0 + 0.0;
             gz = 0.0 + -9.81;
         } else {
-            double dx = v.gcenter[0] - px, dy = v.gcenter[1] - py, dz = v.gcenter[2] - pz;
+            double dx = gm(v.gcenter)[0] - px, dy = gm(v.gcenter)[1] - py, dz = gm(v.gcenter)[2] - pz;
             double dist = sqrt(dot3(dx, dy, dz, dx, dy, dz) + 1e-4);
             gx = 0.0 + (9.81 * dx) / dist;
             gy = 0.0 + (9.81 * dy) / dist;
             gz = 0.0 + (9.81 * dz) / dist;
         }
-        const uint32_t c = B.cnt[gi];
+        const uint32_t c = gm(B.cnt)[gi];
         size_t e = ELL_BASE(B, v, i);
-#pragma unroll 4
+#pragma unroll UNROLL
         for (uint32_t k = 0; k < c; k++, e += 64) {
-            uint32_t j = B.ell_j[e];
-            double rx = px - v.x[j], ry = py - v.y[j], rz = pz - v.z[j];
-            double cf = v.nu_f * 2. * (3. + 2.) * v.V[j] *
-                        dot3(ux - v.vx[j], uy - v.vy[j], uz - v.vz[j], rx, ry, rz) /
+            uint32_t j = gm(B.ell_j)[e];
+            const rec4 qj = ld4(v.q4, j);
+            double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
+            double cf = v.nu_f * 2. * (3. + 2.) * qj.w *
+                        dot3(ux - gm(v.vx)[j], uy - gm(v.vy)[j], uz - gm(v.vz)[j], rx, ry, rz) /
                         (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
-            acx += cf * B.ell_gx[e];
-            acy += cf * B.ell_gy[e];
-            acz += cf * B.ell_gz[e];
+            double wx, wy, wz;
+            pair_grad<CACHED>(B, K, e, rx, ry, rz, wx, wy, wz);
+            acx += cf * wx;
+            acy += cf * wy;
+            acz += cf * wz;
         }
     }
     for (uint32_t bi = 0; bi < v.nb; bi++) {
@@ -383,23 +450,23 @@ __global__ void kb_nonpressure(BatchDev B) {
         double f[6] = {0., 0., 0., 0., 0., 0.};
         if (live) {
             if (b.kind == 0) {
-                for (uint32_t e = b.soff[i]; e < b.soff[i + 1]; e++) {
-                    uint32_t s = b.sidx[e];
-                    double rx = px - b.sx[s], ry = py - b.sy[s], rz = pz - b.sz[s];
+                for (uint32_t e = gm(b.soff)[i]; e < gm(b.soff)[i + 1]; e++) {
+                    uint32_t s = gm(b.sidx)[e];
+                    double rx = px - gm(b.sx)[s], ry = py - gm(b.sy)[s], rz = pz - gm(b.sz)[s];
                     double wx, wy, wz;
                     kernel_grad(K, rx, ry, rz, wx, wy, wz);
                     // static samples have zero velocity
-                    double cf = v.nu_b * 2. * (3. + 2.) * b.svol[s] *
+                    double cf = v.nu_b * 2. * (3. + 2.) * gm(b.svol)[s] *
                                 dot3(ux - 0.0, uy - 0.0, uz - 0.0, rx, ry, rz) /
                                 (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
                     acx += cf * wx;
                     acy += cf * wy;
                     acz += cf * wz;
                 }
-            } else if (b.flag[i]) {
+            } else if (gm(b.flag)[i]) {
                 double vp[4][3];
-                visc_samples(px, py, pz, b.bx[i], b.by[i], b.bz[i], K.h, vp);
-                double vv = 0.25 * b.bvol[i];
+                visc_samples(px, py, pz, gm(b.bx)[i], gm(b.by)[i], gm(b.bz)[i], K.h, vp);
+                double vv = 0.25 * gm(b.bvol)[i];
                 for (int k = 0; k < 4; k++) {
                     double sxk = vp[k][0], syk = vp[k][1], szk = vp[k][2];
                     double svx, svy, svz;
@@ -414,7 +481,7 @@ __global__ void kb_nonpressure(BatchDev B) {
                     if (b.dynamic) {
                         double fx = -(mi * a0), fy = -(mi * a1), fz = -(mi * a2);
                         double tx, ty, tz;
-                        cross3(sxk - b.com[0], syk - b.com[1], szk - b.com[2], fx, fy, fz, tx,
+                        cross3(sxk - gm(b.com)[0], syk - gm(b.com)[1], szk - gm(b.com)[2], fx, fy, fz, tx,
                                ty, tz);
                         f[0] += fx; f[1] += fy; f[2] += fz;
                         f[3] += tx; f[4] += ty; f[5] += tz;
@@ -434,16 +501,16 @@ __global__ void kb_nonpressure(BatchDev B) {
         }
     }
     if (live) {
-        v.ax[i] = gx + acx;
-        v.ay[i] = gy + acy;
-        v.az[i] = gz + acz;
+        gm(v.ax)[i] = gx + acx;
+        gm(v.ay)[i] = gy + acy;
+        gm(v.az)[i] = gz + acz;
     }
 }
 
 // SESPH state equation (sesph.rs:40-51)
 __global__ void kb_sesph_pressure(BatchDev B) {
     PARTICLE_THREAD(B);
-    if (live) v.p[i] = v.stiffness * fmax(v.V0 / v.V[i] - 1., 0.);
+    if (live) gm(v.p)[i] = v.stiffness * fmax(v.V0 / gm(v.V)[i] - 1., 0.);
 }
 
 // ------------------------------------------------------------------------
@@ -451,7 +518,8 @@ __global__ void kb_sesph_pressure(BatchDev B) {
 // acceleration or the solver's scratch (IISPH pressure_acc_f). Scenes whose
 // solve has converged are skipped when the target is the scratch.
 // ------------------------------------------------------------------------
-__global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror) {
+template <bool CACHED, int UNROLL, int WAVES>
+__global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror) {
     __shared__ double lds[6 * kBlock];
     PARTICLE_THREAD(B);
     if (target_scratch && !v.active) return;  // uniform per workgroup
@@ -460,20 +528,36 @@ __global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite,
     double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0;
     double acx = 0., acy = 0., acz = 0.;
     if (live) {
-        Vi = v.V[i]; mi = v.m[i]; pi = v.p[i];
-        const uint32_t c = B.cnt[gi];
-        size_t e = ELL_BASE(B, v, i);
+        const rec4 qi = ld4(v.q4, i);
+        px = qi.x; py = qi.y; pz = qi.z; Vi = qi.w;
+        mi = gm(v.m)[i]; pi = gm(v.p)[i];
+        const uint32_t c = gm(B.cnt)[gi];
+        const size_t e0 = ELL_BASE(B, v, i);
         const double vm = Vi / mi;
-#pragma unroll 4
-        for (uint32_t k = 0; k < c; k++, e += 64) {
-            uint32_t j = B.ell_j[e];
-            double cf = vm * v.V[j] * (pi + v.p[j]);
-            acx -= cf * B.ell_gx[e];
-            acy -= cf * B.ell_gy[e];
-            acz -= cf * B.ell_gz[e];
-        }
-        if (v.nb) {
-            px = v.x[i]; py = v.y[i]; pz = v.z[i];
+        if (c > 0) {
+            // software pipeline: slot k+1's record and slot k+2's index are in
+            // flight while slot k is evaluated (prefetch slots clamp to c-1)
+            const GLOBAL_AS uint32_t *ej = gm(B.ell_j) + e0;
+            uint32_t j1 = ej[64 * (size_t)min(1u, c - 1)];
+            const uint32_t j0 = ej[0];
+            rec4 q0 = ld4(v.q4, j0);
+            double p0 = gm(v.p)[j0];
+#pragma unroll UNROLL
+            for (uint32_t k = 0; k < c; k++) {
+                const uint32_t j2 = ej[64 * (size_t)min(k + 2, c - 1)];
+                const rec4 q1 = ld4(v.q4, j1);
+                const double p1 = gm(v.p)[j1];
+                double cf = vm * q0.w * (pi + p0);
+                double wx, wy, wz;
+                pair_grad<CACHED>(B, K, e0 + 64 * (size_t)k, px - q0.x, py - q0.y, pz - q0.z,
+                                  wx, wy, wz);
+                acx -= cf * wx;
+                acy -= cf * wy;
+                acz -= cf * wz;
+                j1 = j2;
+                q0 = q1;
+                p0 = p1;
+            }
         }
     }
     for (uint32_t bi = 0; bi < v.nb; bi++) {
@@ -481,24 +565,24 @@ __global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite,
         double f[6] = {0., 0., 0., 0., 0., 0.};
         if (live) {
             if (b.kind == 0) {
-                for (uint32_t e = b.soff[i]; e < b.soff[i + 1]; e++) {
-                    uint32_t s = b.sidx[e];
+                for (uint32_t e = gm(b.soff)[i]; e < gm(b.soff)[i + 1]; e++) {
+                    uint32_t s = gm(b.sidx)[e];
                     double wx, wy, wz;
-                    kernel_grad(K, px - b.sx[s], py - b.sy[s], pz - b.sz[s], wx, wy, wz);
-                    double cf = 2. * gamma * Vi * b.svol[s] * pi;
+                    kernel_grad(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
+                    double cf = 2. * gamma * Vi * gm(b.svol)[s] * pi;
                     acx -= (cf * wx) / mi;
                     acy -= (cf * wy) / mi;
                     acz -= (cf * wz) / mi;
                 }
-            } else if (b.flag[i]) {
-                double sxk = b.bx[i], syk = b.by[i], szk = b.bz[i];
+            } else if (gm(b.flag)[i]) {
+                double sxk = gm(b.bx)[i], syk = gm(b.by)[i], szk = gm(b.bz)[i];
                 double wx, wy, wz;
                 kernel_grad(K, px - sxk, py - syk, pz - szk, wx, wy, wz);
-                double cf = 2. * gamma * Vi * b.bvol[i] * pi;
+                double cf = 2. * gamma * Vi * gm(b.bvol)[i] * pi;
                 double fx = cf * wx, fy = cf * wy, fz = cf * wz;
                 if (mirror && b.dynamic) {
                     double tx, ty, tz;
-                    cross3(sxk - b.com[0], syk - b.com[1], szk - b.com[2], fx, fy, fz, tx, ty,
+                    cross3(sxk - gm(b.com)[0], syk - gm(b.com)[1], szk - gm(b.com)[2], fx, fy, fz, tx, ty,
                            tz);
                     f[0] += fx; f[1] += fy; f[2] += fz;
                     f[3] += tx; f[4] += ty; f[5] += tz;
@@ -516,10 +600,10 @@ __global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite,
             }
         }
     }
-    if (live) {
-        double *tx_ = target_scratch ? v.pax : v.ax;
-        double *ty_ = target_scratch ? v.pay : v.ay;
-        double *tz_ = target_scratch ? v.paz : v.az;
+    if (live && target_scratch) {  // the solver's scratch is always overwritten
+        st4(v.pa4, i, acx, acy, acz, 0.0);
+    } else if (live) {
+        double *tx_ = v.ax, *ty_ = v.ay, *tz_ = v.az;
         if (overwrite) {
             tx_[i] = acx;
             ty_[i] = acy;
@@ -537,74 +621,79 @@ __global__ void kb_pred_vel(BatchDev B, int to_pred) {
     PARTICLE_THREAD(B);
     if (!live) return;
     const double dt = v.S.dt;
-    double bx = to_pred ? v.svx[i] : v.vx[i], by = to_pred ? v.svy[i] : v.vy[i],
-           bz = to_pred ? v.svz[i] : v.vz[i];
-    v.svx[i] = bx + dt * v.ax[i];
-    v.svy[i] = by + dt * v.ay[i];
-    v.svz[i] = bz + dt * v.az[i];
+    double bx = to_pred ? gm(v.svx)[i] : gm(v.vx)[i], by = to_pred ? gm(v.svy)[i] : gm(v.vy)[i],
+           bz = to_pred ? gm(v.svz)[i] : gm(v.vz)[i];
+    gm(v.svx)[i] = bx + dt * gm(v.ax)[i];
+    gm(v.svy)[i] = by + dt * gm(v.ay)[i];
+    gm(v.svz)[i] = bz + dt * gm(v.az)[i];
 }
 
 // source term (vp or vde), diagonal element and initial pressure in one pass
 // (iisph.rs:117-376, clamp_pressure = true)
-__global__ void kb_source_diag(BatchDev B, int vp) {
+template <bool CACHED, int UNROLL, int WAVES>
+__global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
     PARTICLE_THREAD(B);
     if (!live) return;
     const KernelConst K = v.K;
     const SolveConst S = v.S;
-    const double px = v.x[i], py = v.y[i], pz = v.z[i];
-    const double ux = v.svx[i], uy = v.svy[i], uz = v.svz[i];
-    const double Vi = v.V[i];
+    const rec4 qi = ld4(v.q4, i);
+    const double px = qi.x, py = qi.y, pz = qi.z;
+    const double ux = gm(v.svx)[i], uy = gm(v.svy)[i], uz = gm(v.svz)[i];
+    const double Vi = qi.w;
     const double dt = S.dt, dt2 = S.dt * S.dt;
     double src = vp ? 1. - S.V0 / Vi : 0.;
     double sfx = 0., sfy = 0., sfz = 0., sf2 = 0., sbx = 0., sby = 0., sbz = 0.;
-    const uint32_t c = B.cnt[gi];
+    const uint32_t c = gm(B.cnt)[gi];
     size_t e = ELL_BASE(B, v, i);
-#pragma unroll 4
+#pragma unroll UNROLL
     for (uint32_t k = 0; k < c; k++, e += 64) {
-        uint32_t j = B.ell_j[e];
-        double wx = B.ell_gx[e], wy = B.ell_gy[e], wz = B.ell_gz[e];
-        double Vj = v.V[j];
-        src -= dt * Vj * dot3(ux - v.svx[j], uy - v.svy[j], uz - v.svz[j], wx, wy, wz);
+        uint32_t j = gm(B.ell_j)[e];
+        const rec4 qj = ld4(v.q4, j);
+        double wx, wy, wz;
+        pair_grad<CACHED>(B, K, e, px - qj.x, py - qj.y, pz - qj.z, wx, wy, wz);
+        double Vj = qj.w;
+        src -= dt * Vj * dot3(ux - gm(v.svx)[j], uy - gm(v.svy)[j], uz - gm(v.svz)[j], wx, wy, wz);
         sfx += Vj * wx;
         sfy += Vj * wy;
         sfz += Vj * wz;
-        sf2 -= dt2 * Vi * powi2(Vj) / v.m[j] * dot3(wx, wy, wz, wx, wy, wz);
+        sf2 -= dt2 * Vi * powi2(Vj) / gm(v.m)[j] * dot3(wx, wy, wz, wx, wy, wz);
     }
     for (uint32_t bi = 0; bi < v.nb; bi++) {
         const BodyDev &b = B.bodies[v.body_base + bi];
         if (b.kind == 0) {
-            for (uint32_t k = b.soff[i]; k < b.soff[i + 1]; k++) {
-                uint32_t s = b.sidx[k];
+            for (uint32_t k = gm(b.soff)[i]; k < gm(b.soff)[i + 1]; k++) {
+                uint32_t s = gm(b.sidx)[k];
                 double wx, wy, wz;
-                kernel_grad(K, px - b.sx[s], py - b.sy[s], pz - b.sz[s], wx, wy, wz);
-                double Vb = b.svol[s];
+                kernel_grad(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
+                double Vb = gm(b.svol)[s];
                 src -= dt * Vb * dot3(ux - 0.0, uy - 0.0, uz - 0.0, wx, wy, wz);
                 sbx += Vb * wx;
                 sby += Vb * wy;
                 sbz += Vb * wz;
             }
-        } else if (b.flag[i]) {
+        } else if (gm(b.flag)[i]) {
             double wx, wy, wz;
-            kernel_grad(K, px - b.bx[i], py - b.by[i], pz - b.bz[i], wx, wy, wz);
-            double Vb = b.bvol[i];
-            src -= dt * Vb * dot3(ux - b.bvx[i], uy - b.bvy[i], uz - b.bvz[i], wx, wy, wz);
+            kernel_grad(K, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i], wx, wy, wz);
+            double Vb = gm(b.bvol)[i];
+            src -= dt * Vb * dot3(ux - gm(b.bvx)[i], uy - gm(b.bvy)[i], uz - gm(b.bvz)[i], wx, wy, wz);
             sbx += Vb * wx;
             sby += Vb * wy;
             sbz += Vb * wz;
         }
     }
-    double ci = -Vi / v.m[i];
+    double ci = -Vi / gm(v.m)[i];
     double cfx = ci * (sfx + 2. * S.gamma * sbx), cfy = ci * (sfy + 2. * S.gamma * sby),
            cfz = ci * (sfz + 2. * S.gamma * sbz);
     double aii = dt2 * dot3(cfx, cfy, cfz, sfx + sbx, sfy + sby, sfz + sbz) + sf2;
-    v.s_f[i] = src;
-    v.a_ff[i] = aii;
-    v.p[i] = (aii > S.eps || aii < -S.eps) ? fmax(S.omega * src / aii, 0.) : 0.;
+    gm(v.s_f)[i] = src;
+    gm(v.a_ff)[i] = aii;
+    gm(v.p)[i] = (aii > S.eps || aii < -S.eps) ? fmax(S.omega * src / aii, 0.) : 0.;
 }
 
 // relaxed Jacobi update and per-workgroup predicted density error
 // (iisph.rs:440-531); converged scenes are skipped
-__global__ void kb_jacobi(BatchDev B) {
+template <bool CACHED, int UNROLL, int WAVES>
+__global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
     __shared__ double lds[kBlock];
     PARTICLE_THREAD(B);
     if (!v.active) return;  // uniform per workgroup
@@ -612,47 +701,57 @@ __global__ void kb_jacobi(BatchDev B) {
     const SolveConst S = v.S;
     double err = 0.;
     if (live) {
-        const double ux = v.pax[i], uy = v.pay[i], uz = v.paz[i];
+        const rec4 ai = ld4(v.pa4, i);
+        const double ux = ai.x, uy = ai.y, uz = ai.z;
         const double dt2 = S.dt * S.dt;
         double adp = 0.;
-        const uint32_t c = B.cnt[gi];
-        size_t e = ELL_BASE(B, v, i);
-#pragma unroll 4
-        for (uint32_t k = 0; k < c; k++, e += 64) {
-            uint32_t j = B.ell_j[e];
-            adp += dt2 * v.V[j] *
-                   dot3(ux - v.pax[j], uy - v.pay[j], uz - v.paz[j], B.ell_gx[e], B.ell_gy[e],
-                        B.ell_gz[e]);
-        }
-        double px = 0, py = 0, pz = 0;
-        if (v.nb) {
-            px = v.x[i]; py = v.y[i]; pz = v.z[i];
+        const rec4 qi = ld4(v.q4, i);
+        const double px = qi.x, py = qi.y, pz = qi.z;
+        const uint32_t c = gm(B.cnt)[gi];
+        const size_t e0 = ELL_BASE(B, v, i);
+        if (c > 0) {  // software-pipelined as in kb_pressure_accel
+            const GLOBAL_AS uint32_t *ej = gm(B.ell_j) + e0;
+            uint32_t j1 = ej[64 * (size_t)min(1u, c - 1)];
+            const uint32_t j0 = ej[0];
+            rec4 q0 = ld4(v.q4, j0), a0 = ld4(v.pa4, j0);
+#pragma unroll UNROLL
+            for (uint32_t k = 0; k < c; k++) {
+                const uint32_t j2 = ej[64 * (size_t)min(k + 2, c - 1)];
+                const rec4 q1 = ld4(v.q4, j1), a1 = ld4(v.pa4, j1);
+                double wx, wy, wz;
+                pair_grad<CACHED>(B, K, e0 + 64 * (size_t)k, px - q0.x, py - q0.y, pz - q0.z,
+                                  wx, wy, wz);
+                adp += dt2 * q0.w * dot3(ux - a0.x, uy - a0.y, uz - a0.z, wx, wy, wz);
+                j1 = j2;
+                q0 = q1;
+                a0 = a1;
+            }
         }
         for (uint32_t bi = 0; bi < v.nb; bi++) {
             const BodyDev &b = B.bodies[v.body_base + bi];
             if (b.kind == 0) {
-                for (uint32_t k = b.soff[i]; k < b.soff[i + 1]; k++) {
-                    uint32_t s = b.sidx[k];
+                for (uint32_t k = gm(b.soff)[i]; k < gm(b.soff)[i + 1]; k++) {
+                    uint32_t s = gm(b.sidx)[k];
                     double wx, wy, wz;
-                    kernel_grad(K, px - b.sx[s], py - b.sy[s], pz - b.sz[s], wx, wy, wz);
-                    adp += dt2 * b.svol[s] * dot3(ux, uy, uz, wx, wy, wz);
+                    kernel_grad(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
+                    adp += dt2 * gm(b.svol)[s] * dot3(ux, uy, uz, wx, wy, wz);
                 }
-            } else if (b.flag[i]) {
+            } else if (gm(b.flag)[i]) {
                 double wx, wy, wz;
-                kernel_grad(K, px - b.bx[i], py - b.by[i], pz - b.bz[i], wx, wy, wz);
-                adp += dt2 * b.bvol[i] * dot3(ux, uy, uz, wx, wy, wz);
+                kernel_grad(K, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i], wx, wy, wz);
+                adp += dt2 * gm(b.bvol)[i] * dot3(ux, uy, uz, wx, wy, wz);
             }
         }
-        const double aii = v.a_ff[i], si = v.s_f[i];
+        const double aii = gm(v.a_ff)[i], si = gm(v.s_f)[i];
         if (aii < -S.eps) {
-            double pn = v.p[i] + S.omega * (si - adp) / aii;
-            v.p[i] = fmax(pn, 0.);
+            double pn = gm(v.p)[i] + S.omega * (si - adp) / aii;
+            gm(v.p)[i] = fmax(pn, 0.);
         }
         err = si < 0. ? fabs(adp - si) : 0.;
     }
     double r[1] = {err};
     block_sum<1>(r, lds);
-    if (threadIdx.x == 0) B.partials[lb] = r[0];
+    if (threadIdx.x == 0) gm(B.partials)[lb] = r[0];
 }
 
 // IISPHwOST: x* = x + dt v*   (iisph_optimized_source_term.rs:103-113)
@@ -660,9 +759,9 @@ __global__ void kb_pred_pos(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
     const double dt = v.S.dt;
-    v.sxp[i] = v.x[i] + dt * v.svx[i];
-    v.syp[i] = v.y[i] + dt * v.svy[i];
-    v.szp[i] = v.z[i] + dt * v.svz[i];
+    gm(v.sxp)[i] = gm(v.x)[i] + dt * gm(v.svx)[i];
+    gm(v.syp)[i] = gm(v.y)[i] + dt * gm(v.svy)[i];
+    gm(v.szp)[i] = gm(v.z)[i] + dt * gm(v.svz)[i];
 }
 
 // IISPHwOST: x* += dt^2 a, v* = v* + dt a   (iisph_optimized_source_term.rs:136-203)
@@ -670,12 +769,12 @@ __global__ void kb_ost_commit(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
     const double dt = v.S.dt, dt2 = dt * dt;
-    v.sxp[i] = v.sxp[i] + dt2 * v.ax[i];
-    v.syp[i] = v.syp[i] + dt2 * v.ay[i];
-    v.szp[i] = v.szp[i] + dt2 * v.az[i];
-    v.svx[i] = v.svx[i] + dt * v.ax[i];
-    v.svy[i] = v.svy[i] + dt * v.ay[i];
-    v.svz[i] = v.svz[i] + dt * v.az[i];
+    gm(v.sxp)[i] = gm(v.sxp)[i] + dt2 * gm(v.ax)[i];
+    gm(v.syp)[i] = gm(v.syp)[i] + dt2 * gm(v.ay)[i];
+    gm(v.szp)[i] = gm(v.szp)[i] + dt2 * gm(v.az)[i];
+    gm(v.svx)[i] = gm(v.svx)[i] + dt * gm(v.ax)[i];
+    gm(v.svy)[i] = gm(v.svy)[i] + dt * gm(v.ay)[i];
+    gm(v.svz)[i] = gm(v.svz)[i] + dt * gm(v.az)[i];
 }
 
 // measurement sums (sph/mod.rs:444-564) per workgroup
@@ -684,14 +783,14 @@ __global__ void kb_measure(BatchDev B) {
     PARTICLE_THREAD(B);
     double r[3] = {0., 0., 0.};
     if (live) {
-        double Vi = v.V[i], mi = v.m[i];
+        double Vi = gm(v.V)[i], mi = gm(v.m)[i];
         r[0] = Vi < v.V0 ? mi / Vi : mi / v.V0;
         r[1] = Vi < v.V0 ? v.V0 / Vi : 1.;
-        r[2] = 0.5 * mi * dot3(v.vx[i], v.vy[i], v.vz[i], v.vx[i], v.vy[i], v.vz[i]);
+        r[2] = 0.5 * mi * dot3(gm(v.vx)[i], gm(v.vy)[i], gm(v.vz)[i], gm(v.vx)[i], gm(v.vy)[i], gm(v.vz)[i]);
     }
     block_sum<3>(r, lds);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) B.partials[(size_t)lb * 3 + k] = r[k];
+        for (int k = 0; k < 3; k++) gm(B.partials)[(size_t)lb * 3 + k] = r[k];
 }
 
 // ------------------------------------------------------------------------
@@ -753,5 +852,14 @@ __global__ void kb_fold_bbox(const int *partials, const uint32_t *blk, int ns, i
     }
     if (threadIdx.x < 6) out[6 * s + threadIdx.x] = lds[threadIdx.x][0];
 }
+
+#define INSTANTIATE_PAIR_PASSES(C, U, W)                                                 \
+    template __global__ void kb_pairs_volume<C, U, W>(BatchDev);                        \
+    template __global__ void kb_nonpressure<C, U, W>(BatchDev);                         \
+    template __global__ void kb_pressure_accel<C, U, W>(BatchDev, int, int, int);       \
+    template __global__ void kb_source_diag<C, U, W>(BatchDev, int);                    \
+    template __global__ void kb_jacobi<C, U, W>(BatchDev);
+#define INSTANTIATE_BOTH(U, W) INSTANTIATE_PAIR_PASSES(true, U, W) INSTANTIATE_PAIR_PASSES(false, U, W)
+SPH_PAIR_VARIANTS(INSTANTIATE_BOTH)
 
 }  // namespace sph

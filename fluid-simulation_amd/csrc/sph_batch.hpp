@@ -38,7 +38,9 @@ struct SceneView {
     double gcenter[3];
     GridDev grid;         // occupied-cell box; start = batch table + cell_base
     double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *m, *V, *p;
-    double *s_f, *a_ff, *pax, *pay, *paz, *svx, *svy, *svz, *sxp, *syp, *szp;
+    double *q4;   // {x, y, z, V} per particle: the record neighbor loops gather
+    double *pa4;  // {pressure acceleration, 0} per particle: the Jacobi scratch
+    double *s_f, *a_ff, *svx, *svy, *svz, *sxp, *syp, *szp;
     double *tx, *ty, *tz, *tvx, *tvy, *tvz, *tm;  // permutation targets
     uint32_t *fid, *ref, *tfid, *tref;
 };
@@ -66,6 +68,13 @@ struct BatchDev {
     double *partials;      // per workgroup
 };
 
+// Pair passes take CACHED = true to read grad W from the pair cache (written by
+// kb_pairs_volume<true>) or false to recompute it from positions, and a code
+// shape: the neighbor loop's unroll factor and a floor on waves per SIMD (which
+// caps registers per lane). Every shape evaluates the same expressions in the
+// same order, so results do not depend on the shape.
+#define SPH_PAIR_VARIANTS(X) X(4, 0) X(2, 0) X(4, 4) X(8, 4) X(2, 6) X(1, 8)
+constexpr int kPairVariants = SPH_SHAPES;
 __global__ void kb_integrate_ec(BatchDev B);
 __global__ void kb_speed_below(BatchDev B);
 __global__ void kb_cells(BatchDev B);
@@ -76,16 +85,22 @@ __global__ void kb_permute(BatchDev B);
 __global__ void kb_count_ell(BatchDev B);
 __global__ void kb_fill_ell(BatchDev B);
 __global__ void kb_vm_sample(BatchDev B);
+template <bool CACHED, int UNROLL, int WAVES>
 __global__ void kb_pairs_volume(BatchDev B);
+template <bool CACHED, int UNROLL, int WAVES>
 __global__ void kb_nonpressure(BatchDev B);
 __global__ void kb_sesph_pressure(BatchDev B);
+template <bool CACHED, int UNROLL, int WAVES>
 __global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror);
 __global__ void kb_pred_vel(BatchDev B, int to_pred);
+template <bool CACHED, int UNROLL, int WAVES>
 __global__ void kb_source_diag(BatchDev B, int vp);
+template <bool CACHED, int UNROLL, int WAVES>
 __global__ void kb_jacobi(BatchDev B);
 __global__ void kb_pred_pos(BatchDev B);
 __global__ void kb_ost_commit(BatchDev B);
 __global__ void kb_measure(BatchDev B);
+__global__ void kb_pack_volume(BatchDev B);
 // per-scene folds of per-workgroup partials: out[s*rows + r] = sum (or max) over the
 // scene's workgroups of partials[lb*rows + r]
 __global__ void kb_fold_sum(const double *partials, const uint32_t *blk, int ns, int rows,

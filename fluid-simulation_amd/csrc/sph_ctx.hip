@@ -93,6 +93,11 @@ struct DBuf {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
     }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
 };
 
 // ------------------------------------------------------------------------
@@ -373,7 +378,8 @@ struct Scene {
     DBuf<double> x, y, z, vx, vy, vz, ax, ay, az, m, V, p;
     DBuf<uint32_t> fid, ref;
     // solver and integrator slots
-    DBuf<double> s_f, a_ff, pax, pay, paz, svx, svy, svz, sxp, syp, szp, ix, iy, iz, ivx, ivy, ivz;
+    DBuf<double> s_f, a_ff, svx, svy, svz, sxp, syp, szp, ix, iy, iz, ivx, ivy, ivz;
+    DBuf<double> q4, pa4;  // packed {x, y, z, V} and {pressure acceleration, 0}
     // permutation targets
     DBuf<double> tx, ty, tz, tvx, tvy, tvz, tm;
     DBuf<uint32_t> tfid, tref;
@@ -438,7 +444,7 @@ struct Scene {
         v.ax = ax.p; v.ay = ay.p; v.az = az.p;
         v.m = m.p; v.V = V.p; v.p = p.p;
         v.s_f = s_f.p; v.a_ff = a_ff.p;
-        v.pax = pax.p; v.pay = pay.p; v.paz = paz.p;
+        v.q4 = q4.p; v.pa4 = pa4.p;
         v.svx = svx.p; v.svy = svy.p; v.svz = svz.p;
         v.sxp = sxp.p; v.syp = syp.p; v.szp = szp.p;
         v.tx = tx.p; v.ty = ty.p; v.tz = tz.p;
@@ -446,6 +452,25 @@ struct Scene {
         v.fid = fid.p; v.ref = ref.p; v.tfid = tfid.p; v.tref = tref.p;
         return v;
     }
+};
+
+// ------------------------------------------------------------------------
+// online tuning of the pair passes' code shapes
+// ------------------------------------------------------------------------
+// Every shape computes the same bits, so shapes are measured on live steps:
+// step k of a tuning round runs every pair pass in shape k / steps_per_shape,
+// and after all shapes each pass keeps the one with the least time per
+// particle. A batch is tuned on its first steps and again when its particle
+// count has changed by 2x since.
+enum { PP_VOLUME, PP_NONPRESSURE, PP_SOURCE_DIAG, PP_PRESSURE_ACCEL, PP_JACOBI, kPairPasses };
+
+struct Tuner {
+    int steps_per_shape = 1;  // 0: off
+    int state = 0;            // 0 untuned, 1 measuring, 2 tuned
+    int shape = 0, step_in = 0;
+    size_t tuned_particles = 0;
+    double cost[kPairPasses][kPairVariants];
+    bool measuring() const { return state == 1; }
 };
 
 // ------------------------------------------------------------------------
@@ -460,6 +485,14 @@ struct Batch {
     char *arena = nullptr;
     size_t arena_cap = 0, arena_off = 0;
     std::vector<Scene *> sc;
+    // pair passes read grad W from the pair cache (24 B per slot) or recompute it
+    // from positions (the default: less HBM traffic and memory, same bits)
+    bool grad_cache = false;
+    // code shape of each pair pass (an index into SPH_PAIR_VARIANTS) and the
+    // online tuner that picks them from timed steps
+    int variant[kPairPasses] = {0, 0, 0, 0, 0};
+    Tuner tuner;
+    Prof tprof;
 
     DBuf<int> ci, ipart;
     DBuf<uint32_t> perm, cnt, slice_len, slice_off, ell_j, cell_cnt, cell_start, cursor;
@@ -513,6 +546,15 @@ struct Batch {
         arena_off += need;
         memcpy(h, src, bytes);
         HIPCK(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s));
+    }
+    // free the launch-time buffers (they are reserved again on the next use)
+    void release_scratch() {
+        sync();
+        ci.release(); ipart.release(); perm.release(); cnt.release(); slice_len.release();
+        slice_off.release(); ell_j.release(); cell_cnt.release(); cell_start.release();
+        cursor.release(); scan_a.release(); scan_b.release(); scan_c.release();
+        ell_gx.release(); ell_gy.release(); ell_gz.release(); partials.release();
+        ucnt.release(); ucursor.release();
     }
     ~Batch() {
         if (s) (void)hipStreamSynchronize(s);
@@ -579,6 +621,88 @@ static void check_device() {
             if (t_) (B).prof.end((B).s, r_);                                            \
         }                                                                               \
     } while (0)
+
+// pair-pass kernels indexed [gradient cache][code shape]
+typedef void (*KernB)(BatchDev);
+typedef void (*KernBI)(BatchDev, int);
+typedef void (*KernBIII)(BatchDev, int, int, int);
+#define PV_F(U, W) &kb_pairs_volume<false, U, W>,
+#define PV_T(U, W) &kb_pairs_volume<true, U, W>,
+#define NP_F(U, W) &kb_nonpressure<false, U, W>,
+#define NP_T(U, W) &kb_nonpressure<true, U, W>,
+#define SD_F(U, W) &kb_source_diag<false, U, W>,
+#define SD_T(U, W) &kb_source_diag<true, U, W>,
+#define PA_F(U, W) &kb_pressure_accel<false, U, W>,
+#define PA_T(U, W) &kb_pressure_accel<true, U, W>,
+#define JA_F(U, W) &kb_jacobi<false, U, W>,
+#define JA_T(U, W) &kb_jacobi<true, U, W>,
+static const KernB k_volume_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(PV_F)},
+                                                     {SPH_PAIR_VARIANTS(PV_T)}};
+static const KernB k_nonpressure_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(NP_F)},
+                                                          {SPH_PAIR_VARIANTS(NP_T)}};
+static const KernBI k_source_diag_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(SD_F)},
+                                                           {SPH_PAIR_VARIANTS(SD_T)}};
+static const KernBIII k_pressure_accel_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(PA_F)},
+                                                                {SPH_PAIR_VARIANTS(PA_T)}};
+static const KernB k_jacobi_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(JA_F)},
+                                                     {SPH_PAIR_VARIANTS(JA_T)}};
+
+// launch of pair pass `pass` over all particles of the batch in the batch's
+// gradient mode and the pass's code shape; timed under `pid` when profiling
+// and under `pass` while tuning
+template <class Tab, class... A>
+static void pair_pass(Batch &B, int pass, int pid, const Tab &tab, A... args) {
+    if (B.total_pblocks == 0) return;
+    auto kern = tab[B.grad_cache ? 1 : 0][B.variant[pass]];
+    Prof::Rec r{pid, B.total_particles, nullptr, nullptr};
+    Prof::Rec t{pass, B.total_particles, nullptr, nullptr};
+    const bool timed = B.prof.on && pid < SPH_PROF_KERNELS, tuning = B.tuner.measuring();
+    if (timed) B.prof.begin(B.s, r);
+    if (tuning) B.tprof.begin(B.s, t);
+    hipLaunchKernelGGL(kern, dim3((unsigned)B.total_pblocks), dim3(kBlock), 0, B.s, args...);
+    HIPCK(hipGetLastError());
+    if (tuning) B.tprof.end(B.s, t);
+    if (timed) B.prof.end(B.s, r);
+}
+
+static void tune_begin_step(Batch &B, size_t particles) {
+    Tuner &T = B.tuner;
+    if (T.steps_per_shape <= 0 || particles == 0) return;
+    const bool changed = T.state == 2 && (particles > 2 * T.tuned_particles ||
+                                          2 * particles < T.tuned_particles);
+    if (T.state == 0 || changed) {
+        T.state = 1;
+        T.shape = 0;
+        T.step_in = 0;
+        for (int p = 0; p < kPairPasses; p++)
+            for (int v = 0; v < kPairVariants; v++) T.cost[p][v] = 0.0;
+        B.tprof.reset();
+    }
+    if (T.state == 1)
+        for (int p = 0; p < kPairPasses; p++) B.variant[p] = T.shape;
+}
+
+static void tune_end_step(Batch &B, size_t particles) {
+    Tuner &T = B.tuner;
+    if (T.state != 1) return;
+    B.sync();
+    B.tprof.drain();
+    for (int p = 0; p < kPairPasses; p++) {
+        T.cost[p][T.shape] += B.tprof.units[p] ? B.tprof.ms[p] / (double)B.tprof.units[p] : 0.0;
+    }
+    B.tprof.reset();
+    if (++T.step_in < T.steps_per_shape) return;
+    T.step_in = 0;
+    if (++T.shape < kPairVariants) return;
+    for (int p = 0; p < kPairPasses; p++) {
+        int best = 0;
+        for (int v = 1; v < kPairVariants; v++)
+            if (T.cost[p][v] < T.cost[p][best]) best = v;
+        B.variant[p] = best;
+    }
+    T.state = 2;
+    T.tuned_particles = particles;
+}
 
 // exclusive scan of n uint32 into out[0..n]; out[n] = total (returned)
 static uint32_t scan_u32(Batch &B, const uint32_t *in, size_t n, uint32_t *out) {
@@ -709,12 +833,14 @@ static void cell_sort(Batch &B, size_t cnt, const int *ci, const GridDev &g,
 // ------------------------------------------------------------------------
 static void reserve_scene(Scene &c, size_t n) {
     const size_t k = std::max<size_t>(n, 1);
-    DBuf<double> *f64s[] = {&c.x,   &c.y,   &c.z,   &c.vx,  &c.vy,   &c.vz,  &c.ax,  &c.ay,
-                            &c.az,  &c.m,   &c.V,   &c.p,   &c.s_f,  &c.a_ff, &c.pax, &c.pay,
-                            &c.paz, &c.svx, &c.svy, &c.svz, &c.sxp,  &c.syp, &c.szp, &c.ix,
-                            &c.iy,  &c.iz,  &c.ivx, &c.ivy, &c.ivz,  &c.tx,  &c.ty,  &c.tz,
-                            &c.tvx, &c.tvy, &c.tvz, &c.tm};
+    DBuf<double> *f64s[] = {&c.x,   &c.y,   &c.z,   &c.vx,  &c.vy,  &c.vz,  &c.ax,  &c.ay,
+                            &c.az,  &c.m,   &c.V,   &c.p,   &c.s_f, &c.a_ff, &c.svx, &c.svy,
+                            &c.svz, &c.sxp, &c.syp, &c.szp, &c.ix,  &c.iy,  &c.iz,  &c.ivx,
+                            &c.ivy, &c.ivz, &c.tx,  &c.ty,  &c.tz,  &c.tvx, &c.tvy, &c.tvz,
+                            &c.tm};
     for (auto *b : f64s) b->reserve(k);
+    c.q4.reserve(4 * k);
+    c.pa4.reserve(4 * k);
     c.fid.reserve(k);
     c.ref.reserve(k);
     c.tfid.reserve(k);
@@ -1007,9 +1133,11 @@ static void neighbor_counts(Batch &B) {
     uint64_t slots = (uint64_t)rows * 64;
     if (slots > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "neighbor cache exceeds 2^32 slots");
     B.ell_j.reserve(std::max<uint64_t>(slots, 1));
-    B.ell_gx.reserve(std::max<uint64_t>(slots, 1));
-    B.ell_gy.reserve(std::max<uint64_t>(slots, 1));
-    B.ell_gz.reserve(std::max<uint64_t>(slots, 1));
+    if (B.grad_cache) {
+        B.ell_gx.reserve(std::max<uint64_t>(slots, 1));
+        B.ell_gy.reserve(std::max<uint64_t>(slots, 1));
+        B.ell_gz.reserve(std::max<uint64_t>(slots, 1));
+    }
 }
 
 // B::find_boundary_samples (sph/mod.rs:707-712)
@@ -1057,7 +1185,7 @@ static void boundary_samples(Batch &B) {
 static void pairs_and_volume(Batch &B) {
     BatchDev d = batch_dev(B);
     PLAUNCH(B, SPH_PROF_PAIRS_VOLUME, B.total_particles, kb_fill_ell, B.total_pblocks, d);
-    PLAUNCH(B, SPH_PROF_PAIRS_VOLUME, B.total_particles, kb_pairs_volume, B.total_pblocks, d);
+    pair_pass(B, PP_VOLUME, SPH_PROF_PAIRS_VOLUME, k_volume_tab, d);
 }
 
 static bool any_dynamic(const Batch &B) {
@@ -1092,8 +1220,8 @@ static void apply_body_partials(Batch &B) {
 
 static void pressure_accel(Batch &B, int overwrite) {
     BatchDev d = batch_dev(B);
-    PLAUNCH(B, SPH_PROF_PRESSURE_ACCEL, B.total_particles, kb_pressure_accel, B.total_pblocks, d,
-            0, overwrite, 1);
+    pair_pass(B, PP_PRESSURE_ACCEL, SPH_PROF_PRESSURE_ACCEL, k_pressure_accel_tab, d, 0,
+              overwrite, 1);
     apply_body_partials(B);
 }
 
@@ -1104,7 +1232,7 @@ static void pressure_accel(Batch &B, int overwrite) {
 static void resolve_pressure(Batch &B, int vp, int after) {
     const size_t ns = B.sc.size();
     BatchDev d = batch_dev(B);
-    PLAUNCH(B, SPH_PROF_SOURCE_DIAG, B.total_particles, kb_source_diag, B.total_pblocks, d, vp);
+    pair_pass(B, PP_SOURCE_DIAG, SPH_PROF_SOURCE_DIAG, k_source_diag_tab, d, vp);
     for (Scene *c : B.sc) {
         c->iter = 0;
         c->err = INFINITY;
@@ -1132,9 +1260,9 @@ static void resolve_pressure(Batch &B, int vp, int after) {
             upload_views(B);
             dirty = false;
         }
-        PLAUNCH(B, SPH_PROF_PRESSURE_ACCEL, B.total_particles, kb_pressure_accel, B.total_pblocks,
-                d, 1, 1, 0);
-        PLAUNCH(B, SPH_PROF_JACOBI, B.total_particles, kb_jacobi, B.total_pblocks, d);
+        pair_pass(B, PP_PRESSURE_ACCEL, SPH_PROF_PRESSURE_ACCEL, k_pressure_accel_tab, d, 1, 1,
+                  0);
+        pair_pass(B, PP_JACOBI, SPH_PROF_JACOBI, k_jacobi_tab, d);
         hipLaunchKernelGGL(kb_fold_sum, dim3((unsigned)ns), dim3(kBlock), 0, B.s, B.partials.p,
                            B.pblk.p, (int)ns, 1, B.red.p);
         HIPCK(hipGetLastError());
@@ -1217,7 +1345,7 @@ static void nonpressure(Batch &B) {
         }
     }
     BatchDev d = batch_dev(B);
-    PLAUNCH(B, SPH_PROF_NONPRESSURE, B.total_particles, kb_nonpressure, B.total_pblocks, d);
+    pair_pass(B, PP_NONPRESSURE, SPH_PROF_NONPRESSURE, k_nonpressure_tab, d);
     apply_body_partials(B);
 }
 
@@ -1286,6 +1414,7 @@ static void update(Batch &B) {
 // integrator + rigid bodies of every scene, then update (sph/mod.rs:125-144)
 static void step_batch(Batch &B) {
     layout(B);
+    tune_begin_step(B, B.total_particles);
     BatchDev d = batch_dev(B);
     PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_integrate_ec, B.total_pblocks, d);
     for (Scene *c : B.sc) {
@@ -1304,6 +1433,7 @@ static void step_batch(Batch &B) {
         c->time += dt;
     }
     update(B);
+    tune_end_step(B, B.total_particles);
 }
 
 // a context's scene alone in the context's own batch
@@ -1390,6 +1520,12 @@ static void download_field(sph_ctx *c, const DBuf<double> &f, std::vector<double
     c->own.sync();
     out.assign(n, 0.0);
     for (size_t k = 0; k < n; k++) out[r[k]] = h[k];
+}
+
+// q4.w = V after a caller-provided volume replaced the computed one
+static void pack_volume(Batch &B) {
+    BatchDev d = batch_dev(B);
+    PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_pack_volume, B.total_pblocks, d);
 }
 
 static void check_solo(const sph_ctx *c) {
@@ -1867,6 +2003,7 @@ int sph_nonpressure_accel(sph_ctx *c, size_t n, const double *x, const double *v
         prepare_external(c, n, x, v, m);
         Scene &S = c->scene;
         upload_sorted(c, V, 1, 0, S.V);
+        pack_volume(c->own);
         for (auto &b : S.bodies)
             for (int k = 0; k < 6; k++) b->applied[k] = 0;
         std::vector<Rigid> saved;
@@ -1894,6 +2031,7 @@ int sph_pressure_solve(sph_ctx *c, size_t n, const double *x, const double *v, c
         prepare_external(c, n, x, v, m);
         Scene &S = c->scene;
         upload_sorted(c, V, 1, 0, S.V);
+        pack_volume(c->own);
         upload_sorted(c, a_inout, 3, 0, S.ax);
         upload_sorted(c, a_inout, 3, 1, S.ay);
         upload_sorted(c, a_inout, 3, 2, S.az);
@@ -1971,7 +2109,7 @@ int sph_world_attach(sph_world *w, sph_ctx *c) {
         if (!c->scene.initialized) throw Error(SPH_ERR_STATE, "sph_system_init has not run");
         if (!w->members.empty() && w->members[0]->scene.P.solver != c->scene.P.solver)
             throw Error(SPH_ERR_UNSUPPORTED, "all systems of a world use one pressure solver");
-        c->own.sync();
+        c->own.release_scratch();  // the world's batch holds the launch-time buffers
         c->scene.in_world = true;
         w->members.push_back(c);
     });
@@ -2029,6 +2167,63 @@ int sph_world_profile_read(sph_world *w, int kernel, uint64_t *launches, uint64_
         if (launches) *launches = w->batch.prof.launches[kernel];
         if (units) *units = w->batch.prof.units[kernel];
         if (ms) *ms = w->batch.prof.ms[kernel];
+    });
+}
+
+static void set_option(Batch &B, int option, int value) {
+    B.sync();
+    if (option == SPH_OPT_CACHE_GRADIENTS) {
+        B.grad_cache = value != 0;
+        if (B.tuner.state == 2) B.tuner.state = 0;  // re-tune in the new mode
+    } else if (option == SPH_OPT_AUTOTUNE) {
+        if (value < 0) throw Error(SPH_ERR_ARG, "steps per shape must be >= 0");
+        B.tuner.steps_per_shape = value;
+        B.tuner.state = 0;
+    } else if (option == SPH_OPT_SHAPE) {
+        if (value < 0 || value >= kPairVariants) throw Error(SPH_ERR_ARG, "no such code shape");
+        for (int p = 0; p < kPairPasses; p++) B.variant[p] = value;
+        B.tuner.steps_per_shape = 0;
+        B.tuner.state = 0;
+    } else {
+        throw Error(SPH_ERR_ARG, "unknown option");
+    }
+}
+
+static int get_option(const Batch &B, int option) {
+    if (option == SPH_OPT_CACHE_GRADIENTS) return B.grad_cache ? 1 : 0;
+    if (option == SPH_OPT_AUTOTUNE) return B.tuner.steps_per_shape;
+    if (option == SPH_OPT_TUNE_STATE) return B.tuner.state;
+    if (option >= SPH_OPT_SHAPE_VOLUME && option < SPH_OPT_SHAPE_VOLUME + kPairPasses)
+        return B.variant[option - SPH_OPT_SHAPE_VOLUME];
+    throw Error(SPH_ERR_ARG, "unknown option");
+}
+
+int sph_get_option(const sph_ctx *c, int option, int *value) {
+    return guarded([&] {
+        if (!c || !value) throw Error(SPH_ERR_ARG, "null argument");
+        *value = get_option(c->own, option);
+    });
+}
+
+int sph_world_get_option(const sph_world *w, int option, int *value) {
+    return guarded([&] {
+        if (!w || !value) throw Error(SPH_ERR_ARG, "null argument");
+        *value = get_option(w->batch, option);
+    });
+}
+
+int sph_set_option(sph_ctx *c, int option, int value) {
+    return guarded([&] {
+        if (!c) throw Error(SPH_ERR_ARG, "null argument");
+        // every step and per-trait call rebuilds the pair cache before using it
+        set_option(c->own, option, value);
+    });
+}
+
+int sph_world_set_option(sph_world *w, int option, int value) {
+    return guarded([&] {
+        if (!w) throw Error(SPH_ERR_ARG, "null argument");
+        set_option(w->batch, option, value);
     });
 }
 

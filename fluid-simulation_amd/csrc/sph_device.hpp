@@ -86,18 +86,19 @@ __device__ __forceinline__ double kernel_fn(const KernelConst &K, double rx, dou
     return K.wfac * kernel_w(rn / K.h);
 }
 
-// grad W(r, h) (kernel/mod.rs:37-45)
+// grad W(r, h) (kernel/mod.rs:37-45): ((gfac r) / |r|) w'(|r|/h), zero at
+// r = 0. Branch-free, so that neighbor loads can be scheduled across it: at
+// |r| = 0 the divisor is replaced by 1 and w'(0) = -3 + 12/4 is exactly 0, so
+// the self pair (r = +0) yields +0 as the reference does; for |r| != 0 the
+// expression is the reference's.
 __device__ __forceinline__ void kernel_grad(const KernelConst &K, double rx, double ry,
                                             double rz, double &gx, double &gy, double &gz) {
-    double rn = sqrt(dot3(rx, ry, rz, rx, ry, rz));
-    if (rn == 0.0) {
-        gx = gy = gz = 0.0;
-        return;
-    }
-    double d = kernel_dqw(rn / K.h);
-    gx = ((K.gfac * rx) / rn) * d;
-    gy = ((K.gfac * ry) / rn) * d;
-    gz = ((K.gfac * rz) / rn) * d;
+    const double rn = sqrt(dot3(rx, ry, rz, rx, ry, rz));
+    const double d = kernel_dqw(rn / K.h);
+    const double rs = rn != 0.0 ? rn : 1.0;
+    gx = ((K.gfac * rx) / rs) * d;
+    gy = ((K.gfac * ry) / rs) * d;
+    gz = ((K.gfac * rz) / rs) * d;
 }
 
 __device__ __forceinline__ void cross3(double ax, double ay, double az, double bx, double by,

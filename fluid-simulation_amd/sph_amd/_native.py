@@ -106,7 +106,19 @@ SIGNATURES = [
     ("sph_world_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("sph_world_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
+    ("sph_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("sph_world_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("sph_get_option", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    ("sph_world_get_option", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
 ]
+
+OPT_CACHE_GRADIENTS = 0
+OPT_AUTOTUNE = 1
+OPT_SHAPE = 2
+OPT_TUNE_STATE = 3
+OPT_SHAPE_VOLUME = 4
+SHAPES = 6
+PAIR_PASSES = ["volume", "nonpressure", "source_diag", "pressure_accel", "jacobi"]
 
 _lib = None
 

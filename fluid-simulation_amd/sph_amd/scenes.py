@@ -422,3 +422,56 @@ def small_rigid_scene(side=(14, 14, 8), seed=2, dx=0.02, solver="IISPH") -> Scen
     s_l, v_l = volume_map_fields(lambda q: sd_box_solid(q, lo_l, hi_l), lo_l, hi_l, h, dx)
     return Scene("small_rigid", p, x, v, m, fid,
                  [VolumeMap(sdf, vm, None, "box"), VolumeMap(s_l, v_l, rb, "cube")])
+
+
+# --------------------------------------------------------------------------
+# serving stream scenes
+# --------------------------------------------------------------------------
+_CONTAINER_CACHE = {}
+
+
+def container_fields(shape: str, size: float, h: float, dx: float):
+    """Volume-map fields of a box ([0, size]^3) or sphere (radius ``size``
+    centred at the origin) container, cached per (shape, size, h, dx): the
+    field construction is per-scene setup, and requests share containers."""
+    key = (shape, round(size, 9), h, dx)
+    if key not in _CONTAINER_CACHE:
+        if shape == "box":
+            lo, hi = np.zeros(3), np.full(3, size)
+            f = volume_map_fields(lambda q: sd_box_container(q, lo, hi), lo, hi, h, dx)
+        else:
+            c = np.zeros(3)
+            f = volume_map_fields(lambda q: sd_sphere_container(q, c, size), c - size,
+                                  c + size, h, dx)
+        _CONTAINER_CACHE[key] = f
+    return _CONTAINER_CACHE[key]
+
+
+def serving_scene(n: int, rho0: float, nu: float, shape: str, seed: int, dx: float = 0.02,
+                  steps_hint: int = 200) -> Scene:
+    """One request of the serving stream: about ``n`` IISPH particles of rest
+    density ``rho0`` and viscosity ``nu`` as a block in a box or sphere
+    volume-map container (sizes quantized so that containers are shared);
+    CFL dt (0.4, max 1e-3), target 0.1 %."""
+    rng = np.random.default_rng(seed)
+    side = max(2, int(round(n ** (1.0 / 3.0))))
+    nz = max(1, int(round(n / (side * side))))
+    counts = [side, side, nz]
+    ext = np.array(counts, dtype=np.float64) * dx
+    h = 2 * dx
+    if shape == "box":
+        step = 8 * dx
+        size = math.ceil((max(ext) + 4 * dx) / step) * step
+        lo = [0.5 * (size - ext[0]), 0.5 * (size - ext[1]), 2 * dx]
+    else:
+        step = 4 * dx
+        size = math.ceil((0.5 * float(np.linalg.norm(ext)) + 3 * dx) / step) * step
+        lo = list(-0.5 * ext)
+    pos = lattice_block(lo, counts, dx, rng, 1e-4)
+    x, v, m, fid = _fluid([(pos, rho0, 0)], dx)
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, fluid_viscosity=nu,
+               boundary_viscosity=nu, solver="IISPH", integrator="EulerCromer",
+               target_density_error=0.1, relaxation_factor=0.5)
+    sdf, vm = container_fields(shape, size, h, dx)
+    return Scene(f"serve_{shape}_{len(x)}", p, x, v, m, fid, [VolumeMap(sdf, vm, None, shape)])

@@ -1,0 +1,290 @@
+"""Continuous-batching service for many SPH scenes on one GPU.
+
+The reference runs one simulation per worker process and steps it alone
+(``sci-phi-backend`` -> ``simulation_worker``: one ``Box<dyn SPHSystem>`` per
+job, ``System::step_forward_in_time`` in a loop). Here a stream of scene
+requests is served by one process and one device:
+
+* a request is admitted when a slot and its device-memory estimate fit the
+  HBM budget; its state (particles, neighbor and cell structures, volume-map
+  lattices) then stays resident until it finishes;
+* every admitted scene joins one ``World``: each time step of the world is one
+  sequence of multi-scene launches that advances every resident scene by one
+  of its own time steps (each keeps its own CFL step size and pressure-solve
+  termination);
+* requests that arrive while others run are admitted between world steps, and
+  finished scenes leave between world steps, freeing their slot and memory.
+
+Each scene's results are bit-identical to running it alone (the world's
+per-scene arithmetic and reductions do not depend on the other scenes).
+
+The sequential baseline serves the same stream one request at a time with
+``System.step_forward_in_time``, the plain per-scene GPU path.
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import math
+import time
+from collections import deque
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import scenes as S
+
+# device bytes per particle while resident in a world: 36 binary64 and 4 u32
+# per-particle fields of the scene, the world's per-particle sort and neighbor
+# slots (about 45 pair slots of 4 B in the default recompute mode) and one
+# volume-map slot (flag + 7 binary64) per boundary
+BYTES_PER_PARTICLE = 36 * 8 + 4 * 4 + 20 + 48 * 4
+BYTES_PER_PARTICLE_BODY = 1 + 7 * 8
+BYTES_PER_SCENE = 1 << 20          # stream, pinned staging, small tables
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float                  # seconds after the stream starts
+    scene: S.Scene
+    steps: int
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    n: int
+    steps: int
+    arrival: float
+    admitted: float
+    finished: float
+    measurement: dict
+    digest: str                     # sha256 of the final x, v (reference order)
+    state: Optional[dict] = None
+
+
+@dataclasses.dataclass
+class Report:
+    mode: str
+    results: Dict[int, Result]
+    step_ms: List[float]            # one entry per (request, step)
+    makespan_s: float               # first arrival -> last completion
+    busy_s: float                   # time inside stepping calls
+    admit_s: float                  # time spent creating and initializing systems
+    peak_resident: int
+    peak_bytes: int
+
+    @property
+    def particle_steps(self) -> int:
+        return sum(r.n * r.steps for r in self.results.values())
+
+    def summary(self) -> dict:
+        lat = np.asarray(self.step_ms) if self.step_ms else np.zeros(1)
+        req = np.asarray([r.finished - r.arrival for r in self.results.values()] or [0.0])
+        derr = [r.measurement["density_error"] for r in self.results.values()]
+        return {
+            "mode": self.mode,
+            "requests": len(self.results),
+            "particle_steps": self.particle_steps,
+            "makespan_s": self.makespan_s,
+            "throughput_particle_steps_per_s": self.particle_steps / max(self.makespan_s, 1e-12),
+            "busy_throughput_particle_steps_per_s": self.particle_steps / max(self.busy_s, 1e-12),
+            "busy_s": self.busy_s,
+            "admit_s": self.admit_s,
+            "p50_step_ms": float(np.percentile(lat, 50)),
+            "p99_step_ms": float(np.percentile(lat, 99)),
+            "p50_request_s": float(np.percentile(req, 50)),
+            "p99_request_s": float(np.percentile(req, 99)),
+            "peak_resident_scenes": self.peak_resident,
+            "peak_resident_bytes_est": self.peak_bytes,
+            "density_error_pct_max": float(max(derr)) if derr else 0.0,
+        }
+
+
+def make_stream(n_requests: int = 256, rate: float = 20.0, n_min: int = 4096,
+                n_max: int = 262144, steps: int = 200, seed: int = 0,
+                dx: float = 0.02) -> List[Request]:
+    """The seeded serving stream: Poisson arrivals at ``rate`` per second,
+    particle counts log-uniform in [n_min, n_max], rest density U[800, 1500],
+    viscosity U[0.005, 0.05], box or sphere container, ``steps`` IISPH steps."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    t -= t[0]
+    ns = np.exp(rng.uniform(math.log(n_min), math.log(n_max), size=n_requests)).astype(int)
+    rho = rng.uniform(800.0, 1500.0, size=n_requests)
+    nu = rng.uniform(0.005, 0.05, size=n_requests)
+    shape = rng.integers(0, 2, size=n_requests)
+    seeds = rng.integers(0, 2**31 - 1, size=n_requests)
+    out = []
+    for k in range(n_requests):
+        sc = S.serving_scene(int(ns[k]), float(rho[k]), float(nu[k]),
+                             "box" if shape[k] == 0 else "sphere", int(seeds[k]), dx=dx)
+        out.append(Request(k, float(t[k]), sc, int(steps)))
+    return out
+
+
+def lattice_bytes(lat: S.Lattice) -> int:
+    return int(lat.values.size) * 8 + int(np.prod(lat.n)) + 256
+
+
+def estimate_device_bytes(scene: S.Scene) -> int:
+    """Device memory a scene holds while resident in a world (an estimate used
+    for admission; the library allocates on demand)."""
+    n = scene.n
+    b = BYTES_PER_SCENE + n * BYTES_PER_PARTICLE
+    for bd in scene.boundaries:
+        if hasattr(bd, "sdf"):
+            b += n * BYTES_PER_PARTICLE_BODY + lattice_bytes(bd.sdf) + lattice_bytes(bd.vm)
+        else:
+            b += int(len(bd.positions)) * 64
+    return int(b)
+
+
+class Admission:
+    """Slots plus an HBM budget in bytes."""
+
+    def __init__(self, budget_bytes: int, max_slots: int):
+        self.budget = int(budget_bytes)
+        self.max_slots = int(max_slots)
+        self.used = 0
+        self.slots = 0
+
+    def fits(self, nbytes: int) -> bool:
+        return self.slots < self.max_slots and self.used + nbytes <= self.budget
+
+    def take(self, nbytes: int):
+        if not self.fits(nbytes):
+            raise RuntimeError("admission over budget")
+        self.used += nbytes
+        self.slots += 1
+
+    def give(self, nbytes: int):
+        self.used -= nbytes
+        self.slots -= 1
+        assert self.used >= 0 and self.slots >= 0
+
+
+def _digest(state: dict) -> str:
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(state["x"]).tobytes())
+    h.update(np.ascontiguousarray(state["v"]).tobytes())
+    return h.hexdigest()
+
+
+def _finish(req: Request, sim, admitted: float, now: float, keep_state: bool) -> Result:
+    st = sim.state()
+    return Result(req.rid, len(sim), req.steps, req.arrival, admitted, now,
+                  sim.take_measurement(), _digest(st), st if keep_state else None)
+
+
+def _check_fits(req: Request, adm: Admission):
+    need = estimate_device_bytes(req.scene)
+    if need > adm.budget:
+        raise ValueError(f"request {req.rid} needs {need} B, more than the whole budget")
+    return need
+
+
+class BatchedServer:
+    """Continuous batching: admitted scenes are stepped together in a World."""
+
+    def __init__(self, hbm_budget_bytes: int, max_slots: int = 64,
+                 clock: Callable[[], float] = time.perf_counter, keep_state: bool = False):
+        from . import system  # loads the HIP library
+        self._system = system
+        self.adm = Admission(hbm_budget_bytes, max_slots)
+        self.clock = clock
+        self.keep_state = keep_state
+
+    def run(self, stream: List[Request]) -> Report:
+        system = self._system
+        pending = deque(sorted(stream, key=lambda r: r.arrival))
+        queue: deque = deque()
+        active = []   # [req, sim, steps_done, bytes, admitted_at]
+        results: Dict[int, Result] = {}
+        step_ms: List[float] = []
+        world = system.World()
+        busy = admit = 0.0
+        peak_n = peak_b = 0
+        t0 = self.clock()
+        try:
+            while pending or queue or active:
+                now = self.clock() - t0
+                while pending and pending[0].arrival <= now:
+                    queue.append(pending.popleft())
+                # admit in arrival order while a slot and the budget allow
+                while queue:
+                    need = _check_fits(queue[0], self.adm)
+                    if not self.adm.fits(need):
+                        break
+                    req = queue.popleft()
+                    a0 = self.clock()
+                    sim = system.System(req.scene)
+                    world.attach(sim)
+                    admit += self.clock() - a0
+                    self.adm.take(need)
+                    active.append([req, sim, 0, need, self.clock() - t0])
+                peak_n = max(peak_n, len(active))
+                peak_b = max(peak_b, self.adm.used)
+                if not active:
+                    if pending:  # idle until the next arrival
+                        time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
+                    continue
+                s0 = self.clock()
+                world.step(1)
+                dt = self.clock() - s0
+                busy += dt
+                step_ms.extend([dt * 1e3] * len(active))
+                done = []
+                for a in active:
+                    a[2] += 1
+                    if a[2] >= a[0].steps:
+                        done.append(a)
+                for a in done:
+                    req, sim, _, need, t_adm = a
+                    world.detach(sim)
+                    results[req.rid] = _finish(req, sim, t_adm, self.clock() - t0,
+                                               self.keep_state)
+                    sim._ctx.close()
+                    self.adm.give(need)
+                    active.remove(a)
+        finally:
+            world.close()
+        makespan = max(r.finished for r in results.values()) - min(r.arrival for r in stream)
+        return Report("batched", results, step_ms, makespan, busy, admit, peak_n, peak_b)
+
+
+class SequentialServer:
+    """The baseline: one request at a time, each stepped alone."""
+
+    def __init__(self, clock: Callable[[], float] = time.perf_counter, keep_state: bool = False):
+        from . import system
+        self._system = system
+        self.clock = clock
+        self.keep_state = keep_state
+
+    def run(self, stream: List[Request]) -> Report:
+        system = self._system
+        results: Dict[int, Result] = {}
+        step_ms: List[float] = []
+        busy = admit = 0.0
+        t0 = self.clock()
+        for req in sorted(stream, key=lambda r: r.arrival):
+            wait = req.arrival - (self.clock() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            a0 = self.clock()
+            sim = system.System(req.scene)
+            admit += self.clock() - a0
+            t_adm = self.clock() - t0
+            for _ in range(req.steps):
+                s0 = self.clock()
+                sim.step_forward_in_time(1)
+                dt = self.clock() - s0
+                busy += dt
+                step_ms.append(dt * 1e3)
+            results[req.rid] = _finish(req, sim, t_adm, self.clock() - t0, self.keep_state)
+            sim._ctx.close()
+        makespan = max(r.finished for r in results.values()) - min(r.arrival for r in stream)
+        return Report("sequential", results, step_ms, makespan, busy, admit, 1,
+                      max(estimate_device_bytes(r.scene) for r in stream))

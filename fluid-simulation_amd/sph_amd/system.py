@@ -221,6 +221,22 @@ class System:
     def profile_read(self) -> dict:
         return _read_profile(N.lib().sph_profile_read, self._ctx.h)
 
+    def set_cache_gradients(self, on: bool):
+        """Read grad W from the pair cache (True) or recompute it (False, default)."""
+        N.check(N.lib().sph_set_option(self._ctx.h, N.OPT_CACHE_GRADIENTS, int(bool(on))))
+
+    def set_option(self, option: int, value: int):
+        N.check(N.lib().sph_set_option(self._ctx.h, int(option), int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int()
+        N.check(N.lib().sph_get_option(self._ctx.h, int(option), C.byref(v)))
+        return int(v.value)
+
+    def tuning(self) -> dict:
+        """Tuner state and the code shape of each neighbor pass."""
+        return _tuning(self.get_option)
+
     def clone(self) -> "System":
         """Deep copy into an independent device context (DynClone)."""
         h = C.c_void_p()
@@ -286,6 +302,13 @@ def _read_profile(fn, handle) -> dict:
     return out
 
 
+def _tuning(get) -> dict:
+    out = {"state": get(N.OPT_TUNE_STATE), "steps_per_shape": get(N.OPT_AUTOTUNE)}
+    for k, name in enumerate(N.PAIR_PASSES):
+        out[name] = get(N.OPT_SHAPE_VOLUME + k)
+    return out
+
+
 class World:
     """Many systems advanced together by shared multi-system launches.
 
@@ -320,6 +343,20 @@ class World:
         ns, npart = C.c_size_t(), C.c_size_t()
         N.check(N.lib().sph_world_size(self.h, C.byref(ns), C.byref(npart)))
         return int(ns.value), int(npart.value)
+
+    def set_cache_gradients(self, on: bool):
+        N.check(N.lib().sph_world_set_option(self.h, N.OPT_CACHE_GRADIENTS, int(bool(on))))
+
+    def set_option(self, option: int, value: int):
+        N.check(N.lib().sph_world_set_option(self.h, int(option), int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int()
+        N.check(N.lib().sph_world_get_option(self.h, int(option), C.byref(v)))
+        return int(v.value)
+
+    def tuning(self) -> dict:
+        return _tuning(self.get_option)
 
     def profile(self, enable: bool = True):
         N.check(N.lib().sph_world_profile(self.h, int(bool(enable))))

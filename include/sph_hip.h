@@ -237,6 +237,35 @@ int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
 /* free and total device memory (hipMemGetInfo), for admission control */
 int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
 
+/* ---- implementation options (no counterpart in the reference) ----------------
+ * None of them changes a result bit; they trade memory and speed.
+ *   SPH_OPT_CACHE_GRADIENTS  1 = the neighbor passes read grad W from the pair
+ *       cache (24 B more HBM per pair slot), 0 = they recompute it (default).
+ *   SPH_OPT_AUTOTUNE  steps per code shape of the online tuner (default 1;
+ *       0 = off): the first steps of a context or world, and the steps after
+ *       its particle count changed 2x, time each code shape (unroll, waves per
+ *       SIMD) of the five neighbor passes, which then keep their fastest.
+ *   SPH_OPT_SHAPE  (set) run every neighbor pass in this code shape
+ *       (0 .. SPH_SHAPES-1) and turn the tuner off.
+ *   SPH_OPT_TUNE_STATE  (get) 0 untuned, 1 measuring, 2 tuned.
+ *   SPH_OPT_SHAPE_VOLUME .. SPH_OPT_SHAPE_VOLUME+4  (get) the code shape of the
+ *       volume, non-pressure, source/diagonal, pressure-acceleration and
+ *       Jacobi passes.
+ * The options of a context apply when it steps alone, those of a world to
+ * the world's steps. */
+enum {
+    SPH_OPT_CACHE_GRADIENTS = 0,
+    SPH_OPT_AUTOTUNE = 1,
+    SPH_OPT_SHAPE = 2,
+    SPH_OPT_TUNE_STATE = 3,
+    SPH_OPT_SHAPE_VOLUME = 4
+};
+enum { SPH_SHAPES = 6 };
+int sph_set_option(sph_ctx *ctx, int option, int value);
+int sph_get_option(const sph_ctx *ctx, int option, int *value);
+int sph_world_set_option(sph_world *world, int option, int value);
+int sph_world_get_option(const sph_world *world, int option, int *value);
+
 #ifdef __cplusplus
 }
 #endif

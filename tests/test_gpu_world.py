@@ -110,3 +110,50 @@ def test_world_rejects_mixed_solvers(native):
     with pytest.raises(native.N.SphError):
         w.attach(b)
     w.step(1)
+
+
+def test_gradient_cache_modes_bit_identical(native):
+    """Pair passes reading cached grad W and recomputing it give the same bits."""
+    scs = [scenes.small_rigid_scene(), scenes.small_iisph_box(side=9, seed=2, jitter=2e-3)]
+    a = [native.System(sc) for sc in scs]
+    b = [native.System(sc) for sc in scs]
+    for s in a:
+        s.set_cache_gradients(True)
+    w = native.World(b)
+    w.set_cache_gradients(True)
+    for k in range(5):
+        for s in a:
+            s.step_forward_in_time(1)
+        w.step(1)
+        if k == 2:  # switch modes mid-run
+            for s in a:
+                s.set_cache_gradients(False)
+            w.set_cache_gradients(False)
+    for x, y in zip(a, b):
+        assert_same(x, y)
+    solo = native.System(scs[1])
+    solo.step_forward_in_time(5)
+    assert_same(solo, a[1])
+
+
+def test_code_shapes_bit_identical_and_tuner(native):
+    """Every code shape of the neighbor passes gives the same bits; the online
+    tuner measures all of them on live steps and then keeps one per pass."""
+    from sph_amd import _native as N
+    sc = scenes.small_iisph_box(side=11, seed=3, solver="IISPHwOST", jitter=2e-3)
+    ref = native.System(sc)
+    ref.set_option(N.OPT_SHAPE, 0)
+    ref.step_forward_in_time(N.SHAPES + 2)
+    for shape in range(1, N.SHAPES):
+        s = native.System(sc)
+        s.set_option(N.OPT_SHAPE, shape)
+        s.step_forward_in_time(N.SHAPES + 2)
+        assert_same(ref, s)
+    tuned = native.System(sc)
+    assert tuned.tuning()["state"] == 0
+    tuned.step_forward_in_time(N.SHAPES)
+    t = tuned.tuning()
+    assert t["state"] == 2
+    assert all(0 <= t[p] < N.SHAPES for p in N.PAIR_PASSES)
+    tuned.step_forward_in_time(2)
+    assert_same(ref, tuned)

@@ -1,0 +1,93 @@
+"""Serving host logic on the CPU: the seeded request stream, the device-memory
+estimate and slot/HBM-budget admission (no GPU calls)."""
+import math
+
+import numpy as np
+import pytest
+
+from sph_amd import scenes, serving
+
+
+def test_stream_is_seeded_and_in_range():
+    a = serving.make_stream(n_requests=12, rate=20.0, n_min=512, n_max=4096, steps=7, seed=3)
+    b = serving.make_stream(n_requests=12, rate=20.0, n_min=512, n_max=4096, steps=7, seed=3)
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    for ra, rb in zip(a, b):
+        assert np.array_equal(ra.scene.x, rb.scene.x)
+        assert ra.scene.params == rb.scene.params
+    arr = np.array([r.arrival for r in a])
+    assert arr[0] == 0.0 and np.all(np.diff(arr) > 0)
+    for r in a:
+        p = r.scene.params
+        assert r.steps == 7
+        assert 0.4 * 512 <= r.scene.n <= 1.6 * 4096
+        assert 800.0 <= r.scene.m[0] / p.rest_density_grid_spacing ** 3 <= 1500.0
+        assert 0.005 <= p.fluid_viscosity <= 0.05 and p.boundary_viscosity == p.fluid_viscosity
+        assert p.solver == "IISPH" and p.cfl and p.target_density_error == 0.1
+    kinds = {r.scene.boundaries[0].kind for r in serving.make_stream(n_requests=16, n_min=512,
+                                                                        n_max=1024, seed=1)}
+    assert kinds == {"box", "sphere"}
+
+
+def test_stream_arrival_rate():
+    # mean gap of the Poisson process is 1/rate (checked on a long seeded draw)
+    rng = np.random.default_rng(0)
+    gaps = rng.exponential(1.0 / 20.0, size=20000)
+    assert abs(gaps.mean() - 0.05) < 0.002
+
+
+@pytest.mark.parametrize("shape", ["box", "sphere"])
+def test_serving_scene_fluid_inside_container(shape):
+    sc = scenes.serving_scene(3000, 1000.0, 0.01, shape, 5)
+    lat = sc.boundaries[0].sdf
+    h = sc.params.kernel_support_radius
+    if shape == "box":
+        size = lat.x_max[0] - 3.1 * h
+        lo, hi = np.zeros(3), np.full(3, size)
+        d = np.minimum(sc.x - lo, hi - sc.x).min()
+    else:
+        r = lat.x_max[0] - 3.1 * h
+        d = (r - np.linalg.norm(sc.x, axis=1)).min()
+    assert d > sc.params.rest_density_grid_spacing  # at least one spacing from the wall
+
+
+def test_containers_are_shared():
+    a = scenes.serving_scene(3000, 900.0, 0.01, "box", 1)
+    b = scenes.serving_scene(3100, 1200.0, 0.02, "box", 2)
+    assert a.boundaries[0].sdf is b.boundaries[0].sdf
+
+
+def test_device_estimate_grows_with_particles():
+    small = scenes.serving_scene(2000, 1000.0, 0.01, "box", 1)
+    big = scenes.serving_scene(16000, 1000.0, 0.01, "box", 1)
+    es, eb = serving.estimate_device_bytes(small), serving.estimate_device_bytes(big)
+    assert eb > es > small.n * serving.BYTES_PER_PARTICLE
+    assert eb - es >= (big.n - small.n) * serving.BYTES_PER_PARTICLE
+
+
+def test_admission_slots_and_budget():
+    adm = serving.Admission(budget_bytes=1000, max_slots=2)
+    assert adm.fits(600)
+    adm.take(600)
+    assert not adm.fits(500)      # over budget
+    assert adm.fits(400)
+    adm.take(400)
+    assert not adm.fits(0)        # out of slots
+    with pytest.raises(RuntimeError):
+        adm.take(0)
+    adm.give(600)
+    assert adm.fits(600) and not adm.fits(601)
+    adm.give(400)
+    assert adm.used == 0 and adm.slots == 0
+
+
+def test_report_summary_percentiles():
+    res = {k: serving.Result(k, 100 * (k + 1), 10, 0.1 * k, 0.1 * k, 0.1 * k + 1.0,
+                             {"density_error": 0.01 * k}, "") for k in range(4)}
+    rep = serving.Report("batched", res, [1.0] * 99 + [5.0], 2.0, 1.0, 0.1, 4, 123)
+    s = rep.summary()
+    assert s["particle_steps"] == 10 * 100 * (1 + 2 + 3 + 4)
+    assert math.isclose(s["throughput_particle_steps_per_s"], s["particle_steps"] / 2.0)
+    assert s["p50_step_ms"] == 1.0 and 1.0 < s["p99_step_ms"] <= 5.0
+    assert math.isclose(s["p50_request_s"], 1.0)
+    assert math.isclose(s["density_error_pct_max"], 0.03)
