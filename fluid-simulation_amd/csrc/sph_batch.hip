@@ -29,9 +29,10 @@ __device__ __forceinline__ int scene_of(const uint32_t *blk, int ns, uint32_t lb
 
 #define PARTICLE_THREAD(B)                                                 \
     const unsigned lb = xcd_block();                                       \
-    const int sid = scene_of((B).pblk, (B).ns, lb);                        \
+    const int sk_ = scene_of((B).pblk, (B).ns, lb);                        \
+    const int sid = (B).smap ? (int)(B).smap[sk_] : sk_;                   \
     const SceneView &v = (B).views[sid];                                   \
-    const size_t i = (size_t)(lb - (B).pblk[sid]) * kBlock + threadIdx.x;  \
+    const size_t i = (size_t)(lb - (B).pblk[sk_]) * kBlock + threadIdx.x;  \
     const bool live = i < v.n;                                             \
     const size_t gi = (size_t)v.part_base + i;                             \
     (void)gi;                                                              \
@@ -522,7 +523,7 @@ template <bool CACHED, int UNROLL, int WAVES>
 __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror) {
     __shared__ double lds[6 * kBlock];
     PARTICLE_THREAD(B);
-    if (target_scratch && !v.active) return;  // uniform per workgroup
+    if (target_scratch && !B.active[sid]) return;  // uniform per workgroup
     const KernelConst K = v.K;
     const double gamma = v.S.gamma;
     double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0;
@@ -696,7 +697,7 @@ template <bool CACHED, int UNROLL, int WAVES>
 __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
     __shared__ double lds[kBlock];
     PARTICLE_THREAD(B);
-    if (!v.active) return;  // uniform per workgroup
+    if (!B.active[sid]) return;  // uniform per workgroup
     const KernelConst K = v.K;
     const SolveConst S = v.S;
     double err = 0.;
@@ -806,6 +807,32 @@ __global__ void kb_fold_sum(const double *partials, const uint32_t *blk, int ns,
     double v[1] = {acc};
     block_sum<1>(v, lds);
     if (threadIdx.x == 0) out[(size_t)s * rows + r] = v[0];
+}
+
+__global__ void kb_fold_converge(BatchDev B, int after, int k) {
+    __shared__ double lds[kBlock];
+    const int sk = blockIdx.x;
+    const int s = B.smap ? (int)B.smap[sk] : sk;
+    double acc = 0.0;  // the shape of kb_fold_sum with one row
+    for (uint32_t q = B.pblk[sk] + threadIdx.x; q < B.pblk[sk + 1]; q += kBlock)
+        acc += B.partials[q];
+    double r[1] = {acc};
+    block_sum<1>(r, lds);
+    if (threadIdx.x == 0) {
+        const size_t slot = (size_t)k * B.ns_all + s;
+        if (B.active[s]) {
+            const SceneView &v = B.views[s];
+            const double err = r[0] / (double)v.n * 100.0;
+            const uint32_t it = B.iters[s] + 1;
+            B.iters[s] = it;
+            B.chunk_err[slot] = err;
+            B.chunk_ran[slot] = 1;
+            const bool done = after >= 0 ? it > (uint32_t)after : (it > 2 && err < v.target_err);
+            if (done) B.active[s] = 0;
+        } else {
+            B.chunk_ran[slot] = 0;
+        }
+    }
 }
 
 __global__ void kb_fold_speed(const double *partials, const uint32_t *blk, int ns, double *out) {

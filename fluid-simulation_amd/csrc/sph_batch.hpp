@@ -27,7 +27,6 @@ struct SceneView {
     uint32_t cell_base;   // first cell in the batch-global cell table
     uint32_t body_base;   // first body in the batch body table
     uint32_t nb;          // bodies of this scene
-    int active;           // pressure solve still iterating
     int integrator;       // SPH_INTEGRATOR_*
     int reach;
     int gravity_mode;
@@ -35,6 +34,7 @@ struct SceneView {
     SolveConst S;         // dt, V0, min diagonal, omega, gamma
     double V0, cs, dthr, dxr, vmax_ball, disable_below, stiffness;
     double nu_f, nu_b, eps2;
+    double target_err;    // target average density error (percent)
     double gcenter[3];
     GridDev grid;         // occupied-cell box; start = batch table + cell_base
     double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *m, *V, *p;
@@ -50,6 +50,10 @@ struct BatchDev {
     const uint32_t *pblk;  // workgroup prefix over particles (ns + 1)
     const uint32_t *cblk;  // workgroup prefix over cells (ns + 1)
     int ns;
+    // a launch may cover a subset of the scenes: entry k of pblk then belongs
+    // to scene smap[k] (smap == nullptr: all scenes, in order)
+    const uint32_t *smap;
+    int ns_all;
     int *ci;               // 3 per particle
     uint32_t *perm;
     uint32_t *cnt;
@@ -66,6 +70,12 @@ struct BatchDev {
     uint32_t *cursor;
     const BodyDev *bodies;
     double *partials;      // per workgroup
+    // pressure solve state per scene, advanced on the device: a scene leaves
+    // the Jacobi loop on its own termination test
+    uint32_t *active;      // still iterating
+    uint32_t *iters;       // iterations done
+    double *chunk_err;     // [k][scene] predicted density error of iteration k of a chunk
+    uint32_t *chunk_ran;   // [k][scene] 1 if the scene ran iteration k of the chunk
 };
 
 // Pair passes take CACHED = true to read grad W from the pair cache (written by
@@ -105,6 +115,9 @@ __global__ void kb_pack_volume(BatchDev B);
 // scene's workgroups of partials[lb*rows + r]
 __global__ void kb_fold_sum(const double *partials, const uint32_t *blk, int ns, int rows,
                             double *out);
+// fold of the Jacobi error partials per scene + the termination test
+// (iisph.rs:19-22, 386-393, 530-531): iteration k of the current chunk
+__global__ void kb_fold_converge(BatchDev B, int after, int k);
 __global__ void kb_fold_speed(const double *partials, const uint32_t *blk, int ns, double *out);
 __global__ void kb_fold_bbox(const int *partials, const uint32_t *blk, int ns, int *out);
 

@@ -415,7 +415,6 @@ struct Scene {
         v.cell_base = cell_base;
         v.body_base = body_base;
         v.nb = (uint32_t)bodies.size();
-        v.active = active ? 1 : 0;
         v.integrator = P.integrator;
         v.reach = reach;
         v.gravity_mode = P.gravity_mode;
@@ -436,6 +435,7 @@ struct Scene {
         v.nu_f = P.fluid_viscosity;
         v.nu_b = P.boundary_viscosity;
         v.eps2 = 0.01 * (P.rest_density_grid_spacing * P.rest_density_grid_spacing);
+        v.target_err = P.target_density_error;
         for (int k = 0; k < 3; k++) v.gcenter[k] = P.gravity_center[k];
         v.grid = grid;
         v.grid.start = cell_start ? cell_start + cell_base : nullptr;
@@ -453,6 +453,8 @@ struct Scene {
         return v;
     }
 };
+
+constexpr int kMaxChunk = 16;  // Jacobi iterations launched between host waits
 
 // ------------------------------------------------------------------------
 // online tuning of the pair passes' code shapes
@@ -497,7 +499,17 @@ struct Batch {
     DBuf<int> ci, ipart;
     DBuf<uint32_t> perm, cnt, slice_len, slice_off, ell_j, cell_cnt, cell_start, cursor;
     DBuf<uint32_t> pblk, cblk, scan_a, scan_b, scan_c;
-    DBuf<double> ell_gx, ell_gy, ell_gz, partials, red;
+    DBuf<double> ell_gx, ell_gy, ell_gz, partials, red, chunk_err;
+    DBuf<uint32_t> d_active, d_iters, chunk_ran, ablk, amap;
+    uint32_t iter_hint = 3;  // iterations of the last target-terminated solve
+    // Jacobi chunks captured as HIP graphs, keyed by their launch arguments
+    bool use_graphs = true;
+    struct ChunkGraph {
+        std::string key;
+        hipGraphExec_t exec;
+    };
+    std::vector<ChunkGraph> graphs;  // most recently used last
+    std::vector<std::string> seen;   // keys launched once without a graph
     DBuf<SceneView> views;
     DBuf<BodyDev> bodies;
     // utility scratch (standalone queries and static samples)
@@ -556,8 +568,14 @@ struct Batch {
         ell_gx.release(); ell_gy.release(); ell_gz.release(); partials.release();
         ucnt.release(); ucursor.release();
     }
+    void drop_graphs() {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+        seen.clear();
+    }
     ~Batch() {
         if (s) (void)hipStreamSynchronize(s);
+        drop_graphs();
         if (pinned) (void)hipHostFree(pinned);
         if (arena) (void)hipHostFree(arena);
         if (s) (void)hipStreamDestroy(s);
@@ -651,18 +669,24 @@ static const KernB k_jacobi_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(JA_F)},
 // gradient mode and the pass's code shape; timed under `pid` when profiling
 // and under `pass` while tuning
 template <class Tab, class... A>
-static void pair_pass(Batch &B, int pass, int pid, const Tab &tab, A... args) {
-    if (B.total_pblocks == 0) return;
+static void pair_pass_grid(Batch &B, size_t grid, int pass, int pid, const Tab &tab, A... args) {
+    if (grid == 0) return;
     auto kern = tab[B.grad_cache ? 1 : 0][B.variant[pass]];
     Prof::Rec r{pid, B.total_particles, nullptr, nullptr};
     Prof::Rec t{pass, B.total_particles, nullptr, nullptr};
     const bool timed = B.prof.on && pid < SPH_PROF_KERNELS, tuning = B.tuner.measuring();
     if (timed) B.prof.begin(B.s, r);
     if (tuning) B.tprof.begin(B.s, t);
-    hipLaunchKernelGGL(kern, dim3((unsigned)B.total_pblocks), dim3(kBlock), 0, B.s, args...);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kBlock), 0, B.s, args...);
     HIPCK(hipGetLastError());
     if (tuning) B.tprof.end(B.s, t);
     if (timed) B.prof.end(B.s, r);
+}
+
+// pair pass over every particle of the batch
+template <class Tab, class... A>
+static void pair_pass(Batch &B, int pass, int pid, const Tab &tab, A... args) {
+    pair_pass_grid(B, B.total_pblocks, pass, pid, tab, args...);
 }
 
 static void tune_begin_step(Batch &B, size_t particles) {
@@ -928,6 +952,8 @@ static BatchDev batch_dev(Batch &B) {
     d.pblk = B.pblk.p;
     d.cblk = B.cblk.p;
     d.ns = (int)B.sc.size();
+    d.smap = nullptr;
+    d.ns_all = (int)B.sc.size();
     d.ci = B.ci.p;
     d.perm = B.perm.p;
     d.cnt = B.cnt.p;
@@ -942,6 +968,10 @@ static BatchDev batch_dev(Batch &B) {
     d.cursor = B.cursor.p;
     d.bodies = B.bodies.p;
     d.partials = B.partials.p;
+    d.active = B.d_active.p;
+    d.iters = B.d_iters.p;
+    d.chunk_err = B.chunk_err.p;
+    d.chunk_ran = B.chunk_ran.p;
     return d;
 }
 
@@ -977,7 +1007,11 @@ static void layout(Batch &B) {
     B.slice_off.reserve(slices + 1);
     B.partials.reserve((blocks + 1) * kMaxBodiesPerScene * 6 + 64);
     B.red.reserve(std::max<size_t>(ns, 1) * kMaxBodiesPerScene * 6 + 64);
-    B.ensure_pinned(std::max<size_t>(ns, 1) * kMaxBodiesPerScene * 6 + 64);
+    B.ensure_pinned(std::max<size_t>(ns, 1) * (kMaxBodiesPerScene * 6 + 2 * kMaxChunk + 2) + 64);
+    B.d_active.reserve(std::max<size_t>(ns, 1));
+    B.d_iters.reserve(std::max<size_t>(ns, 1));
+    B.chunk_err.reserve(std::max<size_t>(ns, 1) * kMaxChunk);
+    B.chunk_ran.reserve(std::max<size_t>(ns, 1) * kMaxChunk);
     upload_bodies(B);
     upload_views(B);
 }
@@ -1225,64 +1259,153 @@ static void pressure_accel(Batch &B, int overwrite) {
     apply_body_partials(B);
 }
 
+// `chunk` Jacobi iterations of every active scene: pressure acceleration into
+// the scratch, the Jacobi update, and the per-scene fold + termination test
+// (d covers the scenes still iterating: d.ns of them, `grid` workgroups)
+static void jacobi_chunk_launches(Batch &B, const BatchDev &d, size_t grid, int after,
+                                  int chunk) {
+    for (int k = 0; k < chunk; k++) {
+        pair_pass_grid(B, grid, PP_PRESSURE_ACCEL, SPH_PROF_PRESSURE_ACCEL, k_pressure_accel_tab,
+                       d, 1, 1, 0);
+        pair_pass_grid(B, grid, PP_JACOBI, SPH_PROF_JACOBI, k_jacobi_tab, d);
+        hipLaunchKernelGGL(kb_fold_converge, dim3((unsigned)d.ns), dim3(kBlock), 0, B.s, d,
+                           after, k);
+        HIPCK(hipGetLastError());
+    }
+}
+
+// A chunk whose launch arguments repeat (the same layout, buffers, code shapes
+// and chunk length) is replayed from a HIP graph: one submission instead of
+// three per iteration. A key is captured on its second use; timing and tuning
+// runs launch directly.
+static void jacobi_chunk(Batch &B, const BatchDev &d, size_t grid, int after, int chunk) {
+    if (!B.use_graphs || B.prof.on || B.tuner.measuring()) {
+        jacobi_chunk_launches(B, d, grid, after, chunk);
+        return;
+    }
+    std::string key(reinterpret_cast<const char *>(&d), sizeof d);
+    const int meta[6] = {after, chunk, (int)grid, B.grad_cache ? 1 : 0,
+                         B.variant[PP_PRESSURE_ACCEL], B.variant[PP_JACOBI]};
+    key.append(reinterpret_cast<const char *>(meta), sizeof meta);
+    for (size_t g = 0; g < B.graphs.size(); g++)
+        if (B.graphs[g].key == key) {
+            Batch::ChunkGraph hit = B.graphs[g];
+            B.graphs.erase(B.graphs.begin() + g);
+            B.graphs.push_back(hit);
+            HIPCK(hipGraphLaunch(hit.exec, B.s));
+            return;
+        }
+    if (std::find(B.seen.begin(), B.seen.end(), key) == B.seen.end()) {
+        B.seen.push_back(key);
+        if (B.seen.size() > 16) B.seen.erase(B.seen.begin());
+        jacobi_chunk_launches(B, d, grid, after, chunk);
+        return;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCK(hipStreamBeginCapture(B.s, hipStreamCaptureModeThreadLocal));
+    try {
+        jacobi_chunk_launches(B, d, grid, after, chunk);
+    } catch (...) {
+        (void)hipStreamEndCapture(B.s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPCK(hipStreamEndCapture(B.s, &graph));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCK(e);
+    if (B.graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(B.graphs.front().exec);
+        B.graphs.erase(B.graphs.begin());
+    }
+    B.graphs.push_back({key, exec});
+    HIPCK(hipGraphLaunch(exec, B.s));
+}
+
 // IISPH::resolve_pressure_globally (iisph.rs:378-560) for every scene at once;
-// after < 0: target density error, else AfterIteration(after). Each scene
-// leaves the loop on its own termination test; finished scenes' workgroups
-// return at once.
+// after < 0: target density error, else AfterIteration(after). Iterations are
+// launched in chunks; after each iteration kb_fold_converge applies every
+// scene's own termination test on the device, and the iterations of finished
+// scenes return at once, so a chunk may run past a scene's last iteration
+// without changing it. The host waits once per chunk: the first chunk covers
+// the iterations the solve needs at least (after + 1, or 3 and the count of
+// the last target-terminated solve), later chunks two each.
 static void resolve_pressure(Batch &B, int vp, int after) {
     const size_t ns = B.sc.size();
     BatchDev d = batch_dev(B);
     pair_pass(B, PP_SOURCE_DIAG, SPH_PROF_SOURCE_DIAG, k_source_diag_tab, d, vp);
-    for (Scene *c : B.sc) {
-        c->iter = 0;
-        c->err = INFINITY;
-        c->err_hist.clear();
-        c->active = c->n > 0;
+    std::vector<uint32_t> act(ns), zero(ns, 0u);
+    bool any = false;
+    for (size_t s = 0; s < ns; s++) {
+        Scene &c = *B.sc[s];
+        c.iter = 0;
+        c.err = INFINITY;
+        c.err_hist.clear();
+        c.active = c.n > 0;
+        act[s] = c.active ? 1u : 0u;
+        any = any || c.active;
     }
-    bool dirty = true;
-    for (;;) {
-        bool any = false;
-        for (Scene *c : B.sc) {
-            if (!c->active) continue;
-            bool conv = after >= 0 ? (c->iter > (uint32_t)after)
-                                   : (c->iter > 2 && c->err < c->P.target_density_error);
-            if (conv) {
-                c->active = false;
-                dirty = true;
-                continue;
+    B.put(B.d_active.p, act.data(), ns * sizeof(uint32_t));
+    B.put(B.d_iters.p, zero.data(), ns * sizeof(uint32_t));
+    int chunk = after >= 0 ? after + 1 : std::max<int>(3, (int)B.iter_hint);
+    uint32_t most = 0;
+    std::vector<uint32_t> ablk, amap;
+    while (any) {
+        chunk = std::min(std::max(chunk, 1), kMaxChunk);
+        // launch over the scenes still iterating only
+        ablk.assign(1, 0u);
+        amap.clear();
+        for (size_t s = 0; s < ns; s++)
+            if (B.sc[s]->active) {
+                amap.push_back((uint32_t)s);
+                ablk.push_back(ablk.back() + (B.hp_blk[s + 1] - B.hp_blk[s]));
             }
-            if (c->iter >= 100000u)
-                throw Error(SPH_ERR_UNSUPPORTED, "pressure solve did not converge");
-            any = true;
+        BatchDev da = d;
+        if (amap.size() < ns) {
+            B.ablk.reserve(ns + 1);
+            B.amap.reserve(ns);
+            B.put(B.ablk.p, ablk.data(), ablk.size() * sizeof(uint32_t));
+            B.put(B.amap.p, amap.data(), amap.size() * sizeof(uint32_t));
+            da.pblk = B.ablk.p;
+            da.smap = B.amap.p;
+            da.ns = (int)amap.size();
         }
-        if (!any) break;
-        if (dirty) {
-            upload_views(B);
-            dirty = false;
-        }
-        pair_pass(B, PP_PRESSURE_ACCEL, SPH_PROF_PRESSURE_ACCEL, k_pressure_accel_tab, d, 1, 1,
-                  0);
-        pair_pass(B, PP_JACOBI, SPH_PROF_JACOBI, k_jacobi_tab, d);
-        hipLaunchKernelGGL(kb_fold_sum, dim3((unsigned)ns), dim3(kBlock), 0, B.s, B.partials.p,
-                           B.pblk.p, (int)ns, 1, B.red.p);
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(B.pinned, B.red.p, ns * sizeof(double), hipMemcpyDeviceToHost, B.s));
+        jacobi_chunk(B, da, ablk.back(), after, chunk);
+        double *errs = B.pinned;
+        uint32_t *ran = reinterpret_cast<uint32_t *>(errs + (size_t)chunk * ns);
+        uint32_t *alive = ran + (size_t)chunk * ns;
+        HIPCK(hipMemcpyAsync(errs, B.chunk_err.p, (size_t)chunk * ns * sizeof(double),
+                             hipMemcpyDeviceToHost, B.s));
+        HIPCK(hipMemcpyAsync(ran, B.chunk_ran.p, (size_t)chunk * ns * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, B.s));
+        HIPCK(hipMemcpyAsync(alive, B.d_active.p, ns * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             B.s));
         B.sync();
         if (B.prof.on) B.prof.drain();
+        any = false;
         for (size_t s = 0; s < ns; s++) {
             Scene &c = *B.sc[s];
-            if (!c.active) continue;
-            c.err = B.pinned[s] / (double)c.n * 100.0;
-            c.err_hist.push_back(c.err);
-            c.iter++;
+            for (int k = 0; k < chunk; k++)
+                if (ran[(size_t)k * ns + s]) {
+                    c.err = errs[(size_t)k * ns + s];
+                    c.err_hist.push_back(c.err);
+                    c.iter++;
+                }
+            c.active = alive[s] != 0;
+            most = std::max(most, c.iter);
+            if (c.active && c.iter >= 100000u)
+                throw Error(SPH_ERR_UNSUPPORTED, "pressure solve did not converge");
+            any = any || c.active;
         }
+        chunk = 2;
     }
+    if (after < 0 && most > 0) B.iter_hint = most;
     for (Scene *c : B.sc) {
         c->last_iter = c->iter;
         c->last_err = c->n ? c->err : 0.0;
         c->active = false;
     }
-    upload_views(B);
 }
 
 // PressureSolver::solve_and_add_acceleration for every scene (one solver per batch)
@@ -2179,6 +2302,9 @@ static void set_option(Batch &B, int option, int value) {
         if (value < 0) throw Error(SPH_ERR_ARG, "steps per shape must be >= 0");
         B.tuner.steps_per_shape = value;
         B.tuner.state = 0;
+    } else if (option == SPH_OPT_GRAPHS) {
+        B.use_graphs = value != 0;
+        if (!B.use_graphs) B.drop_graphs();
     } else if (option == SPH_OPT_SHAPE) {
         if (value < 0 || value >= kPairVariants) throw Error(SPH_ERR_ARG, "no such code shape");
         for (int p = 0; p < kPairPasses; p++) B.variant[p] = value;
@@ -2193,6 +2319,7 @@ static int get_option(const Batch &B, int option) {
     if (option == SPH_OPT_CACHE_GRADIENTS) return B.grad_cache ? 1 : 0;
     if (option == SPH_OPT_AUTOTUNE) return B.tuner.steps_per_shape;
     if (option == SPH_OPT_TUNE_STATE) return B.tuner.state;
+    if (option == SPH_OPT_GRAPHS) return B.use_graphs ? 1 : 0;
     if (option >= SPH_OPT_SHAPE_VOLUME && option < SPH_OPT_SHAPE_VOLUME + kPairPasses)
         return B.variant[option - SPH_OPT_SHAPE_VOLUME];
     throw Error(SPH_ERR_ARG, "unknown option");

@@ -251,6 +251,8 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *   SPH_OPT_SHAPE_VOLUME .. SPH_OPT_SHAPE_VOLUME+4  (get) the code shape of the
  *       volume, non-pressure, source/diagonal, pressure-acceleration and
  *       Jacobi passes.
+ *   SPH_OPT_GRAPHS  1 = replay repeated Jacobi iteration chunks from HIP
+ *       graphs (default), 0 = launch every kernel directly.
  * The options of a context apply when it steps alone, those of a world to
  * the world's steps. */
 enum {
@@ -258,7 +260,8 @@ enum {
     SPH_OPT_AUTOTUNE = 1,
     SPH_OPT_SHAPE = 2,
     SPH_OPT_TUNE_STATE = 3,
-    SPH_OPT_SHAPE_VOLUME = 4
+    SPH_OPT_SHAPE_VOLUME = 4,
+    SPH_OPT_GRAPHS = 9
 };
 enum { SPH_SHAPES = 6 };
 int sph_set_option(sph_ctx *ctx, int option, int value);

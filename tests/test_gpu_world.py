@@ -157,3 +157,32 @@ def test_code_shapes_bit_identical_and_tuner(native):
     assert all(0 <= t[p] < N.SHAPES for p in N.PAIR_PASSES)
     tuned.step_forward_in_time(2)
     assert_same(ref, tuned)
+
+
+def test_graph_replay_bit_identical(native):
+    """Jacobi chunks replayed from HIP graphs give the eager launches' bits on
+    every replay (graphs are captured on a key's second use and then replayed
+    every step, with eager work and host waits in between)."""
+    from sph_amd import _native as N
+    for solver in ("IISPH", "IISPHwOST"):
+        sc = scenes.small_iisph_box(side=10, seed=5, solver=solver, jitter=2e-3)
+        eager, graph = native.System(sc), native.System(sc)
+        for s in (eager, graph):
+            s.set_option(N.OPT_SHAPE, 1)      # fixed shapes: keys repeat from step 1
+        eager.set_option(N.OPT_GRAPHS, 0)
+        assert graph.get_option(N.OPT_GRAPHS) == 1
+        for _ in range(8):
+            eager.step_forward_in_time(1)
+            graph.step_forward_in_time(1)
+            assert_same(eager, graph)
+    w = native.World([native.System(scenes.small_iisph_box(side=9, seed=k, jitter=2e-3))
+                      for k in (1, 2)])
+    solo = [native.System(scenes.small_iisph_box(side=9, seed=k, jitter=2e-3)) for k in (1, 2)]
+    for s in solo:
+        s.set_option(N.OPT_GRAPHS, 0)
+    for _ in range(6):
+        w.step(1)
+        for s in solo:
+            s.step_forward_in_time(1)
+    for a, b in zip(solo, w.members):
+        assert_same(a, b)
