@@ -67,15 +67,15 @@ __device__ __forceinline__ void st4(double *a, size_t i, double x, double y, dou
 // grad W of pair slot e: read from the pair cache (CACHED) or recomputed from
 // r = x_i - x_j with the same expression (bit-identical either way)
 template <bool CACHED>
-__device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &K, size_t e,
-                                          double rx, double ry, double rz, double &wx,
+__device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &K, double rh,
+                                          size_t e, double rx, double ry, double rz, double &wx,
                                           double &wy, double &wz) {
     if (CACHED) {
         wx = gm(B.ell_gx)[e];
         wy = gm(B.ell_gy)[e];
         wz = gm(B.ell_gz)[e];
     } else {
-        kernel_grad(K, rx, ry, rz, wx, wy, wz);
+        kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
     }
 }
 
@@ -355,6 +355,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pairs_volume(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
     const KernelConst K = v.K;
+    const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const rec4 qi = ld4(v.q4, i);
     const double px = qi.x, py = qi.y, pz = qi.z;
     const uint32_t c = gm(B.cnt)[gi];
@@ -366,14 +367,15 @@ __global__ PAIR_PASS_BOUNDS void kb_pairs_volume(BatchDev B) {
         const rec4 qj = ld4(v.q4, j);  // .w is being written by this pass; unused here
         double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
         double rn = sqrt((rx * rx + ry * ry) + rz * rz);
-        double q = rn / K.h;
+        double q = div_rcp(rn, K.h, rh);
         accu += v.V0 * (K.wfac * kernel_w(q));
-        if (CACHED) {  // the expression of kernel_grad
+        if (CACHED) {  // the expression of kernel_grad_rh
             const double d = kernel_dqw(q);
             const double rs = rn != 0.0 ? rn : 1.0;
-            const double wx = ((K.gfac * rx) / rs) * d;
-            const double wy = ((K.gfac * ry) / rs) * d;
-            const double wz = ((K.gfac * rz) / rs) * d;
+            const double yr = refined_rcp(rs);
+            const double wx = div_rcp(K.gfac * rx, rs, yr) * d;
+            const double wy = div_rcp(K.gfac * ry, rs, yr) * d;
+            const double wz = div_rcp(K.gfac * rz, rs, yr) * d;
             gm(B.ell_gx)[e] = wx;
             gm(B.ell_gy)[e] = wy;
             gm(B.ell_gz)[e] = wz;
@@ -410,6 +412,7 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
     __shared__ double lds[6 * kBlock];
     PARTICLE_THREAD(B);
     const KernelConst K = v.K;
+    const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     double px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, mi = 0;
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double acx = 0., acy = 0., acz = 0.;
@@ -440,7 +443,7 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
                         dot3(ux - gm(v.vx)[j], uy - gm(v.vy)[j], uz - gm(v.vz)[j], rx, ry, rz) /
                         (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
             double wx, wy, wz;
-            pair_grad<CACHED>(B, K, e, rx, ry, rz, wx, wy, wz);
+            pair_grad<CACHED>(B, K, rh, e, rx, ry, rz, wx, wy, wz);
             acx += cf * wx;
             acy += cf * wy;
             acz += cf * wz;
@@ -455,7 +458,7 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
                     uint32_t s = gm(b.sidx)[e];
                     double rx = px - gm(b.sx)[s], ry = py - gm(b.sy)[s], rz = pz - gm(b.sz)[s];
                     double wx, wy, wz;
-                    kernel_grad(K, rx, ry, rz, wx, wy, wz);
+                    kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
                     // static samples have zero velocity
                     double cf = v.nu_b * 2. * (3. + 2.) * gm(b.svol)[s] *
                                 dot3(ux - 0.0, uy - 0.0, uz - 0.0, rx, ry, rz) /
@@ -474,7 +477,7 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
                     body_velocity(b, sxk, syk, szk, svx, svy, svz);
                     double rx = px - sxk, ry = py - syk, rz = pz - szk;
                     double wx, wy, wz;
-                    kernel_grad(K, rx, ry, rz, wx, wy, wz);
+                    kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
                     double cf = v.nu_b * 2. * (3. + 2.) * vv *
                                 dot3(ux - svx, uy - svy, uz - svz, rx, ry, rz) /
                                 (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
@@ -525,6 +528,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
     PARTICLE_THREAD(B);
     if (target_scratch && !B.active[sid]) return;  // uniform per workgroup
     const KernelConst K = v.K;
+    const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const double gamma = v.S.gamma;
     double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0;
     double acx = 0., acy = 0., acz = 0.;
@@ -550,7 +554,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
                 const double p1 = gm(v.p)[j1];
                 double cf = vm * q0.w * (pi + p0);
                 double wx, wy, wz;
-                pair_grad<CACHED>(B, K, e0 + 64 * (size_t)k, px - q0.x, py - q0.y, pz - q0.z,
+                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - q0.x, py - q0.y, pz - q0.z,
                                   wx, wy, wz);
                 acx -= cf * wx;
                 acy -= cf * wy;
@@ -569,7 +573,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
                 for (uint32_t e = gm(b.soff)[i]; e < gm(b.soff)[i + 1]; e++) {
                     uint32_t s = gm(b.sidx)[e];
                     double wx, wy, wz;
-                    kernel_grad(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
+                    kernel_grad_rh(K, rh, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
                     double cf = 2. * gamma * Vi * gm(b.svol)[s] * pi;
                     acx -= (cf * wx) / mi;
                     acy -= (cf * wy) / mi;
@@ -578,7 +582,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
             } else if (gm(b.flag)[i]) {
                 double sxk = gm(b.bx)[i], syk = gm(b.by)[i], szk = gm(b.bz)[i];
                 double wx, wy, wz;
-                kernel_grad(K, px - sxk, py - syk, pz - szk, wx, wy, wz);
+                kernel_grad_rh(K, rh, px - sxk, py - syk, pz - szk, wx, wy, wz);
                 double cf = 2. * gamma * Vi * gm(b.bvol)[i] * pi;
                 double fx = cf * wx, fy = cf * wy, fz = cf * wz;
                 if (mirror && b.dynamic) {
@@ -636,6 +640,7 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
     PARTICLE_THREAD(B);
     if (!live) return;
     const KernelConst K = v.K;
+    const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const SolveConst S = v.S;
     const rec4 qi = ld4(v.q4, i);
     const double px = qi.x, py = qi.y, pz = qi.z;
@@ -651,7 +656,7 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
         uint32_t j = gm(B.ell_j)[e];
         const rec4 qj = ld4(v.q4, j);
         double wx, wy, wz;
-        pair_grad<CACHED>(B, K, e, px - qj.x, py - qj.y, pz - qj.z, wx, wy, wz);
+        pair_grad<CACHED>(B, K, rh, e, px - qj.x, py - qj.y, pz - qj.z, wx, wy, wz);
         double Vj = qj.w;
         src -= dt * Vj * dot3(ux - gm(v.svx)[j], uy - gm(v.svy)[j], uz - gm(v.svz)[j], wx, wy, wz);
         sfx += Vj * wx;
@@ -665,7 +670,7 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
             for (uint32_t k = gm(b.soff)[i]; k < gm(b.soff)[i + 1]; k++) {
                 uint32_t s = gm(b.sidx)[k];
                 double wx, wy, wz;
-                kernel_grad(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
+                kernel_grad_rh(K, rh, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
                 double Vb = gm(b.svol)[s];
                 src -= dt * Vb * dot3(ux - 0.0, uy - 0.0, uz - 0.0, wx, wy, wz);
                 sbx += Vb * wx;
@@ -674,7 +679,7 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
             }
         } else if (gm(b.flag)[i]) {
             double wx, wy, wz;
-            kernel_grad(K, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i], wx, wy, wz);
+            kernel_grad_rh(K, rh, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i], wx, wy, wz);
             double Vb = gm(b.bvol)[i];
             src -= dt * Vb * dot3(ux - gm(b.bvx)[i], uy - gm(b.bvy)[i], uz - gm(b.bvz)[i], wx, wy, wz);
             sbx += Vb * wx;
@@ -699,6 +704,7 @@ __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!B.active[sid]) return;  // uniform per workgroup
     const KernelConst K = v.K;
+    const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const SolveConst S = v.S;
     double err = 0.;
     if (live) {
@@ -720,7 +726,7 @@ __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
                 const uint32_t j2 = ej[64 * (size_t)min(k + 2, c - 1)];
                 const rec4 q1 = ld4(v.q4, j1), a1 = ld4(v.pa4, j1);
                 double wx, wy, wz;
-                pair_grad<CACHED>(B, K, e0 + 64 * (size_t)k, px - q0.x, py - q0.y, pz - q0.z,
+                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - q0.x, py - q0.y, pz - q0.z,
                                   wx, wy, wz);
                 adp += dt2 * q0.w * dot3(ux - a0.x, uy - a0.y, uz - a0.z, wx, wy, wz);
                 j1 = j2;
@@ -734,12 +740,12 @@ __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
                 for (uint32_t k = gm(b.soff)[i]; k < gm(b.soff)[i + 1]; k++) {
                     uint32_t s = gm(b.sidx)[k];
                     double wx, wy, wz;
-                    kernel_grad(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
+                    kernel_grad_rh(K, rh, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
                     adp += dt2 * gm(b.svol)[s] * dot3(ux, uy, uz, wx, wy, wz);
                 }
             } else if (gm(b.flag)[i]) {
                 double wx, wy, wz;
-                kernel_grad(K, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i], wx, wy, wz);
+                kernel_grad_rh(K, rh, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i], wx, wy, wz);
                 adp += dt2 * gm(b.bvol)[i] * dot3(ux, uy, uz, wx, wy, wz);
             }
         }

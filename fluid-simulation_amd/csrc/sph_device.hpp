@@ -86,6 +86,43 @@ __device__ __forceinline__ double kernel_fn(const KernelConst &K, double rx, dou
     return K.wfac * kernel_w(rn / K.h);
 }
 
+// Binary64 quotients through a shared reciprocal. refined_rcp(b) and
+// div_rcp(a, b, y) perform the operation sequence of the compiler's double
+// division (reciprocal estimate, two Newton steps, quotient with one FMA
+// residual correction; sign of a zero quotient from a * y) without its
+// exponent pre-scaling and special-case fix-up, which are the identity for
+// finite nonzero b with |b| and |1/b| normal and a zero or |a| > 2^-960 with
+// a/b normal: the operands of the kernel gradient (|r| <= h, h > 0, and
+// gfac r). For those the result is the correctly rounded a / b, so one
+// refined reciprocal serves several numerators.
+__device__ __forceinline__ double refined_rcp(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+__device__ __forceinline__ double div_rcp(double a, double b, double y) {
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    const double res = __builtin_fma(r, y, q);
+    return a == 0.0 ? q : res;
+}
+
+// grad W with 1/h refined once per thread (rh = refined_rcp(K.h)): the
+// expression of kernel_grad below, bit for bit
+__device__ __forceinline__ void kernel_grad_rh(const KernelConst &K, double rh, double rx,
+                                               double ry, double rz, double &gx, double &gy,
+                                               double &gz) {
+    const double rn = sqrt(dot3(rx, ry, rz, rx, ry, rz));
+    const double d = kernel_dqw(div_rcp(rn, K.h, rh));
+    const double rs = rn != 0.0 ? rn : 1.0;
+    const double yr = refined_rcp(rs);
+    gx = div_rcp(K.gfac * rx, rs, yr) * d;
+    gy = div_rcp(K.gfac * ry, rs, yr) * d;
+    gz = div_rcp(K.gfac * rz, rs, yr) * d;
+}
+
 // grad W(r, h) (kernel/mod.rs:37-45): ((gfac r) / |r|) w'(|r|/h), zero at
 // r = 0. Branch-free, so that neighbor loads can be scheduled across it: at
 // |r| = 0 the divisor is replaced by 1 and w'(0) = -3 + 12/4 is exactly 0, so
