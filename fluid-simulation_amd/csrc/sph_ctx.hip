@@ -454,7 +454,8 @@ struct Scene {
     }
 };
 
-constexpr int kMaxChunk = 16;  // Jacobi iterations launched between host waits
+constexpr int kMaxChunk = 16;      // Jacobi iterations launched between host waits
+constexpr size_t kMaxGraphs = 16;  // executable Jacobi-chunk graphs per batch
 
 // ------------------------------------------------------------------------
 // online tuning of the pair passes' code shapes
@@ -502,7 +503,9 @@ struct Batch {
     DBuf<double> ell_gx, ell_gy, ell_gz, partials, red, chunk_err;
     DBuf<uint32_t> d_active, d_iters, chunk_ran, ablk, amap;
     uint32_t iter_hint = 3;  // iterations of the last target-terminated solve
-    // Jacobi chunks captured as HIP graphs, keyed by their launch arguments
+    // Jacobi chunks captured as HIP graphs, keyed by their launch arguments (on
+    // by default for a context, off for a world, whose layout changes with
+    // every admission)
     bool use_graphs = true;
     struct ChunkGraph {
         std::string key;
@@ -1277,7 +1280,9 @@ static void jacobi_chunk_launches(Batch &B, const BatchDev &d, size_t grid, int 
 // A chunk whose launch arguments repeat (the same layout, buffers, code shapes
 // and chunk length) is replayed from a HIP graph: one submission instead of
 // three per iteration. A key is captured on its second use; timing and tuning
-// runs launch directly.
+// runs launch directly. Executable graphs live until the batch ends or graphs
+// are switched off (none is destroyed while stepping); once kMaxGraphs exist,
+// new keys launch directly.
 static void jacobi_chunk(Batch &B, const BatchDev &d, size_t grid, int after, int chunk) {
     if (!B.use_graphs || B.prof.on || B.tuner.measuring()) {
         jacobi_chunk_launches(B, d, grid, after, chunk);
@@ -1295,7 +1300,8 @@ static void jacobi_chunk(Batch &B, const BatchDev &d, size_t grid, int after, in
             HIPCK(hipGraphLaunch(hit.exec, B.s));
             return;
         }
-    if (std::find(B.seen.begin(), B.seen.end(), key) == B.seen.end()) {
+    if (B.graphs.size() >= kMaxGraphs ||
+        std::find(B.seen.begin(), B.seen.end(), key) == B.seen.end()) {
         B.seen.push_back(key);
         if (B.seen.size() > 16) B.seen.erase(B.seen.begin());
         jacobi_chunk_launches(B, d, grid, after, chunk);
@@ -1315,10 +1321,6 @@ static void jacobi_chunk(Batch &B, const BatchDev &d, size_t grid, int after, in
     hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIPCK(e);
-    if (B.graphs.size() >= 8) {
-        (void)hipGraphExecDestroy(B.graphs.front().exec);
-        B.graphs.erase(B.graphs.begin());
-    }
     B.graphs.push_back({key, exec});
     HIPCK(hipGraphLaunch(exec, B.s));
 }
@@ -2213,6 +2215,7 @@ int sph_world_create(sph_world **out) {
         check_device();
         std::unique_ptr<sph_world> w(new sph_world());
         w->batch.init();
+        w->batch.use_graphs = false;
         *out = w.release();
     });
 }

@@ -252,7 +252,8 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *       volume, non-pressure, source/diagonal, pressure-acceleration and
  *       Jacobi passes.
  *   SPH_OPT_GRAPHS  1 = replay repeated Jacobi iteration chunks from HIP
- *       graphs (default), 0 = launch every kernel directly.
+ *       graphs (default for a context), 0 = launch every kernel directly
+ *       (default for a world).
  * The options of a context apply when it steps alone, those of a world to
  * the world's steps. */
 enum {

@@ -177,6 +177,7 @@ def test_graph_replay_bit_identical(native):
             assert_same(eager, graph)
     w = native.World([native.System(scenes.small_iisph_box(side=9, seed=k, jitter=2e-3))
                       for k in (1, 2)])
+    assert w.get_option(N.OPT_GRAPHS) == 0   # worlds launch directly by default
     solo = [native.System(scenes.small_iisph_box(side=9, seed=k, jitter=2e-3)) for k in (1, 2)]
     for s in solo:
         s.set_option(N.OPT_GRAPHS, 0)
