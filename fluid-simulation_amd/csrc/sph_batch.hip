@@ -225,9 +225,11 @@ __global__ void kb_permute(BatchDev B) {
     gm(v.ty)[i] = y;
     gm(v.tz)[i] = z;
     st4(v.q4, i, x, y, z, 0.0);
-    gm(v.tvx)[i] = gm(v.vx)[src];
-    gm(v.tvy)[i] = gm(v.vy)[src];
-    gm(v.tvz)[i] = gm(v.vz)[src];
+    const double ux = gm(v.vx)[src], uy = gm(v.vy)[src], uz = gm(v.vz)[src];
+    gm(v.tvx)[i] = ux;
+    gm(v.tvy)[i] = uy;
+    gm(v.tvz)[i] = uz;
+    st4(v.v4, i, ux, uy, uz, 0.0);
     gm(v.tm)[i] = gm(v.m)[src];
     gm(v.tfid)[i] = gm(v.fid)[src];
     gm(v.tref)[i] = gm(v.ref)[src];
@@ -433,20 +435,29 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
             gz = 0.0 + (9.81 * dz) / dist;
         }
         const uint32_t c = gm(B.cnt)[gi];
-        size_t e = ELL_BASE(B, v, i);
+        const size_t e0 = ELL_BASE(B, v, i);
+        if (c > 0) {  // software-pipelined as in kb_pressure_accel
+            const GLOBAL_AS uint32_t *ej = gm(B.ell_j) + e0;
+            uint32_t j1 = ej[64 * (size_t)min(1u, c - 1)];
+            const uint32_t j0 = ej[0];
+            rec4 q0 = ld4(v.q4, j0), u0 = ld4(v.v4, j0);
 #pragma unroll UNROLL
-        for (uint32_t k = 0; k < c; k++, e += 64) {
-            uint32_t j = gm(B.ell_j)[e];
-            const rec4 qj = ld4(v.q4, j);
-            double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
-            double cf = v.nu_f * 2. * (3. + 2.) * qj.w *
-                        dot3(ux - gm(v.vx)[j], uy - gm(v.vy)[j], uz - gm(v.vz)[j], rx, ry, rz) /
-                        (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
-            double wx, wy, wz;
-            pair_grad<CACHED>(B, K, rh, e, rx, ry, rz, wx, wy, wz);
-            acx += cf * wx;
-            acy += cf * wy;
-            acz += cf * wz;
+            for (uint32_t k = 0; k < c; k++) {
+                const uint32_t j2 = ej[64 * (size_t)min(k + 2, c - 1)];
+                const rec4 q1 = ld4(v.q4, j1), u1 = ld4(v.v4, j1);
+                double rx = px - q0.x, ry = py - q0.y, rz = pz - q0.z;
+                double cf = v.nu_f * 2. * (3. + 2.) * q0.w *
+                            dot3(ux - u0.x, uy - u0.y, uz - u0.z, rx, ry, rz) /
+                            (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
+                double wx, wy, wz;
+                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, rx, ry, rz, wx, wy, wz);
+                acx += cf * wx;
+                acy += cf * wy;
+                acz += cf * wz;
+                j1 = j2;
+                q0 = q1;
+                u0 = u1;
+            }
         }
     }
     for (uint32_t bi = 0; bi < v.nb; bi++) {
@@ -628,9 +639,12 @@ __global__ void kb_pred_vel(BatchDev B, int to_pred) {
     const double dt = v.S.dt;
     double bx = to_pred ? gm(v.svx)[i] : gm(v.vx)[i], by = to_pred ? gm(v.svy)[i] : gm(v.vy)[i],
            bz = to_pred ? gm(v.svz)[i] : gm(v.vz)[i];
-    gm(v.svx)[i] = bx + dt * gm(v.ax)[i];
-    gm(v.svy)[i] = by + dt * gm(v.ay)[i];
-    gm(v.svz)[i] = bz + dt * gm(v.az)[i];
+    const double nx = bx + dt * gm(v.ax)[i], ny = by + dt * gm(v.ay)[i],
+                 nz = bz + dt * gm(v.az)[i];
+    gm(v.svx)[i] = nx;
+    gm(v.svy)[i] = ny;
+    gm(v.svz)[i] = nz;
+    st4(v.sv4, i, nx, ny, nz, gm(v.m)[i]);  // the record the source pass gathers
 }
 
 // source term (vp or vde), diagonal element and initial pressure in one pass
@@ -650,19 +664,29 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
     double src = vp ? 1. - S.V0 / Vi : 0.;
     double sfx = 0., sfy = 0., sfz = 0., sf2 = 0., sbx = 0., sby = 0., sbz = 0.;
     const uint32_t c = gm(B.cnt)[gi];
-    size_t e = ELL_BASE(B, v, i);
+    const size_t e0 = ELL_BASE(B, v, i);
+    if (c > 0) {  // software-pipelined as in kb_pressure_accel
+        const GLOBAL_AS uint32_t *ej = gm(B.ell_j) + e0;
+        uint32_t j1 = ej[64 * (size_t)min(1u, c - 1)];
+        const uint32_t j0 = ej[0];
+        rec4 q0 = ld4(v.q4, j0), s0 = ld4(v.sv4, j0);
 #pragma unroll UNROLL
-    for (uint32_t k = 0; k < c; k++, e += 64) {
-        uint32_t j = gm(B.ell_j)[e];
-        const rec4 qj = ld4(v.q4, j);
-        double wx, wy, wz;
-        pair_grad<CACHED>(B, K, rh, e, px - qj.x, py - qj.y, pz - qj.z, wx, wy, wz);
-        double Vj = qj.w;
-        src -= dt * Vj * dot3(ux - gm(v.svx)[j], uy - gm(v.svy)[j], uz - gm(v.svz)[j], wx, wy, wz);
-        sfx += Vj * wx;
-        sfy += Vj * wy;
-        sfz += Vj * wz;
-        sf2 -= dt2 * Vi * powi2(Vj) / gm(v.m)[j] * dot3(wx, wy, wz, wx, wy, wz);
+        for (uint32_t k = 0; k < c; k++) {
+            const uint32_t j2 = ej[64 * (size_t)min(k + 2, c - 1)];
+            const rec4 q1 = ld4(v.q4, j1), s1 = ld4(v.sv4, j1);
+            double wx, wy, wz;
+            pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - q0.x, py - q0.y, pz - q0.z,
+                              wx, wy, wz);
+            const double Vj = q0.w;
+            src -= dt * Vj * dot3(ux - s0.x, uy - s0.y, uz - s0.z, wx, wy, wz);
+            sfx += Vj * wx;
+            sfy += Vj * wy;
+            sfz += Vj * wz;
+            sf2 -= dt2 * Vi * powi2(Vj) / s0.w * dot3(wx, wy, wz, wx, wy, wz);
+            j1 = j2;
+            q0 = q1;
+            s0 = s1;
+        }
     }
     for (uint32_t bi = 0; bi < v.nb; bi++) {
         const BodyDev &b = B.bodies[v.body_base + bi];

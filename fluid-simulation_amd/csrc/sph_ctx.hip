@@ -379,7 +379,9 @@ struct Scene {
     DBuf<uint32_t> fid, ref;
     // solver and integrator slots
     DBuf<double> s_f, a_ff, svx, svy, svz, sxp, syp, szp, ix, iy, iz, ivx, ivy, ivz;
-    DBuf<double> q4, pa4;  // packed {x, y, z, V} and {pressure acceleration, 0}
+    // packed gather records: {x, y, z, V}, {pressure acceleration, 0},
+    // {predicted velocity, m}, {velocity, 0}
+    DBuf<double> q4, pa4, sv4, v4;
     // permutation targets
     DBuf<double> tx, ty, tz, tvx, tvy, tvz, tm;
     DBuf<uint32_t> tfid, tref;
@@ -444,7 +446,7 @@ struct Scene {
         v.ax = ax.p; v.ay = ay.p; v.az = az.p;
         v.m = m.p; v.V = V.p; v.p = p.p;
         v.s_f = s_f.p; v.a_ff = a_ff.p;
-        v.q4 = q4.p; v.pa4 = pa4.p;
+        v.q4 = q4.p; v.pa4 = pa4.p; v.sv4 = sv4.p; v.v4 = v4.p;
         v.svx = svx.p; v.svy = svy.p; v.svz = svz.p;
         v.sxp = sxp.p; v.syp = syp.p; v.szp = szp.p;
         v.tx = tx.p; v.ty = ty.p; v.tz = tz.p;
@@ -868,6 +870,8 @@ static void reserve_scene(Scene &c, size_t n) {
     for (auto *b : f64s) b->reserve(k);
     c.q4.reserve(4 * k);
     c.pa4.reserve(4 * k);
+    c.sv4.reserve(4 * k);
+    c.v4.reserve(4 * k);
     c.fid.reserve(k);
     c.ref.reserve(k);
     c.tfid.reserve(k);
@@ -2354,6 +2358,97 @@ int sph_world_set_option(sph_world *w, int option, int value) {
     return guarded([&] {
         if (!w) throw Error(SPH_ERR_ARG, "null argument");
         set_option(w->batch, option, value);
+    });
+}
+
+int sph_volume_map_build(sph_ctx *c, const sph_lattice *sdf, double h, const double x_min[3],
+                         const int64_t n[3], double dx, const double *gl_x, const double *gl_w,
+                         int degree, double *values_out, uint8_t *present_out) {
+    return guarded([&] {
+        if (!sdf || !sdf->values || !sdf->present || !x_min || !n || !gl_x || !gl_w ||
+            !values_out || !present_out)
+            throw Error(SPH_ERR_ARG, "null argument");
+        if (!(h > 0) || !(dx > 0)) throw Error(SPH_ERR_ARG, "h and dx must be > 0");
+        if (degree < 1 || degree > kVmMaxDegree)
+            throw Error(SPH_ERR_UNSUPPORTED, "quadrature degree outside 1..32");
+        VmGrid g{};
+        size_t nodes = 1;
+        for (int d = 0; d < 3; d++) {
+            if (n[d] <= 0 || n[d] > (1 << 20)) throw Error(SPH_ERR_ARG, "lattice n out of range");
+            g.x_min[d] = x_min[d];
+            g.n[d] = (int)n[d];
+            nodes *= 3 * (size_t)n[d] + 1;
+        }
+        if (nodes >= ((size_t)1 << 31)) throw Error(SPH_ERR_UNSUPPORTED, "volume-map lattice too large");
+        g.dx = dx;
+        // a context lends its stream; without one the build runs on a stream of its own
+        Batch local;
+        if (!c) {
+            check_device();
+            local.init();
+        }
+        Batch &B = c ? c->own : local;
+        LatticeHost L;
+        L.load(*sdf, B.s);
+        // node tables: radial, polar and azimuthal Gauss-Legendre points mapped
+        // as ((b - a) x + (b + a)) / 2, with host sin/cos
+        const double PI = 3.14159265358979323846;
+        std::vector<double> r(degree), st(degree), ct(degree), sp(degree), cp(degree);
+        for (int i = 0; i < degree; i++) {
+            r[i] = 0.5 * ((h - 0.0) * gl_x[i] + (h + 0.0));
+            const double th = 0.5 * ((PI - 0.0) * gl_x[i] + (PI + 0.0));
+            const double ph = 0.5 * ((2. * PI - 0.0) * gl_x[i] + (2. * PI + 0.0));
+            st[i] = std::sin(th);
+            ct[i] = std::cos(th);
+            sp[i] = std::sin(ph);
+            cp[i] = std::cos(ph);
+        }
+        DBuf<double> tw, tr, tst, tct, tsp, tcp, values;
+        tw.upload(gl_w, degree, B.s);
+        tr.upload(r.data(), degree, B.s);
+        tst.upload(st.data(), degree, B.s);
+        tct.upload(ct.data(), degree, B.s);
+        tsp.upload(sp.data(), degree, B.s);
+        tcp.upload(cp.data(), degree, B.s);
+        values.reserve(nodes);
+        DBuf<uint8_t> status, present;
+        status.reserve(nodes);
+        present.reserve(nodes);
+        DBuf<uint32_t> band, nband;
+        band.reserve(nodes);
+        nband.reserve(1);
+        HIPCK(hipMemsetAsync(present.p, 0, nodes, B.s));
+        LAUNCH(k_fill_u32, 1, B.s, nband.p, (size_t)1, 0u);
+        const double bv = 4.0 / 3.0 * PI * ((h * h) * h);  // ball_volume (:34-36)
+        LAUNCH(k_vm_classify, nodes, B.s, L.d, g, h, bv, values.p, status.p, band.p,
+               (unsigned int *)nband.p);
+        uint32_t nb = 0;
+        nband.download(&nb, 1, B.s);
+        B.sync();
+        VmQuad Q{};
+        Q.h = h;
+        Q.bv = bv;
+        Q.wfac = (16. / PI) / ((h * h) * h);
+        Q.degree = degree;
+        Q.w = tw.p;
+        Q.r = tr.p;
+        Q.st = tst.p;
+        Q.ct = tct.p;
+        Q.sp = tsp.p;
+        Q.cp = tcp.p;
+        // one workgroup per band node, in launches of at most 2^22 workgroups
+        for (uint32_t off = 0; off < nb; off += (1u << 22)) {
+            const uint32_t cnt = std::min<uint32_t>(nb - off, 1u << 22);
+            hipLaunchKernelGGL(k_vm_quadrature, dim3(cnt), dim3(kBlock), 0, B.s, L.d, g, Q,
+                               band.p + off, values.p, status.p);
+            HIPCK(hipGetLastError());
+        }
+        const size_t ncell = (size_t)n[0] * n[1] * n[2];
+        LAUNCH(k_vm_cells, ncell, B.s, g, status.p, present.p);
+        LAUNCH(k_vm_finish, nodes, B.s, nodes, present.p, values.p);
+        values.download(values_out, nodes, B.s);
+        present.download(present_out, nodes, B.s);
+        B.sync();
     });
 }
 

@@ -296,3 +296,131 @@ __global__ void k_widen_u32(size_t n, const uint32_t *in, uint64_t *out) {
 }
 
 }  // namespace sph
+
+namespace sph {
+
+// ------------------------------------------------------------------------
+// Volume-map construction (DiscretizedBoundaryFields::new,
+// volume_map_boundary.rs:1593-1624): the node values of the volume map on
+// the lattice (VmGrid) from a discretized signed-distance field
+// ------------------------------------------------------------------------
+__device__ __forceinline__ bool vm_is_node(int ix, int iy, int iz) {
+    // serendipity nodes: corners (all offsets 0 mod 3) and edge nodes (one
+    // offset 1 or 2 mod 3)
+    return ((ix % 3 != 0) + (iy % 3 != 0) + (iz % 3 != 0)) <= 1;
+}
+
+// cubic_extension_fn (volume_map_boundary.rs:1716-1726)
+__device__ __forceinline__ double vm_cubic_extension(double sd, double h, double wfac) {
+    if (sd >= h) return 0.;
+    if (sd <= 0.) return 1.;
+    const double q = sqrt(dot3(sd, 0., 0., sd, 0., 0.)) / h;
+    const double q0 = sqrt(dot3(0., 0., 0., 0., 0., 0.)) / h;
+    return (wfac * kernel_w(q)) / (wfac * kernel_w(q0));
+}
+
+__global__ void k_vm_classify(LatticeDev sdf, VmGrid g, double h, double bv, double *values,
+                              uint8_t *status, uint32_t *band, unsigned int *nband) {
+    const size_t nx = 3 * (size_t)g.n[0] + 1, ny = 3 * (size_t)g.n[1] + 1,
+                 nz = 3 * (size_t)g.n[2] + 1;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nx * ny * nz) return;
+    const int ix = (int)(k % nx), iy = (int)((k / nx) % ny), iz = (int)(k / (nx * ny));
+    values[k] = 0.0;
+    if (!vm_is_node(ix, iy, iz)) {
+        status[k] = 0;
+        return;
+    }
+    // x_min + key / 3 * dx (discretization.rs:171-175)
+    const double px = g.x_min[0] + (double)ix / 3.0 * g.dx,
+                 py = g.x_min[1] + (double)iy / 3.0 * g.dx,
+                 pz = g.x_min[2] + (double)iz / 3.0 * g.dx;
+    double sd;
+    status[k] = 1;
+    if (serendipity(sdf, px, py, pz, sd)) return;  // outside the band: 0 (SDFnWrapper::volume)
+    if (sd >= 2.0 * h) return;
+    if (sd <= -h) {
+        values[k] = 4.0 / 3.0 * 3.14159265358979323846 * ((h * h) * h);
+        return;
+    }
+    status[k] = 2;  // quadrature pending
+    band[atomicAdd(nband, 1u)] = (uint32_t)k;
+}
+
+// gauss_legendre_integrate_ball (utilities/discretization.rs:33-79) of the
+// cubic extension of the SDF around one band node per workgroup: thread t
+// sums the phi rule for (r_i, theta_j) pairs, then the theta and r rules are
+// summed in node order, as nested sequential sums
+__global__ void k_vm_quadrature(LatticeDev sdf, VmGrid g, VmQuad Q, const uint32_t *band,
+                                double *values, uint8_t *status) {
+    __shared__ double tj[kVmMaxDegree][kVmMaxDegree + 1];
+    __shared__ double ti[kVmMaxDegree];
+    __shared__ int err;
+    const double PI = 3.14159265358979323846;
+    const size_t nx = 3 * (size_t)g.n[0] + 1, ny = 3 * (size_t)g.n[1] + 1;
+    const uint32_t k = band[blockIdx.x];
+    const int ix = (int)(k % nx), iy = (int)((k / nx) % ny), iz = (int)(k / (nx * ny));
+    const double px = g.x_min[0] + (double)ix / 3.0 * g.dx,
+                 py = g.x_min[1] + (double)iy / 3.0 * g.dx,
+                 pz = g.x_min[2] + (double)iz / 3.0 * g.dx;
+    const int D = Q.degree;
+    if (threadIdx.x == 0) err = 0;
+    __syncthreads();
+    for (int ij = threadIdx.x; ij < D * D; ij += blockDim.x) {
+        const int i = ij / D, j = ij % D;
+        const double r = Q.r[i], st = Q.st[j], ct = Q.ct[j];
+        double acc = 0.0;
+        int e = 0;
+        for (int kk = 0; kk < D; kk++) {
+            double sd;
+            e = serendipity(sdf, px + r * st * Q.cp[kk], py + r * st * Q.sp[kk], pz + r * ct, sd);
+            if (e) break;
+            acc += vm_cubic_extension(sd, Q.h, Q.wfac) * Q.w[kk];
+        }
+        if (e) err = 1;
+        tj[i][j] = st * (0.5 * (2. * PI - 0.0) * acc) * Q.w[j];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < D) {
+        const int i = threadIdx.x;
+        double acc = 0.0;
+        for (int j = 0; j < D; j++) acc += tj[i][j];
+        ti[i] = (Q.r[i] * Q.r[i]) * (0.5 * (PI - 0.0) * acc) * Q.w[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (err) {
+            status[k] = 3;  // an SDF evaluation failed: the node has no value
+            return;
+        }
+        double acc = 0.0;
+        for (int i = 0; i < D; i++) acc += ti[i];
+        double v = 0.5 * (Q.h - 0.0) * acc;
+        v = v < 0.0 ? 0.0 : (v > Q.bv ? Q.bv : v);  // clamp(0, ball_volume)
+        values[k] = v;
+        status[k] = 1;
+    }
+}
+
+// cells whose 32 nodes all have values are kept (prune lower bound 0 removes
+// nothing else, values being >= 0); their nodes are present
+__global__ void k_vm_cells(VmGrid g, const uint8_t *status, uint8_t *present) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
+    if (c >= ncell) return;
+    const size_t nx = 3 * (size_t)g.n[0] + 1, ny = 3 * (size_t)g.n[1] + 1;
+    const size_t cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / ((size_t)g.n[0] * g.n[1]);
+    const size_t base = (3 * cz * ny + 3 * cy) * nx + 3 * cx;
+    for (int i = 0; i < 32; i++)
+        if (status[base + ((size_t)kNodeOff[i][2] * ny + kNodeOff[i][1]) * nx + kNodeOff[i][0]] != 1)
+            return;
+    for (int i = 0; i < 32; i++)
+        present[base + ((size_t)kNodeOff[i][2] * ny + kNodeOff[i][1]) * nx + kNodeOff[i][0]] = 1;
+}
+
+__global__ void k_vm_finish(size_t n, const uint8_t *present, double *values) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && !present[k]) values[k] = 0.0;
+}
+
+}  // namespace sph

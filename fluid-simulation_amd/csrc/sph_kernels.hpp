@@ -50,6 +50,28 @@ __global__ void k_static_volume(size_t n, KernelConst K, double gamma_v, const d
                                 const double *y, const double *z, const uint32_t *off,
                                 const uint32_t *idx, double *vol);
 __global__ void k_map_indices(size_t n, const uint32_t *perm, const uint32_t *in, uint64_t *out);
+
+// volume-map construction
+constexpr int kVmMaxDegree = 32;
+struct VmGrid {
+    double x_min[3];
+    int n[3];  // cells per axis of the volume-map lattice
+    double dx;
+};
+struct VmQuad {
+    double h, bv, wfac;  // support radius, ball volume, alpha / h^3
+    int degree;
+    const double *w;               // Gauss-Legendre weights on [-1, 1]
+    const double *r;               // radial nodes on [0, h]
+    const double *st, *ct;         // sin, cos of the polar nodes on [0, pi]
+    const double *sp, *cp;         // sin, cos of the azimuthal nodes on [0, 2 pi]
+};
+__global__ void k_vm_classify(LatticeDev sdf, VmGrid g, double h, double bv, double *values,
+                              uint8_t *status, uint32_t *band, unsigned int *nband);
+__global__ void k_vm_quadrature(LatticeDev sdf, VmGrid g, VmQuad Q, const uint32_t *band,
+                                double *values, uint8_t *status);
+__global__ void k_vm_cells(VmGrid g, const uint8_t *status, uint8_t *present);
+__global__ void k_vm_finish(size_t n, const uint8_t *present, double *values);
 __global__ void k_widen_u32(size_t n, const uint32_t *in, uint64_t *out);
 
 }  // namespace sph

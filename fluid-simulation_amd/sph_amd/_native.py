@@ -25,6 +25,7 @@ _u64p = C.POINTER(C.c_uint64)
 _u32p = C.POINTER(C.c_uint32)
 _u8p = C.POINTER(C.c_uint8)
 _szp = C.POINTER(C.c_size_t)
+_i64p = C.POINTER(C.c_int64)
 
 
 class SphError(RuntimeError):
@@ -106,6 +107,8 @@ SIGNATURES = [
     ("sph_world_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("sph_world_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
+    ("sph_volume_map_build", C.c_int, [C.c_void_p, C.POINTER(SphLattice), C.c_double, _dp, _i64p,
+                                       C.c_double, _dp, _dp, C.c_int, _dp, _u8p]),
     ("sph_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("sph_world_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("sph_get_option", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),

@@ -184,10 +184,16 @@ def _present_from_kept(keep, n):
     return present
 
 
-def _lattice(aabb_min, aabb_max, pad, dx, sdf, prune_lo, prune_hi, value_fn):
+def lattice_grid(aabb_min, aabb_max, pad, dx):
+    """x_min, x_max and cell counts of a lattice over the padded box."""
     x_min = np.asarray(aabb_min, dtype=np.float64) - pad
     x_max = np.asarray(aabb_max, dtype=np.float64) + pad
     n = np.array([int(round((x_max[d] - x_min[d]) / dx)) for d in range(3)], dtype=np.int64)
+    return x_min, x_max, n
+
+
+def _lattice(aabb_min, aabb_max, pad, dx, sdf, prune_lo, prune_hi, value_fn):
+    x_min, x_max, n = lattice_grid(aabb_min, aabb_max, pad, dx)
     pts = _node_positions(x_min, n, dx)
     vals = value_fn(pts)
     cells = _cell_node_values(vals, n)
@@ -248,6 +254,26 @@ def volume_map_fields(sd_fn, aabb_min, aabb_max, h, dx_particle):
     pad_vm = 2.0 * h + cell / 6.0                    # :1595
     vm = _lattice(aabb_min, aabb_max, pad_vm, cell, sd_fn, 0.0, None,
                   lambda pts: halfspace_volume(sd_fn(pts), h))
+    return sdf, vm
+
+
+def volume_map_quadrature(sd_fn, aabb_min, aabb_max, h, dx_particle, builder, degree=30):
+    """SDF + volume-map lattices on the reference's grid rule, the volume map
+    integrated from the SDF lattice with the degree-`degree` Gauss-Legendre
+    ball rule (``volume_map_boundary.rs:1593-1624``).
+
+    ``builder(sdf, h, x_min, n, dx, gl_x, gl_w) -> (values, present)`` runs the
+    integration (``sph_amd.system.build_volume_map`` on the device). The
+    reference takes its nodes from the gauss-quad crate; here they are numpy's
+    ``leggauss`` nodes in ascending order."""
+    cell = 4.0 * dx_particle
+    pad_sd = 3.1 * h
+    sdf = _lattice(aabb_min, aabb_max, pad_sd, cell, sd_fn, -pad_sd, pad_sd, sd_fn)
+    x_min, x_max, n = lattice_grid(aabb_min, aabb_max, 2.0 * h + cell / 6.0, cell)
+    gl_x, gl_w = np.polynomial.legendre.leggauss(degree)
+    vals, pres = builder(sdf, h, x_min, n, cell, gl_x, gl_w)
+    vm = Lattice(x_min, x_max, float(cell), n, np.ascontiguousarray(vals, dtype=np.float64),
+                 np.ascontiguousarray(pres, dtype=np.uint8))
     return sdf, vm
 
 

@@ -34,11 +34,11 @@ import numpy as np
 
 from . import scenes as S
 
-# device bytes per particle while resident in a world: 41 binary64 and 4 u32
+# device bytes per particle while resident in a world: 49 binary64 and 4 u32
 # per-particle fields of the scene, the world's per-particle sort scratch and
 # neighbor slots (about 45 pair slots of 4 B in the default recompute mode) and
 # one volume-map slot (flag + 7 binary64) per boundary
-BYTES_PER_PARTICLE = 41 * 8 + 4 * 4 + 20 + 48 * 4
+BYTES_PER_PARTICLE = 49 * 8 + 4 * 4 + 20 + 48 * 4
 BYTES_PER_PARTICLE_BODY = 1 + 7 * 8
 BYTES_PER_SCENE = 1 << 20          # stream, pinned staging, small tables
 

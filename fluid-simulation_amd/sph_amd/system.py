@@ -377,6 +377,29 @@ class World:
             pass
 
 
+def build_volume_map(sdf, h: float, x_min, n, dx: float, gl_x, gl_w, system=None):
+    """Volume-map node values and present flags on the lattice (x_min, n, dx)
+    from the SDF lattice `sdf` (``DiscretizedBoundaryFields::new``,
+    ``volume_map_boundary.rs:1593-1624``), with the Gauss-Legendre rule
+    (gl_x, gl_w) on [-1, 1]; runs on the stream of `system` if given."""
+    s = _lattice_struct(sdf)
+    xm = np.ascontiguousarray(x_min, dtype=np.float64)
+    nn = np.ascontiguousarray(n, dtype=np.int64)
+    gx = np.ascontiguousarray(gl_x, dtype=np.float64)
+    gw = np.ascontiguousarray(gl_w, dtype=np.float64)
+    if xm.shape != (3,) or nn.shape != (3,) or gx.shape != gw.shape or gx.ndim != 1:
+        raise ValueError("x_min and n need 3 entries; gl_x and gl_w one per node")
+    shape = (3 * int(nn[2]) + 1, 3 * int(nn[1]) + 1, 3 * int(nn[0]) + 1)
+    vals = np.empty(shape, dtype=np.float64)
+    pres = np.empty(shape, dtype=np.uint8)
+    handle = system._ctx.h if system is not None else None
+    N.check(N.lib().sph_volume_map_build(handle, C.byref(s), float(h), N.dptr(xm),
+                                         nn.ctypes.data_as(N._i64p), float(dx), N.dptr(gx),
+                                         N.dptr(gw), len(gx), N.dptr(vals),
+                                         pres.ctypes.data_as(N._u8p)))
+    return vals, pres
+
+
 def device_memory():
     """(free, total) bytes of device memory."""
     f, t = C.c_size_t(), C.c_size_t()

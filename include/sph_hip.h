@@ -234,6 +234,25 @@ int sph_world_size(const sph_world *world, size_t *systems, size_t *particles);
 int sph_world_profile(sph_world *world, int enable);
 int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
                            uint64_t *units, double *ms);
+/* ---- boundary-field construction (setup of VolumeMapBoundary) ----------------
+ * The volume map of DiscretizedBoundaryFields::new (volume_map_boundary.rs:
+ * 1593-1624) over the cells (x_min, n, dx): for every serendipity node p,
+ * SDFnWrapper::volume (:1684-1714) clamped to [0, ball_volume(h)], i.e. 0
+ * where `sdf` cannot be evaluated at p or sdf(p) >= 2h, the ball volume where
+ * sdf(p) <= -h, else gauss_legendre_integrate_ball (utilities/discretization.rs:
+ * 33-79) of cubic_extension_fn(sdf, h) (:1716-1726) with the `degree`-point
+ * Gauss-Legendre rule (gl_x, gl_w) on [-1, 1] (each of the r, theta, phi rules
+ * maps x to ((b - a) x + (b + a)) / 2 and scales its node-order sum by
+ * (b - a) / 2). A node whose quadrature hits an unevaluable sdf point has no
+ * value; a cell is kept iff all 32 of its nodes have values
+ * (CubicSerendipityDiscretization::new, discretization.rs:99-245, lower bound
+ * 0). Outputs: values and present flags of the dense (3n+1)^3 nodes, x
+ * fastest. degree <= 32. `ctx` may be NULL (the build then uses a stream of
+ * its own); otherwise it runs on the context's stream. */
+int sph_volume_map_build(sph_ctx *ctx, const sph_lattice *sdf, double h, const double x_min[3],
+                         const int64_t n[3], double dx, const double *gl_x, const double *gl_w,
+                         int degree, double *values_out, uint8_t *present_out);
+
 /* free and total device memory (hipMemGetInfo), for admission control */
 int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
 

@@ -1470,3 +1470,119 @@ void ref_system_set_threads(int nthreads) {
     (void)nthreads;
 #endif
 }
+
+
+/* =========================================================================
+ * Volume-map construction (setup of VolumeMapBoundary)
+ * ========================================================================= */
+
+/* cubic_extension_fn (volume_map_boundary.rs:1716-1726):
+ * W((sd, 0, 0), h) / W(0, h) on (0, h), 1 below, 0 above */
+static double ref_cubic_extension(double sd, double h) {
+    if (sd >= h) return 0.;
+    if (sd <= 0.) return 1.;
+    double r[3] = {sd, 0., 0.}, z[3] = {0., 0., 0.};
+    return ref_kernel_function(r, h) / ref_kernel_function(z, h);
+}
+
+/* ball_volume (volume_map_boundary.rs:34-36) */
+static double ref_ball_volume(double h) {
+    return 4.0 / 3.0 * 3.14159265358979323846 * ((h * h) * h);
+}
+
+int ref_vm_integrate_ball(const ref_lattice *sdf, const double center[3], double h,
+                          const double *gl_x, const double *gl_w, int degree, double *out) {
+    const double PI = 3.14159265358979323846;
+    const int n = degree;
+    double acc_r = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double r = 0.5 * ((h - 0.0) * gl_x[i] + (h + 0.0));
+        double acc_t = 0.0;
+        for (int j = 0; j < n; j++) {
+            const double theta = 0.5 * ((PI - 0.0) * gl_x[j] + (PI + 0.0));
+            const double st = sin(theta), ct = cos(theta);
+            double acc_p = 0.0;
+            for (int k = 0; k < n; k++) {
+                const double phi = 0.5 * ((2. * PI - 0.0) * gl_x[k] + (2. * PI + 0.0));
+                const double p[3] = {center[0] + r * st * cos(phi), center[1] + r * st * sin(phi),
+                                     center[2] + r * ct};
+                double sd;
+                int e = ref_serendipity_function(sdf, p, &sd);
+                if (e) return e;
+                acc_p += ref_cubic_extension(sd, h) * gl_w[k];
+            }
+            const double inner_p = 0.5 * (2. * PI - 0.0) * acc_p;
+            acc_t += st * inner_p * gl_w[j];
+        }
+        const double inner_t = 0.5 * (PI - 0.0) * acc_t;
+        acc_r += (r * r) * inner_t * gl_w[i];
+    }
+    *out = 0.5 * (h - 0.0) * acc_r;
+    return 0;
+}
+
+int ref_vm_node_value(const ref_lattice *sdf, const double p[3], double h, const double *gl_x,
+                      const double *gl_w, int degree, double *out) {
+    const double bv = ref_ball_volume(h);
+    double sd, v;
+    if (ref_serendipity_function(sdf, p, &sd)) {
+        v = 0.0;  /* pruned cell or outside -> treated as outside the band */
+    } else if (sd >= 2.0 * h) {
+        v = 0.0;
+    } else if (sd <= -h) {
+        v = 4.0 / 3.0 * 3.14159265358979323846 * ((h * h) * h);
+    } else {
+        int e = ref_vm_integrate_ball(sdf, p, h, gl_x, gl_w, degree, &v);
+        if (e) return e;
+    }
+    *out = v < 0.0 ? 0.0 : (v > bv ? bv : v);  /* f64::clamp(0, bv) */
+    return 0;
+}
+
+void ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const int64_t n[3],
+                  double dx, const double *gl_x, const double *gl_w, int degree, double *values,
+                  uint8_t *present) {
+    const int64_t nx = 3 * n[0] + 1, ny = 3 * n[1] + 1, nz = 3 * n[2] + 1;
+    const size_t nodes = (size_t)(nx * ny * nz);
+    uint8_t *ok = (uint8_t *)calloc(nodes, 1);
+    uint8_t *isnode = (uint8_t *)calloc(nodes, 1);
+    for (int64_t cz = 0; cz < n[2]; cz++)
+        for (int64_t cy = 0; cy < n[1]; cy++)
+            for (int64_t cx = 0; cx < n[0]; cx++)
+                for (int i = 0; i < 32; i++)
+                    isnode[((3 * cz + NODE_OFFS[i][2]) * ny + (3 * cy + NODE_OFFS[i][1])) * nx +
+                           (3 * cx + NODE_OFFS[i][0])] = 1;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < (int64_t)nodes; k++) {
+        values[k] = 0.0;
+        present[k] = 0;
+        if (!isnode[k]) continue;
+        const int64_t ix = k % nx, iy = (k / nx) % ny, iz = k / (nx * ny);
+        /* x_min + key / 3 * dx (discretization.rs:171-175) */
+        const double p[3] = {x_min[0] + (double)ix / 3.0 * dx, x_min[1] + (double)iy / 3.0 * dx,
+                             x_min[2] + (double)iz / 3.0 * dx};
+        double v;
+        if (ref_vm_node_value(sdf, p, h, gl_x, gl_w, degree, &v) == 0) {
+            values[k] = v;
+            ok[k] = 1;
+        }
+    }
+    /* a cell is kept iff all 32 nodes evaluated (values are >= 0, so the
+     * lower bound 0 prunes nothing else) */
+    for (int64_t cz = 0; cz < n[2]; cz++)
+        for (int64_t cy = 0; cy < n[1]; cy++)
+            for (int64_t cx = 0; cx < n[0]; cx++) {
+                int good = 1;
+                for (int i = 0; i < 32 && good; i++)
+                    good = ok[((3 * cz + NODE_OFFS[i][2]) * ny + (3 * cy + NODE_OFFS[i][1])) * nx +
+                              (3 * cx + NODE_OFFS[i][0])];
+                if (!good) continue;
+                for (int i = 0; i < 32; i++)
+                    present[((3 * cz + NODE_OFFS[i][2]) * ny + (3 * cy + NODE_OFFS[i][1])) * nx +
+                            (3 * cx + NODE_OFFS[i][0])] = 1;
+            }
+    for (size_t k = 0; k < nodes; k++)
+        if (!present[k]) values[k] = 0.0;
+    free(ok);
+    free(isnode);
+}

@@ -175,6 +175,27 @@ size_t ref_system_boundary_samples(const ref_system *s, int b, size_t i, int mod
 double ref_system_static_volume(const ref_system *s, int b, size_t k);
 void ref_system_set_threads(int nthreads);
 
+/* ---- volume-map construction (setup; DiscretizedBoundaryFields::new) ------
+ * gauss_legendre_integrate_ball (utilities/discretization.rs:33-79) of
+ * cubic_extension_fn(sdf(p), h) (volume_map_boundary.rs:1716-1726) around
+ * `center`, with the Gauss-Legendre rule (gl_x, gl_w) of `degree` points on
+ * [-1, 1] mapped to [a, b] as x -> ((b - a) x + (b + a)) / 2, sum times
+ * (b - a) / 2, terms accumulated in node order. Returns 0, or the first
+ * error status of an sdf evaluation (1 OutOfBounds, 2 PrunedCell). */
+int ref_vm_integrate_ball(const ref_lattice *sdf, const double center[3], double h,
+                          const double *gl_x, const double *gl_w, int degree, double *out);
+/* SDFnWrapper::volume (volume_map_boundary.rs:1684-1714) clamped to
+ * [0, ball_volume(h)] as in DiscretizedBoundaryFields::new (:1606-1624). */
+int ref_vm_node_value(const ref_lattice *sdf, const double p[3], double h, const double *gl_x,
+                      const double *gl_w, int degree, double *out);
+/* The volume-map lattice over the cells of (x_min, n, dx)
+ * (CubicSerendipityDiscretization::new, discretization.rs:99-245, prune lower
+ * bound 0): values[] and present[] of the dense (3n+1)^3 nodes. A cell is kept
+ * iff all 32 of its node values evaluate; present = node of a kept cell. */
+void ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const int64_t n[3],
+                  double dx, const double *gl_x, const double *gl_w, int degree, double *values,
+                  uint8_t *present);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ _dp = C.POINTER(C.c_double)
 _u64p = C.POINTER(C.c_uint64)
 _u32p = C.POINTER(C.c_uint32)
 _u8p = C.POINTER(C.c_uint8)
+_i64p = C.POINTER(C.c_int64)
 
 
 class RefLattice(C.Structure):
@@ -123,6 +124,14 @@ def lib():
         L.ref_system_static_volume.restype = C.c_double
         L.ref_system_static_volume.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.ref_system_set_threads.argtypes = [C.c_int]
+        L.ref_vm_integrate_ball.restype = C.c_int
+        L.ref_vm_integrate_ball.argtypes = [C.POINTER(RefLattice), _dp, C.c_double, _dp, _dp,
+                                            C.c_int, _dp]
+        L.ref_vm_node_value.restype = C.c_int
+        L.ref_vm_node_value.argtypes = [C.POINTER(RefLattice), _dp, C.c_double, _dp, _dp,
+                                        C.c_int, _dp]
+        L.ref_vm_build.argtypes = [C.POINTER(RefLattice), C.c_double, _dp, _i64p, C.c_double,
+                                   _dp, _dp, C.c_int, _dp, _u8p]
         _lib = L
     return _lib
 
@@ -198,6 +207,40 @@ def serendipity_gradient(lat, p):
     out = np.zeros(3)
     st = lib().ref_serendipity_gradient(C.byref(s), dptr(p), dptr(out))
     return st, out
+
+
+def vm_integrate_ball(sdf, center, h, gl_x, gl_w):
+    """gauss_legendre_integrate_ball of the cubic extension of `sdf`; returns
+    (status, value), status nonzero when an SDF evaluation failed."""
+    s = lattice_struct(sdf)
+    c, gx, gw = f64c(center), f64c(gl_x), f64c(gl_w)
+    out = C.c_double()
+    st = lib().ref_vm_integrate_ball(C.byref(s), dptr(c), float(h), dptr(gx), dptr(gw),
+                                     len(gx), C.byref(out))
+    return st, out.value
+
+
+def vm_node_value(sdf, p, h, gl_x, gl_w):
+    """SDFnWrapper::volume clamped to [0, ball volume]; (status, value)."""
+    s = lattice_struct(sdf)
+    c, gx, gw = f64c(p), f64c(gl_x), f64c(gl_w)
+    out = C.c_double()
+    st = lib().ref_vm_node_value(C.byref(s), dptr(c), float(h), dptr(gx), dptr(gw), len(gx),
+                                 C.byref(out))
+    return st, out.value
+
+
+def vm_build(sdf, h, x_min, n, dx, gl_x, gl_w):
+    """Volume-map node values and present flags, shape (3n_z+1, 3n_y+1, 3n_x+1)."""
+    s = lattice_struct(sdf)
+    xm, gx, gw = f64c(x_min), f64c(gl_x), f64c(gl_w)
+    nn = np.ascontiguousarray(n, dtype=np.int64)
+    shape = (3 * int(nn[2]) + 1, 3 * int(nn[1]) + 1, 3 * int(nn[0]) + 1)
+    vals = np.zeros(shape, dtype=np.float64)
+    pres = np.zeros(shape, dtype=np.uint8)
+    lib().ref_vm_build(C.byref(s), float(h), dptr(xm), nn.ctypes.data_as(_i64p), float(dx),
+                       dptr(gx), dptr(gw), len(gx), dptr(vals), pres.ctypes.data_as(_u8p))
+    return vals, pres
 
 
 def params_struct(P):
