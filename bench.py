@@ -22,6 +22,11 @@ of IISPH scene requests of mixed sizes served by continuous batching (one
 world, HBM-budget admission) and, as the baseline, one request at a time with
 the plain per-scene path; it reports throughput and p50/p99 step latency of
 both.
+
+--workload volume_map times the construction of the same workload's boundary
+field instead: the volume map of the radius-1 m sphere container (h = 0.02,
+cell 0.04) integrated from its SDF lattice with the degree-30 Gauss-Legendre
+ball rule, in ball quadratures per second; a step is one whole build.
 """
 import argparse
 import json
@@ -50,6 +55,15 @@ ALGO_BYTES = {
     "pressure_accel": {"base": 72, "per_body": 33},   # x,m,V,p r; a w
     "jacobi": {"base": 88, "per_body": 33},           # x,V,a_p,s,a_ii r; p r/w
 }
+
+
+# Algorithmic binary64 operations per SDF evaluation in the volume-map ball
+# quadrature (+, -, *, /, sqrt each one; common factors counted once): the
+# serendipity interpolation 247 (cell and local coordinates 29, shape-function
+# factors 26, 32 shape products 128, weighted sum 64), the cubic extension 13,
+# the quadrature point 7, the weighted phi sum 2. degree^3 evaluations per node.
+VM_FLOPS_PER_EVAL = 269
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(*a):
@@ -158,6 +172,107 @@ def serving_bench(args):
     print(json.dumps(out))
 
 
+def volume_map_bench(args):
+    """Volume-map construction of the two-fluid ball's sphere container."""
+    from sph_amd import scenes, system
+    degree = 30
+    dx, radius = 0.01, 1.0
+    h = 2 * dx
+    c = np.zeros(3)
+    sd = lambda q: scenes.sd_sphere_container(q, c, radius)
+    sdf, x_min, _, n, cell = scenes.sdf_and_volume_map_grid(sd, c - radius, c + radius, h, dx)
+    gx, gw = np.polynomial.legendre.leggauss(degree)
+    sim = system.System(scenes.small_iisph_box())  # lends its stream and profiler
+    stats = {}
+    for _ in range(args.warmup):
+        system.build_volume_map(sdf, h, x_min, n, cell, gx, gw, system=sim, stats=stats)
+    sim.profile(True)
+    lat = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        vals, pres = system.build_volume_map(sdf, h, x_min, n, cell, gx, gw, system=sim,
+                                             stats=stats)
+        lat.append(time.perf_counter() - t)
+    prof = sim.profile_read()["vm_quadrature"]
+    sim.profile(False)
+    q = stats["quadrature_nodes"]
+    total = float(np.sum(lat))
+    avg_s = prof["ms"] / 1e3 / max(prof["launches"], 1)
+    flops = VM_FLOPS_PER_EVAL * degree ** 3 * prof["units"] / max(prof["launches"], 1)
+    achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
+    out = {
+        "metric": "volume-map construction (degree-30 ball quadrature), quadrature nodes/s",
+        "value": q * args.steps / total,
+        "unit": "quadrature nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "p50_step_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_step_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic sphere SDF lattice)",
+        "config": {"workload": "volume_map_sphere_r1_h0.02", "cells": [int(v) for v in n],
+                   "nodes": int(vals.size), "quadrature_nodes": q, "degree": degree,
+                   "present_nodes": int(pres.sum())},
+        "roofline": {
+            "bound": "valu_fp64",
+            "kernel": "k_vm_quadrature",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "algorithmic_flops_per_launch": flops,
+            "avg_launch_ms": avg_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    sim._ctx.close()
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        try:
+            out["cpu_baseline"] = vm_cpu_baseline(sdf, h, x_min, n, cell, gx, gw,
+                                                  args.cpu_seconds, threads)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline unavailable: {e}")
+    print(json.dumps(out))
+
+
+def vm_cpu_baseline(sdf, h, x_min, n, cell, gx, gw, budget_s, threads):
+    """The oracle's volume-map build on a slab of cells through the sphere's
+    equator, as thick as fits the time budget."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcpu
+    refcpu.lib().ref_system_set_threads(threads)
+    mid = int(n[2]) // 2
+    cells, rate, q, dt = 1, 0.0, 0, 0.0
+    while True:
+        sub_min = np.array([x_min[0], x_min[1], x_min[2] + mid * cell])
+        sub_n = np.array([n[0], n[1], cells])
+        st = {}
+        t = time.perf_counter()
+        refcpu.vm_build(sdf, h, sub_min, sub_n, cell, gx, gw, stats=st)
+        dt = time.perf_counter() - t
+        q = st["quadrature_nodes"]
+        rate = q / dt
+        if dt > 0.3 * budget_s or cells >= int(n[2]) - mid:
+            break
+        cells = min(int(n[2]) - mid, max(cells + 1, int(cells * 0.8 * budget_s / max(dt, 1e-3))))
+    return {
+        "value": rate,
+        "unit": "quadrature nodes/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle ref_vm_build (binary64 C restatement, OpenMP) on a slab of "
+                   f"{cells} cell layer(s) through the sphere's equator: {q} quadrature "
+                   f"nodes in {dt:.1f} s"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +304,9 @@ def main():
 
     if args.workload == "serving":
         serving_bench(args)
+        return
+    if args.workload == "volume_map":
+        volume_map_bench(args)
         return
 
     from sph_amd import system

@@ -2363,7 +2363,8 @@ int sph_world_set_option(sph_world *w, int option, int value) {
 
 int sph_volume_map_build(sph_ctx *c, const sph_lattice *sdf, double h, const double x_min[3],
                          const int64_t n[3], double dx, const double *gl_x, const double *gl_w,
-                         int degree, double *values_out, uint8_t *present_out) {
+                         int degree, double *values_out, uint8_t *present_out,
+                         uint64_t *quadrature_nodes) {
     return guarded([&] {
         if (!sdf || !sdf->values || !sdf->present || !x_min || !n || !gl_x || !gl_w ||
             !values_out || !present_out)
@@ -2425,6 +2426,7 @@ int sph_volume_map_build(sph_ctx *c, const sph_lattice *sdf, double h, const dou
         uint32_t nb = 0;
         nband.download(&nb, 1, B.s);
         B.sync();
+        if (quadrature_nodes) *quadrature_nodes = nb;
         VmQuad Q{};
         Q.h = h;
         Q.bv = bv;
@@ -2439,9 +2441,12 @@ int sph_volume_map_build(sph_ctx *c, const sph_lattice *sdf, double h, const dou
         // one workgroup per band node, in launches of at most 2^22 workgroups
         for (uint32_t off = 0; off < nb; off += (1u << 22)) {
             const uint32_t cnt = std::min<uint32_t>(nb - off, 1u << 22);
+            Prof::Rec r{SPH_PROF_VM_QUADRATURE, cnt, nullptr, nullptr};
+            if (B.prof.on) B.prof.begin(B.s, r);
             hipLaunchKernelGGL(k_vm_quadrature, dim3(cnt), dim3(kBlock), 0, B.s, L.d, g, Q,
                                band.p + off, values.p, status.p);
             HIPCK(hipGetLastError());
+            if (B.prof.on) B.prof.end(B.s, r);
         }
         const size_t ncell = (size_t)n[0] * n[1] * n[2];
         LAUNCH(k_vm_cells, ncell, B.s, g, status.p, present.p);

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libsphhip.so")
 
 SPH_OK = 0
 PROF_KERNELS = ["neighbor_count", "pairs_volume", "boundary_sample", "nonpressure",
-                "source_diag", "pressure_accel", "jacobi"]
+                "source_diag", "pressure_accel", "jacobi", "vm_quadrature"]
 STATUS = {-1: "SPH_ERR_ARG", -2: "SPH_ERR_HIP", -3: "SPH_ERR_STATE", -4: "SPH_ERR_CAPACITY",
           -5: "SPH_ERR_UNSUPPORTED", -6: "SPH_ERR_NO_DEVICE"}
 
@@ -108,7 +108,7 @@ SIGNATURES = [
     ("sph_world_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
     ("sph_volume_map_build", C.c_int, [C.c_void_p, C.POINTER(SphLattice), C.c_double, _dp, _i64p,
-                                       C.c_double, _dp, _dp, C.c_int, _dp, _u8p]),
+                                       C.c_double, _dp, _dp, C.c_int, _dp, _u8p, _u64p]),
     ("sph_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("sph_world_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("sph_get_option", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),

@@ -257,6 +257,16 @@ def volume_map_fields(sd_fn, aabb_min, aabb_max, h, dx_particle):
     return sdf, vm
 
 
+def sdf_and_volume_map_grid(sd_fn, aabb_min, aabb_max, h, dx_particle):
+    """The SDF lattice (analytic node values) and the volume-map grid
+    (x_min, x_max, n, cell size) of one boundary on the reference's rule."""
+    cell = 4.0 * dx_particle
+    pad_sd = 3.1 * h
+    sdf = _lattice(aabb_min, aabb_max, pad_sd, cell, sd_fn, -pad_sd, pad_sd, sd_fn)
+    x_min, x_max, n = lattice_grid(aabb_min, aabb_max, 2.0 * h + cell / 6.0, cell)
+    return sdf, x_min, x_max, n, cell
+
+
 def volume_map_quadrature(sd_fn, aabb_min, aabb_max, h, dx_particle, builder, degree=30):
     """SDF + volume-map lattices on the reference's grid rule, the volume map
     integrated from the SDF lattice with the degree-`degree` Gauss-Legendre
@@ -266,10 +276,8 @@ def volume_map_quadrature(sd_fn, aabb_min, aabb_max, h, dx_particle, builder, de
     integration (``sph_amd.system.build_volume_map`` on the device). The
     reference takes its nodes from the gauss-quad crate; here they are numpy's
     ``leggauss`` nodes in ascending order."""
-    cell = 4.0 * dx_particle
-    pad_sd = 3.1 * h
-    sdf = _lattice(aabb_min, aabb_max, pad_sd, cell, sd_fn, -pad_sd, pad_sd, sd_fn)
-    x_min, x_max, n = lattice_grid(aabb_min, aabb_max, 2.0 * h + cell / 6.0, cell)
+    sdf, x_min, x_max, n, cell = sdf_and_volume_map_grid(sd_fn, aabb_min, aabb_max, h,
+                                                         dx_particle)
     gl_x, gl_w = np.polynomial.legendre.leggauss(degree)
     vals, pres = builder(sdf, h, x_min, n, cell, gl_x, gl_w)
     vm = Lattice(x_min, x_max, float(cell), n, np.ascontiguousarray(vals, dtype=np.float64),

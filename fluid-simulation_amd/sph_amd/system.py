@@ -377,11 +377,12 @@ class World:
             pass
 
 
-def build_volume_map(sdf, h: float, x_min, n, dx: float, gl_x, gl_w, system=None):
+def build_volume_map(sdf, h: float, x_min, n, dx: float, gl_x, gl_w, system=None, stats=None):
     """Volume-map node values and present flags on the lattice (x_min, n, dx)
     from the SDF lattice `sdf` (``DiscretizedBoundaryFields::new``,
     ``volume_map_boundary.rs:1593-1624``), with the Gauss-Legendre rule
-    (gl_x, gl_w) on [-1, 1]; runs on the stream of `system` if given."""
+    (gl_x, gl_w) on [-1, 1]; runs on the stream of `system` if given.
+    `stats["quadrature_nodes"]` receives the number of ball quadratures run."""
     s = _lattice_struct(sdf)
     xm = np.ascontiguousarray(x_min, dtype=np.float64)
     nn = np.ascontiguousarray(n, dtype=np.int64)
@@ -393,10 +394,13 @@ def build_volume_map(sdf, h: float, x_min, n, dx: float, gl_x, gl_w, system=None
     vals = np.empty(shape, dtype=np.float64)
     pres = np.empty(shape, dtype=np.uint8)
     handle = system._ctx.h if system is not None else None
+    q = C.c_uint64()
     N.check(N.lib().sph_volume_map_build(handle, C.byref(s), float(h), N.dptr(xm),
                                          nn.ctypes.data_as(N._i64p), float(dx), N.dptr(gx),
                                          N.dptr(gw), len(gx), N.dptr(vals),
-                                         pres.ctypes.data_as(N._u8p)))
+                                         pres.ctypes.data_as(N._u8p), C.byref(q)))
+    if stats is not None:
+        stats["quadrature_nodes"] = int(q.value)
     return vals, pres
 
 

@@ -174,7 +174,8 @@ enum {
     SPH_PROF_SOURCE_DIAG = 4,      /* IISPH source term + diagonal + initial pressure */
     SPH_PROF_PRESSURE_ACCEL = 5,   /* pressure acceleration */
     SPH_PROF_JACOBI = 6,           /* relaxed Jacobi update + error */
-    SPH_PROF_KERNELS = 7
+    SPH_PROF_VM_QUADRATURE = 7,    /* volume-map ball quadrature (units: band nodes) */
+    SPH_PROF_KERNELS = 8
 };
 int sph_profile(sph_ctx *ctx, int enable);
 int sph_profile_read(sph_ctx *ctx, int kernel, uint64_t *launches, uint64_t *units, double *ms);
@@ -248,10 +249,12 @@ int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
  * (CubicSerendipityDiscretization::new, discretization.rs:99-245, lower bound
  * 0). Outputs: values and present flags of the dense (3n+1)^3 nodes, x
  * fastest. degree <= 32. `ctx` may be NULL (the build then uses a stream of
- * its own); otherwise it runs on the context's stream. */
+ * its own); otherwise it runs on the context's stream. `quadrature_nodes`
+ * (may be NULL) receives the number of nodes that ran the ball quadrature. */
 int sph_volume_map_build(sph_ctx *ctx, const sph_lattice *sdf, double h, const double x_min[3],
                          const int64_t n[3], double dx, const double *gl_x, const double *gl_w,
-                         int degree, double *values_out, uint8_t *present_out);
+                         int degree, double *values_out, uint8_t *present_out,
+                         uint64_t *quadrature_nodes);
 
 /* free and total device memory (hipMemGetInfo), for admission control */
 int sph_device_memory(size_t *free_bytes, size_t *total_bytes);

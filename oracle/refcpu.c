@@ -1539,7 +1539,7 @@ int ref_vm_node_value(const ref_lattice *sdf, const double p[3], double h, const
     return 0;
 }
 
-void ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const int64_t n[3],
+uint64_t ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const int64_t n[3],
                   double dx, const double *gl_x, const double *gl_w, int degree, double *values,
                   uint8_t *present) {
     const int64_t nx = 3 * n[0] + 1, ny = 3 * n[1] + 1, nz = 3 * n[2] + 1;
@@ -1552,7 +1552,8 @@ void ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const
                 for (int i = 0; i < 32; i++)
                     isnode[((3 * cz + NODE_OFFS[i][2]) * ny + (3 * cy + NODE_OFFS[i][1])) * nx +
                            (3 * cx + NODE_OFFS[i][0])] = 1;
-#pragma omp parallel for schedule(dynamic, 64)
+    uint64_t quadratures = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : quadratures)
     for (int64_t k = 0; k < (int64_t)nodes; k++) {
         values[k] = 0.0;
         present[k] = 0;
@@ -1561,7 +1562,8 @@ void ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const
         /* x_min + key / 3 * dx (discretization.rs:171-175) */
         const double p[3] = {x_min[0] + (double)ix / 3.0 * dx, x_min[1] + (double)iy / 3.0 * dx,
                              x_min[2] + (double)iz / 3.0 * dx};
-        double v;
+        double v, sc;
+        if (ref_serendipity_function(sdf, p, &sc) == 0 && sc < 2.0 * h && sc > -h) quadratures++;
         if (ref_vm_node_value(sdf, p, h, gl_x, gl_w, degree, &v) == 0) {
             values[k] = v;
             ok[k] = 1;
@@ -1585,4 +1587,5 @@ void ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const
         if (!present[k]) values[k] = 0.0;
     free(ok);
     free(isnode);
+    return quadratures;
 }

@@ -191,8 +191,9 @@ int ref_vm_node_value(const ref_lattice *sdf, const double p[3], double h, const
 /* The volume-map lattice over the cells of (x_min, n, dx)
  * (CubicSerendipityDiscretization::new, discretization.rs:99-245, prune lower
  * bound 0): values[] and present[] of the dense (3n+1)^3 nodes. A cell is kept
- * iff all 32 of its node values evaluate; present = node of a kept cell. */
-void ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const int64_t n[3],
+ * iff all 32 of its node values evaluate; present = node of a kept cell.
+ * Returns the number of nodes that ran the ball quadrature. */
+uint64_t ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], const int64_t n[3],
                   double dx, const double *gl_x, const double *gl_w, int degree, double *values,
                   uint8_t *present);
 

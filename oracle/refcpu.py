@@ -130,6 +130,7 @@ def lib():
         L.ref_vm_node_value.restype = C.c_int
         L.ref_vm_node_value.argtypes = [C.POINTER(RefLattice), _dp, C.c_double, _dp, _dp,
                                         C.c_int, _dp]
+        L.ref_vm_build.restype = C.c_uint64
         L.ref_vm_build.argtypes = [C.POINTER(RefLattice), C.c_double, _dp, _i64p, C.c_double,
                                    _dp, _dp, C.c_int, _dp, _u8p]
         _lib = L
@@ -230,16 +231,19 @@ def vm_node_value(sdf, p, h, gl_x, gl_w):
     return st, out.value
 
 
-def vm_build(sdf, h, x_min, n, dx, gl_x, gl_w):
-    """Volume-map node values and present flags, shape (3n_z+1, 3n_y+1, 3n_x+1)."""
+def vm_build(sdf, h, x_min, n, dx, gl_x, gl_w, stats=None):
+    """Volume-map node values and present flags, shape (3n_z+1, 3n_y+1, 3n_x+1);
+    `stats["quadrature_nodes"]` receives the number of ball quadratures run."""
     s = lattice_struct(sdf)
     xm, gx, gw = f64c(x_min), f64c(gl_x), f64c(gl_w)
     nn = np.ascontiguousarray(n, dtype=np.int64)
     shape = (3 * int(nn[2]) + 1, 3 * int(nn[1]) + 1, 3 * int(nn[0]) + 1)
     vals = np.zeros(shape, dtype=np.float64)
     pres = np.zeros(shape, dtype=np.uint8)
-    lib().ref_vm_build(C.byref(s), float(h), dptr(xm), nn.ctypes.data_as(_i64p), float(dx),
-                       dptr(gx), dptr(gw), len(gx), dptr(vals), pres.ctypes.data_as(_u8p))
+    q = lib().ref_vm_build(C.byref(s), float(h), dptr(xm), nn.ctypes.data_as(_i64p), float(dx),
+                           dptr(gx), dptr(gw), len(gx), dptr(vals), pres.ctypes.data_as(_u8p))
+    if stats is not None:
+        stats["quadrature_nodes"] = int(q)
     return vals, pres
 
 

@@ -27,11 +27,15 @@ def same_bits(a, b):
 
 
 def check_pair(native, sd_fn, lo, hi, h, dx, degree, system=None):
-    build = lambda *a: native.build_volume_map(*a, system=system)
+    sg, sr = {}, {}
+    build = lambda *a: native.build_volume_map(*a, system=system, stats=sg)
     sdf, vm_g = scenes.volume_map_quadrature(sd_fn, lo, hi, h, dx, build, degree=degree)
-    _, vm_r = scenes.volume_map_quadrature(sd_fn, lo, hi, h, dx, refcpu.vm_build, degree=degree)
+    ref = lambda *a: refcpu.vm_build(*a, stats=sr)
+    _, vm_r = scenes.volume_map_quadrature(sd_fn, lo, hi, h, dx, ref, degree=degree)
     print(f"volume map {vm_g.n.tolist()} cells, degree {degree}: "
-          f"{int(vm_g.present.sum())} present nodes", flush=True)
+          f"{int(vm_g.present.sum())} present nodes, {sg['quadrature_nodes']} quadratures",
+          flush=True)
+    assert sg["quadrature_nodes"] == sr["quadrature_nodes"] > 0
     assert np.array_equal(vm_g.present, vm_r.present)
     assert same_bits(vm_g.values, vm_r.values)
     return vm_g
