@@ -23,6 +23,10 @@ world, HBM-budget admission) and, as the baseline, one request at a time with
 the plain per-scene path; it reports throughput and p50/p99 step latency of
 both.
 
+--workload sensor_plane times sensor queries on the same 1M-particle state
+instead: the six ScalarQuantity fields (one per step, in turn) on a
+1024 x 1024 plane through the ball's center (get_quantity_at_positions).
+
 --workload volume_map times the construction of the same workload's boundary
 field instead: the volume map of the radius-1 m sphere container (h = 0.02,
 cell 0.04) integrated from its SDF lattice with the degree-30 Gauss-Legendre
@@ -172,6 +176,89 @@ def serving_bench(args):
     print(json.dumps(out))
 
 
+def sensor_plane_bench(args):
+    """get_quantity_at_positions on a 1024^2 plane through the 1M-particle ball."""
+    from sph_amd import system
+    from sph_amd._native import QUANTITIES
+    scene = build_scene("two_fluid_ball", args.side)
+    sim = system.System(scene)
+    for _ in range(args.warmup):
+        sim.step_forward_in_time(1)
+    res = 1024
+    u = np.linspace(-1.0, 1.0, res)
+    gx, gy = np.meshgrid(u, u, indexing="xy")
+    plane = np.stack([gx.ravel(), gy.ravel(), np.zeros(res * res)], axis=1).astype(np.float32)
+    sim.get_quantity_at_positions("Density", plane)  # first-call allocations
+    sim.profile(True)
+    lat = []
+    for k in range(args.steps):
+        t = time.perf_counter()
+        vals = sim.get_quantity_at_positions(QUANTITIES[k % 6], plane)
+        lat.append(time.perf_counter() - t)
+    prof = sim.profile_read()["quantity"]
+    sim.profile(False)
+    nq = len(plane)
+    total = float(np.sum(lat))
+    avg_s = prof["ms"] / 1e3 / max(prof["launches"], 1)
+    per_query = 24 + 4  # f64 position read, f32 value written
+    achieved = per_query * nq / avg_s / 1e9 if avg_s > 0 else 0.0
+    out = {
+        "metric": "sensor-plane queries/s (get_quantity_at_positions, 1M-particle state)",
+        "value": nq * args.steps / total,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "p50_step_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_step_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lattice + jitter, analytic volume-map boundary)",
+        "config": {"workload": f"sensor_plane_{res}x{res}_{scene.name}", "particles": len(sim),
+                   "queries_per_step": nq, "quantities": QUANTITIES,
+                   "nonzero_fraction_last": float(np.count_nonzero(vals)) / nq},
+        "kernel_queries_per_s": nq / avg_s if avg_s > 0 else 0.0,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_quantity",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": per_query * nq,
+            "avg_launch_ms": avg_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import refcpu
+            refcpu.lib().ref_system_set_threads(threads)
+            o = refcpu.OracleSystem(scene, threads=threads)
+            sub = plane[::16]
+            t0 = time.perf_counter()
+            for k in range(6):
+                o.quantity_at_positions(k, sub)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {
+                "value": 6 * len(sub) / dt, "unit": "queries/s", "cores": threads,
+                "kind": "port",
+                "sample": (f"oracle ref_system_quantity_at_positions (binary64 C restatement, "
+                           f"OpenMP) on every 16th plane point ({len(sub)} queries), all six "
+                           f"quantities, on the initial state (System::new): {dt:.1f} s"),
+            }
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline unavailable: {e}")
+    sim._ctx.close()
+    print(json.dumps(out))
+
+
 def volume_map_bench(args):
     """Volume-map construction of the two-fluid ball's sphere container."""
     from sph_amd import scenes, system
@@ -307,6 +394,9 @@ def main():
         return
     if args.workload == "volume_map":
         volume_map_bench(args)
+        return
+    if args.workload == "sensor_plane":
+        sensor_plane_bench(args)
         return
 
     from sph_amd import system

@@ -325,24 +325,18 @@ __global__ void kb_vm_sample(BatchDev B) {
         const BodyDev &b = B.bodies[v.body_base + bi];
         if (b.kind != 1) continue;
         uint8_t ok = 0;
-        double sd, gx, gy, gz, vol;
-        if (!body_sd(b, px, py, pz, sd) && sd < range && !body_sdg(b, px, py, pz, gx, gy, gz)) {
-            double gn = sqrt(dot3(gx, gy, gz, gx, gy, gz));
-            if (gn > 0.5 && gn < 2.0 && !body_vm(b, px, py, pz, vol) && vol > 0.) {
-                vol = fmin(vol, v.vmax_ball);
-                double d = fmax(sd + 0.25 * v.dxr, v.dxr);
-                double ox = px - (gx / gn) * d, oy = py - (gy / gn) * d, oz = pz - (gz / gn) * d;
-                double vx, vy, vz;
-                body_velocity(b, ox, oy, oz, vx, vy, vz);
-                gm(const_cast<double *>(b.bx))[i] = ox;
-                gm(const_cast<double *>(b.by))[i] = oy;
-                gm(const_cast<double *>(b.bz))[i] = oz;
-                gm(const_cast<double *>(b.bvx))[i] = vx;
-                gm(const_cast<double *>(b.bvy))[i] = vy;
-                gm(const_cast<double *>(b.bvz))[i] = vz;
-                gm(const_cast<double *>(b.bvol))[i] = vol;
-                ok = 1;
-            }
+        double ox, oy, oz, vol;
+        if (vm_normal_sample(b, px, py, pz, range, v.dxr, v.vmax_ball, ox, oy, oz, vol)) {
+            double vx, vy, vz;
+            body_velocity(b, ox, oy, oz, vx, vy, vz);
+            gm(const_cast<double *>(b.bx))[i] = ox;
+            gm(const_cast<double *>(b.by))[i] = oy;
+            gm(const_cast<double *>(b.bz))[i] = oz;
+            gm(const_cast<double *>(b.bvx))[i] = vx;
+            gm(const_cast<double *>(b.bvy))[i] = vy;
+            gm(const_cast<double *>(b.bvz))[i] = vz;
+            gm(const_cast<double *>(b.bvol))[i] = vol;
+            ok = 1;
         }
         gm(const_cast<uint8_t *>(b.flag))[i] = ok;
     }

@@ -842,8 +842,10 @@ static void cell_box(Batch &B, size_t cnt, double cs, const double *X, const dou
     HIPCK(hipGetLastError());
 }
 
+// counting sort by cell; members of a cell ordered by `ref` (or by index)
 static void cell_sort(Batch &B, size_t cnt, const int *ci, const GridDev &g,
-                      DBuf<uint32_t> &start, DBuf<uint32_t> &perm) {
+                      DBuf<uint32_t> &start, DBuf<uint32_t> &perm,
+                      const uint32_t *ref = nullptr) {
     size_t ncell = (size_t)g.dim[0] * g.dim[1] * g.dim[2];
     B.ucnt.reserve(ncell);
     start.reserve(ncell + 1);
@@ -854,7 +856,7 @@ static void cell_sort(Batch &B, size_t cnt, const int *ci, const GridDev &g,
     LAUNCH(k_cell_count, cnt, B.s, cnt, ci, g, B.ucnt.p);
     scan_u32(B, B.ucnt.p, ncell, start.p);
     LAUNCH(k_cell_scatter, cnt, B.s, cnt, ci, g, start.p, B.ucursor.p, perm.p);
-    LAUNCH(k_cell_order, ncell, B.s, ncell, start.p, perm.p, (const uint32_t *)nullptr);
+    LAUNCH(k_cell_order, ncell, B.s, ncell, start.p, perm.p, ref);
 }
 
 // ------------------------------------------------------------------------
@@ -2358,6 +2360,114 @@ int sph_world_set_option(sph_world *w, int option, int value) {
     return guarded([&] {
         if (!w) throw Error(SPH_ERR_ARG, "null argument");
         set_option(w->batch, option, value);
+    });
+}
+
+int sph_quantity_at_positions(sph_ctx *c, int quantity, size_t nq, const float *pos,
+                              float *out) {
+    return guarded([&] {
+        if (!c || (!pos && nq) || (!out && nq)) throw Error(SPH_ERR_ARG, "null argument");
+        if (quantity < SPH_QUANTITY_SPEED || quantity > SPH_QUANTITY_KINETIC_ENERGY)
+            throw Error(SPH_ERR_ARG, "unknown quantity");
+        if (nq >= 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "more than 2^32-1 positions");
+        check_solo(c);
+        Scene &S = c->scene;
+        if (!S.initialized) throw Error(SPH_ERR_STATE, "system not initialized");
+        if (nq == 0) return;
+        Batch &B = c->own;
+        const size_t n = S.n;
+        const double cs = S.P.kernel_support_radius;
+        // f32 positions widened to f64 (sph/mod.rs:247-250)
+        std::vector<double> hx(nq), hy(nq), hz(nq);
+        for (size_t k = 0; k < nq; k++) {
+            hx[k] = (double)pos[3 * k];
+            hy[k] = (double)pos[3 * k + 1];
+            hz[k] = (double)pos[3 * k + 2];
+        }
+        DBuf<double> qx, qy, qz;
+        qx.upload(hx.data(), nq, B.s);
+        qy.upload(hy.data(), nq, B.s);
+        qz.upload(hz.data(), nq, B.s);
+        DBuf<int> fci, bbf;
+        fci.reserve(3 * std::max<size_t>(n, 1));
+        bbf.reserve(6);
+        cell_box(B, n, cs, S.x.p, S.y.p, S.z.p, fci.p, bbf.p);
+        int b1[6];
+        HIPCK(hipMemcpyAsync(b1, bbf.p, sizeof b1, hipMemcpyDeviceToHost, B.s));
+        B.sync();
+        // Duplicate neighbors in the reference need two cells of one stencil
+        // in one hash bucket with occupied members in range, so the guard
+        // covers the occupied cells (fluid and static samples) widened by the
+        // stencil reach; a query elsewhere only meets members out of range.
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        bool occupied = n > 0;
+        if (n)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = b1[k];
+                hi[k] = b1[3 + k];
+            }
+        std::vector<BodyDev> bv;
+        std::vector<GridDev> sg;
+        for (auto &b : S.bodies) {
+            bv.push_back(b->view());
+            sg.push_back(b->sg);
+            if (b->kind == 0 && b->ns)
+                for (int k = 0; k < 3; k++) {
+                    const int a = b->sg.org[k], z = b->sg.org[k] + b->sg.dim[k] - 1;
+                    lo[k] = occupied ? std::min(lo[k], a) : a;
+                    hi[k] = occupied ? std::max(hi[k], z) : z;
+                }
+            occupied = occupied || (b->kind == 0 && b->ns);
+        }
+        if (occupied) {
+            int checked[6] = {1, 1, 1, 0, 0, 0};
+            bool valid = false;
+            hash_guard(B, checked, valid, lo, hi, S.reach);
+        }
+        // the fluid's grid (one empty cell when there is no fluid)
+        const int none[6] = {0, 0, 0, 0, 0, 0};
+        GridDev g = make_grid(n ? b1 : none, n ? b1 + 3 : none + 3);
+        DBuf<uint32_t> start, perm;
+        cell_sort(B, n, fci.p, g, start, perm, S.ref.p);
+        g.start = start.p;
+        DBuf<BodyDev> bodies;
+        DBuf<GridDev> sgrid;
+        if (!bv.empty()) {
+            bodies.upload(bv.data(), bv.size(), B.s);
+            sgrid.upload(sg.data(), sg.size(), B.s);
+        }
+        QueryConst C{};
+        C.K = S.K;
+        C.V0 = S.V0;
+        C.cs = cs;
+        C.dthr = S.dthr;
+        C.dxr = S.P.rest_density_grid_spacing;
+        C.vmax = 4.0 / 3.0 * 3.14159265358979323846 * ((cs * cs) * cs);  // ball_volume
+        C.reach = S.reach;
+        QueryFluid F{};
+        F.x = S.x.p; F.y = S.y.p; F.z = S.z.p;
+        F.vx = S.vx.p; F.vy = S.vy.p; F.vz = S.vz.p;
+        F.m = S.m.p; F.V = S.V.p; F.p = S.p.p;
+        F.perm = perm.p;
+        F.g = g;
+        DBuf<float> res;
+        res.reserve(nq);
+        const int nb = (int)bv.size();
+        Prof::Rec r{SPH_PROF_QUANTITY, nq, nullptr, nullptr};
+        if (B.prof.on) B.prof.begin(B.s, r);
+#define QLAUNCH(Q) LAUNCH(k_quantity<Q>, nq, B.s, nq, qx.p, qy.p, qz.p, C, F, nb, bodies.p, sgrid.p, res.p)
+        switch (quantity) {
+            case SPH_QUANTITY_SPEED: QLAUNCH(kQSpeed); break;
+            case SPH_QUANTITY_VOLUME: QLAUNCH(kQVolume); break;
+            case SPH_QUANTITY_DENSITY: QLAUNCH(kQDensity); break;
+            case SPH_QUANTITY_DENSITY_ERROR: QLAUNCH(kQDensityError); break;
+            case SPH_QUANTITY_PRESSURE: QLAUNCH(kQPressure); break;
+            default: QLAUNCH(kQKineticEnergy); break;
+        }
+#undef QLAUNCH
+        if (B.prof.on) B.prof.end(B.s, r);
+        res.download(out, nq, B.s);
+        B.sync();
     });
 }
 

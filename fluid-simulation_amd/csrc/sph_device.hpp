@@ -321,6 +321,26 @@ __device__ inline int body_vm(const BodyDev &b, double px, double py, double pz,
     return serendipity(b.vm, lx, ly, lz, v);
 }
 
+// The normal-mode boundary sample of one position (volume_map_boundary.rs:
+// 1259-1346): the point d = max(sd + dx/4, dx) inside the surface along the SDF
+// gradient, with the volume-map value capped at the ball volume. False when the
+// position has none (sd not below the range, an unusable gradient or volume).
+__device__ inline bool vm_normal_sample(const BodyDev &b, double px, double py, double pz,
+                                        double range, double dxr, double vmax, double &ox,
+                                        double &oy, double &oz, double &vol) {
+    double sd, gx, gy, gz;
+    if (body_sd(b, px, py, pz, sd) || !(sd < range) || body_sdg(b, px, py, pz, gx, gy, gz))
+        return false;
+    const double gn = sqrt(dot3(gx, gy, gz, gx, gy, gz));
+    if (!(gn > 0.5 && gn < 2.0) || body_vm(b, px, py, pz, vol) || !(vol > 0.)) return false;
+    vol = fmin(vol, vmax);
+    const double d = fmax(sd + 0.25 * dxr, dxr);
+    ox = px - (gx / gn) * d;
+    oy = py - (gy / gn) * d;
+    oz = pz - (gz / gn) * d;
+    return true;
+}
+
 // the four viscosity samples around the normal sample (volume_map_boundary.rs:1347-1376)
 __device__ inline void visc_samples(double px, double py, double pz, double ox, double oy,
                                     double oz, double range, double (&vp)[4][3]) {

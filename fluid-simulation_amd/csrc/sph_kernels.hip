@@ -424,3 +424,129 @@ __global__ void k_vm_finish(size_t n, const uint8_t *present, double *values) {
 }
 
 }  // namespace sph
+
+namespace sph {
+
+// ------------------------------------------------------------------------
+// Quantities at arbitrary positions (SPHSystem::get_quantity_at_positions,
+// sph/mod.rs:241-341; quantities.rs:14-330). One thread per query walks the
+// fluid's (2c+1)^3 stencil in the order of k_neighbors, so every sum takes
+// its terms in the reference's neighbor order; boundaries contribute to the
+// speed and the volume only, as in the reference.
+// ------------------------------------------------------------------------
+template <class Fn>
+__device__ __forceinline__ void stencil_walk(const GridDev &g, int cx, int cy, int cz, int reach,
+                                             Fn &&fn) {
+    int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
+    if (zlo < 0) zlo = 0;
+    if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
+    if (zlo > zhi) return;
+    for (int dx = -reach; dx <= reach; dx++) {
+        const int gx = cx + dx - g.org[0];
+        if (gx < 0 || gx >= g.dim[0]) continue;
+        for (int dy = -reach; dy <= reach; dy++) {
+            const int gy = cy + dy - g.org[1];
+            if (gy < 0 || gy >= g.dim[1]) continue;
+            const uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
+            const uint32_t a = g.start[row + zlo], b = g.start[row + zhi + 1];
+            for (uint32_t j = a; j < b; j++) fn(j);
+        }
+    }
+}
+
+template <int Q>
+__global__ void k_quantity(size_t nq, const double *qx, const double *qy, const double *qz,
+                           QueryConst C, QueryFluid F, int nb, const BodyDev *bodies,
+                           const GridDev *sgrid, float *out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const double px = qx[i], py = qy[i], pz = qz[i];
+    const int cx = sat_i32(floor(px / C.cs)), cy = sat_i32(floor(py / C.cs)),
+              cz = sat_i32(floor(pz / C.cs));
+    const KernelConst K = C.K;
+    double acc = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+    stencil_walk(F.g, cx, cy, cz, C.reach, [&](uint32_t j) {
+        const uint32_t s = F.perm[j];
+        const double rx = px - F.x[s], ry = py - F.y[s], rz = pz - F.z[s];
+        const double d2 = (rx * rx + ry * ry) + rz * rz;
+        if (!(d2 < C.dthr)) return;
+        const double w = K.wfac * kernel_w(sqrt(d2) / K.h);
+        if (Q == kQSpeed) {  // get_speed (quantities.rs:64-120)
+            const double Vs = F.V[s];
+            ax += (F.vx[s] * Vs) * w;
+            ay += (F.vy[s] * Vs) * w;
+            az += (F.vz[s] * Vs) * w;
+        } else if (Q == kQVolume) {  // get_volume (:14-61)
+            acc += C.V0 * w;
+        } else if (Q == kQDensity) {  // get_density (:123-170)
+            acc += F.m[s] * w;
+        } else if (Q == kQDensityError) {
+            // get_density_error (:173-228) over the masses its caller passes
+            // as sample volumes (sph/mod.rs:297-306)
+            const double ms = F.m[s];
+            if (ms < C.V0) acc += ((C.V0 / ms - 1.) * ms) * w;
+        } else if (Q == kQPressure) {  // get_pressure (:231-280)
+            acc += (F.p[s] * F.V[s]) * w;
+        } else {  // get_kinetic_energy (:283-330)
+            const double ux = F.vx[s], uy = F.vy[s], uz = F.vz[s];
+            acc += (((0.5 * F.m[s]) * ((ux * ux + uy * uy) + uz * uz)) * F.V[s]) * w;
+        }
+    });
+    if (Q == kQSpeed || Q == kQVolume) {
+        for (int bi = 0; bi < nb; bi++) {
+            const BodyDev &b = bodies[bi];
+            if (b.kind == 0) {  // static samples (velocity zero)
+                stencil_walk(sgrid[bi], cx, cy, cz, C.reach, [&](uint32_t j) {
+                    const double rx = px - b.sx[j], ry = py - b.sy[j], rz = pz - b.sz[j];
+                    const double d2 = (rx * rx + ry * ry) + rz * rz;
+                    if (!(d2 < C.dthr)) return;
+                    const double w = K.wfac * kernel_w(sqrt(d2) / K.h);
+                    if (Q == kQVolume) {
+                        acc += b.svol[j] * w;
+                    } else {
+                        ax += (0.0 * b.svol[j]) * w;
+                        ay += (0.0 * b.svol[j]) * w;
+                        az += (0.0 * b.svol[j]) * w;
+                    }
+                });
+            } else {
+                double ox, oy, oz, vol;
+                if (!vm_normal_sample(b, px, py, pz, K.h, C.dxr, C.vmax, ox, oy, oz, vol)) continue;
+                const double w = kernel_fn(K, px - ox, py - oy, pz - oz);
+                if (Q == kQVolume) {
+                    acc += vol * w;
+                } else {
+                    double vx, vy, vz;
+                    body_velocity(b, ox, oy, oz, vx, vy, vz);
+                    ax += (vx * vol) * w;
+                    ay += (vy * vol) * w;
+                    az += (vz * vol) * w;
+                }
+            }
+        }
+    }
+    double res;
+    if (Q == kQSpeed)
+        res = sqrt((ax * ax + ay * ay) + az * az);
+    else if (Q == kQVolume)
+        res = C.V0 / acc;
+    else if (Q == kQDensityError)
+        res = 100. * acc;
+    else
+        res = acc;
+    out[i] = (float)res;
+}
+
+#define INSTANTIATE_QUANTITY(Q)                                                                  \
+    template __global__ void k_quantity<Q>(size_t, const double *, const double *,              \
+                                           const double *, QueryConst, QueryFluid, int,          \
+                                           const BodyDev *, const GridDev *, float *);
+INSTANTIATE_QUANTITY(kQSpeed)
+INSTANTIATE_QUANTITY(kQVolume)
+INSTANTIATE_QUANTITY(kQDensity)
+INSTANTIATE_QUANTITY(kQDensityError)
+INSTANTIATE_QUANTITY(kQPressure)
+INSTANTIATE_QUANTITY(kQKineticEnergy)
+#undef INSTANTIATE_QUANTITY
+
+}  // namespace sph

@@ -52,6 +52,24 @@ __global__ void k_static_volume(size_t n, KernelConst K, double gamma_v, const d
 __global__ void k_map_indices(size_t n, const uint32_t *perm, const uint32_t *in, uint64_t *out);
 
 // volume-map construction
+// Sensor queries (SPHSystem::get_quantity_at_positions, sph/mod.rs:241-341):
+// the fluid in (cell, reference index) order through `perm`, read in place
+struct QueryFluid {
+    const double *x, *y, *z, *vx, *vy, *vz, *m, *V, *p;
+    const uint32_t *perm;  // sorted slot -> storage index
+    GridDev g;
+};
+struct QueryConst {
+    KernelConst K;
+    double V0, cs, dthr, dxr, vmax;
+    int reach;
+};
+enum { kQSpeed = 0, kQVolume, kQDensity, kQDensityError, kQPressure, kQKineticEnergy };
+template <int Q>
+__global__ void k_quantity(size_t nq, const double *qx, const double *qy, const double *qz,
+                           QueryConst C, QueryFluid F, int nb, const BodyDev *bodies,
+                           const GridDev *sgrid, float *out);
+
 constexpr int kVmMaxDegree = 32;
 struct VmGrid {
     double x_min[3];

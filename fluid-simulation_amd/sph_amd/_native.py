@@ -15,8 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsphhip.so")
 
 SPH_OK = 0
+# ScalarQuantity (render_info.rs:205-212)
+QUANTITIES = ["Speed", "Volume", "Density", "DensityError", "Pressure", "KineticEnergy"]
 PROF_KERNELS = ["neighbor_count", "pairs_volume", "boundary_sample", "nonpressure",
-                "source_diag", "pressure_accel", "jacobi", "vm_quadrature"]
+                "source_diag", "pressure_accel", "jacobi", "vm_quadrature", "quantity"]
 STATUS = {-1: "SPH_ERR_ARG", -2: "SPH_ERR_HIP", -3: "SPH_ERR_STATE", -4: "SPH_ERR_CAPACITY",
           -5: "SPH_ERR_UNSUPPORTED", -6: "SPH_ERR_NO_DEVICE"}
 
@@ -107,6 +109,8 @@ SIGNATURES = [
     ("sph_world_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("sph_world_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
+    ("sph_quantity_at_positions", C.c_int, [C.c_void_p, C.c_int, C.c_size_t,
+                                            C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("sph_volume_map_build", C.c_int, [C.c_void_p, C.POINTER(SphLattice), C.c_double, _dp, _i64p,
                                        C.c_double, _dp, _dp, C.c_int, _dp, _u8p, _u64p]),
     ("sph_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),

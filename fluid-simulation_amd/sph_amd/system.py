@@ -197,6 +197,20 @@ class System:
                                            out["fluid_id"].ctypes.data_as(N._u32p)))
         return out
 
+    def get_quantity_at_positions(self, quantity, positions) -> np.ndarray:
+        """``SPHSystem::get_quantity_at_positions`` (``sph/mod.rs:241-341``):
+        ``quantity`` is a ScalarQuantity name ("Speed", "Volume", "Density",
+        "DensityError", "Pressure", "KineticEnergy") or its index; positions
+        are [n][3] f32. Returns n f32 values."""
+        q = N.QUANTITIES.index(quantity) if isinstance(quantity, str) else int(quantity)
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        out = np.empty(len(pos), dtype=np.float32)
+        fp = C.POINTER(C.c_float)
+        N.check(N.lib().sph_quantity_at_positions(self._ctx.h, q, len(pos),
+                                                  pos.ctypes.data_as(fp),
+                                                  out.ctypes.data_as(fp)))
+        return out
+
     def error_history(self):
         cnt = C.c_size_t()
         buf = np.zeros(100000)

@@ -175,7 +175,8 @@ enum {
     SPH_PROF_PRESSURE_ACCEL = 5,   /* pressure acceleration */
     SPH_PROF_JACOBI = 6,           /* relaxed Jacobi update + error */
     SPH_PROF_VM_QUADRATURE = 7,    /* volume-map ball quadrature (units: band nodes) */
-    SPH_PROF_KERNELS = 8
+    SPH_PROF_QUANTITY = 8,         /* sensor queries (units: positions) */
+    SPH_PROF_KERNELS = 9
 };
 int sph_profile(sph_ctx *ctx, int enable);
 int sph_profile_read(sph_ctx *ctx, int kernel, uint64_t *launches, uint64_t *units, double *ms);
@@ -235,6 +236,28 @@ int sph_world_size(const sph_world *world, size_t *systems, size_t *particles);
 int sph_world_profile(sph_world *world, int enable);
 int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
                            uint64_t *units, double *ms);
+/* ---- sensor queries -------------------------------------------------------------
+ * SPHSystem::get_quantity_at_positions (sph/mod.rs:241-341): a scalar field
+ * of the current fluid state at nq positions ([nq][3] f32, widened to f64),
+ * each an SPH sum over the fluid neighbors of the position within the kernel
+ * support (and, for the speed and the volume, the boundaries' samples of the
+ * position), in the reference's neighbor order; out: nq f32 values. The
+ * quantities are those of quantities.rs: get_speed (:64-120), get_volume
+ * (:14-61), get_density (:123-170), get_density_error (:173-228, evaluated
+ * over the fluid masses, which the reference passes as its sample volumes),
+ * get_pressure (:231-280), get_kinetic_energy (:283-330). The system's state
+ * is not changed. */
+enum {
+    SPH_QUANTITY_SPEED = 0,  /* ScalarQuantity order (render_info.rs:205-212) */
+    SPH_QUANTITY_VOLUME = 1,
+    SPH_QUANTITY_DENSITY = 2,
+    SPH_QUANTITY_DENSITY_ERROR = 3,
+    SPH_QUANTITY_PRESSURE = 4,
+    SPH_QUANTITY_KINETIC_ENERGY = 5
+};
+int sph_quantity_at_positions(sph_ctx *ctx, int quantity, size_t nq, const float *pos,
+                              float *out);
+
 /* ---- boundary-field construction (setup of VolumeMapBoundary) ----------------
  * The volume map of DiscretizedBoundaryFields::new (volume_map_boundary.rs:
  * 1593-1624) over the cells (x_min, n, dx): for every serendipity node p,

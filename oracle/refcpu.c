@@ -1459,6 +1459,99 @@ size_t ref_system_boundary_samples(const ref_system *s, int bi, size_t i, int mo
     return (size_t)(hi - lo);
 }
 
+/* SPHSystem::get_quantity_at_positions (sph/mod.rs:241-341) with the
+ * quantities of quantities.rs: get_speed (:64-120), get_volume (:14-61),
+ * get_density (:123-170), get_density_error (:173-228; the caller passes the
+ * fluid masses as its `sample_volumes`, sph/mod.rs:297-306, restated as is),
+ * get_pressure (:231-280), get_kinetic_energy (:283-330). Queries are f32
+ * positions widened to f64; results are narrowed to f32. The boundaries'
+ * sample lists are rebuilt for the queries and then for the fluid again. */
+int ref_system_quantity_at_positions(ref_system *s, int quantity, size_t nq, const float *pos,
+                                     float *out) {
+    if (quantity < 0 || quantity > 5) return -1;
+    const double h = s->P.kernel_support_radius, V0 = s->rest_volume;
+    double *q = dalloc(3 * nq);
+    for (size_t k = 0; k < 3 * nq; k++) q[k] = (double)pos[k];
+    ref_nlist nl;
+    memset(&nl, 0, sizeof nl);
+    ref_find_samples(h, h, q, nq, s->x, s->n, &nl);
+    double *x0 = s->x;
+    const size_t n0 = s->n;
+    s->x = q;
+    s->n = nq;
+    find_boundary_samples(s);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < nq; i++) {
+        const double *pi = q + 3 * i;
+        double res = 0.0;
+        if (quantity == 0) { /* Speed */
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (uint64_t e = nl.offsets[i]; e < nl.offsets[i + 1]; e++) {
+                const uint64_t j = nl.indices[e];
+                double r[3];
+                sub3(pi, x0 + 3 * j, r);
+                const double w = ref_kernel_function(r, h);
+                for (int k = 0; k < 3; k++) acc[k] += (s->v[3 * j + k] * s->V[j]) * w;
+            }
+            for (int bi = 0; bi < s->nb; bi++) {
+                const ref_boundary *b = &s->b[bi];
+                for (uint64_t e = b->off[0][i]; e < b->off[0][i + 1]; e++) {
+                    const uint64_t id = b->ids[0][e];
+                    double r[3];
+                    sub3(pi, b->pos + 3 * id, r);
+                    const double w = ref_kernel_function(r, h);
+                    for (int k = 0; k < 3; k++) acc[k] += (b->vel[3 * id + k] * b->vol[id]) * w;
+                }
+            }
+            res = norm3(acc);
+        } else if (quantity == 1) { /* Volume */
+            double acc = 0.0;
+            for (uint64_t e = nl.offsets[i]; e < nl.offsets[i + 1]; e++) {
+                double r[3];
+                sub3(x0 + 3 * nl.indices[e], pi, r);
+                acc += V0 * ref_kernel_function(r, h);
+            }
+            for (int bi = 0; bi < s->nb; bi++) {
+                const ref_boundary *b = &s->b[bi];
+                for (uint64_t e = b->off[0][i]; e < b->off[0][i + 1]; e++) {
+                    const uint64_t id = b->ids[0][e];
+                    double r[3];
+                    sub3(b->pos + 3 * id, pi, r);
+                    acc += b->vol[id] * ref_kernel_function(r, h);
+                }
+            }
+            res = V0 / acc;
+        } else {
+            double acc = 0.0;
+            for (uint64_t e = nl.offsets[i]; e < nl.offsets[i + 1]; e++) {
+                const uint64_t j = nl.indices[e];
+                double r[3];
+                sub3(pi, x0 + 3 * j, r);
+                const double w = ref_kernel_function(r, h);
+                if (quantity == 2) { /* Density */
+                    acc += s->m[j] * w;
+                } else if (quantity == 3) { /* DensityError, masses as volumes */
+                    if (!(s->m[j] < V0)) continue;
+                    const double err = V0 / s->m[j] - 1.;
+                    acc += (err * s->m[j]) * w;
+                } else if (quantity == 4) { /* Pressure */
+                    acc += (s->p[j] * s->V[j]) * w;
+                } else { /* KineticEnergy */
+                    acc += (((0.5 * s->m[j]) * norm_sq3(s->v + 3 * j)) * s->V[j]) * w;
+                }
+            }
+            res = quantity == 3 ? 100. * acc : acc;
+        }
+        out[i] = (float)res;
+    }
+    s->x = x0;
+    s->n = n0;
+    find_boundary_samples(s);
+    ref_nlist_free(&nl);
+    free(q);
+    return 0;
+}
+
 double ref_system_static_volume(const ref_system *s, int bi, size_t k) {
     return s->b[bi].svol[k];
 }

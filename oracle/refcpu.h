@@ -197,6 +197,12 @@ uint64_t ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], c
                   double dx, const double *gl_x, const double *gl_w, int degree, double *values,
                   uint8_t *present);
 
+/* SPHSystem::get_quantity_at_positions (sph/mod.rs:241-341): quantity
+ * 0 Speed, 1 Volume, 2 Density, 3 DensityError, 4 Pressure, 5 KineticEnergy
+ * (render_info.rs:205-212) at nq f32 positions; out = f32 values. */
+int ref_system_quantity_at_positions(ref_system *s, int quantity, size_t nq, const float *pos,
+                                     float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,6 +124,10 @@ def lib():
         L.ref_system_static_volume.restype = C.c_double
         L.ref_system_static_volume.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.ref_system_set_threads.argtypes = [C.c_int]
+        L.ref_system_quantity_at_positions.restype = C.c_int
+        L.ref_system_quantity_at_positions.argtypes = [C.c_void_p, C.c_int, C.c_size_t,
+                                                       C.POINTER(C.c_float),
+                                                       C.POINTER(C.c_float)]
         L.ref_vm_integrate_ball.restype = C.c_int
         L.ref_vm_integrate_ball.argtypes = [C.POINTER(RefLattice), _dp, C.c_double, _dp, _dp,
                                             C.c_int, _dp]
@@ -310,6 +314,17 @@ class OracleSystem:
 
     def step(self, n=1):
         lib().ref_system_step(self.h, int(n))
+
+    def quantity_at_positions(self, quantity, positions):
+        """get_quantity_at_positions: f32 values of quantity 0..5 at f32 positions."""
+        q = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros(len(q), dtype=np.float32)
+        st = lib().ref_system_quantity_at_positions(
+            self.h, int(quantity), len(q), q.ctypes.data_as(C.POINTER(C.c_float)),
+            out.ctypes.data_as(C.POINTER(C.c_float)))
+        if st:
+            raise ValueError(f"quantity {quantity} refused")
+        return out
 
     def __len__(self):
         return int(lib().ref_system_len(self.h))
