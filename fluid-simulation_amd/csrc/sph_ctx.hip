@@ -2061,6 +2061,72 @@ int sph_body_state(sph_ctx *c, int bi, double out[22]) {
     });
 }
 
+int sph_fluid_download_slots(sph_ctx *c, int *nslots, double *xp, double *vp) {
+    return guarded([&] {
+        if (!c || !nslots) throw Error(SPH_ERR_ARG, "null argument");
+        Scene &S = c->scene;
+        // TakePredicted keeps one position and one velocity slot
+        // (take_predicted.rs:9-10); EulerCromer none (integration_schemes/mod.rs:25-28)
+        *nslots = S.P.integrator == SPH_INTEGRATOR_TAKE_PREDICTED ? 1 : 0;
+        if (*nslots == 0) return;
+        std::vector<double> A, B_, C_;
+        if (xp) {
+            download_field(c, S.ix, A);
+            download_field(c, S.iy, B_);
+            download_field(c, S.iz, C_);
+            split3(A, B_, C_, xp);
+        }
+        if (vp) {
+            download_field(c, S.ivx, A);
+            download_field(c, S.ivy, B_);
+            download_field(c, S.ivz, C_);
+            split3(A, B_, C_, vp);
+        }
+    });
+}
+
+int sph_system_restore(sph_ctx *c, uint64_t time_steps_propagated, double current_time,
+                       size_t nbodies, const uint8_t *present, const double *bodies) {
+    return guarded([&] {
+        if (!c || (nbodies && (!present || !bodies))) throw Error(SPH_ERR_ARG, "null argument");
+        check_solo(c);
+        Scene &S = c->scene;
+        // BoundaryHandling::restore_from_checkpoint skips every boundary when
+        // the entry count differs (volume_map_boundary.rs:1504-1518), and a
+        // boundary whose entry does not match its kind (:2099-2120)
+        if (nbodies == S.bodies.size()) {
+            for (size_t b = 0; b < nbodies; b++) {
+                Body &body = *S.bodies[b];
+                if (!body.dynamic || !present[b]) continue;
+                const double *s = bodies + 19 * b;
+                Rigid &r = body.rb;
+                memcpy(r.com, s, 3 * sizeof(double));
+                // the serialized quaternion passes UnitQuaternion::from_quaternion
+                // (rigid_body_motion.rs:273-289), which divides each coordinate
+                // by the root of the squares summed in storage order (i, j, k, w)
+                const double w = s[3], qi = s[4], qj = s[5], qk = s[6];
+                const double nrm = sqrt(((qi * qi + qj * qj) + qk * qk) + w * w);
+                r.q[0] = w / nrm;
+                r.q[1] = qi / nrm;
+                r.q[2] = qj / nrm;
+                r.q[3] = qk / nrm;
+                memcpy(r.v, s + 7, 3 * sizeof(double));
+                memcpy(r.L, s + 10, 3 * sizeof(double));
+                memcpy(r.F, s + 13, 3 * sizeof(double));
+                memcpy(r.T, s + 16, 3 * sizeof(double));
+                r.derive();  // RigidBodyMotion::restore_from_checkpoint (rigid_body_motion.rs:214-222)
+            }
+        }
+        // continue_from_checkpoint (sph/mod.rs:362-378): counters, then update()
+        init_boundaries(c->own, S);
+        S.dt = S.P.cfl ? 0.0 : S.P.time_increment;
+        S.time = current_time;
+        S.steps = time_steps_propagated;
+        update(solo(c));
+        S.initialized = true;
+    });
+}
+
 int sph_body_forces(sph_ctx *c, int bi, double out[6]) {
     return guarded([&] {
         if (!c || !out || bi < 0 || (size_t)bi >= c->scene.bodies.size())

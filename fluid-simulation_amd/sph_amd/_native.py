@@ -93,6 +93,9 @@ SIGNATURES = [
     ("sph_error_history", C.c_int, [C.c_void_p, _dp, C.c_size_t, _szp]),
     ("sph_body_state", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("sph_body_forces", C.c_int, [C.c_void_p, C.c_int, _dp]),
+    ("sph_fluid_download_slots", C.c_int, [C.c_void_p, C.POINTER(C.c_int), _dp, _dp]),
+    ("sph_system_restore", C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_size_t, _u8p,
+                                     _dp]),
     ("sph_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("sph_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
     ("sph_volume", C.c_int, [C.c_void_p, C.c_size_t, _dp, _dp]),

@@ -251,6 +251,127 @@ class System:
         """Tuner state and the code shape of each neighbor pass."""
         return _tuning(self.get_option)
 
+    # ---- checkpoints (sph/mod.rs:362-378, 884-915) -------------------------
+    def get_checkpoint(self):
+        """``SystemCheckpoint`` of the current state (fluid in the reference's
+        particle order, the integrator's slots, and one entry per boundary:
+        the RigidBodyMotion state of a dynamic one, None for a static one)."""
+        from . import state_io
+        st = self.state()
+        n = len(st["m"])
+        ns = C.c_int()
+        xp, vp = np.zeros((n, 3)), np.zeros((n, 3))
+        N.check(N.lib().sph_fluid_download_slots(self._ctx.h, C.byref(ns), N.dptr(xp),
+                                                 N.dptr(vp)))
+        pslots = [xp] if ns.value else []
+        vslots = [vp] if ns.value else []
+        m = self.take_measurement()
+        states = []
+        for b, desc in enumerate(self.scene.boundaries):
+            if getattr(desc, "rigid", None) is None:
+                states.append(None)
+                continue
+            s = self.body_state(b)
+            q = s["q"]  # (w, i, j, k) -> serialized as (i, j, k, w)
+            states.append(state_io.RigidBodyCheckpoint(
+                s["com"].copy(), np.array([q[1], q[2], q[3], q[0]]), s["lin_vel"].copy(),
+                s["ang_mom"].copy(), s["force"].copy(), s["torque"].copy()))
+        fluid = state_io.FluidCheckpoint(st["fluid_id"].copy(), st["x"], st["v"], st["m"],
+                                         pslots, vslots)
+        return state_io.SystemCheckpoint(int(m["steps"]), float(m["time"]), fluid, states)
+
+    @classmethod
+    def from_checkpoint(cls, scene: Scene, cp) -> "System":
+        """``SPHSystem::continue_from_checkpoint``: a system of ``scene``'s
+        parameters and boundaries carrying the checkpoint's fluid, counters
+        and rigid-body states, after the update() that resuming runs."""
+        import dataclasses
+        f = cp.fluid
+        sc = dataclasses.replace(scene, x=np.asarray(f.position, dtype=np.float64).reshape(-1, 3),
+                                 v=np.asarray(f.velocity, dtype=np.float64).reshape(-1, 3),
+                                 m=np.asarray(f.mass, dtype=np.float64).ravel(),
+                                 fluid_id=np.asarray(f.fluid_id, dtype=np.uint32).ravel())
+        sim = cls(sc, init=False)
+        nb = len(cp.dynamic_states)
+        present = np.zeros(max(nb, 1), dtype=np.uint8)
+        bodies = np.zeros((max(nb, 1), 19))
+        for b, d in enumerate(cp.dynamic_states):
+            if d is None:
+                continue
+            present[b] = 1
+            o = np.asarray(d.orientation, dtype=np.float64)  # (i, j, k, w)
+            bodies[b] = np.concatenate([d.center_of_mass, [o[3], o[0], o[1], o[2]],
+                                        d.linear_velocity, d.angular_momentum, d.force,
+                                        d.torque])
+        N.check(N.lib().sph_system_restore(sim._ctx.h, int(cp.time_steps_propagated),
+                                           float(cp.current_time), nb,
+                                           present.ctypes.data_as(N._u8p), N.dptr(bodies)))
+        return sim
+
+    def time_step_info(self, fluid=None, boundary=None) -> dict:
+        """``TimeStepInfo::from_system`` (render_info.rs:80-89) for a recording:
+        fluid = {"kind": "Samples", "coloring": {"kind": "Uniform" | "FluidId" |
+        "QuantityGraded", "quantity": ...}} or {"kind": "SensorPlane",
+        "quantity": ..., "planes": [{"positions", "rows", "cols"}]};
+        boundary = {"kind": "Samples", "positions": [m][3], "coloring": ...}."""
+        fluid = fluid or {"kind": "Samples", "coloring": {"kind": "Uniform"}}
+        m = self.take_measurement()
+        meas = {"time": m["time"], "density": m["density"], "density_error": m["density_error"],
+                "kinetic_energy": m["kinetic_energy"], "time_step_size": m["time_step_size"],
+                "solver_iterations": m["solver_iterations"],
+                "predicted_density_error": m["predicted_density_error"]}
+        P = self.scene.params
+        meas.update(stiffness=P.stiffness, fluid_viscosity=P.fluid_viscosity,
+                    boundary_viscosity=P.boundary_viscosity,
+                    rest_density_grid_spacing=P.rest_density_grid_spacing,
+                    kernel_support_radius=P.kernel_support_radius,
+                    target_density_error=P.target_density_error,
+                    relaxation_factor=P.relaxation_factor)
+        if fluid["kind"] == "Samples":
+            st = self.state()
+            col = fluid.get("coloring", {"kind": "Uniform"})
+            if col["kind"] == "FluidId":
+                col = {"kind": "FluidId", "id": st["fluid_id"],
+                       "max_id": int(st["fluid_id"].max()) if len(st["fluid_id"]) else 0}
+            elif col["kind"] == "QuantityGraded":
+                col = {"kind": "QuantityGraded", "quantity": col["quantity"],
+                       "data": self.quantity_of_fluid_samples(col["quantity"], st)}
+            fv = {"kind": "Samples", "positions": st["x"].astype(np.float32), "coloring": col}
+        else:
+            planes = []
+            for pl in fluid["planes"]:
+                pos = np.asarray(pl["positions"], dtype=np.float32).reshape(-1, 3)
+                planes.append({"positions": pos,
+                               "data": self.get_quantity_at_positions(fluid["quantity"], pos),
+                               "rows": pl["rows"], "cols": pl["cols"]})
+            fv = {"kind": "SensorPlane", "quantity": fluid["quantity"], "planes": planes}
+        bv = boundary or {"kind": "Samples", "positions": np.zeros((0, 3), dtype=np.float32),
+                          "coloring": {"kind": "Uniform"}}
+        return {"time_step_number": int(m["steps"]), "measurement": meas, "fluid": fv,
+                "boundary": bv}
+
+    def quantity_of_fluid_samples(self, quantity, st=None) -> np.ndarray:
+        """``get_quantity_of_fluid_samples`` (sph/mod.rs:202-238): a per-particle
+        quantity in the reference's particle order, as f32."""
+        st = st if st is not None else self.state()
+        V0 = self.scene.params.rest_density_grid_spacing ** 3
+        v, V, m, p = st["v"], st["V"], st["m"], st["p"]
+        if quantity == "Speed":
+            out = np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+        elif quantity == "Volume":
+            out = V
+        elif quantity == "Density":
+            out = m / V
+        elif quantity == "DensityError":
+            out = np.where(V < V0, 100.0 * (V0 / V - 1.0), 0.0)
+        elif quantity == "Pressure":
+            out = p
+        elif quantity == "KineticEnergy":
+            out = (0.5 * m) * ((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+        else:
+            raise ValueError(f"unknown quantity {quantity!r}")
+        return np.asarray(out, dtype=np.float64).astype(np.float32)
+
     def clone(self) -> "System":
         """Deep copy into an independent device context (DynClone)."""
         h = C.c_void_p()

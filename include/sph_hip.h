@@ -161,6 +161,26 @@ int sph_body_state(sph_ctx *ctx, int b, double out[22]);
 /* F[3], tau[3] applied to boundary b during the last update() (FP64) */
 int sph_body_forces(sph_ctx *ctx, int b, double out[6]);
 
+/* ---- checkpoints ------------------------------------------------------------------
+ * The integrator's position and velocity slots of FluidCheckpoint
+ * (fluid.rs:352-381): *nslots = 1 for TakePredicted (take_predicted.rs:9-10),
+ * 0 for EulerCromer; with one slot, xp and vp (either may be NULL) receive
+ * [n][3] in the reference's particle order. */
+int sph_fluid_download_slots(sph_ctx *ctx, int *nslots, double *xp, double *vp);
+/* SPHSystem::continue_from_checkpoint (sph/mod.rs:362-378), called after
+ * sph_fluid_upload of the checkpoint's fluid and in place of sph_system_init:
+ * restores the step count and current time, each dynamic boundary's
+ * RigidBodyMotion state (rigid_body_motion.rs:214-222) and runs update().
+ * bodies holds 19 binary64 per boundary: center of mass[3], orientation
+ * (w, i, j, k; normalized as UnitQuaternion::from_quaternion does,
+ * rigid_body_motion.rs:273-289), linear velocity[3], angular momentum[3],
+ * force[3], torque[3]; present[b] = 0 is a checkpoint entry of None. As in
+ * BoundaryHandling::restore_from_checkpoint (volume_map_boundary.rs:
+ * 1504-1518, 2099-2120), a count mismatch skips every boundary and a kind
+ * mismatch skips that boundary. */
+int sph_system_restore(sph_ctx *ctx, uint64_t time_steps_propagated, double current_time,
+                       size_t nbodies, const uint8_t *present, const double *bodies);
+
 /* ---- kernel timing ------------------------------------------------------------
  * HIP-event timing of the hot kernels on the context's stream (no counterpart
  * in the reference, which only times whole steps, sph/mod.rs:127,143).

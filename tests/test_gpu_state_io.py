@@ -1,0 +1,140 @@
+"""Checkpoint resume and recordings on the device.
+
+A system saved after k steps and resumed from the saved file
+(SPHSystem::continue_from_checkpoint, sph/mod.rs:362-378) continues bit for
+bit as the uninterrupted system when no body moves: the fluid is stored in the
+reference's particle order, which is the order the library sorts ties by, and
+update() recomputes everything else from it. The continued trajectory is also
+checked against the oracle stepping the uninterrupted run.
+
+A moving body resumes differently, as in the reference: the saved orientation
+is re-normalized (UnitQuaternion::from_quaternion, rigid_body_motion.rs:
+273-289), and the saved pending force and torque are restored before the
+resuming update() adds that update's fluid and gravity forces on top
+(rigid_body_motion.rs:214-222; the accumulators are only cleared by the
+body's own step, :183-201), so the first resumed body step integrates twice
+the pending force.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import refcpu
+from sph_amd import scenes, state_io
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def native():
+    from sph_amd import system
+    return system
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+SCENES = {
+    "iisph_box": lambda: scenes.small_iisph_box(side=10, jitter=2e-3),
+    "ost_box": lambda: scenes.small_iisph_box(side=10, solver="IISPHwOST", jitter=2e-3),
+    "sesph": lambda: scenes.dam_break_sesph(side=8),
+}
+
+
+@pytest.mark.parametrize("case", list(SCENES))
+@pytest.mark.parametrize("fmt", ["ron", "bincode"])
+def test_resume_continues_the_run(native, tmp_path, case, fmt):
+    sc = SCENES[case]()
+    print(f"resume {case} {fmt}: {sc.n} particles", flush=True)
+    a = native.System(sc)
+    a.step_forward_in_time(3)
+    path = str(tmp_path / f"state.{fmt}")
+    state_io.save_checkpoint(a.get_checkpoint(), path, fmt)
+    cp = state_io.load_checkpoint(path)
+    assert cp.time_steps_propagated == 3
+    if sc.params.integrator == "TakePredicted":
+        assert len(cp.fluid.integrator_position_slots) == 1
+    b = native.System.from_checkpoint(sc, cp)
+    ma, mb = a.take_measurement(), b.take_measurement()
+    assert mb["steps"] == ma["steps"] and mb["time"] == ma["time"]
+    a.step_forward_in_time(3)
+    b.step_forward_in_time(3)
+    sa, sb = a.state(), b.state()
+    for k in ("x", "v", "a", "V", "p", "m"):
+        assert np.array_equal(bits(sa[k]), bits(sb[k])), (k, int(np.sum(sa[k] != sb[k])))
+    assert np.array_equal(sa["fluid_id"], sb["fluid_id"])
+    ma, mb = a.take_measurement(), b.take_measurement()
+    assert ma["steps"] == mb["steps"] == 6
+    assert ma["solver_iterations"] == mb["solver_iterations"]
+    # the uninterrupted run itself follows the oracle
+    o = refcpu.OracleSystem(sc)
+    o.step(6)
+    so = o.state()
+    for k in ("x", "v", "V", "p"):
+        den = float(np.max(np.abs(so[k]))) or 1.0
+        assert float(np.max(np.abs(sb[k] - so[k]))) / den <= 1e-5, k
+
+
+def test_resume_restores_rigid_state(native):
+    sc = scenes.small_rigid_scene()
+    a = native.System(sc)
+    a.step_forward_in_time(4)
+    cp = a.get_checkpoint()
+    dyn = [b for b, d in enumerate(cp.dynamic_states) if d is not None]
+    assert dyn, "the scene has a moving body"
+    assert all((cp.dynamic_states[b] is None) == (getattr(d, "rigid", None) is None)
+               for b, d in enumerate(sc.boundaries))
+    b = native.System.from_checkpoint(sc, cp)
+    for k in dyn:
+        ra, rb = a.body_state(k), b.body_state(k)
+        for f in ("com", "lin_vel", "ang_mom"):
+            assert np.array_equal(ra[f], rb[f]), f
+        assert np.max(np.abs(ra["q"] - rb["q"])) <= 4e-16
+        # pending force and torque: the saved ones plus the resuming update's
+        # own, which repeat the saved ones up to the re-normalized orientation
+        for f in ("force", "torque"):
+            scale = 1.0 + float(np.max(np.abs(ra[f])))
+            assert np.max(np.abs(rb[f] - 2.0 * ra[f])) <= 1e-9 * scale, f
+        fa, fb = a.body_forces(k), b.body_forces(k)
+        assert np.max(np.abs(fb - fa)) <= 1e-9 * (1.0 + float(np.max(np.abs(fa))))
+    # a checkpoint with the wrong number of boundaries leaves bodies untouched
+    bad = state_io.SystemCheckpoint(cp.time_steps_propagated, cp.current_time, cp.fluid,
+                                    cp.dynamic_states + [None])
+    c = native.System.from_checkpoint(sc, bad)
+    init = native.System(sc, init=True)
+    for k in dyn:
+        assert np.array_equal(c.body_state(k)["com"], init.body_state(k)["com"])
+
+
+def test_recording_of_a_run(native, tmp_path):
+    sc = scenes.small_iisph_box(side=10, jitter=2e-3)
+    sim = native.System(sc)
+    path = str(tmp_path / "run.bin")
+    rec = state_io.Recording(path, state_io.SimulationParameters(
+        float(sc.params.rest_density_grid_spacing), (0.0, 0.0, 5.0), 100, True, True, False))
+    u = np.linspace(0.0, 0.6, 8, dtype=np.float32)
+    gx, gy = np.meshgrid(u, u)
+    plane = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, 0.1, np.float32)], axis=1)
+    for k in range(3):
+        sim.step_forward_in_time(1)
+        rec.append_time_step(sim.time_step_info(
+            {"kind": "Samples", "coloring": {"kind": "QuantityGraded", "quantity": "Density"}}))
+    # the same step number again replaces that step's record
+    rec.append_time_step(sim.time_step_info(
+        {"kind": "SensorPlane", "quantity": "Pressure",
+         "planes": [{"positions": plane, "rows": 8, "cols": 8}]}))
+    sp, steps = state_io.read_recording(path)
+    assert [s["time_step_number"] for s in steps] == [1, 2, 3]
+    last = steps[-1]
+    assert last["fluid"]["kind"] == "SensorPlane"
+    got = last["fluid"]["planes"][0]["data"]
+    assert np.array_equal(got, sim.get_quantity_at_positions("Pressure", plane))
+    st = sim.state()
+    m = sim.take_measurement()
+    assert last["measurement"]["solver_iterations"] == m["solver_iterations"]
+    assert steps[1]["fluid"]["positions"].shape == (sc.n, 3)
+    dens = steps[1]["fluid"]["coloring"]["data"]
+    assert dens.dtype == np.float32 and np.all(dens > 0)
+    assert os.path.getsize(path) > 0 and st["x"].shape == (sc.n, 3)
