@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 # is not counted. `per_body` is added per volume-map boundary (the particle's
 # boundary-sample slot).
 ALGO_BYTES = {
-    "neighbor_count": {"base": 28, "per_body": 0},    # x,y,z r; count w
+    "neighbor_count": {"base": 36, "per_body": 33},   # one sweep: x,y,z r; count, V w
     "pairs_volume": {"base": 32, "per_body": 33},     # x,y,z r; V w
     "boundary_sample": {"base": 81, "per_body": 0},   # x,y,z r; flag + 7 f64 w
     "nonpressure": {"base": 88, "per_body": 33},      # x,v,m,V r; a w

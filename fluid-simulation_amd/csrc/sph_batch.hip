@@ -313,6 +313,112 @@ __global__ void kb_fill_ell(BatchDev B) {
 }
 
 // ------------------------------------------------------------------------
+// One sweep per update: neighbor search into a fixed-width pair cache plus
+// get_volume (spatial_hashing.rs:75-103; quantities.rs:14-61). Slice g owns
+// slots [64 cap g, 64 cap (g+1)); the k-th neighbor of lane l sits at
+// 64 cap g + 64 k + l. Each wavefront collects its first kSweepRows neighbor
+// rows in LDS and writes them out as whole 256-byte rows once its sweep is
+// done; neighbors past kSweepRows go straight to the cache, and past `cap`
+// only count (the caller widens the cache and sweeps again). The volume sum
+// takes the neighbors in the order they are found, which is the order every
+// other pass reads them in, with kb_pairs_volume's expressions; the boundary
+// terms follow as in kb_pairs_volume, so boundary samples must be current.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap) {
+    __shared__ uint32_t rows[kBlock / 64][kSweepRows][64];
+    PARTICLE_THREAD(B);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    uint32_t cnt = 0;
+    // slot of lane l's first neighbor; for lanes past the scene's end it is
+    // the slot the lane would have, inside the slice's allocation
+    const size_t e0 = ((size_t)(v.slice_base + (i >> 6)) * cap) * 64 + (i & 63);
+    if (live) {
+        const KernelConst K = v.K;
+        const double rh = refined_rcp(K.h);
+        const rec4 qi = ld4(v.q4, i);
+        const double px = qi.x, py = qi.y, pz = qi.z;
+        const GridDev &g = v.grid;
+        const int cx = sat_i32(floor(px / v.cs)), cy = sat_i32(floor(py / v.cs)),
+                  cz = sat_i32(floor(pz / v.cs));
+        double accu = 0.;
+        int zlo = cz - v.reach - g.org[2], zhi = cz + v.reach - g.org[2];
+        if (zlo < 0) zlo = 0;
+        if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
+        if (zlo <= zhi) {
+            for (int dx = -v.reach; dx <= v.reach; dx++) {
+                const int gx = cx + dx - g.org[0];
+                if (gx < 0 || gx >= g.dim[0]) continue;
+                for (int dy = -v.reach; dy <= v.reach; dy++) {
+                    const int gy = cy + dy - g.org[1];
+                    if (gy < 0 || gy >= g.dim[1]) continue;
+                    const uint32_t row =
+                        ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
+                    const uint32_t a = gm(g.start)[row + zlo] - v.part_base,
+                                   b = gm(g.start)[row + zhi + 1] - v.part_base;
+                    for (uint32_t j = a; j < b; j++) {
+                        const rec4 qj = ld4(v.q4, j);  // .w is being written by this pass; unused
+                        const double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
+                        const double d2 = (rx * rx + ry * ry) + rz * rz;
+                        if (d2 < v.dthr) {
+                            if (cnt < kSweepRows) rows[w][cnt][l] = j;
+                            else if (cnt < cap) gm(B.ell_j)[e0 + 64 * (size_t)cnt] = j;
+                            const double q = div_rcp(sqrt(d2), K.h, rh);
+                            accu += v.V0 * (K.wfac * kernel_w(q));
+                            cnt++;
+                        }
+                    }
+                }
+            }
+        }
+        for (uint32_t bi = 0; bi < v.nb; bi++) {
+            const BodyDev &b = B.bodies[v.body_base + bi];
+            if (b.kind == 0) {
+                for (uint32_t k = gm(b.soff)[i]; k < gm(b.soff)[i + 1]; k++) {
+                    const uint32_t s = gm(b.sidx)[k];
+                    accu += gm(b.svol)[s] *
+                            kernel_fn(K, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s]);
+                }
+            } else if (gm(b.flag)[i]) {
+                accu += gm(b.bvol)[i] *
+                        kernel_fn(K, px - gm(b.bx)[i], py - gm(b.by)[i], pz - gm(b.bz)[i]);
+            }
+        }
+        const double Vi = v.V0 / accu;
+        gm(v.V)[i] = Vi;
+        gm(v.q4)[4 * i + 3] = Vi;
+        gm(B.cnt)[gi] = cnt;
+    }
+    uint32_t mx = cnt;
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    if (l == 0 && live) gm(B.slice_len)[v.slice_base + (i >> 6)] = mx;
+    // the wavefront's buffered rows as whole rows (lanes past their own count
+    // write a value no pass reads)
+    const uint32_t nrows = min(min(mx, (uint32_t)kSweepRows), cap);
+    if (live)
+        for (uint32_t k = 0; k < nrows; k++) gm(B.ell_j)[e0 + 64 * (size_t)k] = rows[w][k][l];
+}
+
+// largest neighbor count of the batch (slice maxima folded by one workgroup)
+__global__ void kb_max_count(const uint32_t *slice_len, size_t n, uint32_t *out) {
+    __shared__ uint32_t red[kBlock];
+    uint32_t mx = 0;
+    for (size_t k = threadIdx.x; k < n; k += kBlock) mx = max(mx, slice_len[k]);
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// slice_off[g] = g * cap: the fixed-width layout in the form ELL_BASE reads
+__global__ void kb_fixed_slices(uint32_t *slice_off, size_t n, uint32_t cap) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g <= n) slice_off[g] = (uint32_t)(g * cap);
+}
+
+// ------------------------------------------------------------------------
 // VolumeMapBoundary::find_boundary_samples for every volume-map body of the
 // particle's scene (volume_map_boundary.rs:1259-1424)
 // ------------------------------------------------------------------------

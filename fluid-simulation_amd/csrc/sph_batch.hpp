@@ -95,6 +95,11 @@ __global__ void kb_cell_scatter(BatchDev B);
 __global__ void kb_cell_order(BatchDev B);
 __global__ void kb_permute(BatchDev B);
 __global__ void kb_count_ell(BatchDev B);
+// neighbor rows a wavefront of kb_sweep_ell buffers in LDS (48 KB per workgroup)
+constexpr int kSweepRows = 48;
+__global__ void kb_sweep_ell(BatchDev B, uint32_t cap);
+__global__ void kb_max_count(const uint32_t *slice_len, size_t n, uint32_t *out);
+__global__ void kb_fixed_slices(uint32_t *slice_off, size_t n, uint32_t cap);
 __global__ void kb_fill_ell(BatchDev B);
 __global__ void kb_vm_sample(BatchDev B);
 template <bool CACHED, int UNROLL, int WAVES>

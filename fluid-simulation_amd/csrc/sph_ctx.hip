@@ -493,6 +493,11 @@ struct Batch {
     // pair passes read grad W from the pair cache (24 B per slot) or recompute it
     // from positions (the default: less HBM traffic and memory, same bits)
     bool grad_cache = false;
+    // neighbor list and volume from one sweep into a fixed-width pair cache
+    // (kb_sweep_ell) or from the count / fill / volume passes; ell_cap is the
+    // fixed width the next sweep uses (0: not sized yet)
+    bool sweep = true;
+    uint32_t ell_cap = 0;
     // code shape of each pair pass (an index into SPH_PAIR_VARIANTS) and the
     // online tuner that picks them from timed steps
     int variant[kPairPasses] = {0, 0, 0, 0, 0};
@@ -503,7 +508,7 @@ struct Batch {
     DBuf<uint32_t> perm, cnt, slice_len, slice_off, ell_j, cell_cnt, cell_start, cursor;
     DBuf<uint32_t> pblk, cblk, scan_a, scan_b, scan_c;
     DBuf<double> ell_gx, ell_gy, ell_gz, partials, red, chunk_err;
-    DBuf<uint32_t> d_active, d_iters, chunk_ran, ablk, amap;
+    DBuf<uint32_t> d_active, d_iters, chunk_ran, ablk, amap, max_cnt;
     uint32_t iter_hint = 3;  // iterations of the last target-terminated solve
     // Jacobi chunks captured as HIP graphs, keyed by their launch arguments (on
     // by default for a context, off for a world, whose layout changes with
@@ -1224,6 +1229,38 @@ static void boundary_samples(Batch &B) {
     }
 }
 
+// neighbor indices and the volume in one sweep (kb_sweep_ell) into a pair
+// cache of fixed width: the largest neighbor count of the last sweep plus 8,
+// at least 16. A sweep that finds more neighbors than fit runs again with a
+// wider cache. Boundary samples must be current (the volume reads them).
+static void sweep_neighbors(Batch &B) {
+    if (B.total_particles == 0) return;
+    B.max_cnt.reserve(1);
+    uint32_t cap = B.ell_cap ? B.ell_cap : 64;
+    for (;;) {
+        const uint64_t slots = (uint64_t)B.total_slices * 64 * cap;
+        if (slots > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "neighbor cache exceeds 2^32 slots");
+        B.ell_j.reserve(std::max<uint64_t>(slots, 1));
+        LAUNCH(kb_fixed_slices, B.total_slices + 1, B.s, B.slice_off.p, B.total_slices, cap);
+        BatchDev d = batch_dev(B);
+        PLAUNCH(B, SPH_PROF_NEIGHBOR_COUNT, B.total_particles, kb_sweep_ell, B.total_pblocks, d,
+                cap);
+        hipLaunchKernelGGL(kb_max_count, dim3(1), dim3(kBlock), 0, B.s, B.slice_len.p,
+                           B.total_slices, B.max_cnt.p);
+        HIPCK(hipGetLastError());
+        uint32_t mx = 0;
+        HIPCK(hipMemcpyAsync(B.pinned, B.max_cnt.p, sizeof mx, hipMemcpyDeviceToHost, B.s));
+        B.sync();
+        memcpy(&mx, B.pinned, sizeof mx);
+        const uint32_t want = std::max<uint32_t>(16, ((mx + 8 + 7) / 8) * 8);
+        if (mx <= cap) {
+            B.ell_cap = want;
+            return;
+        }
+        cap = want;  // overflow: the cache was too narrow for this sweep
+    }
+}
+
 // neighbor indices, then grad W into the pair cache and the volume (get_volume)
 static void pairs_and_volume(Batch &B) {
     BatchDev d = batch_dev(B);
@@ -1498,9 +1535,14 @@ static void measure_sums(Batch &B, std::vector<double> &out) {
 static void prepare(Batch &B) {
     cells_and_grids(B);
     sort_scenes(B);
-    neighbor_counts(B);
-    boundary_samples(B);
-    pairs_and_volume(B);
+    if (B.sweep && !B.grad_cache) {
+        boundary_samples(B);
+        sweep_neighbors(B);
+    } else {
+        neighbor_counts(B);
+        boundary_samples(B);
+        pairs_and_volume(B);
+    }
 }
 
 // System::update (sph/mod.rs:674-731) for every scene of the batch
@@ -2380,6 +2422,10 @@ static void set_option(Batch &B, int option, int value) {
     } else if (option == SPH_OPT_GRAPHS) {
         B.use_graphs = value != 0;
         if (!B.use_graphs) B.drop_graphs();
+    } else if (option == SPH_OPT_SWEEP) {
+        if (value < 0) throw Error(SPH_ERR_ARG, "sweep option must be >= 0");
+        B.sweep = value != 0;
+        if (value > 1) B.ell_cap = (uint32_t)value;
     } else if (option == SPH_OPT_SHAPE) {
         if (value < 0 || value >= kPairVariants) throw Error(SPH_ERR_ARG, "no such code shape");
         for (int p = 0; p < kPairPasses; p++) B.variant[p] = value;
@@ -2395,6 +2441,7 @@ static int get_option(const Batch &B, int option) {
     if (option == SPH_OPT_AUTOTUNE) return B.tuner.steps_per_shape;
     if (option == SPH_OPT_TUNE_STATE) return B.tuner.state;
     if (option == SPH_OPT_GRAPHS) return B.use_graphs ? 1 : 0;
+    if (option == SPH_OPT_SWEEP) return B.sweep ? 1 : 0;
     if (option >= SPH_OPT_SHAPE_VOLUME && option < SPH_OPT_SHAPE_VOLUME + kPairPasses)
         return B.variant[option - SPH_OPT_SHAPE_VOLUME];
     throw Error(SPH_ERR_ARG, "unknown option");

@@ -128,6 +128,7 @@ OPT_SHAPE = 2
 OPT_TUNE_STATE = 3
 OPT_SHAPE_VOLUME = 4
 OPT_GRAPHS = 9
+OPT_SWEEP = 10
 SHAPES = 6
 PAIR_PASSES = ["volume", "nonpressure", "source_diag", "pressure_accel", "jacobi"]
 

@@ -187,7 +187,7 @@ int sph_system_restore(sph_ctx *ctx, uint64_t time_steps_propagated, double curr
  * sph_profile(ctx, 1) clears and starts; sph_profile_read returns the launch
  * count, the particles processed and the summed kernel milliseconds. */
 enum {
-    SPH_PROF_NEIGHBOR_COUNT = 0,   /* cell sweep: neighbor counts per particle */
+    SPH_PROF_NEIGHBOR_COUNT = 0,   /* cell sweep: neighbor counts (one-sweep path: + pair cache + volume) */
     SPH_PROF_PAIRS_VOLUME = 1,     /* cell sweep: pair cache (index, grad W) + volume */
     SPH_PROF_BOUNDARY_SAMPLE = 2,  /* volume-map boundary samples */
     SPH_PROF_NONPRESSURE = 3,      /* gravity + viscosity */
@@ -319,6 +319,12 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *   SPH_OPT_GRAPHS  1 = replay repeated Jacobi iteration chunks from HIP
  *       graphs (default for a context), 0 = launch every kernel directly
  *       (default for a world).
+ *   SPH_OPT_SWEEP  1 = neighbor list and volume from one cell sweep into a
+ *       fixed-width pair cache (default), 0 = separate count, fill and
+ *       volume passes into a slice-width pair cache. The pair cache with
+ *       stored grad W (SPH_OPT_CACHE_GRADIENTS) always uses the passes. A
+ *       value above 1 also sets the width of the next sweep's cache; a
+ *       sweep that finds more neighbors than fit runs again, wider.
  * The options of a context apply when it steps alone, those of a world to
  * the world's steps. */
 enum {
@@ -327,7 +333,8 @@ enum {
     SPH_OPT_SHAPE = 2,
     SPH_OPT_TUNE_STATE = 3,
     SPH_OPT_SHAPE_VOLUME = 4,
-    SPH_OPT_GRAPHS = 9
+    SPH_OPT_GRAPHS = 9,
+    SPH_OPT_SWEEP = 10
 };
 enum { SPH_SHAPES = 6 };
 int sph_set_option(sph_ctx *ctx, int option, int value);

@@ -136,6 +136,38 @@ def test_gradient_cache_modes_bit_identical(native):
     assert_same(solo, a[1])
 
 
+@pytest.mark.parametrize("width", [1, 5])
+def test_one_sweep_matches_separate_passes(native, width):
+    """The one-sweep neighbor list + volume (fixed-width cache) and the count /
+    fill / volume passes (slice-width cache) give the same bits, alone and in
+    a world; width 5 starts each system with a cache narrower than its
+    neighbor counts, so the first sweep runs again wider."""
+    from sph_amd import _native as N
+    scs = [scenes.small_rigid_scene(), scenes.small_iisph_box(side=9, seed=2, jitter=2e-3),
+           scenes.dam_break_sesph(side=8),
+           scenes.small_iisph_box(side=9, seed=2, solver="IISPHwOST", jitter=2e-3)]
+    a = [native.System(sc, init=False) for sc in scs]        # separate passes
+    b = [native.System(sc, init=False) for sc in scs]        # one sweep
+    for s in a:
+        s.set_option(N.OPT_SWEEP, 0)
+    for s in b:
+        s.set_option(N.OPT_SWEEP, width)
+    for s in a + b:
+        N.check(N.lib().sph_system_init(s.handle))
+    c = [native.System(sc) for sc in scs[:2]]    # one sweep in a world (one solver)
+    w = native.World(c)
+    w.set_option(N.OPT_SWEEP, width)
+    for k in range(4):
+        for s in a + b:
+            s.step_forward_in_time(1)
+        w.step(1)
+        print(f"sweep width {width}: step {k + 1}", flush=True)
+    for x, y in zip(a, b):
+        assert_same(x, y)
+    for x, z in zip(a, c):
+        assert_same(x, z)
+
+
 def test_code_shapes_bit_identical_and_tuner(native):
     """Every code shape of the neighbor passes gives the same bits; the online
     tuner measures all of them on live steps and then keeps one per pass."""
