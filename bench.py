@@ -458,6 +458,7 @@ def main():
         "density_error_pct_max": float(max(errs)),
         "density_error_pct_last": float(errs[-1]),
         "jacobi_iterations_per_step_mean": float(np.mean(iters)),
+        "code_shapes": sim.tuning(),
         "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()},
         "roofline": {
             "bound": "hbm",

@@ -420,11 +420,13 @@ def rigid_coupling(counts=(158, 78, 40), seed=0, dx=0.0125) -> Scene:
     return Scene("rigid_coupling", p, x, v, m, fid, bounds)
 
 
-def small_iisph_box(side=12, seed=1, dx=0.02, box=0.6, solver="IISPH", jitter=1e-4) -> Scene:
-    """A small VolumeMap box scene for parity tests (CPU-oracle sized)."""
+def small_iisph_box(side=12, seed=1, dx=0.02, box=0.6, solver="IISPH", jitter=1e-4,
+                    height=None) -> Scene:
+    """A small VolumeMap box scene for parity tests (CPU-oracle sized): a
+    side x side x height (default side) block of particles."""
     rng = np.random.default_rng(seed)
-    pos = lattice_block([0.5 * (box - side * dx), 0.5 * (box - side * dx), 0.02], [side] * 3,
-                        dx, rng, jitter)
+    pos = lattice_block([0.5 * (box - side * dx), 0.5 * (box - side * dx), 0.02],
+                        [side, side, height or side], dx, rng, jitter)
     x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
     h = 2 * dx
     integ = "TakePredicted" if solver == "IISPHwOST" else "EulerCromer"
