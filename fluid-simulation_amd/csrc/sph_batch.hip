@@ -55,7 +55,7 @@ __device__ __forceinline__ GLOBAL_AS T *gm(T *p) {
 // record instead of separate 8-byte loads): q4[i] = {x, y, z, V},
 // pa4[i] = {pressure acceleration x, y, z, 0}
 typedef double rec4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef double rec2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ rec4 ld4(const double *a, size_t i) {
     return ((const GLOBAL_AS rec4 *)a)[i];
 }
@@ -82,16 +82,19 @@ __device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &
 
 // The neighbor loop of a pair pass: body(k, r) for k = 0 .. c-1 in order,
 // where r = load(j) gathers the records of slot k's neighbor j (ej points at
-// the particle's first slot; slot k is ej[64 k]). PIPE = 1 software-pipelines
-// the loop: slot k+1's records and slot k+2's index are in flight while slot
-// k is evaluated (prefetch slots clamp to c-1, so every load stays in
-// bounds). PIPE = 0 is the plain loop, which needs fewer registers. Both call
-// body in the same order with the same records.
+// the particle's first slot; slot k is ej[64 k]). Three forms, all calling
+// body in the same order with the same records:
+//   PIPE = 0: the plain loop (index, then records, then the arithmetic);
+//   PIPE = 1: software-pipelined: slot k+1's records and slot k+2's index are
+//     in flight while slot k is evaluated;
+//   PIPE = 2: only the next index is prefetched, issued after the current
+//     records so that waiting for the records does not wait for it.
+// Prefetch slots clamp to c-1, so every load stays in bounds.
 template <int PIPE, int UNROLL, class Load, class Body>
 __device__ __forceinline__ void neighbor_loop(const GLOBAL_AS uint32_t *ej, uint32_t c, Load load,
                                               Body body) {
     if (c == 0) return;
-    if (PIPE) {
+    if (PIPE == 1) {
         uint32_t j1 = ej[64 * (size_t)min(1u, c - 1)];
         auto r0 = load(ej[0]);
 #pragma unroll UNROLL
@@ -101,6 +104,18 @@ __device__ __forceinline__ void neighbor_loop(const GLOBAL_AS uint32_t *ej, uint
             body(k, r0);
             j1 = j2;
             r0 = r1;
+        }
+    } else if (PIPE == 2) {
+        uint32_t jn = ej[0];
+#pragma unroll UNROLL
+        for (uint32_t k = 0; k < c; k++) {
+            // the index load stays after the record loads (scheduling
+            // barriers), so that waiting for the records leaves it in flight
+            const auto r = load(jn);
+            __builtin_amdgcn_sched_barrier(0);
+            jn = ej[64 * (size_t)min(k + 1, c - 1)];
+            __builtin_amdgcn_sched_barrier(0);
+            body(k, r);
         }
     } else {
 #pragma unroll UNROLL
@@ -116,97 +131,6 @@ struct QP {  // {x, y, z, V} and the pressure
     rec4 q;
     double p;
 };
-
-// Row-phased neighbor loop over LDS-staged records (StagePlan). For each
-// stencil row the workgroup first copies the row's record ranges into LDS
-// with coalesced loads (fetch(j, st, slot) stores the NF fields of particle j
-// at st[f * kStageRecs + slot]); then every lane evaluates its own neighbors
-// in that row, in slot order, reading them from LDS (make(st, slot) rebuilds
-// the record). rcw holds the lane's per-row neighbor counts (one byte per
-// row) and sc its source column in the plan. Every thread of the workgroup
-// must call it (it synchronizes the workgroup); body sees the same neighbors
-// with the same records in the same order as in neighbor_loop.
-template <int NF, int UNROLL, class Fetch, class Make, class Body>
-__device__ __forceinline__ void staged_loop(const StagePlan *P, double *st, bool live, int sc,
-                                            const uint32_t (&rcw)[3],
-                                            const GLOBAL_AS uint32_t *ej, Fetch fetch, Make make,
-                                            Body body) {
-    uint32_t k = 0;
-    const int nc = P->ncol;
-    for (int r = 0; r < kStageRows; r++) {
-        __syncthreads();  // every lane is done with the previous row's records
-        for (int c = 0; c < nc; c++) {
-            const uint32_t a = P->a[r][c], n = P->n[r][c], off = P->off[r][c];
-            for (uint32_t t = threadIdx.x; t < n; t += kBlock) fetch(a + t, st, off + t);
-        }
-        __syncthreads();
-        if (live) {
-            const uint32_t nr = (rcw[r >> 2] >> (8 * (r & 3))) & 255u;
-            const uint32_t a = P->a[r][sc], off = P->off[r][sc];
-#pragma unroll UNROLL
-            for (uint32_t q = 0; q < nr; q++, k++)
-                body(k, make(st, off + (ej[64 * (size_t)k] - a)));
-        }
-    }
-    __syncthreads();  // the LDS may be reused after the loop
-}
-
-// The neighbor loop of a pair pass over the pair cache: staged through LDS
-// when the code shape asks for it and the workgroup has a plan (a
-// workgroup-uniform choice), otherwise neighbor_loop. Every thread of the
-// workgroup must call it; lanes past the scene's end pass live = false.
-template <int STAGE, int PIPE, int UNROLL, int NF, class Load, class Fetch, class Make,
-          class Body>
-__device__ __forceinline__ void pair_loop(const BatchDev &B, const SceneView &v, uint32_t wblk,
-                                          bool live, size_t gi, double px, double py,
-                                          uint32_t c, size_t e0, double *st, Load load,
-                                          Fetch fetch, Make make, Body body) {
-    const GLOBAL_AS uint32_t *ej = gm(B.ell_j) + e0;
-    if (STAGE && B.plan) {
-        const StagePlan *P = B.plan + v.blk_base + wblk;
-        if (P->ncol > 0) {
-            int sc = 0;
-            uint32_t rcw[3] = {0u, 0u, 0u};
-            if (live) {
-                const GridDev &g = v.grid;
-                const int cx = sat_i32(floor(px / v.cs)) - g.org[0],
-                          cy = sat_i32(floor(py / v.cs)) - g.org[1];
-                // within [0, ncol) by the sort order; clamped so that a plan
-                // never indexes past its own entries
-                sc = min(max(cx * g.dim[1] + cy - P->col0, 0), P->ncol - 1);
-                const u32x4 w = ((const GLOBAL_AS u32x4 *)B.rowcnt)[gi];
-                rcw[0] = w.x;
-                rcw[1] = w.y;
-                rcw[2] = w.z;
-            }
-            staged_loop<NF, UNROLL>(P, st, live, sc, rcw, ej, fetch, make, body);
-            return;
-        }
-    }
-    if (live) neighbor_loop<PIPE, UNROLL>(ej, c, load, body);
-}
-
-// LDS record layouts of the staged passes
-__device__ __forceinline__ void fetch_qu(const double *q4, const double *u4, int nf, uint32_t j,
-                                         double *st, uint32_t s) {
-    const rec4 q = ld4(q4, j), u = ld4(u4, j);
-    st[0 * kStageRecs + s] = q.x;
-    st[1 * kStageRecs + s] = q.y;
-    st[2 * kStageRecs + s] = q.z;
-    st[3 * kStageRecs + s] = q.w;
-    st[4 * kStageRecs + s] = u.x;
-    st[5 * kStageRecs + s] = u.y;
-    st[6 * kStageRecs + s] = u.z;
-    if (nf > 7) st[7 * kStageRecs + s] = u.w;
-}
-__device__ __forceinline__ QU make_qu(const double *st, int nf, uint32_t s) {
-    QU r;
-    r.q = rec4{st[0 * kStageRecs + s], st[1 * kStageRecs + s], st[2 * kStageRecs + s],
-               st[3 * kStageRecs + s]};
-    r.u = rec4{st[4 * kStageRecs + s], st[5 * kStageRecs + s], st[6 * kStageRecs + s],
-               nf > 7 ? st[7 * kStageRecs + s] : 0.0};
-    return r;
-}
 
 // ------------------------------------------------------------------------
 // integration (integration_schemes/euler_cromer.rs:12-25)
@@ -461,7 +385,6 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap)
     // slot of lane l's first neighbor; for lanes past the scene's end it is
     // the slot the lane would have, inside the slice's allocation
     const size_t e0 = ((size_t)(v.slice_base + (i >> 6)) * cap) * 64 + (i & 63);
-    uint32_t rcw[3] = {0u, 0u, 0u};  // neighbors per stencil row (reach 1), one byte each
     if (live) {
         const KernelConst K = v.K;
         const double rh = refined_rcp(K.h);
@@ -485,24 +408,32 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap)
                         ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
                     const uint32_t a = gm(g.start)[row + zlo] - v.part_base,
                                    b = gm(g.start)[row + zhi + 1] - v.part_base;
-                    const uint32_t cnt0 = cnt;
-                    for (uint32_t j = a; j < b; j++) {
-                        const rec4 qj = ld4(v.q4, j);  // .w is being written by this pass; unused
-                        const double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
-                        const double d2 = (rx * rx + ry * ry) + rz * rz;
-                        if (d2 < v.dthr) {
-                            if (cnt < kSweepRows) rows[w][cnt][l] = j;
-                            else if (cnt < cap) gm(B.ell_j)[e0 + 64 * (size_t)cnt] = j;
-                            const double q = div_rcp(sqrt(d2), K.h, rh);
-                            accu += v.V0 * (K.wfac * kernel_w(q));
-                            cnt++;
+                    // candidates in groups of kSweepBatch: all of a group's
+                    // positions are loaded first (indices clamped to b - 1, so
+                    // every load is in bounds), then tested in index order
+                    for (uint32_t j0 = a; j0 < b; j0 += kSweepBatch) {
+                        double cx_[kSweepBatch], cy_[kSweepBatch], cz_[kSweepBatch];
+#pragma unroll
+                        for (int t = 0; t < kSweepBatch; t++) {
+                            const uint32_t jj = min(j0 + (uint32_t)t, b - 1);
+                            const rec2 xy = ((const GLOBAL_AS rec2 *)v.q4)[2 * (size_t)jj];
+                            cx_[t] = xy.x;
+                            cy_[t] = xy.y;
+                            cz_[t] = gm(v.q4)[4 * (size_t)jj + 2];  // .w is being written; unused
                         }
-                    }
-                    if (v.reach == 1) {
-                        const int r = (dx + 1) * 3 + (dy + 1);
-                        const uint32_t nr = cnt - cnt0;
-                        if (nr > 255u) gm(B.unstaged)[v.blk_base + (i >> 8)] = 1u;
-                        rcw[r >> 2] |= min(nr, 255u) << (8 * (r & 3));
+#pragma unroll
+                        for (int t = 0; t < kSweepBatch; t++) {
+                            const uint32_t j = j0 + (uint32_t)t;
+                            const double rx = px - cx_[t], ry = py - cy_[t], rz = pz - cz_[t];
+                            const double d2 = (rx * rx + ry * ry) + rz * rz;
+                            if (j < b && d2 < v.dthr) {
+                                if (cnt < kSweepRows) rows[w][cnt][l] = j;
+                                else if (cnt < cap) gm(B.ell_j)[e0 + 64 * (size_t)cnt] = j;
+                                const double q = div_rcp(sqrt(d2), K.h, rh);
+                                accu += v.V0 * (K.wfac * kernel_w(q));
+                                cnt++;
+                            }
+                        }
                     }
                 }
             }
@@ -524,7 +455,6 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap)
         gm(v.V)[i] = Vi;
         gm(v.q4)[4 * i + 3] = Vi;
         gm(B.cnt)[gi] = cnt;
-        ((GLOBAL_AS u32x4 *)B.rowcnt)[gi] = u32x4{rcw[0], rcw[1], rcw[2], 0u};
     }
     uint32_t mx = cnt;
     for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
@@ -554,57 +484,6 @@ __global__ void kb_max_count(const uint32_t *slice_len, size_t n, uint32_t *out)
 __global__ void kb_fixed_slices(uint32_t *slice_off, size_t n, uint32_t cap) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g <= n) slice_off[g] = (uint32_t)(g * cap);
-}
-
-// StagePlan of every workgroup into plan[] (one thread each; B covers the
-// whole batch; B.plan is not read).
-// The workgroup's particles are consecutive in (column, z) order, so they lie
-// in the columns from its first particle's to its last particle's; the first
-// column holds them from the first particle's z upwards, the last up to the
-// last particle's z, and columns in between whole. A plan needs reach 1 (9
-// stencil rows), at most kStageCols columns, row counts below 256 (the sweep
-// flags the workgroup otherwise) and at most kStageRecs records per row.
-__global__ void kb_stage_plan(BatchDev B, StagePlan *plan) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= B.pblk[B.ns]) return;
-    const int sk = scene_of(B.pblk, B.ns, (uint32_t)t);
-    const SceneView &v = B.views[sk];
-    StagePlan &P = plan[t];
-    P.ncol = 0;
-    const size_t i0 = (t - B.pblk[sk]) * (size_t)kBlock;
-    if (v.reach != 1 || i0 >= v.n || gm(B.unstaged)[t]) return;
-    const size_t i1 = min(i0 + (size_t)kBlock, (size_t)v.n) - 1;
-    const GridDev &g = v.grid;
-    const rec4 q0 = ld4(v.q4, i0), q1 = ld4(v.q4, i1);
-    const int x0 = sat_i32(floor(q0.x / v.cs)) - g.org[0], y0 = sat_i32(floor(q0.y / v.cs)) - g.org[1],
-              z0 = sat_i32(floor(q0.z / v.cs)) - g.org[2];
-    const int x1 = sat_i32(floor(q1.x / v.cs)) - g.org[0], y1 = sat_i32(floor(q1.y / v.cs)) - g.org[1],
-              z1 = sat_i32(floor(q1.z / v.cs)) - g.org[2];
-    const int col0 = x0 * g.dim[1] + y0, ncol = x1 * g.dim[1] + y1 - col0 + 1;
-    if (ncol < 1 || ncol > kStageCols) return;
-    uint32_t tot[kStageRows] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-    for (int c = 0; c < ncol; c++) {
-        const int col = col0 + c, gx = col / g.dim[1], gy = col % g.dim[1];
-        const int zl = max((c == 0 ? z0 : 0) - 1, 0);
-        const int zh = min((c == ncol - 1 ? z1 : g.dim[2] - 1) + 1, g.dim[2] - 1);
-        for (int r = 0; r < kStageRows; r++) {
-            const int tx = gx + r / 3 - 1, ty = gy + r % 3 - 1;
-            uint32_t a = 0u, n = 0u;
-            if (tx >= 0 && tx < g.dim[0] && ty >= 0 && ty < g.dim[1]) {
-                const uint32_t row = ((uint32_t)tx * (uint32_t)g.dim[1] + (uint32_t)ty) * (uint32_t)g.dim[2];
-                a = gm(g.start)[row + zl] - v.part_base;
-                n = gm(g.start)[row + zh + 1] - v.part_base - a;
-            }
-            P.a[r][c] = a;
-            P.n[r][c] = n;
-            P.off[r][c] = tot[r];
-            tot[r] += n;
-        }
-    }
-    for (int r = 0; r < kStageRows; r++)
-        if (tot[r] > (uint32_t)kStageRecs) return;
-    P.col0 = col0;
-    P.ncol = ncol;
 }
 
 // ------------------------------------------------------------------------
@@ -641,7 +520,7 @@ __global__ void kb_vm_sample(BatchDev B) {
 // grad W per cached pair (lockstep, coalesced stores) and get_volume
 // (quantities.rs:14-61)
 // ------------------------------------------------------------------------
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ PAIR_PASS_BOUNDS void kb_pairs_volume(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
@@ -698,18 +577,15 @@ __global__ void kb_pack_volume(BatchDev B) {
 // reset + gravity + artificial viscosity (non_pressure_accelerations.rs:19-185);
 // reactions onto dynamic bodies summed per workgroup in FP64
 // ------------------------------------------------------------------------
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
     __shared__ double lds[6 * kBlock];
-    __shared__ double st[STAGE ? 7 * kStageRecs : 1];
     PARTICLE_THREAD(B);
     const KernelConst K = v.K;
     const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     double px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, mi = 0;
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double acx = 0., acy = 0., acz = 0.;
-    uint32_t c = 0;
-    size_t e0 = 0;
     if (live) {
         const rec4 qi = ld4(v.q4, i);
         px = qi.x; py = qi.y; pz = qi.z;
@@ -726,25 +602,22 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
             gy = 0.0 + (9.81 * dy) / dist;
             gz = 0.0 + (9.81 * dz) / dist;
         }
-        c = gm(B.cnt)[gi];
-        e0 = ELL_BASE(B, v, i);
+        const uint32_t c = gm(B.cnt)[gi];
+        const size_t e0 = ELL_BASE(B, v, i);
+        neighbor_loop<PIPE, UNROLL>(
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.v4, j)}; },
+            [&](uint32_t k, const QU &r) {
+                double rx = px - r.q.x, ry = py - r.q.y, rz = pz - r.q.z;
+                double cf = v.nu_f * 2. * (3. + 2.) * r.q.w *
+                            dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, rx, ry, rz) /
+                            (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
+                double wx, wy, wz;
+                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, rx, ry, rz, wx, wy, wz);
+                acx += cf * wx;
+                acy += cf * wy;
+                acz += cf * wz;
+            });
     }
-    pair_loop<STAGE, PIPE, UNROLL, 7>(
-        B, v, lb - B.pblk[sk_], live, gi, px, py, c, e0, st,
-        [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.v4, j)}; },
-        [&](uint32_t j, double *sb, uint32_t s) { fetch_qu(v.q4, v.v4, 7, j, sb, s); },
-        [&](const double *sb, uint32_t s) { return make_qu(sb, 7, s); },
-        [&](uint32_t k, const QU &r) {
-            double rx = px - r.q.x, ry = py - r.q.y, rz = pz - r.q.z;
-            double cf = v.nu_f * 2. * (3. + 2.) * r.q.w *
-                        dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, rx, ry, rz) /
-                        (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
-            double wx, wy, wz;
-            pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, rx, ry, rz, wx, wy, wz);
-            acx += cf * wx;
-            acy += cf * wy;
-            acz += cf * wz;
-        });
     for (uint32_t bi = 0; bi < v.nb; bi++) {
         const BodyDev &b = B.bodies[v.body_base + bi];
         double f[6] = {0., 0., 0., 0., 0., 0.};
@@ -818,52 +691,35 @@ __global__ void kb_sesph_pressure(BatchDev B) {
 // acceleration or the solver's scratch (IISPH pressure_acc_f). Scenes whose
 // solve has converged are skipped when the target is the scratch.
 // ------------------------------------------------------------------------
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror) {
     __shared__ double lds[6 * kBlock];
-    __shared__ double st[STAGE ? 5 * kStageRecs : 1];
     PARTICLE_THREAD(B);
     if (target_scratch && !B.active[sid]) return;  // uniform per workgroup
     const KernelConst K = v.K;
     const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const double gamma = v.S.gamma;
-    double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0, vm = 0;
+    double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0;
     double acx = 0., acy = 0., acz = 0.;
-    uint32_t c = 0;
-    size_t e0 = 0;
     if (live) {
         const rec4 qi = ld4(v.q4, i);
         px = qi.x; py = qi.y; pz = qi.z; Vi = qi.w;
         mi = gm(v.m)[i]; pi = gm(v.p)[i];
-        c = gm(B.cnt)[gi];
-        e0 = ELL_BASE(B, v, i);
-        vm = Vi / mi;
+        const uint32_t c = gm(B.cnt)[gi];
+        const size_t e0 = ELL_BASE(B, v, i);
+        const double vm = Vi / mi;
+        neighbor_loop<PIPE, UNROLL>(
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QP{ld4(v.q4, j), gm(v.p)[j]}; },
+            [&](uint32_t k, const QP &r) {
+                double cf = vm * r.q.w * (pi + r.p);
+                double wx, wy, wz;
+                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y,
+                                  pz - r.q.z, wx, wy, wz);
+                acx -= cf * wx;
+                acy -= cf * wy;
+                acz -= cf * wz;
+            });
     }
-    pair_loop<STAGE, PIPE, UNROLL, 5>(
-        B, v, lb - B.pblk[sk_], live, gi, px, py, c, e0, st,
-        [&](uint32_t j) { return QP{ld4(v.q4, j), gm(v.p)[j]}; },
-        [&](uint32_t j, double *sb, uint32_t s) {
-            const rec4 q = ld4(v.q4, j);
-            sb[0 * kStageRecs + s] = q.x;
-            sb[1 * kStageRecs + s] = q.y;
-            sb[2 * kStageRecs + s] = q.z;
-            sb[3 * kStageRecs + s] = q.w;
-            sb[4 * kStageRecs + s] = gm(v.p)[j];
-        },
-        [&](const double *sb, uint32_t s) {
-            return QP{rec4{sb[0 * kStageRecs + s], sb[1 * kStageRecs + s], sb[2 * kStageRecs + s],
-                           sb[3 * kStageRecs + s]},
-                      sb[4 * kStageRecs + s]};
-        },
-        [&](uint32_t k, const QP &r) {
-            double cf = vm * r.q.w * (pi + r.p);
-            double wx, wy, wz;
-            pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y, pz - r.q.z,
-                              wx, wy, wz);
-            acx -= cf * wx;
-            acy -= cf * wy;
-            acz -= cf * wz;
-        });
     for (uint32_t bi = 0; bi < v.nb; bi++) {
         const BodyDev &b = B.bodies[v.body_base + bi];
         double f[6] = {0., 0., 0., 0., 0., 0.};
@@ -937,32 +793,24 @@ __global__ void kb_pred_vel(BatchDev B, int to_pred) {
 
 // source term (vp or vde), diagonal element and initial pressure in one pass
 // (iisph.rs:117-376, clamp_pressure = true)
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
-    __shared__ double st[STAGE ? 8 * kStageRecs : 1];
     PARTICLE_THREAD(B);
-    if (!STAGE && !live) return;
+    if (!live) return;
     const KernelConst K = v.K;
     const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const SolveConst S = v.S;
-    double px = 0., py = 0., pz = 0., ux = 0., uy = 0., uz = 0., Vi = 1.;
-    uint32_t c = 0;
-    size_t e0 = 0;
-    if (live) {
-        const rec4 qi = ld4(v.q4, i);
-        px = qi.x; py = qi.y; pz = qi.z; Vi = qi.w;
-        ux = gm(v.svx)[i]; uy = gm(v.svy)[i]; uz = gm(v.svz)[i];
-        c = gm(B.cnt)[gi];
-        e0 = ELL_BASE(B, v, i);
-    }
+    const rec4 qi = ld4(v.q4, i);
+    const double px = qi.x, py = qi.y, pz = qi.z;
+    const double ux = gm(v.svx)[i], uy = gm(v.svy)[i], uz = gm(v.svz)[i];
+    const double Vi = qi.w;
     const double dt = S.dt, dt2 = S.dt * S.dt;
     double src = vp ? 1. - S.V0 / Vi : 0.;
     double sfx = 0., sfy = 0., sfz = 0., sf2 = 0., sbx = 0., sby = 0., sbz = 0.;
-    pair_loop<STAGE, PIPE, UNROLL, 8>(
-        B, v, lb - B.pblk[sk_], live, gi, px, py, c, e0, st,
-        [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.sv4, j)}; },
-        [&](uint32_t j, double *sb, uint32_t s) { fetch_qu(v.q4, v.sv4, 8, j, sb, s); },
-        [&](const double *sb, uint32_t s) { return make_qu(sb, 8, s); },
+    const uint32_t c = gm(B.cnt)[gi];
+    const size_t e0 = ELL_BASE(B, v, i);
+    neighbor_loop<PIPE, UNROLL>(
+        gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.sv4, j)}; },
         [&](uint32_t k, const QU &r) {
             double wx, wy, wz;
             pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y, pz - r.q.z,
@@ -1008,40 +856,32 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
 
 // relaxed Jacobi update and per-workgroup predicted density error
 // (iisph.rs:440-531); converged scenes are skipped
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
     __shared__ double lds[kBlock];
-    __shared__ double st[STAGE ? 7 * kStageRecs : 1];
     PARTICLE_THREAD(B);
     if (!B.active[sid]) return;  // uniform per workgroup
     const KernelConst K = v.K;
     const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const SolveConst S = v.S;
     double err = 0.;
-    const double dt2 = S.dt * S.dt;
-    double ux = 0., uy = 0., uz = 0., px = 0., py = 0., pz = 0., adp = 0.;
-    uint32_t c = 0;
-    size_t e0 = 0;
     if (live) {
         const rec4 ai = ld4(v.pa4, i);
-        ux = ai.x; uy = ai.y; uz = ai.z;
+        const double ux = ai.x, uy = ai.y, uz = ai.z;
+        const double dt2 = S.dt * S.dt;
+        double adp = 0.;
         const rec4 qi = ld4(v.q4, i);
-        px = qi.x; py = qi.y; pz = qi.z;
-        c = gm(B.cnt)[gi];
-        e0 = ELL_BASE(B, v, i);
-    }
-    pair_loop<STAGE, PIPE, UNROLL, 7>(
-        B, v, lb - B.pblk[sk_], live, gi, px, py, c, e0, st,
-        [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.pa4, j)}; },
-        [&](uint32_t j, double *sb, uint32_t s) { fetch_qu(v.q4, v.pa4, 7, j, sb, s); },
-        [&](const double *sb, uint32_t s) { return make_qu(sb, 7, s); },
-        [&](uint32_t k, const QU &r) {
-            double wx, wy, wz;
-            pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y, pz - r.q.z,
-                              wx, wy, wz);
-            adp += dt2 * r.q.w * dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, wx, wy, wz);
-        });
-    if (live) {
+        const double px = qi.x, py = qi.y, pz = qi.z;
+        const uint32_t c = gm(B.cnt)[gi];
+        const size_t e0 = ELL_BASE(B, v, i);
+        neighbor_loop<PIPE, UNROLL>(
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.pa4, j)}; },
+            [&](uint32_t k, const QU &r) {
+                double wx, wy, wz;
+                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y,
+                                  pz - r.q.z, wx, wy, wz);
+                adp += dt2 * r.q.w * dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, wx, wy, wz);
+            });
         for (uint32_t bi = 0; bi < v.nb; bi++) {
             const BodyDev &b = B.bodies[v.body_base + bi];
             if (b.kind == 0) {
@@ -1194,14 +1034,14 @@ __global__ void kb_fold_bbox(const int *partials, const uint32_t *blk, int ns, i
     if (threadIdx.x < 6) out[6 * s + threadIdx.x] = lds[threadIdx.x][0];
 }
 
-#define INSTANTIATE_PAIR_PASSES(C, U, W, P, T)                                           \
-    template __global__ void kb_pairs_volume<C, U, W, P, T>(BatchDev);                  \
-    template __global__ void kb_nonpressure<C, U, W, P, T>(BatchDev);                   \
-    template __global__ void kb_pressure_accel<C, U, W, P, T>(BatchDev, int, int, int); \
-    template __global__ void kb_source_diag<C, U, W, P, T>(BatchDev, int);              \
-    template __global__ void kb_jacobi<C, U, W, P, T>(BatchDev);
-#define INSTANTIATE_BOTH(U, W, P, T) \
-    INSTANTIATE_PAIR_PASSES(true, U, W, P, T) INSTANTIATE_PAIR_PASSES(false, U, W, P, T)
+#define INSTANTIATE_PAIR_PASSES(C, U, W, P)                                              \
+    template __global__ void kb_pairs_volume<C, U, W, P>(BatchDev);                     \
+    template __global__ void kb_nonpressure<C, U, W, P>(BatchDev);                      \
+    template __global__ void kb_pressure_accel<C, U, W, P>(BatchDev, int, int, int);    \
+    template __global__ void kb_source_diag<C, U, W, P>(BatchDev, int);                 \
+    template __global__ void kb_jacobi<C, U, W, P>(BatchDev);
+#define INSTANTIATE_BOTH(U, W, P) \
+    INSTANTIATE_PAIR_PASSES(true, U, W, P) INSTANTIATE_PAIR_PASSES(false, U, W, P)
 SPH_PAIR_VARIANTS(INSTANTIATE_BOTH)
 
 }  // namespace sph

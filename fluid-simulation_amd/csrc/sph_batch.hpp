@@ -24,7 +24,6 @@ struct SceneView {
     uint32_t n;           // particles
     uint32_t part_base;   // first index of this scene in batch-global per-particle arrays
     uint32_t slice_base;  // first 64-particle slice in the batch-global slice arrays
-    uint32_t blk_base;    // first workgroup of this scene in the batch-global workgroup order
     uint32_t cell_base;   // first cell in the batch-global cell table
     uint32_t body_base;   // first body in the batch body table
     uint32_t nb;          // bodies of this scene
@@ -48,25 +47,6 @@ struct SceneView {
     uint32_t *fid, *ref, *tfid, *tref;
 };
 
-// Staging plan of one workgroup for the row-phased neighbor loops. The
-// workgroup's particles lie in ncol consecutive cell columns, the first being
-// col0 = gx * dim[1] + gy (grid-relative). For stencil row r = (dx, dy) in the
-// reference's order and source column c, the neighbors of the workgroup's
-// particles in column c lie in the scene-local index range
-// [a[r][c], a[r][c] + n[r][c]) of target column (gx + dx, gy + dy), and are
-// staged at LDS record off[r][c]; row r stages rown[r] records in all.
-// ncol = 0: the workgroup runs the unstaged loop.
-constexpr int kStageCols = 4;
-constexpr int kStageRows = 9;
-constexpr int kStageRecs = 576;  // LDS records per row phase
-struct StagePlan {
-    int ncol;
-    int col0;
-    uint32_t a[kStageRows][kStageCols];
-    uint32_t n[kStageRows][kStageCols];
-    uint32_t off[kStageRows][kStageCols];
-};
-
 struct BatchDev {
     const SceneView *views;
     const uint32_t *pblk;  // workgroup prefix over particles (ns + 1)
@@ -86,11 +66,6 @@ struct BatchDev {
     uint32_t *slice_len;
     const uint32_t *slice_off;
     uint32_t *ell_j;                   // neighbor's scene-local sorted index
-    // per particle, its neighbor count in each of the 9 stencil rows (bytes
-    // 0..8 of a 16-byte record; written by kb_sweep_ell when reach is 1)
-    uint32_t *rowcnt;
-    uint32_t *unstaged;                // per workgroup: 1 = a row count exceeded 255
-    StagePlan *plan;                   // per workgroup (batch-global order)
     double *ell_gx, *ell_gy, *ell_gz;  // grad W(x_i - x_j)
     uint32_t *cell_cnt;
     uint32_t *cell_start;
@@ -108,13 +83,12 @@ struct BatchDev {
 // Pair passes take CACHED = true to read grad W from the pair cache (written by
 // kb_pairs_volume<true>) or false to recompute it from positions, and a code
 // shape: the neighbor loop's unroll factor, a floor on waves per SIMD (which
-// caps registers per lane), whether the loop is software-pipelined, and
-// whether neighbor records are staged through LDS row by row (StagePlan).
-// Every shape evaluates the same expressions in the same order, so results do
-// not depend on the shape.
-#define SPH_PAIR_VARIANTS(X)                                                              \
-    X(4, 0, 1, 0) X(2, 0, 1, 0) X(4, 4, 1, 0) X(2, 6, 1, 0) X(1, 0, 0, 0) X(2, 0, 0, 0)     \
-    X(4, 0, 0, 0) X(1, 8, 0, 0) X(1, 0, 0, 1) X(2, 0, 0, 1) X(1, 4, 0, 1)
+// caps registers per lane) and the loop form (neighbor_loop's PIPE). Every
+// shape evaluates the same expressions in the same order, so results do not
+// depend on the shape.
+#define SPH_PAIR_VARIANTS(X)                                                      \
+    X(4, 0, 1) X(2, 0, 1) X(4, 4, 1) X(1, 0, 0) X(2, 0, 0) X(1, 8, 0) X(1, 0, 2)  \
+    X(2, 0, 2) X(1, 8, 2)
 constexpr int kPairVariants = SPH_SHAPES;
 __global__ void kb_integrate_ec(BatchDev B);
 __global__ void kb_speed_below(BatchDev B);
@@ -126,23 +100,24 @@ __global__ void kb_permute(BatchDev B);
 __global__ void kb_count_ell(BatchDev B);
 // neighbor rows a wavefront of kb_sweep_ell buffers in LDS (48 KB per workgroup)
 constexpr int kSweepRows = 48;
+// candidates whose positions kb_sweep_ell loads together
+constexpr int kSweepBatch = 4;
 __global__ void kb_sweep_ell(BatchDev B, uint32_t cap);
 __global__ void kb_max_count(const uint32_t *slice_len, size_t n, uint32_t *out);
 __global__ void kb_fixed_slices(uint32_t *slice_off, size_t n, uint32_t cap);
-__global__ void kb_stage_plan(BatchDev B, StagePlan *plan);
 __global__ void kb_fill_ell(BatchDev B);
 __global__ void kb_vm_sample(BatchDev B);
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_pairs_volume(BatchDev B);
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_nonpressure(BatchDev B);
 __global__ void kb_sesph_pressure(BatchDev B);
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror);
 __global__ void kb_pred_vel(BatchDev B, int to_pred);
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_source_diag(BatchDev B, int vp);
-template <bool CACHED, int UNROLL, int WAVES, int PIPE, int STAGE>
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_jacobi(BatchDev B);
 __global__ void kb_pred_pos(BatchDev B);
 __global__ void kb_ost_commit(BatchDev B);

@@ -396,7 +396,7 @@ struct Scene {
     uint32_t iter = 0;
     double err = 0;
     // placement in the batch that steps it
-    uint32_t part_base = 0, slice_base = 0, cell_base = 0, body_base = 0, blk_base = 0;
+    uint32_t part_base = 0, slice_base = 0, cell_base = 0, body_base = 0;
 
     bool boundaries_initialized = false;
     bool initialized = false;
@@ -414,7 +414,6 @@ struct Scene {
         v.n = (uint32_t)n;
         v.part_base = part_base;
         v.slice_base = slice_base;
-        v.blk_base = blk_base;
         v.cell_base = cell_base;
         v.body_base = body_base;
         v.nb = (uint32_t)bodies.size();
@@ -509,11 +508,7 @@ struct Batch {
     DBuf<uint32_t> perm, cnt, slice_len, slice_off, ell_j, cell_cnt, cell_start, cursor;
     DBuf<uint32_t> pblk, cblk, scan_a, scan_b, scan_c;
     DBuf<double> ell_gx, ell_gy, ell_gz, partials, red, chunk_err;
-    DBuf<uint32_t> d_active, d_iters, chunk_ran, ablk, amap, max_cnt, rowcnt, unstaged;
-    // LDS staging plans of the pair passes' workgroups (valid after a sweep
-    // of the current layout; the staged code shapes run unstaged otherwise)
-    DBuf<StagePlan> plan;
-    bool plans_valid = false;
+    DBuf<uint32_t> d_active, d_iters, chunk_ran, ablk, amap, max_cnt;
     uint32_t iter_hint = 3;  // iterations of the last target-terminated solve
     // Jacobi chunks captured as HIP graphs, keyed by their launch arguments (on
     // by default for a context, off for a world, whose layout changes with
@@ -659,16 +654,16 @@ static void check_device() {
 typedef void (*KernB)(BatchDev);
 typedef void (*KernBI)(BatchDev, int);
 typedef void (*KernBIII)(BatchDev, int, int, int);
-#define PV_F(U, W, P, T) &kb_pairs_volume<false, U, W, P, T>,
-#define PV_T(U, W, P, T) &kb_pairs_volume<true, U, W, P, T>,
-#define NP_F(U, W, P, T) &kb_nonpressure<false, U, W, P, T>,
-#define NP_T(U, W, P, T) &kb_nonpressure<true, U, W, P, T>,
-#define SD_F(U, W, P, T) &kb_source_diag<false, U, W, P, T>,
-#define SD_T(U, W, P, T) &kb_source_diag<true, U, W, P, T>,
-#define PA_F(U, W, P, T) &kb_pressure_accel<false, U, W, P, T>,
-#define PA_T(U, W, P, T) &kb_pressure_accel<true, U, W, P, T>,
-#define JA_F(U, W, P, T) &kb_jacobi<false, U, W, P, T>,
-#define JA_T(U, W, P, T) &kb_jacobi<true, U, W, P, T>,
+#define PV_F(U, W, P) &kb_pairs_volume<false, U, W, P>,
+#define PV_T(U, W, P) &kb_pairs_volume<true, U, W, P>,
+#define NP_F(U, W, P) &kb_nonpressure<false, U, W, P>,
+#define NP_T(U, W, P) &kb_nonpressure<true, U, W, P>,
+#define SD_F(U, W, P) &kb_source_diag<false, U, W, P>,
+#define SD_T(U, W, P) &kb_source_diag<true, U, W, P>,
+#define PA_F(U, W, P) &kb_pressure_accel<false, U, W, P>,
+#define PA_T(U, W, P) &kb_pressure_accel<true, U, W, P>,
+#define JA_F(U, W, P) &kb_jacobi<false, U, W, P>,
+#define JA_T(U, W, P) &kb_jacobi<true, U, W, P>,
 static const KernB k_volume_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(PV_F)},
                                                      {SPH_PAIR_VARIANTS(PV_T)}};
 static const KernB k_nonpressure_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(NP_F)},
@@ -982,9 +977,6 @@ static BatchDev batch_dev(Batch &B) {
     d.ell_gx = B.ell_gx.p;
     d.ell_gy = B.ell_gy.p;
     d.ell_gz = B.ell_gz.p;
-    d.rowcnt = B.rowcnt.p;
-    d.unstaged = B.unstaged.p;
-    d.plan = B.plans_valid ? B.plan.p : nullptr;
     d.cell_cnt = B.cell_cnt.p;
     d.cell_start = B.cell_start.p;
     d.cursor = B.cursor.p;
@@ -1009,7 +1001,6 @@ static void layout(Batch &B) {
         c.part_base = (uint32_t)parts;
         c.slice_base = (uint32_t)slices;
         c.body_base = (uint32_t)bodies;
-        c.blk_base = (uint32_t)blocks;
         B.hp_blk[s] = (uint32_t)blocks;
         parts += c.n;
         slices += (c.n + 63) / 64;
@@ -1026,10 +1017,6 @@ static void layout(Batch &B) {
     B.ci.reserve(3 * std::max<size_t>(parts, 1));
     B.perm.reserve(std::max<size_t>(parts, 1));
     B.cnt.reserve(std::max<size_t>(parts, 1));
-    B.rowcnt.reserve(4 * std::max<size_t>(parts, 1));
-    B.unstaged.reserve(std::max<size_t>(blocks, 1));
-    B.plan.reserve(std::max<size_t>(blocks, 1));
-    B.plans_valid = false;
     B.slice_len.reserve(slices + 1);
     B.slice_off.reserve(slices + 1);
     B.partials.reserve((blocks + 1) * kMaxBodiesPerScene * 6 + 64);
@@ -1255,8 +1242,6 @@ static void sweep_neighbors(Batch &B) {
         if (slots > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "neighbor cache exceeds 2^32 slots");
         B.ell_j.reserve(std::max<uint64_t>(slots, 1));
         LAUNCH(kb_fixed_slices, B.total_slices + 1, B.s, B.slice_off.p, B.total_slices, cap);
-        LAUNCH(k_fill_u32, B.total_pblocks, B.s, B.unstaged.p, B.total_pblocks, 0u);
-        B.plans_valid = false;
         BatchDev d = batch_dev(B);
         PLAUNCH(B, SPH_PROF_NEIGHBOR_COUNT, B.total_particles, kb_sweep_ell, B.total_pblocks, d,
                 cap);
@@ -1270,8 +1255,6 @@ static void sweep_neighbors(Batch &B) {
         const uint32_t want = std::max<uint32_t>(16, ((mx + 8 + 7) / 8) * 8);
         if (mx <= cap) {
             B.ell_cap = want;
-            LAUNCH(kb_stage_plan, B.total_pblocks, B.s, d, B.plan.p);
-            B.plans_valid = true;
             return;
         }
         cap = want;  // overflow: the cache was too narrow for this sweep
@@ -1556,7 +1539,6 @@ static void prepare(Batch &B) {
         boundary_samples(B);
         sweep_neighbors(B);
     } else {
-        B.plans_valid = false;
         neighbor_counts(B);
         boundary_samples(B);
         pairs_and_volume(B);
@@ -2460,16 +2442,6 @@ static int get_option(const Batch &B, int option) {
     if (option == SPH_OPT_TUNE_STATE) return B.tuner.state;
     if (option == SPH_OPT_GRAPHS) return B.use_graphs ? 1 : 0;
     if (option == SPH_OPT_SWEEP) return B.sweep ? 1 : 0;
-    if (option == SPH_OPT_STAGED) {  // workgroups with a staging plan
-        if (!B.plans_valid || B.total_pblocks == 0) return 0;
-        std::vector<StagePlan> h(B.total_pblocks);
-        HIPCK(hipMemcpyAsync(h.data(), B.plan.p, h.size() * sizeof(StagePlan),
-                             hipMemcpyDeviceToHost, B.s));
-        HIPCK(hipStreamSynchronize(B.s));
-        int n = 0;
-        for (const StagePlan &P : h) n += P.ncol > 0 ? 1 : 0;
-        return n;
-    }
     if (option >= SPH_OPT_SHAPE_VOLUME && option < SPH_OPT_SHAPE_VOLUME + kPairPasses)
         return B.variant[option - SPH_OPT_SHAPE_VOLUME];
     throw Error(SPH_ERR_ARG, "unknown option");

@@ -129,8 +129,7 @@ OPT_TUNE_STATE = 3
 OPT_SHAPE_VOLUME = 4
 OPT_GRAPHS = 9
 OPT_SWEEP = 10
-OPT_STAGED = 11
-SHAPES = 11
+SHAPES = 9
 PAIR_PASSES = ["volume", "nonpressure", "source_diag", "pressure_accel", "jacobi"]
 
 _lib = None

@@ -309,8 +309,8 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *   SPH_OPT_AUTOTUNE  steps per code shape of the online tuner (default 1;
  *       0 = off): the first steps of a context or world, and the steps after
  *       its particle count changed 2x, time each code shape (unroll, waves per
- *       SIMD, software-pipelined or plain loop) of the five neighbor passes,
- *       which then keep their fastest.
+ *       SIMD, loop form: plain, software-pipelined, next index prefetched) of
+ *       the five neighbor passes, which then keep their fastest.
  *   SPH_OPT_SHAPE  (set) run every neighbor pass in this code shape
  *       (0 .. SPH_SHAPES-1) and turn the tuner off.
  *   SPH_OPT_TUNE_STATE  (get) 0 untuned, 1 measuring, 2 tuned.
@@ -326,10 +326,6 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *       stored grad W (SPH_OPT_CACHE_GRADIENTS) always uses the passes. A
  *       value above 1 also sets the width of the next sweep's cache; a
  *       sweep that finds more neighbors than fit runs again, wider.
- *   SPH_OPT_STAGED  (get) the number of workgroups whose neighbor records the
- *       staged code shapes copy through LDS row by row (the others run those
- *       shapes unstaged): workgroups within at most 4 cell columns, when the
- *       search reach is one cell and the one-sweep path built the cache.
  * The options of a context apply when it steps alone, those of a world to
  * the world's steps. */
 enum {
@@ -339,10 +335,9 @@ enum {
     SPH_OPT_TUNE_STATE = 3,
     SPH_OPT_SHAPE_VOLUME = 4,
     SPH_OPT_GRAPHS = 9,
-    SPH_OPT_SWEEP = 10,
-    SPH_OPT_STAGED = 11
+    SPH_OPT_SWEEP = 10
 };
-enum { SPH_SHAPES = 11 };
+enum { SPH_SHAPES = 9 };
 int sph_set_option(sph_ctx *ctx, int option, int value);
 int sph_get_option(const sph_ctx *ctx, int option, int *value);
 int sph_world_set_option(sph_world *world, int option, int value);

@@ -168,38 +168,6 @@ def test_one_sweep_matches_separate_passes(native, width):
         assert_same(x, z)
 
 
-def test_staged_shapes_bit_identical(native):
-    """The code shapes that stage neighbor records through LDS row by row give
-    the plain loops' bits, alone and in a world, on columns tall enough that
-    most workgroups get a staging plan."""
-    from sph_amd import _native as N
-    staged = [8, 9, 10]  # the STAGE = 1 entries of SPH_PAIR_VARIANTS
-    for solver in ("IISPH", "IISPHwOST"):
-        scs = [scenes.small_iisph_box(side=6, height=60, box=1.4, seed=7, solver=solver,
-                                      jitter=2e-3),
-               scenes.small_iisph_box(side=7, height=40, box=1.4, seed=8, solver=solver,
-                                      jitter=2e-3)]
-        refs = [native.System(sc) for sc in scs]
-        for r in refs:
-            r.set_option(N.OPT_SHAPE, 4)  # plain loop, unroll 1
-            r.step_forward_in_time(4)
-        for shape in staged:
-            s = native.System(scs[0])
-            s.set_option(N.OPT_SHAPE, shape)
-            blocks = s.get_option(N.OPT_STAGED)
-            print(f"{solver} shape {shape}: {blocks} staged workgroups", flush=True)
-            assert blocks > 0
-            s.step_forward_in_time(4)
-            assert_same(refs[0], s)
-        mates = [native.System(sc) for sc in scs]
-        w = native.World(mates)
-        w.set_option(N.OPT_SHAPE, staged[1])
-        w.step(4)
-        assert w.get_option(N.OPT_STAGED) > 0
-        for r, m in zip(refs, mates):
-            assert_same(r, m)
-
-
 def test_code_shapes_bit_identical_and_tuner(native):
     """Every code shape of the neighbor passes gives the same bits; the online
     tuner measures all of them on live steps and then keeps one per pass."""
