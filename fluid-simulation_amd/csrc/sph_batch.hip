@@ -377,7 +377,7 @@ __global__ void kb_fill_ell(BatchDev B) {
 // other pass reads them in, with kb_pairs_volume's expressions; the boundary
 // terms follow as in kb_pairs_volume, so boundary samples must be current.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap) {
+__global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap, uint32_t *maxc) {
     __shared__ uint32_t rows[kBlock / 64][kSweepRows][64];
     PARTICLE_THREAD(B);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -458,26 +458,15 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap)
     }
     uint32_t mx = cnt;
     for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-    if (l == 0 && live) gm(B.slice_len)[v.slice_base + (i >> 6)] = mx;
+    if (l == 0 && live) {
+        gm(B.slice_len)[v.slice_base + (i >> 6)] = mx;
+        atomicMax(maxc, mx);  // the batch's largest count (zeroed before the launch)
+    }
     // the wavefront's buffered rows as whole rows (lanes past their own count
     // write a value no pass reads)
     const uint32_t nrows = min(min(mx, (uint32_t)kSweepRows), cap);
     if (live)
         for (uint32_t k = 0; k < nrows; k++) gm(B.ell_j)[e0 + 64 * (size_t)k] = rows[w][k][l];
-}
-
-// largest neighbor count of the batch (slice maxima folded by one workgroup)
-__global__ void kb_max_count(const uint32_t *slice_len, size_t n, uint32_t *out) {
-    __shared__ uint32_t red[kBlock];
-    uint32_t mx = 0;
-    for (size_t k = threadIdx.x; k < n; k += kBlock) mx = max(mx, slice_len[k]);
-    red[threadIdx.x] = mx;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = red[0];
 }
 
 // slice_off[g] = g * cap: the fixed-width layout in the form ELL_BASE reads

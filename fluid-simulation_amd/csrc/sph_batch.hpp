@@ -102,8 +102,7 @@ __global__ void kb_count_ell(BatchDev B);
 constexpr int kSweepRows = 48;
 // candidates whose positions kb_sweep_ell loads together
 constexpr int kSweepBatch = 4;
-__global__ void kb_sweep_ell(BatchDev B, uint32_t cap);
-__global__ void kb_max_count(const uint32_t *slice_len, size_t n, uint32_t *out);
+__global__ void kb_sweep_ell(BatchDev B, uint32_t cap, uint32_t *maxc);
 __global__ void kb_fixed_slices(uint32_t *slice_off, size_t n, uint32_t cap);
 __global__ void kb_fill_ell(BatchDev B);
 __global__ void kb_vm_sample(BatchDev B);

@@ -739,7 +739,10 @@ static void tune_end_step(Batch &B, size_t particles) {
 }
 
 // exclusive scan of n uint32 into out[0..n]; out[n] = total (returned)
-static uint32_t scan_u32(Batch &B, const uint32_t *in, size_t n, uint32_t *out) {
+// exclusive scan of n counts into out[0..n]; out[n] = the total. A caller that
+// knows the total passes it (known_total >= 0): no read-back, no host wait.
+static uint32_t scan_u32(Batch &B, const uint32_t *in, size_t n, uint32_t *out,
+                         int64_t known_total = -1) {
     const size_t BS = 1024;
     if (n == 0) {
         uint32_t z = 0;
@@ -769,6 +772,11 @@ static uint32_t scan_u32(Batch &B, const uint32_t *in, size_t n, uint32_t *out) 
         }
         hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb1), dim3(512), 0, B.s, out, n, B.scan_b.p);
         HIPCK(hipGetLastError());
+    }
+    if (known_total >= 0) {
+        const uint32_t t32 = (uint32_t)known_total;
+        B.put(out + n, &t32, sizeof t32);
+        return t32;
     }
     uint32_t *last = reinterpret_cast<uint32_t *>(B.pinned);
     HIPCK(hipMemcpyAsync(&last[0], out + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
@@ -1155,7 +1163,8 @@ static void sort_scenes(Batch &B) {
     LAUNCH(k_fill_u32, B.total_cells, B.s, B.cell_cnt.p, B.total_cells, 0u);
     LAUNCH(k_fill_u32, B.total_cells, B.s, B.cursor.p, B.total_cells, 0u);
     PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cell_count, B.total_pblocks, d);
-    scan_u32(B, B.cell_cnt.p, B.total_cells, B.cell_start.p);
+    // every particle of the batch lies in one cell: the total is known
+    scan_u32(B, B.cell_cnt.p, B.total_cells, B.cell_start.p, (int64_t)B.total_particles);
     PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cell_scatter, B.total_pblocks, d);
     PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cell_order, B.hc_blk.back(), d);
     PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_permute, B.total_pblocks, d);
@@ -1242,12 +1251,10 @@ static void sweep_neighbors(Batch &B) {
         if (slots > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "neighbor cache exceeds 2^32 slots");
         B.ell_j.reserve(std::max<uint64_t>(slots, 1));
         LAUNCH(kb_fixed_slices, B.total_slices + 1, B.s, B.slice_off.p, B.total_slices, cap);
+        LAUNCH(k_fill_u32, 1, B.s, B.max_cnt.p, (size_t)1, 0u);
         BatchDev d = batch_dev(B);
         PLAUNCH(B, SPH_PROF_NEIGHBOR_COUNT, B.total_particles, kb_sweep_ell, B.total_pblocks, d,
-                cap);
-        hipLaunchKernelGGL(kb_max_count, dim3(1), dim3(kBlock), 0, B.s, B.slice_len.p,
-                           B.total_slices, B.max_cnt.p);
-        HIPCK(hipGetLastError());
+                cap, B.max_cnt.p);
         uint32_t mx = 0;
         HIPCK(hipMemcpyAsync(B.pinned, B.max_cnt.p, sizeof mx, hipMemcpyDeviceToHost, B.s));
         B.sync();
