@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -334,6 +335,11 @@ struct Body {
     DBuf<uint8_t> flag;
     DBuf<double> bx, by, bz, bvx, bvy, bvz, bvol;
     double applied[6] = {0, 0, 0, 0, 0, 0};
+    // body-frame bounding box of the boundary's mesh (LocalAABB,
+    // volume_map_boundary.rs:1137-1150); without one the boundary takes no
+    // part in rigid-rigid contact
+    bool has_aabb = false;
+    double aabb_c[3] = {0, 0, 0}, aabb_h[3] = {0, 0, 0};
 
     BodyDev view() const {
         BodyDev b{};
@@ -406,6 +412,18 @@ struct Scene {
     uint32_t last_iter = 0, eqs1_iter = 0;
     double last_err = 0, avg_density = 0;
     std::vector<double> err_hist;
+    // rigid-rigid contact of a volume-map system: the Gauss-Legendre rule of
+    // the contact-cell quadrature (degree 0: off), the last step's active
+    // cells (kContactRec doubles each) and NNCG iterations, device scratch
+    int contact_degree = 0;
+    double contact_x[kContactMaxDegree] = {0}, contact_w[kContactMaxDegree] = {0};
+    std::vector<double> contact_rec;
+    uint32_t contact_iters = 0;
+    DBuf<ContactPairDev> cpairs;
+    DBuf<BodyDev> cbodies;
+    DBuf<unsigned int> cnact;
+    DBuf<uint32_t> cidx;
+    DBuf<double> csums;
 
     size_t ncells() const { return (size_t)grid.dim[0] * grid.dim[1] * grid.dim[2]; }
 
@@ -1591,6 +1609,470 @@ static void update(Batch &B) {
     }
 }
 
+// ------------------------------------------------------------------------
+// rigid-rigid contact (VolumeMapBoundary::step_forward_in_time,
+// volume_map_boundary.rs:1440-1453): the broad phase and sub-cell grid on the
+// host, the contact cells' quadrature on the device (k_contact_cells), then
+// compute_diagonal's closing terms and the NNCG solve (solve_for_p_c,
+// :186-637) over the active cells on the host, in binary64 with the
+// reference's operation order
+// ------------------------------------------------------------------------
+// a contact-cell record: body a, body b, volume, center[3], f_a[3], tau_a[3],
+// f_b[3], tau_b[3], a_cc pressure, a_cc friction, p, lambda[3]
+constexpr int kContactRec = 24;
+namespace contact {
+constexpr double kCellSizeFactor = 0.5;  // CONTACT_CELL_SIZE_FACTOR
+constexpr double kFriction = 0.5;        // FRICTION_COEFFICIENT
+constexpr double kRelaxation = 0.5;      // RELAXATION
+constexpr double kMinDiagonal = 1e-35;   // MIN_DIAGONAL_ELEMENT
+constexpr double kFbPressure = 1e-6, kFbFriction = 1e-3;
+// the reference iterates until converged (warning past 100 iterations)
+constexpr uint32_t kMaxIterations = 100000;
+enum { A = 0, B = 1, VOL = 2, CEN = 3, FA = 6, TA = 9, FB = 12, TB = 15, ACCP = 18, ACCF = 19,
+       P = 20, LAM = 21 };
+
+static inline double dot(const double *a, const double *b) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline double norm(const double *a) { return std::sqrt(dot(a, a)); }
+static inline void cross(const double *a, const double *b, double *o) {
+    const double x = a[1] * b[2] - a[2] * b[1];
+    const double y = a[2] * b[0] - a[0] * b[2];
+    const double z = a[0] * b[1] - a[1] * b[0];
+    o[0] = x;
+    o[1] = y;
+    o[2] = z;
+}
+static inline void sub(const double *a, const double *b, double *o) {
+    for (int k = 0; k < 3; k++) o[k] = a[k] - b[k];
+}
+// UnitQuaternion * Vector3: t = 2 (q_v x v); v' = t w + q_v x t + v
+static void rotate(const double q[4], const double v[3], double o[3]) {
+    const double qv[3] = {q[1], q[2], q[3]};
+    double t[3], c[3];
+    cross(qv, v, t);
+    for (int k = 0; k < 3; k++) t[k] *= 2.0;
+    cross(qv, t, c);
+    for (int k = 0; k < 3; k++) o[k] = t[k] * q[0] + c[k] + v[k];
+}
+
+// BoundaryType::world_aabb (:2122-2147): |R| applied to the half extents
+// around the posed center
+static void world_aabb(const Body &b, double mn[3], double mx[3]) {
+    if (!b.dynamic) {
+        for (int k = 0; k < 3; k++) {
+            mn[k] = b.aabb_c[k] - b.aabb_h[k];
+            mx[k] = b.aabb_c[k] + b.aabb_h[k];
+        }
+        return;
+    }
+    double R[9], hw[3], c[3];
+    Rigid::quat_matrix(b.rb.q, R);
+    for (double &r : R) r = std::fabs(r);
+    Rigid::mv(R, b.aabb_h, hw);
+    rotate(b.rb.q, b.aabb_c, c);
+    for (int k = 0; k < 3; k++) c[k] = c[k] + b.rb.com[k];
+    for (int k = 0; k < 3; k++) {
+        mn[k] = c[k] - hw[k];
+        mx[k] = c[k] + hw[k];
+    }
+}
+
+// compute_diagonal (:884-946) from the device sums of one cell
+static void diagonal(const Scene &S, const double *sums, double *r, double dt) {
+    const Body &Ab = *S.bodies[(int)r[A]], &Bb = *S.bodies[(int)r[B]];
+    r[VOL] = sums[0];
+    for (int k = 0; k < 3; k++) {
+        r[CEN + k] = sums[1 + k] / r[VOL];
+        r[FA + k] = sums[4 + k];
+        r[TA + k] = sums[7 + k];
+        r[FB + k] = sums[10 + k];
+        r[TB + k] = sums[13 + k];
+    }
+    double m[3], term_a = 0.0, term_b = 0.0;
+    if (Ab.dynamic) {
+        Rigid::mv(Ab.rb.Iinv_world, r + TA, m);
+        term_a = dot(r + FA, r + FA) / Ab.rb.mass + dot(r + TA, m);
+    }
+    if (Bb.dynamic) {
+        Rigid::mv(Bb.rb.Iinv_world, r + TB, m);
+        term_b = dot(r + FB, r + FB) / Bb.rb.mass + dot(r + TB, m);
+    }
+    r[ACCP] = dt * dt * (term_a + term_b);
+    // a_cc friction = 1/3 tr(dt T (K_a + K_b)), K = I/m - [c - com]x I^-1 [c - com]x
+    const double fn = std::fmax(norm(r + FA), 1e-15);
+    const double nrm[3] = {r[FA] / fn, r[FA + 1] / fn, r[FA + 2] / fn};
+    double T[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) T[3 * i + j] = (i == j ? 1.0 : 0.0) - nrm[i] * nrm[j];
+    double Ksum[9];
+    const Body *pair[2] = {&Ab, &Bb};
+    double Kb[2][9];
+    for (int e = 0; e < 2; e++) {
+        const Body &bb = *pair[e];
+        if (!bb.dynamic) {
+            for (double &v : Kb[e]) v = 0.0;
+            continue;
+        }
+        double d[3], X[9], XI[9], XIX[9];
+        sub(r + CEN, bb.rb.com, d);
+        X[0] = 0.0; X[1] = -d[2]; X[2] = d[1];
+        X[3] = d[2]; X[4] = 0.0; X[5] = -d[0];
+        X[6] = -d[1]; X[7] = d[0]; X[8] = 0.0;
+        Rigid::mm(X, bb.rb.Iinv_world, XI);
+        Rigid::mm(XI, X, XIX);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                Kb[e][3 * i + j] = (i == j ? 1.0 : 0.0) / bb.rb.mass - XIX[3 * i + j];
+    }
+    for (int k = 0; k < 9; k++) Ksum[k] = Kb[0][k] + Kb[1][k];
+    double dT[9], Pm[9];
+    for (int k = 0; k < 9; k++) dT[k] = dt * T[k];
+    Rigid::mm(dT, Ksum, Pm);
+    r[ACCF] = 1. / 3. * (((0.0 + Pm[0]) + Pm[4]) + Pm[8]);
+}
+
+// solve_for_p_c (:186-637) over the active cells of one scene
+struct Solver {
+    const Scene &S;
+    double *rec;
+    size_t nc;
+    int nb;
+    double dt;
+    std::vector<double> T, p, verr, lam, vtang, sp, sl, pj, lj, gp, gl, pp, vtn, dev, ef, et, vt,
+        wt;
+    std::vector<uint32_t> nact;
+
+    Solver(const Scene &s, double *r, size_t n, double step)
+        : S(s), rec(r), nc(n), nb((int)s.bodies.size()), dt(step), T(9 * n), p(n), verr(n),
+          lam(3 * n), vtang(3 * n), sp(n), sl(3 * n), pj(n), lj(3 * n), gp(n), gl(3 * n), pp(n),
+          vtn(n), dev(n), ef(3 * nb), et(3 * nb), vt(3 * nb), wt(3 * nb), nact(nb) {
+        for (size_t c = 0; c < nc; c++) {  // tangential projection I - n n^T (:927-928)
+            const double *r = cell(c);
+            const double fn = std::fmax(norm(r + FA), 1e-15);
+            const double nrm[3] = {r[FA] / fn, r[FA + 1] / fn, r[FA + 2] / fn};
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    T[9 * c + 3 * i + j] = (i == j ? 1.0 : 0.0) - nrm[i] * nrm[j];
+        }
+    }
+    const double *cell(size_t c) const { return rec + c * kContactRec; }
+    int body(size_t c, int side) const { return (int)cell(c)[side ? B : A]; }
+    const Body &bd(int i) const { return *S.bodies[i]; }
+
+    // accumulate_extra_forces (:948-986): each body over its cells in cell order
+    void extra() {
+        for (int bi = 0; bi < nb; bi++) {
+            double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
+            const bool dyn = bd(bi).dynamic;
+            for (size_t c = 0; c < nc; c++) {
+                const double *r = cell(c);
+                const double *l = &lam[3 * c];
+                for (int side = 0; side < 2; side++) {
+                    if (body(c, side) != bi) continue;
+                    const double *fs = r + (side ? FB : FA), *ts = r + (side ? TB : TA);
+                    for (int k = 0; k < 3; k++)
+                        f[k] += side ? p[c] * fs[k] - l[k] : p[c] * fs[k] + l[k];
+                    if (dyn) {
+                        double d[3], cr[3];
+                        sub(r + CEN, bd(bi).rb.com, d);
+                        cross(d, l, cr);
+                        for (int k = 0; k < 3; k++)
+                            t[k] += side ? p[c] * ts[k] - cr[k] : p[c] * ts[k] + cr[k];
+                    }
+                }
+            }
+            for (int k = 0; k < 3; k++) {
+                ef[3 * bi + k] = f[k];
+                et[3 * bi + k] = t[k];
+            }
+        }
+    }
+    // compute_trial_velocities (:988-1012); RigidBodyMotion::trial_velocity,
+    // trial_angular_velocity (rigid_body_motion.rs:166-181)
+    void trial() {
+        for (int bi = 0; bi < nb; bi++) {
+            const Body &b = bd(bi);
+            if (!b.dynamic) {
+                for (int k = 0; k < 3; k++) vt[3 * bi + k] = wt[3 * bi + k] = 0.0;
+                continue;
+            }
+            double l[3];
+            for (int k = 0; k < 3; k++) {
+                vt[3 * bi + k] = b.rb.v[k] + dt / b.rb.mass * (b.rb.F[k] + ef[3 * bi + k]);
+                l[k] = b.rb.L[k] + dt * (b.rb.T[k] + et[3 * bi + k]);
+            }
+            Rigid::mv(b.rb.Iinv_world, l, &wt[3 * bi]);
+        }
+    }
+    // predicted_volume_error, predicted_tangential_velocity (:996-1033)
+    void predict() {
+        for (size_t c = 0; c < nc; c++) {
+            const double *r = cell(c);
+            const int a = body(c, 0), b = body(c, 1);
+            const double *va = &vt[3 * a], *wa = &wt[3 * a], *vb = &vt[3 * b], *wb = &wt[3 * b];
+            verr[c] = -r[VOL] +
+                      dt * (((dot(va, r + FA) + dot(wa, r + TA)) + dot(vb, r + FB)) + dot(wb, r + TB));
+            if (bd(a).dynamic && bd(b).dynamic) {
+                double ra[3], rb[3], ca[3], cb[3], vr[3];
+                sub(r + CEN, bd(a).rb.com, ra);
+                sub(r + CEN, bd(b).rb.com, rb);
+                cross(wa, ra, ca);
+                cross(wb, rb, cb);
+                for (int k = 0; k < 3; k++) vr[k] = ((va[k] + ca[k]) - vb[k]) - cb[k];
+                Rigid::mv(&T[9 * c], vr, &vtang[3 * c]);
+            } else {
+                vtang[3 * c] = vtang[3 * c + 1] = vtang[3 * c + 2] = 0.0;
+            }
+        }
+    }
+    // projected_jacobi (:1035-1102)
+    void jacobi() {
+        for (int bi = 0; bi < nb; bi++) {
+            uint32_t n = 0;
+            for (size_t c = 0; c < nc; c++) {
+                const int hits = (body(c, 0) == bi) + (body(c, 1) == bi);
+                if (hits && (verr[c] < 0.0 || p[c] > 0.0)) n += (uint32_t)hits;
+            }
+            nact[bi] = n;
+        }
+        for (size_t c = 0; c < nc; c++) {
+            const double *r = cell(c);
+            const uint32_t nm = std::max(std::max(nact[body(c, 0)], nact[body(c, 1)]), 1u);
+            const double n_max = (double)nm;
+            const double alpha = kRelaxation / n_max / std::fmax(r[ACCP], kMinDiagonal);
+            pj[c] = std::fmax(p[c] - alpha * verr[c], 0.0);
+            const double lmax = kFriction * p[c] * norm(r + FA);
+            const double d = std::fmax(r[ACCF], kMinDiagonal);
+            double raw[3], star[3], nw[3];
+            for (int k = 0; k < 3; k++) raw[k] = -vtang[3 * c + k] / d;
+            const double rn = norm(raw);
+            for (int k = 0; k < 3; k++) star[k] = rn > lmax ? lmax * raw[k] / rn : raw[k];
+            for (int k = 0; k < 3; k++) nw[k] = lam[3 * c + k] + kRelaxation / n_max * star[k];
+            const double nn = norm(nw);
+            for (int k = 0; k < 3; k++) lj[3 * c + k] = nn > lmax ? lmax * nw[k] / nn : nw[k];
+        }
+    }
+    // the merit inputs of the termination test (:310-334)
+    void merit() {
+        for (size_t c = 0; c < nc; c++) {
+            const double *r = cell(c);
+            pp[c] = p[c] * std::fmax(r[ACCP], kMinDiagonal);
+            vtn[c] = norm(&vtang[3 * c]);
+            dev[c] = std::fmax(r[ACCF], kMinDiagonal) *
+                     (kFriction * p[c] * norm(r + FA) - norm(&lam[3 * c]));
+        }
+    }
+    // mean_fischer_burmeister (utilities/mod.rs:27-41), summed in index order
+    static double mean_fb(const std::vector<double> &a, const std::vector<double> &b) {
+        double sum = 0.0;
+        for (size_t i = 0; i < a.size(); i++)
+            sum += std::fabs(a[i] + b[i] - std::sqrt(a[i] * a[i] + b[i] * b[i]));
+        return sum / (double)a.size();
+    }
+    void refresh() {
+        extra();
+        trial();
+        predict();
+    }
+    // the Jacobi step and its gradient (the negated step); returns |g_p|^2, |g_lambda|^2
+    void step_gradient(double &g2p, double &g2l) {
+        jacobi();
+        for (size_t c = 0; c < nc; c++) {
+            gp[c] = -(pj[c] - p[c]);
+            for (int k = 0; k < 3; k++) gl[3 * c + k] = -(lj[3 * c + k] - lam[3 * c + k]);
+        }
+        g2p = 0.0;
+        g2l = 0.0;
+        for (size_t c = 0; c < nc; c++) g2p += gp[c] * gp[c];
+        for (size_t c = 0; c < nc; c++) g2l += dot(&gl[3 * c], &gl[3 * c]);
+    }
+
+    uint32_t solve() {
+        refresh();
+        double prev_p2, prev_l2;
+        step_gradient(prev_p2, prev_l2);
+        for (size_t c = 0; c < nc; c++) {
+            sp[c] = -gp[c];
+            for (int k = 0; k < 3; k++) sl[3 * c + k] = -gl[3 * c + k];
+        }
+        p = pj;
+        lam = lj;
+        refresh();
+        merit();
+        uint32_t it = 0;
+        while (!(mean_fb(pp, verr) < kFbPressure && mean_fb(vtn, dev) < kFbFriction)) {
+            if (it >= kMaxIterations) break;
+            double g2p, g2l;
+            step_gradient(g2p, g2l);
+            const double beta =
+                (g2p + g2l) / std::fmax(prev_p2 + prev_l2, std::numeric_limits<double>::min());
+            if (beta > 1.0) {  // restart from the Jacobi iterate
+                std::fill(sp.begin(), sp.end(), 0.0);
+                std::fill(sl.begin(), sl.end(), 0.0);
+                p = pj;
+                lam = lj;
+            } else {
+                // friction first: its cone bound takes the pressure before this update
+                for (size_t c = 0; c < nc; c++) {
+                    double ln[3], tp[3];
+                    for (int k = 0; k < 3; k++) ln[k] = lj[3 * c + k] + beta * sl[3 * c + k];
+                    Rigid::mv(&T[9 * c], ln, tp);
+                    const double lmax = kFriction * p[c] * norm(cell(c) + FA);
+                    const bool out = norm(tp) > lmax;
+                    for (int k = 0; k < 3; k++) {
+                        lam[3 * c + k] = out ? lj[3 * c + k] : tp[k];
+                        sl[3 * c + k] = out ? 0.0 : beta * sl[3 * c + k] - gl[3 * c + k];
+                    }
+                }
+                for (size_t c = 0; c < nc; c++) {
+                    const double pn = pj[c] + beta * sp[c];
+                    const bool neg = pn < 0.0;
+                    p[c] = neg ? pj[c] : pn;
+                    sp[c] = neg ? 0.0 : beta * sp[c] - gp[c];
+                }
+            }
+            prev_p2 = g2p;
+            prev_l2 = g2l;
+            refresh();
+            merit();
+            it++;
+        }
+        extra();
+        for (size_t c = 0; c < nc; c++) {
+            rec[c * kContactRec + P] = p[c];
+            for (int k = 0; k < 3; k++) rec[c * kContactRec + LAM + k] = lam[3 * c + k];
+        }
+        return it;
+    }
+};
+}  // namespace contact
+
+// contact handling of one scene before its bodies step
+static void contact_step(Batch &B, Scene &S, double dt) {
+    S.contact_rec.clear();
+    S.contact_iters = 0;
+    if (S.contact_degree <= 0 || S.bodies.size() < 2) return;
+    bool dyn = false;
+    for (auto &b : S.bodies) {
+        if (b->kind != 1) return;
+        dyn |= b->dynamic;
+    }
+    if (!dyn) return;
+    // broad_phase (:70-92), generate_contact_cells (:98-115), subdivide_overlap
+    // (:1813-1844)
+    const double h = S.P.kernel_support_radius;
+    const double target = h * contact::kCellSizeFactor;
+    std::vector<ContactPairDev> pairs;
+    uint64_t total = 0;
+    const int nb = (int)S.bodies.size();
+    for (int i = 0; i < nb; i++)
+        for (int j = i + 1; j < nb; j++) {
+            const Body &A = *S.bodies[i], &Bd = *S.bodies[j];
+            if (!A.has_aabb || !Bd.has_aabb) continue;
+            if (!A.dynamic && !Bd.dynamic) continue;
+            double amn[3], amx[3], bmn[3], bmx[3];
+            contact::world_aabb(A, amn, amx);
+            contact::world_aabb(Bd, bmn, bmx);
+            if (!(amn[0] <= bmx[0] && amx[0] >= bmn[0] && amn[1] <= bmx[1] && amx[1] >= bmn[1] &&
+                  amn[2] <= bmx[2] && amx[2] >= bmn[2]))
+                continue;
+            ContactPairDev P{};
+            P.a = i;
+            P.b = j;
+            double omx[3];
+            for (int k = 0; k < 3; k++) {
+                P.omn[k] = std::fmax(amn[k], bmn[k]);
+                omx[k] = std::fmin(amx[k], bmx[k]);
+            }
+            if (!(P.omn[0] <= omx[0] && P.omn[1] <= omx[1] && P.omn[2] <= omx[2])) continue;
+            uint64_t cells = 1;
+            for (int k = 0; k < 3; k++) {
+                const double ext = omx[k] - P.omn[k];
+                const double c = std::ceil(ext / target);
+                // f64 as usize saturates (NaN and negatives to 0), then max(1)
+                const uint64_t ck = c > 0.0 ? (c < 4294967295.0 ? (uint64_t)c : 4294967295u) : 0;
+                P.nn[k] = (uint32_t)std::max<uint64_t>(ck, 1);
+                P.cs[k] = ext / (double)P.nn[k];
+                cells *= P.nn[k];
+            }
+            if (total + cells > 0x7fffffffu)
+                throw Error(SPH_ERR_UNSUPPORTED, "contact overlap exceeds 2^31 sub-cells");
+            P.first = (uint32_t)total;
+            total += cells;
+            pairs.push_back(P);
+        }
+    if (pairs.empty()) return;
+    std::vector<BodyDev> views;
+    for (auto &b : S.bodies) views.push_back(b->view());
+    S.cpairs.reserve(pairs.size());
+    S.cbodies.reserve(views.size());
+    S.cnact.reserve(1);
+    B.put(S.cpairs.p, pairs.data(), pairs.size() * sizeof(ContactPairDev));
+    B.put(S.cbodies.p, views.data(), views.size() * sizeof(BodyDev));
+    ContactQuadDev Q{};
+    Q.K = S.K;
+    Q.degree = S.contact_degree;
+    for (int k = 0; k < S.contact_degree; k++) {
+        Q.x[k] = S.contact_x[k];
+        Q.w[k] = S.contact_w[k];
+    }
+    // room for active cells: what the buffers hold (at least 4096), else all
+    // of them after a first pass that only counts
+    size_t cap = std::min<size_t>(total, std::max<size_t>(S.cidx.cap, 4096));
+    size_t nc = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        S.cidx.reserve(cap);
+        S.csums.reserve(cap * kContactSums);
+        LAUNCH(k_fill_u32, 1, B.s, (uint32_t *)S.cnact.p, (size_t)1, 0u);
+        constexpr uint64_t kChunk = 1u << 20;  // workgroups per launch
+        for (uint64_t base = 0; base < total; base += kChunk) {
+            const uint64_t nblk = std::min<uint64_t>(kChunk, total - base);
+            hipLaunchKernelGGL(k_contact_cells, dim3((unsigned)nblk), dim3(kContactBlock), 0,
+                               B.s, S.cpairs.p, (int)pairs.size(), S.cbodies.p, Q,
+                               (uint32_t)base, S.cnact.p, (uint32_t)cap, S.cidx.p, S.csums.p);
+            HIPCK(hipGetLastError());
+        }
+        unsigned int *hn = reinterpret_cast<unsigned int *>(B.pinned);
+        HIPCK(hipMemcpyAsync(hn, S.cnact.p, sizeof(unsigned int), hipMemcpyDeviceToHost, B.s));
+        B.sync();
+        nc = *hn;
+        if (nc > total) throw Error(SPH_ERR_HIP, "contact cell count out of range");
+        if (nc <= cap) break;
+        cap = nc;
+    }
+    if (nc == 0) return;
+    std::vector<uint32_t> idx(nc);
+    std::vector<double> sums(nc * kContactSums);
+    S.cidx.download(idx.data(), nc, B.s);
+    S.csums.download(sums.data(), nc * kContactSums, B.s);
+    B.sync();
+    // the reference's cell order: candidate index (pair, then z, y, x)
+    std::vector<uint32_t> order(nc);
+    for (size_t k = 0; k < nc; k++) order[k] = (uint32_t)k;
+    std::sort(order.begin(), order.end(), [&](uint32_t u, uint32_t v) { return idx[u] < idx[v]; });
+    S.contact_rec.assign(nc * kContactRec, 0.0);
+    for (size_t k = 0; k < nc; k++) {
+        const uint32_t ci = idx[order[k]];
+        size_t pi = 0;
+        while (pi + 1 < pairs.size() && pairs[pi + 1].first <= ci) pi++;
+        double *r = &S.contact_rec[k * kContactRec];
+        r[contact::A] = (double)pairs[pi].a;
+        r[contact::B] = (double)pairs[pi].b;
+        contact::diagonal(S, &sums[(size_t)order[k] * kContactSums], r, dt);
+    }
+    contact::Solver solver(S, S.contact_rec.data(), nc, dt);
+    S.contact_iters = solver.solve();
+    for (int bi = 0; bi < nb; bi++) {  // add_force_and_torque (rigid_body_motion.rs:156-159)
+        Body &b = *S.bodies[bi];
+        if (!b.dynamic) continue;
+        for (int k = 0; k < 3; k++) {
+            b.rb.F[k] += solver.ef[3 * bi + k];
+            b.rb.T[k] += solver.et[3 * bi + k];
+        }
+    }
+}
+
 // integrator + rigid bodies of every scene, then update (sph/mod.rs:125-144)
 static void step_batch(Batch &B) {
     layout(B);
@@ -1607,8 +2089,9 @@ static void step_batch(Batch &B) {
             c->vy.swap(c->ivy);
             c->vz.swap(c->ivz);
         }
+        contact_step(B, *c, dt);  // B::step_forward_in_time: contact, then the bodies
         for (auto &b : c->bodies)
-            if (b->dynamic) b->rb.step(dt);  // B::step_forward_in_time
+            if (b->dynamic) b->rb.step(dt);
         c->steps++;
         c->time += dt;
     }
@@ -1817,6 +2300,9 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
             b->ns = bp->ns;
             b->spos = bp->spos;
             memcpy(b->applied, bp->applied, sizeof b->applied);
+            b->has_aabb = bp->has_aabb;
+            memcpy(b->aabb_c, bp->aabb_c, sizeof b->aabb_c);
+            memcpy(b->aabb_h, bp->aabb_h, sizeof b->aabb_h);
             if (bp->kind == 1) {
                 b->sdf.copy_from(bp->sdf, cs);
                 b->vm.copy_from(bp->vm, cs);
@@ -1834,6 +2320,11 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
         d.last_err = s.last_err;
         d.avg_density = s.avg_density;
         d.err_hist = s.err_hist;
+        d.contact_degree = s.contact_degree;
+        memcpy(d.contact_x, s.contact_x, sizeof d.contact_x);
+        memcpy(d.contact_w, s.contact_w, sizeof d.contact_w);
+        d.contact_rec = s.contact_rec;
+        d.contact_iters = s.contact_iters;
         if (s.initialized) {  // rebuild neighbor/boundary state so the clone is self-contained
             Batch &B = solo(c);
             layout(B);
@@ -1985,6 +2476,52 @@ int sph_boundary_add_volume_map(sph_ctx *c, const sph_lattice *sdf, const sph_la
         idx = (int)c->scene.bodies.size() - 1;
     });
     return st ? st : idx;
+}
+
+int sph_boundary_set_aabb(sph_ctx *c, int bi, const double center[3],
+                          const double half_extents[3]) {
+    return guarded([&] {
+        if (!c || !center || !half_extents) throw Error(SPH_ERR_ARG, "null argument");
+        if (bi < 0 || (size_t)bi >= c->scene.bodies.size())
+            throw Error(SPH_ERR_ARG, "bad boundary index");
+        Body &b = *c->scene.bodies[bi];
+        b.has_aabb = true;
+        memcpy(b.aabb_c, center, sizeof b.aabb_c);
+        memcpy(b.aabb_h, half_extents, sizeof b.aabb_h);
+    });
+}
+
+int sph_contact_set_quadrature(sph_ctx *c, const double *gl_x, const double *gl_w, int degree) {
+    return guarded([&] {
+        if (!c || (degree > 0 && (!gl_x || !gl_w))) throw Error(SPH_ERR_ARG, "null argument");
+        if (degree < 0 || degree > kContactMaxDegree)
+            throw Error(SPH_ERR_UNSUPPORTED, "contact quadrature degree outside 0..16");
+        Scene &S = c->scene;
+        S.contact_degree = degree;
+        for (int k = 0; k < kContactMaxDegree; k++) {
+            S.contact_x[k] = k < degree ? gl_x[k] : 0.0;
+            S.contact_w[k] = k < degree ? gl_w[k] : 0.0;
+        }
+    });
+}
+
+int sph_contact_stats(sph_ctx *c, uint32_t out[2]) {
+    return guarded([&] {
+        if (!c || !out) throw Error(SPH_ERR_ARG, "null argument");
+        out[0] = (uint32_t)(c->scene.contact_rec.size() / kContactRec);
+        out[1] = c->scene.contact_iters;
+    });
+}
+
+int sph_contact_cells(sph_ctx *c, double *out, size_t cap, size_t *count) {
+    return guarded([&] {
+        if (!c || !count || (!out && cap)) throw Error(SPH_ERR_ARG, "null argument");
+        const std::vector<double> &r = c->scene.contact_rec;
+        const size_t n = r.size() / kContactRec;
+        *count = n;
+        if (n > cap) throw Error(SPH_ERR_CAPACITY, "buffer too small");
+        if (n) memcpy(out, r.data(), r.size() * sizeof(double));
+    });
 }
 
 int sph_fluid_upload(sph_ctx *c, size_t n, const double *x, const double *v, const double *m,

@@ -549,4 +549,130 @@ INSTANTIATE_QUANTITY(kQPressure)
 INSTANTIATE_QUANTITY(kQKineticEnergy)
 #undef INSTANTIATE_QUANTITY
 
+// ------------------------------------------------------------------------
+// Rigid-rigid contact cells: one workgroup per candidate sub-cell
+// ------------------------------------------------------------------------
+// cubic_extension_fn_derivative (volume_map_boundary.rs:1734-1741): the x
+// component of grad W at (sd, 0, 0) over W(0)
+__device__ __forceinline__ double contact_ext_derivative(const KernelConst &K, double sd) {
+    if (sd >= K.h || sd <= 0.0) return 0.0;
+    double gx, gy, gz;
+    kernel_grad(K, sd, 0.0, 0.0, gx, gy, gz);
+    return gx / kernel_fn(K, 0.0, 0.0, 0.0);
+}
+
+// The candidate test of generate_contact_cells (:124-146) at the sub-cell
+// centre, then build_contact_cell (:702-768) and the sums of
+// compute_diagonal (:835-882) over the degree^3 Gauss-Legendre points
+// (quadrature_points, :805-830). Each thread evaluates one point per round
+// and leaves its terms in LDS (+0 for a point that is skipped, which leaves
+// a sum that starts at +0 unchanged); threads 0..16 then add one term
+// column each in point order, so every sum is the reference's sequential
+// sum over the kept points. Active cells take a slot from `nact` and are
+// stored while the slot is below `cap`; the host orders them by candidate
+// index.
+__global__ __launch_bounds__(kContactBlock) void k_contact_cells(
+    const ContactPairDev *pairs, int npairs, const BodyDev *bodies, ContactQuadDev Q,
+    uint32_t base, unsigned int *nact, uint32_t cap, uint32_t *cell_idx, double *sums) {
+    __shared__ double terms[kContactBlock][kContactSums];
+    __shared__ double total[kContactSums];
+    const uint32_t c = base + blockIdx.x;
+    int pi = 0;
+    while (pi + 1 < npairs && pairs[pi + 1].first <= c) pi++;
+    const ContactPairDev &P = pairs[pi];
+    const BodyDev &A = bodies[P.a];
+    const BodyDev &B = bodies[P.b];
+    const uint32_t local = c - P.first;
+    const uint32_t ix = local % P.nn[0], iy = (local / P.nn[0]) % P.nn[1],
+                   iz = local / (P.nn[0] * P.nn[1]);
+    const double fx = P.omn[0] + (double)ix * P.cs[0], fy = P.omn[1] + (double)iy * P.cs[1],
+                 fz = P.omn[2] + (double)iz * P.cs[2];
+    const double tx = fx + P.cs[0], ty = fy + P.cs[1], tz = fz + P.cs[2];
+    const double hx = (tx - fx) * 0.5, hy = (ty - fy) * 0.5, hz = (tz - fz) * 0.5;
+    const double cx = fx + hx, cy = fy + hy, cz = fz + hz;
+    {
+        const double dx = tx - fx, dy = ty - fy, dz = tz - fz;
+        const double bound = 2.0 * (sqrt(dot3(dx, dy, dz, dx, dy, dz)) * 0.5);
+        double pa, pb;
+        // uniform over the workgroup: every thread evaluates the same centre
+        if (body_sd(A, cx, cy, cz, pa) || body_sd(B, cx, cy, cz, pb)) return;
+        if (!(pa - bound < 0. && pb - bound < 0.)) return;
+    }
+    const KernelConst &K = Q.K;
+    const int n = Q.degree, npts = n * n * n;
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int r0 = 0; r0 < npts; r0 += kContactBlock) {
+        double v[kContactSums];
+#pragma unroll
+        for (int k = 0; k < kContactSums; k++) v[k] = 0.0;
+        const int q = r0 + t;
+        if (q < npts) {
+            const int i = q / (n * n), j = (q / n) % n, k = q % n;
+            const double px = cx + Q.x[i] * hx, py = cy + Q.x[j] * hy, pz = cz + Q.x[k] * hz;
+            const double w = Q.w[i] * Q.w[j] * Q.w[k] * hx * hy * hz;
+            double pa, pb, gax, gay, gaz, gbx, gby, gbz;
+            if (!body_sd(A, px, py, pz, pa) && !body_sd(B, px, py, pz, pb)) {
+                const double ua = 2.0 * pa + K.h, ub = 2.0 * pb + K.h;
+                const double sa = vm_cubic_extension(ua, K.h, K.wfac);
+                const double sb = vm_cubic_extension(ub, K.h, K.wfac);
+                if (sa > 0.0 && sb > 0.0 && !body_sdg(A, px, py, pz, gax, gay, gaz) &&
+                    !body_sdg(B, px, py, pz, gbx, gby, gbz)) {
+                    const double spa = 2.0 * contact_ext_derivative(K, ua);
+                    const double spb = 2.0 * contact_ext_derivative(K, ub);
+                    const double wg = w * sa * sb;
+                    v[0] = wg;
+                    v[1] = wg * px;
+                    v[2] = wg * py;
+                    v[3] = wg * pz;
+                    const double wa = w * spa * sb;
+                    v[4] = wa * gax;
+                    v[5] = wa * gay;
+                    v[6] = wa * gaz;
+                    if (A.dynamic) {
+                        double ox, oy, oz;
+                        cross3(px - A.com[0], py - A.com[1], pz - A.com[2], gax, gay, gaz, ox,
+                               oy, oz);
+                        v[7] = wa * ox;
+                        v[8] = wa * oy;
+                        v[9] = wa * oz;
+                    }
+                    const double wb = w * spb * sa;
+                    v[10] = wb * gbx;
+                    v[11] = wb * gby;
+                    v[12] = wb * gbz;
+                    if (B.dynamic) {
+                        double ox, oy, oz;
+                        cross3(px - B.com[0], py - B.com[1], pz - B.com[2], gbx, gby, gbz, ox,
+                               oy, oz);
+                        v[13] = wb * ox;
+                        v[14] = wb * oy;
+                        v[15] = wb * oz;
+                    }
+                    v[16] = 1.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kContactSums; k++) terms[t][k] = v[k];
+        __syncthreads();
+        if (t < kContactSums) {
+            const int lim = min(kContactBlock, npts - r0);
+            for (int u = 0; u < lim; u++) acc += terms[u][t];
+        }
+        __syncthreads();
+    }
+    if (t < kContactSums) total[t] = acc;
+    __syncthreads();
+    if (t == 0 && total[0] > 0.0 && total[16] > 0.0) {  // v_c > 0 with points kept
+        const unsigned int slot = atomicAdd(nact, 1u);
+        if (slot < cap) {  // the host reruns with room for every active cell
+            cell_idx[slot] = c;
+            for (int k = 0; k < kContactSums; k++)
+                sums[(size_t)slot * kContactSums + k] = total[k];
+        }
+    }
+}
+
 }  // namespace sph
+

@@ -92,4 +92,28 @@ __global__ void k_vm_cells(VmGrid g, const uint8_t *status, uint8_t *present);
 __global__ void k_vm_finish(size_t n, const uint8_t *present, double *values);
 __global__ void k_widen_u32(size_t n, const uint32_t *in, uint64_t *out);
 
+// Rigid-rigid contact cells (VolumeMapBoundary::generate_contact_cells,
+// volume_map_boundary.rs:98-161, 702-768, 835-882): candidate sub-cells of
+// the overlapping world AABBs of a body pair, numbered pair by pair in
+// (z, y, x) order from `first`
+constexpr int kContactMaxDegree = 16;
+constexpr int kContactBlock = 256;
+// per active cell: volume, sum of weighted positions[3], f_a[3], tau_a[3],
+// f_b[3], tau_b[3], quadrature points kept
+constexpr int kContactSums = 17;
+struct ContactPairDev {
+    int a, b;
+    uint32_t first;
+    uint32_t nn[3];
+    double omn[3], cs[3];
+};
+struct ContactQuadDev {
+    KernelConst K;
+    int degree;
+    double x[kContactMaxDegree], w[kContactMaxDegree];  // Gauss-Legendre rule on [-1, 1]
+};
+__global__ void k_contact_cells(const ContactPairDev *pairs, int npairs, const BodyDev *bodies,
+                                ContactQuadDev Q, uint32_t base, unsigned int *nact,
+                                uint32_t cap, uint32_t *cell_idx, double *sums);
+
 }  // namespace sph

@@ -84,6 +84,10 @@ SIGNATURES = [
                                            C.POINTER(SphRigidInit)]),
     ("sph_boundary_add_volume_map", C.c_int, [C.c_void_p, C.POINTER(SphLattice),
                                               C.POINTER(SphLattice), C.POINTER(SphRigidInit)]),
+    ("sph_boundary_set_aabb", C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    ("sph_contact_set_quadrature", C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
+    ("sph_contact_stats", C.c_int, [C.c_void_p, _u32p]),
+    ("sph_contact_cells", C.c_int, [C.c_void_p, _dp, C.c_size_t, _szp]),
     ("sph_fluid_upload", C.c_int, [C.c_void_p, C.c_size_t, _dp, _dp, _dp, _u32p]),
     ("sph_system_init", C.c_int, [C.c_void_p]),
     ("sph_step", C.c_int, [C.c_void_p, C.c_int, C.POINTER(SphMeasurement)]),

@@ -115,6 +115,27 @@ class VolumeMap:
     vm: Lattice
     rigid: Optional[Rigid] = None
     kind: str = "box"                    # description only
+    # body-frame bounding box (center, half extents) of the boundary's mesh
+    # (LocalAABB, volume_map_boundary.rs:1137-1150); None keeps the boundary
+    # out of rigid-rigid contact
+    aabb: Optional[tuple] = None
+
+
+def aabb_of(lo, hi) -> tuple:
+    """(center, half extents) of the box [lo, hi] (parry's Aabb::center and
+    half_extents)."""
+    lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    return ((lo + hi) * 0.5, (hi - lo) * 0.5)
+
+
+# Gauss-Legendre degree of the contact-cell quadrature
+# (VolumeMapBoundary::QUADRATURE_DEGREE, volume_map_boundary.rs:57)
+CONTACT_DEGREE = 8
+
+
+def contact_rule(degree: int = CONTACT_DEGREE):
+    """Nodes (ascending on [-1, 1]) and weights of the contact quadrature."""
+    return np.polynomial.legendre.leggauss(degree)
 
 
 @dataclasses.dataclass
@@ -126,6 +147,9 @@ class Scene:
     m: np.ndarray
     fluid_id: np.ndarray
     boundaries: List[object]
+    # rigid-rigid contact quadrature degree of a volume-map system (the
+    # reference always runs it; 0 turns it off)
+    contact_degree: int = CONTACT_DEGREE
 
     @property
     def n(self) -> int:
@@ -358,7 +382,8 @@ def block_drop_iisph(side=50, seed=0, dx=0.01, box=1.0) -> Scene:
                target_density_error=0.1, relaxation_factor=0.5)
     lo_b, hi_b = np.zeros(3), np.full(3, box)
     sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
-    return Scene("iisph_block_drop", p, x, v, m, fid, [VolumeMap(sdf, vm, None, "box")])
+    return Scene("iisph_block_drop", p, x, v, m, fid,
+                 [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))])
 
 
 def two_fluid_ball(side=100, seed=0, dx=0.01, radius=1.0, solver="IISPHwOST") -> Scene:
@@ -380,7 +405,8 @@ def two_fluid_ball(side=100, seed=0, dx=0.01, radius=1.0, solver="IISPHwOST") ->
     c = np.zeros(3)
     sdf, vm = volume_map_fields(lambda q: sd_sphere_container(q, c, radius), c - radius,
                                 c + radius, h, dx)
-    return Scene(f"two_fluid_ball_{solver}", p, x, v, m, fid, [VolumeMap(sdf, vm, None, "sphere")])
+    return Scene(f"two_fluid_ball_{solver}", p, x, v, m, fid,
+                 [VolumeMap(sdf, vm, None, "sphere", aabb_of(c - radius, c + radius))])
 
 
 def rigid_coupling(counts=(158, 78, 40), seed=0, dx=0.0125) -> Scene:
@@ -396,7 +422,7 @@ def rigid_coupling(counts=(158, 78, 40), seed=0, dx=0.0125) -> Scene:
                max_time_increment=1e-3, cfl_number=0.4, solver="IISPH", integrator="EulerCromer")
     lo_b, hi_b = np.zeros(3), np.array([2.0, 1.0, 1.0])
     sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
-    bounds = [VolumeMap(sdf, vm, None, "box")]
+    bounds = [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))]
     surface = (counts[2] + 1) * dx
     bodies = [("cube", 0.2, 500.0), ("cube", 0.2, 800.0), ("sphere", 0.15, 1200.0)]
     for k, (kind, size, rho) in enumerate(bodies):
@@ -416,7 +442,7 @@ def rigid_coupling(counts=(158, 78, 40), seed=0, dx=0.0125) -> Scene:
             fn = (lambda r: (lambda q: sd_sphere_solid(q, np.zeros(3), r)))(size)
         s_l, v_l = volume_map_fields(fn, lo_l, hi_l, h, dx)
         rb = Rigid(mass=mass, inertia_body=inertia, com=np.array([cx, cy, cz]))
-        bounds.append(VolumeMap(s_l, v_l, rb, kind))
+        bounds.append(VolumeMap(s_l, v_l, rb, kind, aabb_of(lo_l, hi_l)))
     return Scene("rigid_coupling", p, x, v, m, fid, bounds)
 
 
@@ -434,7 +460,8 @@ def small_iisph_box(side=12, seed=1, dx=0.02, box=0.6, solver="IISPH", jitter=1e
                max_time_increment=1e-3, cfl_number=0.4, solver=solver, integrator=integ)
     lo_b, hi_b = np.zeros(3), np.full(3, box)
     sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
-    return Scene(f"small_box_{solver}", p, x, v, m, fid, [VolumeMap(sdf, vm, None, "box")])
+    return Scene(f"small_box_{solver}", p, x, v, m, fid,
+                 [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))])
 
 
 def small_rigid_scene(side=(14, 14, 8), seed=2, dx=0.02, solver="IISPH") -> Scene:
@@ -457,7 +484,86 @@ def small_rigid_scene(side=(14, 14, 8), seed=2, dx=0.02, solver="IISPH") -> Scen
                ang_vel=np.array([0.3, -0.2, 0.5]))
     s_l, v_l = volume_map_fields(lambda q: sd_box_solid(q, lo_l, hi_l), lo_l, hi_l, h, dx)
     return Scene("small_rigid", p, x, v, m, fid,
-                 [VolumeMap(sdf, vm, None, "box"), VolumeMap(s_l, v_l, rb, "cube")])
+                 [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b)),
+                  VolumeMap(s_l, v_l, rb, "cube", aabb_of(lo_l, hi_l))])
+
+
+def _rot_x(angle: float) -> np.ndarray:
+    return np.array([math.cos(0.5 * angle), math.sin(0.5 * angle), 0.0, 0.0])
+
+
+def contact_stack(seed=3, dx=0.02, depth=0.002, tilt=0.05, sphere=True,
+                  solver="IISPH") -> Scene:
+    """Rigid-rigid contact case: inside a 1 x 1 x 0.6 m box container, a static
+    solid slab [0.2, 0.8]^2 x [0, 0.1], a dynamic cube (edge 0.2 m, density
+    600, tilted by ``tilt`` about x) whose lowest edge sinks about ``depth``
+    below the slab's top face and, with ``sphere``, a dynamic sphere (radius
+    0.08 m, density 900, sliding at 0.3 m/s along x) that sinks ``depth`` into
+    the cube's top face; a 4 x 4 x 3 fluid block sits in a far corner. IISPH;
+    CFL dt."""
+    rng = np.random.default_rng(seed)
+    pos = lattice_block([0.86, 0.86, dx], [4, 4, 3], dx, rng, 1e-4)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    integ = "TakePredicted" if solver == "IISPHwOST" else "EulerCromer"
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver=solver, integrator=integ)
+    lo_b, hi_b = np.zeros(3), np.array([1.0, 1.0, 0.6])
+    sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
+    bounds = [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))]
+    lo_s, hi_s = np.array([0.2, 0.2, 0.0]), np.array([0.8, 0.8, 0.1])
+    s_s, v_s = volume_map_fields(lambda q: sd_box_solid(q, lo_s, hi_s), lo_s, hi_s, h, dx)
+    bounds.append(VolumeMap(s_s, v_s, None, "slab", aabb_of(lo_s, hi_s)))
+    half = 0.1
+    lo_c, hi_c = np.full(3, -half), np.full(3, half)
+    drop = half * (math.cos(tilt) + abs(math.sin(tilt)))  # com to lowest edge
+    com_c = np.array([0.5 + rng.uniform(-0.01, 0.01), 0.5 + rng.uniform(-0.01, 0.01),
+                      0.1 + drop - depth])
+    mass_c = 600.0 * (2 * half) ** 3
+    s_c, v_c = volume_map_fields(lambda q: sd_box_solid(q, lo_c, hi_c), lo_c, hi_c, h, dx)
+    bounds.append(VolumeMap(s_c, v_c, Rigid(mass=mass_c, inertia_body=np.eye(3) * (
+        mass_c * (2 * half) ** 2 / 6.0), com=com_c, q=_rot_x(tilt)), "cube",
+        aabb_of(lo_c, hi_c)))
+    if sphere:
+        r = 0.08
+        top = com_c[2] + half / math.cos(tilt)  # the top face above the com
+        com_s = np.array([com_c[0] + 0.02, com_c[1] - 0.01, top + r - depth])
+        mass_s = 900.0 * 4.0 / 3.0 * math.pi * r ** 3
+        s_r, v_r = volume_map_fields(lambda q: sd_sphere_solid(q, np.zeros(3), r),
+                                     np.full(3, -r), np.full(3, r), h, dx)
+        bounds.append(VolumeMap(s_r, v_r, Rigid(mass=mass_s, inertia_body=np.eye(3) * (
+            0.4 * mass_s * r * r), com=com_s, lin_vel=np.array([0.3, 0.0, 0.0])), "sphere",
+            aabb_of(np.full(3, -r), np.full(3, r))))
+    return Scene(f"contact_stack_{solver}", p, x, v, m, fid, bounds)
+
+
+def halfspace_contact(depth, dx=0.02, half=0.1, offset=(0.013, -0.007), dt=1e-3,
+                      floor_half=0.3) -> Scene:
+    """Planar rigid-rigid contact: a static floor (solid below z = 0.5, AABB
+    [0.5 -+ floor_half]^2 x [0.4, 0.5]) and a dynamic block whose signed
+    distance is that of its bottom face alone (solid above it; AABB a cube of
+    half edge ``half``), the face ``depth`` below the floor's top. Both signed
+    distances are linear, so the lattices reproduce them exactly. Four fluid
+    particles sit far outside both lattices; fixed dt."""
+    h = 2 * dx
+    zt = 0.5
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=False,
+               time_increment=dt, solver="IISPH", integrator="EulerCromer")
+    x = np.array([[3.0 + 0.02 * i, 3.0, 3.0] for i in range(4)])
+    v = np.zeros_like(x)
+    m = np.full(len(x), 1000.0 * dx ** 3)
+    fid = np.zeros(len(x), dtype=np.uint32)
+    lo_f = np.array([0.5 - floor_half, 0.5 - floor_half, zt - 0.1])
+    hi_f = np.array([0.5 + floor_half, 0.5 + floor_half, zt])
+    s_f, v_f = volume_map_fields(lambda q: q[..., 2] - zt, lo_f, hi_f, h, dx)
+    lo_b, hi_b = np.full(3, -half), np.full(3, half)
+    s_b, v_b = volume_map_fields(lambda q: -half - q[..., 2], lo_b, hi_b, h, dx)
+    mass = 500.0 * (2 * half) ** 3
+    com = np.array([0.5 + offset[0], 0.5 + offset[1], zt - depth + half])
+    rb = Rigid(mass=mass, inertia_body=np.eye(3) * (mass * (2 * half) ** 2 / 6.0), com=com)
+    return Scene("halfspace_contact", p, x, v, m, fid,
+                 [VolumeMap(s_f, v_f, None, "floor", aabb_of(lo_f, hi_f)),
+                  VolumeMap(s_b, v_b, rb, "block", aabb_of(lo_b, hi_b))])
 
 
 # --------------------------------------------------------------------------
@@ -510,4 +616,7 @@ def serving_scene(n: int, rho0: float, nu: float, shape: str, seed: int, dx: flo
                boundary_viscosity=nu, solver="IISPH", integrator="EulerCromer",
                target_density_error=0.1, relaxation_factor=0.5)
     sdf, vm = container_fields(shape, size, h, dx)
-    return Scene(f"serve_{shape}_{len(x)}", p, x, v, m, fid, [VolumeMap(sdf, vm, None, shape)])
+    box = aabb_of(np.zeros(3), np.full(3, size)) if shape == "box" else \
+        aabb_of(np.full(3, -size), np.full(3, size))
+    return Scene(f"serve_{shape}_{len(x)}", p, x, v, m, fid,
+                 [VolumeMap(sdf, vm, None, shape, box)])

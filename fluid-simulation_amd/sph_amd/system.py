@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .scenes import INTEGRATORS, SOLVERS, Params, Scene
+from .scenes import INTEGRATORS, SOLVERS, Params, Scene, contact_rule
 
 
 def _params_struct(P: Params) -> N.SphParams:
@@ -161,6 +161,16 @@ class System:
                 sdf, vm = _lattice_struct(b.sdf), _lattice_struct(b.vm)
                 N.check(L.sph_boundary_add_volume_map(self._ctx.h, C.byref(sdf), C.byref(vm),
                                                       rp))
+        for i, b in enumerate(scene.boundaries):
+            if getattr(b, "aabb", None) is not None:
+                c, hw = N.f64(b.aabb[0]), N.f64(b.aabb[1])
+                N.check(L.sph_boundary_set_aabb(self._ctx.h, i, N.dptr(c), N.dptr(hw)))
+        # rigid-rigid contact runs in every volume-map system of the reference
+        degree = int(getattr(scene, "contact_degree", 0) or 0)
+        if degree and all(not hasattr(b, "positions") for b in scene.boundaries):
+            gx, gw = (N.f64(a) for a in contact_rule(degree))
+            self._keep += [gx, gw]
+            N.check(L.sph_contact_set_quadrature(self._ctx.h, N.dptr(gx), N.dptr(gw), degree))
         x, v, m = N.f64(scene.x), N.f64(scene.v), N.f64(scene.m)
         fid = np.ascontiguousarray(scene.fluid_id, dtype=np.uint32)
         N.check(L.sph_fluid_upload(self._ctx.h, len(x), N.dptr(x), N.dptr(v), N.dptr(m),
@@ -226,6 +236,22 @@ class System:
     def body_forces(self, b: int):
         out = np.zeros(6)
         N.check(N.lib().sph_body_forces(self._ctx.h, int(b), N.dptr(out)))
+        return out
+
+    def contact_stats(self):
+        """(active contact cells, NNCG iterations) of the last step."""
+        out = np.zeros(2, dtype=np.uint32)
+        N.check(N.lib().sph_contact_stats(self._ctx.h, out.ctypes.data_as(N._u32p)))
+        return int(out[0]), int(out[1])
+
+    def contact_cells(self) -> np.ndarray:
+        """The last step's active contact cells in solve order, [n][24]: body a,
+        body b, volume, center, f_a, tau_a, f_b, tau_b, a_cc pressure, a_cc
+        friction, p, lambda."""
+        n = self.contact_stats()[0]
+        out = np.zeros((n, 24))
+        cnt = C.c_size_t()
+        N.check(N.lib().sph_contact_cells(self._ctx.h, N.dptr(out), n, C.byref(cnt)))
         return out
 
     def profile(self, enable: bool = True):

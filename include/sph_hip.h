@@ -140,6 +140,33 @@ int sph_boundary_add_samples(sph_ctx *ctx, const double *pos, size_t m,
 int sph_boundary_add_volume_map(sph_ctx *ctx, const sph_lattice *sdf, const sph_lattice *vm,
                                 const sph_rigid_init *rigid);
 
+/* ---- rigid-rigid contact (VolumeMapBoundary::step_forward_in_time,
+ * volume_map_boundary.rs:1440-1453) -----------------------------------------
+ * Before the bodies of a volume-map system step, the overlapping world AABBs
+ * of every body pair with a dynamic member are cut into sub-cells of h/2
+ * (generate_contact_cells, :98-161); Gauss-Legendre quadrature of the
+ * cubic-extended signed distances gives each cell's contact volume and
+ * sensitivities (:702-946), and an NNCG-accelerated projected Jacobi with
+ * Coulomb friction (solve_for_p_c, :186-637) adds the contact forces and
+ * torques to the dynamic bodies.
+ * The body-frame bounding box of boundary b's mesh (LocalAABB, :1137-1150 and
+ * :1209-1222: parry's local_aabb center and half extents); a boundary without
+ * one takes no part in contact. */
+int sph_boundary_set_aabb(sph_ctx *ctx, int b, const double center[3],
+                          const double half_extents[3]);
+/* The contact-cell quadrature rule: degree Gauss-Legendre nodes (ascending on
+ * [-1, 1]) and weights, degree 8 in the reference (:57); 1..16, or 0 to turn
+ * contact off (the default). */
+int sph_contact_set_quadrature(sph_ctx *ctx, const double *gl_x, const double *gl_w,
+                               int degree);
+/* active contact cells and NNCG iterations of the last step */
+int sph_contact_stats(sph_ctx *ctx, uint32_t out[2]);
+/* The last step's active cells in solve order, 24 binary64 each: body a,
+ * body b, volume, center[3], f_a[3], tau_a[3], f_b[3], tau_b[3], a_cc
+ * pressure, a_cc friction, p, lambda[3]; *count receives the cell count
+ * (SPH_ERR_CAPACITY if it exceeds cap). */
+int sph_contact_cells(sph_ctx *ctx, double *out, size_t cap, size_t *count);
+
 /* ---- device-resident system ----------------------------------------------- */
 /* Fluid::add_samples equivalent: the fluid state (sph/fluid.rs:26-104). */
 int sph_fluid_upload(sph_ctx *ctx, size_t n, const double *x, const double *v, const double *m,

@@ -570,6 +570,11 @@ typedef struct ref_boundary {
     const uint64_t *ids[2];
     const double *pos, *vel, *vol;
     double applied[6]; /* F, tau applied during the last update */
+    /* body-frame bounding box of the boundary's mesh (LocalAABB,
+     * volume_map_boundary.rs:1772-1776); has_aabb = 0 keeps the boundary
+     * out of rigid-rigid contact */
+    int has_aabb;
+    double aabb_c[3], aabb_h[3];
 } ref_boundary;
 
 struct ref_system {
@@ -590,6 +595,12 @@ struct ref_system {
     ref_boundary *b;
     double avg_density;
     uint32_t last_iter, eqs1_iter;
+    /* rigid-rigid contact: Gauss-Legendre rule of the contact-cell quadrature
+     * (degree 0: off) and the last step's active cells and NNCG iterations */
+    int contact_degree;
+    double contact_x[32], contact_w[32];
+    uint32_t contact_cells, contact_iters;
+    double *contact_rec; /* REF_CONTACT_REC doubles per active cell of the last step */
     double last_err;
     double *err_hist;
     size_t err_hist_len, err_hist_cap;
@@ -635,6 +646,7 @@ void ref_system_destroy(ref_system *s) {
     free(s->fid); free(s->origin);
     free(s->ix); free(s->iv); free(s->sx); free(s->sv);
     free(s->s_f); free(s->a_ff); free(s->pacc); free(s->err_hist);
+    free(s->contact_rec);
     ref_nlist_free(&s->nl);
     for (int i = 0; i < s->nb; i++) {
         ref_boundary *b = &s->b[i];
@@ -1355,6 +1367,652 @@ int ref_system_init(ref_system *s) {
     return 0;
 }
 
+/* ===================================================================== */
+/* rigid-rigid contact: VolumeMapBoundary::step_forward_in_time            */
+/* (volume_map_boundary.rs:56-1102, 1440-1453)                             */
+/* ===================================================================== */
+#define CONTACT_CELL_SIZE_FACTOR 0.5
+#define CONTACT_FRICTION 0.5
+#define CONTACT_RELAXATION 0.5
+#define CONTACT_MIN_DIAGONAL 1e-35
+#define CONTACT_FB_TOL_PRESSURE 1e-6
+#define CONTACT_FB_TOL_FRICTION 1e-3
+#define CONTACT_MAX_ITERATIONS 100000u /* the reference has none (it warns at 100) */
+
+/* BoundaryType::world_aabb (:2122-2147): |R| applied to the half extents */
+static void contact_world_aabb(const ref_boundary *b, double mn[3], double mx[3]) {
+    if (!b->dynamic) {
+        for (int k = 0; k < 3; k++) {
+            mn[k] = b->aabb_c[k] - b->aabb_h[k];
+            mx[k] = b->aabb_c[k] + b->aabb_h[k];
+        }
+        return;
+    }
+    double R[9], hw[3], c[3];
+    ref_quat_to_matrix(b->rb.q, R);
+    for (int k = 0; k < 9; k++) R[k] = fabs(R[k]);
+    matvec3(R, b->aabb_h, hw);
+    ref_rigid_local_to_world(&b->rb, b->aabb_c, c); /* pose.transform_point */
+    for (int k = 0; k < 3; k++) {
+        mn[k] = c[k] - hw[k];
+        mx[k] = c[k] + hw[k];
+    }
+}
+
+static double ref_cubic_extension(double sd, double h);
+
+/* cubic_extension_fn_derivative (:1734-1741) */
+static double ref_cubic_extension_derivative(double sd, double h) {
+    if (sd >= h || sd <= 0.0) return 0.0;
+    double r[3] = {sd, 0., 0.}, z[3] = {0., 0., 0.}, g[3];
+    ref_kernel_gradient(r, h, g);
+    return g[0] / ref_kernel_function(z, h);
+}
+
+typedef struct contact_qp { /* QuadPointData (:832-842) */
+    double pos[3], weight, ga[3], gb[3], sa, sb, spa, spb;
+} contact_qp;
+
+typedef struct contact_cell { /* ContactCell (:686-700) */
+    int a, b;
+    contact_qp *qp;
+    size_t nqp;
+    double volume, center[3], fa[3], taua[3], fb[3], taub[3], T[9], acc_p, acc_f;
+} contact_cell;
+
+/* build_contact_cell (:702-768): the degree^3 Gauss-Legendre points of the
+ * sub-cell (:854-880) where both SDFs and gradients evaluate and both cubic
+ * extensions of 2 phi + h are positive */
+static int contact_build_cell(const ref_system *s, int ia, int ib, const double from[3],
+                              const double to[3], contact_cell *cell) {
+    const ref_boundary *A = &s->b[ia], *Bd = &s->b[ib];
+    const double h = s->P.kernel_support_radius, scaling = 2.0;
+    const int n = s->contact_degree;
+    double half[3], center[3];
+    for (int k = 0; k < 3; k++) half[k] = (to[k] - from[k]) * 0.5;
+    for (int k = 0; k < 3; k++) center[k] = from[k] + half[k];
+    double vc = 0.0;
+    cell->qp = (contact_qp *)malloc((size_t)n * n * n * sizeof(contact_qp));
+    cell->nqp = 0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++) {
+                double pos[3] = {center[0] + s->contact_x[i] * half[0],
+                                 center[1] + s->contact_x[j] * half[1],
+                                 center[2] + s->contact_x[k] * half[2]};
+                double w = s->contact_w[i] * s->contact_w[j] * s->contact_w[k] * half[0] *
+                           half[1] * half[2];
+                double pa, pb, ga[3], gb[3];
+                if (vm_sd(A, pos, &pa) || vm_sd(Bd, pos, &pb)) continue;
+                double sa = ref_cubic_extension(scaling * pa + h, h);
+                double sb = ref_cubic_extension(scaling * pb + h, h);
+                if (sa <= 0.0 || sb <= 0.0) continue;
+                if (vm_sdg(A, pos, ga) || vm_sdg(Bd, pos, gb)) continue;
+                vc += w * sa * sb;
+                contact_qp *q = &cell->qp[cell->nqp++];
+                memcpy(q->pos, pos, sizeof pos);
+                q->weight = w;
+                memcpy(q->ga, ga, sizeof ga);
+                memcpy(q->gb, gb, sizeof gb);
+                q->sa = sa;
+                q->sb = sb;
+                q->spa = scaling * ref_cubic_extension_derivative(scaling * pa + h, h);
+                q->spb = scaling * ref_cubic_extension_derivative(scaling * pb + h, h);
+            }
+    if (vc <= 0.0 || cell->nqp == 0) {
+        free(cell->qp);
+        cell->qp = NULL;
+        return 0;
+    }
+    cell->a = ia;
+    cell->b = ib;
+    cell->volume = vc;
+    return 1;
+}
+
+/* generate_contact_cells (:98-161) with broad_phase (:70-92) and
+ * subdivide_overlap (:1813-1844): cells in the order of the pairs (i < j, one
+ * of them dynamic, overlapping world AABBs) and of the sub-cells (z, y, x) */
+typedef struct contact_task { /* one candidate sub-cell of a body pair */
+    int a, b;
+    double from[3], to[3];
+} contact_task;
+
+static contact_cell *contact_generate(const ref_system *s, size_t *ncell) {
+    const double h = s->P.kernel_support_radius;
+    size_t tcap = 64, nt = 0;
+    contact_task *tasks = (contact_task *)malloc(tcap * sizeof(contact_task));
+    for (int i = 0; i < s->nb; i++)
+        for (int j = i + 1; j < s->nb; j++) {
+            const ref_boundary *A = &s->b[i], *Bd = &s->b[j];
+            if (!A->has_aabb || !Bd->has_aabb) continue;
+            if (!A->dynamic && !Bd->dynamic) continue;
+            double amn[3], amx[3], bmn[3], bmx[3];
+            contact_world_aabb(A, amn, amx);
+            contact_world_aabb(Bd, bmn, bmx);
+            if (!(amn[0] <= bmx[0] && amx[0] >= bmn[0] && amn[1] <= bmx[1] &&
+                  amx[1] >= bmn[1] && amn[2] <= bmx[2] && amx[2] >= bmn[2]))
+                continue;
+            double omn[3], omx[3];
+            for (int k = 0; k < 3; k++) {
+                omn[k] = fmax(amn[k], bmn[k]);
+                omx[k] = fmin(amx[k], bmx[k]);
+            }
+            if (!(omn[0] <= omx[0] && omn[1] <= omx[1] && omn[2] <= omx[2])) continue;
+            const double target = h * CONTACT_CELL_SIZE_FACTOR;
+            double ext[3], cs[3];
+            size_t nn[3];
+            for (int k = 0; k < 3; k++) {
+                ext[k] = omx[k] - omn[k];
+                double c = ceil(ext[k] / target);
+                size_t ck = c > 0.0 ? (size_t)c : 0; /* f64 as usize saturates at 0 */
+                nn[k] = ck > 1 ? ck : 1;
+                cs[k] = ext[k] / (double)nn[k];
+            }
+            for (size_t iz = 0; iz < nn[2]; iz++)
+                for (size_t iy = 0; iy < nn[1]; iy++)
+                    for (size_t ix = 0; ix < nn[0]; ix++) {
+                        if (nt == tcap) {
+                            tcap *= 2;
+                            tasks = (contact_task *)realloc(tasks, tcap * sizeof(contact_task));
+                        }
+                        contact_task *t = &tasks[nt++];
+                        t->a = i;
+                        t->b = j;
+                        t->from[0] = omn[0] + (double)ix * cs[0];
+                        t->from[1] = omn[1] + (double)iy * cs[1];
+                        t->from[2] = omn[2] + (double)iz * cs[2];
+                        for (int k = 0; k < 3; k++) t->to[k] = t->from[k] + cs[k];
+                    }
+        }
+    /* the candidate test (:124-146) and build_contact_cell per task; kept
+     * cells in task order */
+    contact_cell *tmp = (contact_cell *)calloc(nt ? nt : 1, sizeof(contact_cell));
+    unsigned char *keep = (unsigned char *)calloc(nt ? nt : 1, 1);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (size_t t = 0; t < nt; t++) {
+        const contact_task *tk = &tasks[t];
+        const ref_boundary *A = &s->b[tk->a], *Bd = &s->b[tk->b];
+        double d[3], center[3], diff[3];
+        for (int k = 0; k < 3; k++) d[k] = (tk->to[k] - tk->from[k]) * 0.5;
+        for (int k = 0; k < 3; k++) center[k] = tk->from[k] + d[k];
+        sub3(tk->to, tk->from, diff);
+        const double bound = 2.0 * (norm3(diff) * 0.5);
+        double pa, pb;
+        if (vm_sd(A, center, &pa) || vm_sd(Bd, center, &pb)) continue;
+        if (!(pa - bound < 0. && pb - bound < 0.)) continue;
+        keep[t] = (unsigned char)contact_build_cell(s, tk->a, tk->b, tk->from, tk->to, &tmp[t]);
+    }
+    size_t nc = 0;
+    for (size_t t = 0; t < nt; t++)
+        if (keep[t]) tmp[nc++] = tmp[t];
+    free(keep);
+    free(tasks);
+    *ncell = nc;
+    return tmp;
+}
+
+/* compute_diagonal (:884-946) */
+static void contact_diagonal(const ref_system *s, contact_cell *c, double dt) {
+    const ref_boundary *A = &s->b[c->a], *Bd = &s->b[c->b];
+    double x[3] = {0., 0., 0.}, fa[3] = {0., 0., 0.}, ta[3] = {0., 0., 0.},
+           fb[3] = {0., 0., 0.}, tb[3] = {0., 0., 0.};
+    for (size_t q = 0; q < c->nqp; q++) {
+        const contact_qp *p = &c->qp[q];
+        double wg = p->weight * p->sa * p->sb;
+        for (int k = 0; k < 3; k++) x[k] += wg * p->pos[k];
+        double wa = p->weight * p->spa * p->sb, r[3], cr[3];
+        for (int k = 0; k < 3; k++) fa[k] += wa * p->ga[k];
+        if (A->dynamic) {
+            sub3(p->pos, A->rb.com, r);
+            cross3(r, p->ga, cr);
+            for (int k = 0; k < 3; k++) ta[k] += wa * cr[k];
+        }
+        double wb = p->weight * p->spb * p->sa;
+        for (int k = 0; k < 3; k++) fb[k] += wb * p->gb[k];
+        if (Bd->dynamic) {
+            sub3(p->pos, Bd->rb.com, r);
+            cross3(r, p->gb, cr);
+            for (int k = 0; k < 3; k++) tb[k] += wb * cr[k];
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        c->center[k] = x[k] / c->volume;
+        c->fa[k] = fa[k];
+        c->taua[k] = ta[k];
+        c->fb[k] = fb[k];
+        c->taub[k] = tb[k];
+    }
+    const double fn = fmax(norm3(fa), 1e-15);
+    double nrm[3] = {fa[0] / fn, fa[1] / fn, fa[2] / fn};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) c->T[3 * i + j] = (i == j ? 1.0 : 0.0) - nrm[i] * nrm[j];
+    double term_a = 0.0, term_b = 0.0, m[3];
+    if (A->dynamic) {
+        matvec3(A->rb.inertia_inv_world, ta, m);
+        term_a = dot3(fa, fa) / A->rb.mass + dot3(ta, m);
+    }
+    if (Bd->dynamic) {
+        matvec3(Bd->rb.inertia_inv_world, tb, m);
+        term_b = dot3(fb, fb) / Bd->rb.mass + dot3(tb, m);
+    }
+    c->acc_p = dt * dt * (term_a + term_b);
+    /* I/m - [c - com]x Iinv [c - com]x of each dynamic body (zero for a static one) */
+    double M[2][9];
+    const ref_boundary *bodies[2] = {A, Bd};
+    for (int e = 0; e < 2; e++) {
+        const ref_boundary *bb = bodies[e];
+        if (!bb->dynamic) {
+            for (int k = 0; k < 9; k++) M[e][k] = 0.0;
+            continue;
+        }
+        double r[3], X[9], XI[9], XIX[9];
+        sub3(c->center, bb->rb.com, r);
+        X[0] = 0.0; X[1] = -r[2]; X[2] = r[1];
+        X[3] = r[2]; X[4] = 0.0; X[5] = -r[0];
+        X[6] = -r[1]; X[7] = r[0]; X[8] = 0.0;
+        matmul3(X, bb->rb.inertia_inv_world, XI);
+        matmul3(XI, X, XIX);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                M[e][3 * i + j] = (i == j ? 1.0 : 0.0) / bb->rb.mass - XIX[3 * i + j];
+    }
+    double S[9], dT[9], P[9];
+    for (int k = 0; k < 9; k++) S[k] = M[0][k] + M[1][k];
+    for (int k = 0; k < 9; k++) dT[k] = dt * c->T[k];
+    matmul3(dT, S, P);
+    c->acc_f = 1. / 3. * (((0.0 + P[0]) + P[4]) + P[8]);
+}
+
+/* Cell-level contact solve. A cell is a REF_CONTACT_REC record (body a,
+ * body b, volume, center, f_a, tau_a, f_b, tau_b, a_cc pressure, a_cc friction,
+ * p, lambda); a body is a REF_CONTACT_BODY record (dynamic flag, mass, com,
+ * linear velocity, angular momentum, force, torque, world inverse inertia). */
+#define CR_A 0
+#define CR_B 1
+#define CR_VOL 2
+#define CR_CENTER 3
+#define CR_FA 6
+#define CR_TA 9
+#define CR_FB 12
+#define CR_TB 15
+#define CR_ACCP 18
+#define CR_ACCF 19
+#define CR_P 20
+#define CR_LAM 21
+#define CB_DYN 0
+#define CB_MASS 1
+#define CB_COM 2
+#define CB_V 5
+#define CB_L 8
+#define CB_F 11
+#define CB_T 14
+#define CB_IINV 17
+
+static inline const double *crec(const double *cells, size_t c) {
+    return cells + c * REF_CONTACT_REC;
+}
+static inline const double *cbody(const double *bodies, int b) {
+    return bodies + (size_t)b * REF_CONTACT_BODY;
+}
+static inline int cell_body(const double *r, int side) { return (int)r[side ? CR_B : CR_A]; }
+
+/* the tangential projection I - n n^T, n = f_a / max(|f_a|, 1e-15) (:927-928) */
+static void contact_tangential(const double *r, double T[9]) {
+    const double fn = fmax(norm3(r + CR_FA), 1e-15);
+    double nrm[3] = {r[CR_FA] / fn, r[CR_FA + 1] / fn, r[CR_FA + 2] / fn};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) T[3 * i + j] = (i == j ? 1.0 : 0.0) - nrm[i] * nrm[j];
+}
+
+/* accumulate_extra_forces (:948-986): per body over its cells in cell order */
+static void contact_extra(const double *bodies, int nb, const double *cells, size_t nc,
+                          const double *p, const double *lam, double *ef, double *et) {
+    for (int bi = 0; bi < nb; bi++) {
+        const double *own = cbody(bodies, bi);
+        double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
+        for (size_t c = 0; c < nc; c++) {
+            const double *r = crec(cells, c);
+            for (int side = 0; side < 2; side++) {
+                if (cell_body(r, side) != bi) continue;
+                const double *l = lam + 3 * c;
+                const double *fs = r + (side ? CR_FB : CR_FA), *ts = r + (side ? CR_TB : CR_TA);
+                double d[3], cr[3];
+                for (int k = 0; k < 3; k++)
+                    f[k] += side ? p[c] * fs[k] - l[k] : p[c] * fs[k] + l[k];
+                if (own[CB_DYN] != 0.0) {
+                    sub3(r + CR_CENTER, own + CB_COM, d);
+                    cross3(d, l, cr);
+                    for (int k = 0; k < 3; k++)
+                        t[k] += side ? p[c] * ts[k] - cr[k] : p[c] * ts[k] + cr[k];
+                }
+            }
+        }
+        memcpy(ef + 3 * bi, f, sizeof f);
+        memcpy(et + 3 * bi, t, sizeof t);
+    }
+}
+
+/* compute_trial_velocities (:988-1012), RigidBodyMotion::trial_velocity and
+ * trial_angular_velocity (rigid_body_motion.rs:166-181) */
+static void contact_trial(const double *bodies, int nb, double dt, const double *ef,
+                          const double *et, double *vt, double *wt) {
+    for (int bi = 0; bi < nb; bi++) {
+        const double *b = cbody(bodies, bi);
+        if (b[CB_DYN] == 0.0) {
+            for (int k = 0; k < 3; k++) vt[3 * bi + k] = wt[3 * bi + k] = 0.0;
+            continue;
+        }
+        double l[3];
+        for (int k = 0; k < 3; k++) {
+            vt[3 * bi + k] = b[CB_V + k] + dt / b[CB_MASS] * (b[CB_F + k] + ef[3 * bi + k]);
+            l[k] = b[CB_L + k] + dt * (b[CB_T + k] + et[3 * bi + k]);
+        }
+        matvec3(b + CB_IINV, l, wt + 3 * bi);
+    }
+}
+
+/* predicted_volume_error and predicted_tangential_velocity (:996-1033) */
+static void contact_predict(const double *bodies, const double *cells, size_t nc,
+                            const double *T, double dt, const double *vt, const double *wt,
+                            double *verr, double *vtang) {
+    for (size_t c = 0; c < nc; c++) {
+        const double *r = crec(cells, c);
+        const int a = cell_body(r, 0), b = cell_body(r, 1);
+        const double *va = vt + 3 * a, *wa = wt + 3 * a, *vb = vt + 3 * b, *wb = wt + 3 * b;
+        verr[c] = -r[CR_VOL] + dt * (((dot3(va, r + CR_FA) + dot3(wa, r + CR_TA)) +
+                                      dot3(vb, r + CR_FB)) +
+                                     dot3(wb, r + CR_TB));
+        const double *A = cbody(bodies, a), *B = cbody(bodies, b);
+        if (A[CB_DYN] != 0.0 && B[CB_DYN] != 0.0) {
+            double ra[3], rb[3], ca[3], cb[3], vr[3];
+            sub3(r + CR_CENTER, A + CB_COM, ra);
+            sub3(r + CR_CENTER, B + CB_COM, rb);
+            cross3(wa, ra, ca);
+            cross3(wb, rb, cb);
+            for (int k = 0; k < 3; k++) vr[k] = ((va[k] + ca[k]) - vb[k]) - cb[k];
+            matvec3(T + 9 * c, vr, vtang + 3 * c);
+        } else {
+            vtang[3 * c] = vtang[3 * c + 1] = vtang[3 * c + 2] = 0.0;
+        }
+    }
+}
+
+/* projected_jacobi (:1035-1102) */
+static void contact_jacobi(int nb, const double *cells, size_t nc, const double *p,
+                           const double *verr, const double *lam, const double *vtang,
+                           uint32_t *nact, double *pn, double *ln) {
+    for (int bi = 0; bi < nb; bi++) {
+        uint32_t n = 0;
+        for (size_t c = 0; c < nc; c++) {
+            const double *r = crec(cells, c);
+            int hits = (cell_body(r, 0) == bi) + (cell_body(r, 1) == bi);
+            if (hits && (verr[c] < 0.0 || p[c] > 0.0)) n += (uint32_t)hits;
+        }
+        nact[bi] = n;
+    }
+    for (size_t c = 0; c < nc; c++) {
+        const double *r = crec(cells, c);
+        const uint32_t na = nact[cell_body(r, 0)], nb2 = nact[cell_body(r, 1)];
+        const uint32_t nm = na > nb2 ? na : nb2;
+        const double n_max = (double)(nm > 1u ? nm : 1u);
+        const double alpha = CONTACT_RELAXATION / n_max / fmax(r[CR_ACCP], CONTACT_MIN_DIAGONAL);
+        pn[c] = fmax(p[c] - alpha * verr[c], 0.0);
+        const double lmax = CONTACT_FRICTION * p[c] * norm3(r + CR_FA);
+        const double d = fmax(r[CR_ACCF], CONTACT_MIN_DIAGONAL);
+        double raw[3], star[3], nw[3];
+        for (int k = 0; k < 3; k++) raw[k] = -vtang[3 * c + k] / d;
+        const double rn = norm3(raw);
+        for (int k = 0; k < 3; k++) star[k] = rn > lmax ? lmax * raw[k] / rn : raw[k];
+        for (int k = 0; k < 3; k++) nw[k] = lam[3 * c + k] + CONTACT_RELAXATION / n_max * star[k];
+        const double nn = norm3(nw);
+        for (int k = 0; k < 3; k++) ln[3 * c + k] = nn > lmax ? lmax * nw[k] / nn : nw[k];
+    }
+}
+
+/* mean_fischer_burmeister (utilities/mod.rs:27-41), summed in index order */
+static double contact_mean_fb(const double *a, const double *b, size_t n) {
+    double sum = 0.0;
+    for (size_t i = 0; i < n; i++) sum += fabs(a[i] + b[i] - sqrt(a[i] * a[i] + b[i] * b[i]));
+    return sum / (double)n;
+}
+
+/* the three merit inputs of solve_for_p_c's termination test (:310-334) */
+static void contact_merit(const double *cells, size_t nc, const double *p, const double *lam,
+                          const double *vtang, double *pp, double *vtn, double *dev) {
+    for (size_t c = 0; c < nc; c++) {
+        const double *r = crec(cells, c);
+        pp[c] = p[c] * fmax(r[CR_ACCP], CONTACT_MIN_DIAGONAL);
+        vtn[c] = norm3(vtang + 3 * c);
+        dev[c] = fmax(r[CR_ACCF], CONTACT_MIN_DIAGONAL) *
+                 (CONTACT_FRICTION * p[c] * norm3(r + CR_FA) - norm3(lam + 3 * c));
+    }
+}
+
+/* solve_for_p_c (:186-637) after compute_diagonal: NNCG-accelerated projected
+ * Jacobi over the cells; writes p and lambda into the cell records and the
+ * committed per-body force and torque into extra[6 * nb]; returns the
+ * iteration count */
+uint32_t ref_contact_solve(const double *bodies, int nb, double *cells, size_t nc, double dt,
+                           double *extra) {
+    for (size_t k = 0; k < 6 * (size_t)nb; k++) extra[k] = 0.0;
+    if (nc == 0) return 0;
+    double *T = (double *)malloc(9 * nc * sizeof(double));
+    for (size_t c = 0; c < nc; c++) contact_tangential(crec(cells, c), T + 9 * c);
+    double *p = (double *)calloc(nc, sizeof(double)), *verr = (double *)calloc(nc, sizeof(double));
+    double *lam = (double *)calloc(3 * nc, sizeof(double)), *vtang = (double *)calloc(3 * nc, sizeof(double));
+    double *sp = (double *)calloc(nc, sizeof(double)), *sl = (double *)calloc(3 * nc, sizeof(double));
+    double *pj = (double *)calloc(nc, sizeof(double)), *lj = (double *)calloc(3 * nc, sizeof(double));
+    double *gp = (double *)calloc(nc, sizeof(double)), *gl = (double *)calloc(3 * nc, sizeof(double));
+    double *pp = (double *)calloc(nc, sizeof(double)), *vtn = (double *)calloc(nc, sizeof(double));
+    double *dev = (double *)calloc(nc, sizeof(double));
+    double *ef = (double *)calloc(3 * (size_t)nb, sizeof(double)), *et = (double *)calloc(3 * (size_t)nb, sizeof(double));
+    double *vt = (double *)calloc(3 * (size_t)nb, sizeof(double)), *wt = (double *)calloc(3 * (size_t)nb, sizeof(double));
+    uint32_t *nact = (uint32_t *)calloc((size_t)nb, sizeof(uint32_t));
+
+    contact_extra(bodies, nb, cells, nc, p, lam, ef, et);
+    contact_trial(bodies, nb, dt, ef, et, vt, wt);
+    contact_predict(bodies, cells, nc, T, dt, vt, wt, verr, vtang);
+    contact_jacobi(nb, cells, nc, p, verr, lam, vtang, nact, pj, lj);
+    double prev_p2 = 0.0, prev_l2 = 0.0;
+    for (size_t c = 0; c < nc; c++) {
+        gp[c] = -(pj[c] - p[c]);
+        for (int k = 0; k < 3; k++) gl[3 * c + k] = -(lj[3 * c + k] - lam[3 * c + k]);
+    }
+    for (size_t c = 0; c < nc; c++) {
+        sp[c] = -gp[c];
+        for (int k = 0; k < 3; k++) sl[3 * c + k] = -gl[3 * c + k];
+    }
+    memcpy(p, pj, nc * sizeof(double));
+    memcpy(lam, lj, 3 * nc * sizeof(double));
+    for (size_t c = 0; c < nc; c++) prev_p2 += gp[c] * gp[c];
+    for (size_t c = 0; c < nc; c++) prev_l2 += dot3(gl + 3 * c, gl + 3 * c);
+    contact_extra(bodies, nb, cells, nc, p, lam, ef, et);
+    contact_trial(bodies, nb, dt, ef, et, vt, wt);
+    contact_predict(bodies, cells, nc, T, dt, vt, wt, verr, vtang);
+    contact_merit(cells, nc, p, lam, vtang, pp, vtn, dev);
+
+    uint32_t it = 0;
+    while (!(contact_mean_fb(pp, verr, nc) < CONTACT_FB_TOL_PRESSURE &&
+             contact_mean_fb(vtn, dev, nc) < CONTACT_FB_TOL_FRICTION)) {
+        if (it >= CONTACT_MAX_ITERATIONS) break;
+        contact_jacobi(nb, cells, nc, p, verr, lam, vtang, nact, pj, lj);
+        double gp2 = 0.0, gl2 = 0.0;
+        for (size_t c = 0; c < nc; c++) {
+            gp[c] = -(pj[c] - p[c]);
+            for (int k = 0; k < 3; k++) gl[3 * c + k] = -(lj[3 * c + k] - lam[3 * c + k]);
+        }
+        for (size_t c = 0; c < nc; c++) gp2 += gp[c] * gp[c];
+        for (size_t c = 0; c < nc; c++) gl2 += dot3(gl + 3 * c, gl + 3 * c);
+        const double beta = (gp2 + gl2) / fmax(prev_p2 + prev_l2, 2.2250738585072014e-308);
+        if (beta > 1.0) { /* NNCG restart */
+            for (size_t c = 0; c < nc; c++) {
+                sp[c] = 0.0;
+                for (int k = 0; k < 3; k++) sl[3 * c + k] = 0.0;
+            }
+            memcpy(p, pj, nc * sizeof(double));
+            memcpy(lam, lj, 3 * nc * sizeof(double));
+        } else {
+            for (size_t c = 0; c < nc; c++) {
+                /* friction first: its cone bound uses the pressure before this update */
+                double ln[3], tp[3];
+                for (int k = 0; k < 3; k++) ln[k] = lj[3 * c + k] + beta * sl[3 * c + k];
+                matvec3(T + 9 * c, ln, tp);
+                const double mag = norm3(tp);
+                const double lmax = CONTACT_FRICTION * p[c] * norm3(crec(cells, c) + CR_FA);
+                if (mag > lmax) {
+                    for (int k = 0; k < 3; k++) {
+                        lam[3 * c + k] = lj[3 * c + k];
+                        sl[3 * c + k] = 0.0;
+                    }
+                } else {
+                    for (int k = 0; k < 3; k++) {
+                        lam[3 * c + k] = tp[k];
+                        sl[3 * c + k] = beta * sl[3 * c + k] - gl[3 * c + k];
+                    }
+                }
+            }
+            for (size_t c = 0; c < nc; c++) {
+                const double pn = pj[c] + beta * sp[c];
+                if (pn < 0.0) {
+                    p[c] = pj[c];
+                    sp[c] = 0.0;
+                } else {
+                    p[c] = pn;
+                    sp[c] = beta * sp[c] - gp[c];
+                }
+            }
+        }
+        prev_p2 = gp2;
+        prev_l2 = gl2;
+        contact_extra(bodies, nb, cells, nc, p, lam, ef, et);
+        contact_trial(bodies, nb, dt, ef, et, vt, wt);
+        contact_predict(bodies, cells, nc, T, dt, vt, wt, verr, vtang);
+        contact_merit(cells, nc, p, lam, vtang, pp, vtn, dev);
+        it++;
+    }
+    contact_extra(bodies, nb, cells, nc, p, lam, ef, et);
+    for (int bi = 0; bi < nb; bi++)
+        for (int k = 0; k < 3; k++) {
+            extra[6 * bi + k] = ef[3 * bi + k];
+            extra[6 * bi + 3 + k] = et[3 * bi + k];
+        }
+    for (size_t c = 0; c < nc; c++) {
+        double *r = cells + c * REF_CONTACT_REC;
+        r[CR_P] = p[c];
+        for (int k = 0; k < 3; k++) r[CR_LAM + k] = lam[3 * c + k];
+    }
+    free(T); free(p); free(verr); free(lam); free(vtang); free(sp); free(sl); free(pj); free(lj);
+    free(gp); free(gl); free(pp); free(vtn); free(dev); free(ef); free(et); free(vt); free(wt);
+    free(nact);
+    return it;
+}
+
+/* solve_for_p_c of a system's active cells: compute_diagonal per cell, the
+ * cell-level solve, then add_force_and_torque on every dynamic body
+ * (rigid_body_motion.rs:156-159) */
+static void contact_solve(ref_system *s, contact_cell *cells, size_t nc, double dt) {
+    s->contact_iters = 0;
+    if (nc == 0) return;
+    const int nb = s->nb;
+    for (size_t c = 0; c < nc; c++) contact_diagonal(s, &cells[c], dt);
+    double *bodies = (double *)calloc((size_t)nb * REF_CONTACT_BODY, sizeof(double));
+    for (int bi = 0; bi < nb; bi++) {
+        const ref_boundary *b = &s->b[bi];
+        double *o = bodies + (size_t)bi * REF_CONTACT_BODY;
+        o[CB_DYN] = b->dynamic ? 1.0 : 0.0;
+        if (!b->dynamic) continue;
+        o[CB_MASS] = b->rb.mass;
+        for (int k = 0; k < 3; k++) {
+            o[CB_COM + k] = b->rb.com[k];
+            o[CB_V + k] = b->rb.lin_vel[k];
+            o[CB_L + k] = b->rb.ang_mom[k];
+            o[CB_F + k] = b->rb.force[k];
+            o[CB_T + k] = b->rb.torque[k];
+        }
+        memcpy(o + CB_IINV, b->rb.inertia_inv_world, 9 * sizeof(double));
+    }
+    s->contact_rec = (double *)realloc(s->contact_rec, nc * REF_CONTACT_REC * sizeof(double));
+    for (size_t c = 0; c < nc; c++) {
+        const contact_cell *cc = &cells[c];
+        double *r = s->contact_rec + c * REF_CONTACT_REC;
+        r[CR_A] = (double)cc->a;
+        r[CR_B] = (double)cc->b;
+        r[CR_VOL] = cc->volume;
+        for (int k = 0; k < 3; k++) {
+            r[CR_CENTER + k] = cc->center[k];
+            r[CR_FA + k] = cc->fa[k];
+            r[CR_TA + k] = cc->taua[k];
+            r[CR_FB + k] = cc->fb[k];
+            r[CR_TB + k] = cc->taub[k];
+            r[CR_LAM + k] = 0.0;
+        }
+        r[CR_ACCP] = cc->acc_p;
+        r[CR_ACCF] = cc->acc_f;
+        r[CR_P] = 0.0;
+    }
+    double *extra = (double *)calloc(6 * (size_t)nb, sizeof(double));
+    s->contact_iters = ref_contact_solve(bodies, nb, s->contact_rec, nc, dt, extra);
+    for (int bi = 0; bi < nb; bi++) {
+        ref_boundary *b = &s->b[bi];
+        if (!b->dynamic) continue;
+        for (int k = 0; k < 3; k++) {
+            b->rb.force[k] += extra[6 * bi + k];
+            b->rb.torque[k] += extra[6 * bi + 3 + k];
+        }
+    }
+    free(bodies);
+    free(extra);
+}
+
+/* contact handling of a system whose boundaries are all volume maps */
+static void contact_step(ref_system *s, double dt) {
+    s->contact_cells = 0;
+    s->contact_iters = 0;
+    if (s->contact_degree <= 0 || s->nb < 2) return;
+    int dyn = 0;
+    for (int i = 0; i < s->nb; i++) {
+        if (s->b[i].kind != 1) return;
+        dyn |= s->b[i].dynamic;
+    }
+    if (!dyn) return;
+    size_t nc = 0;
+    contact_cell *cells = contact_generate(s, &nc);
+    s->contact_cells = (uint32_t)nc;
+    contact_solve(s, cells, nc, dt);
+    for (size_t c = 0; c < nc; c++) free(cells[c].qp);
+    free(cells);
+}
+
+int ref_system_set_aabb(ref_system *s, int b, const double center[3], const double half[3]) {
+    if (b < 0 || b >= s->nb) return -1;
+    s->b[b].has_aabb = 1;
+    memcpy(s->b[b].aabb_c, center, 3 * sizeof(double));
+    memcpy(s->b[b].aabb_h, half, 3 * sizeof(double));
+    return 0;
+}
+
+int ref_system_set_contact(ref_system *s, const double *gl_x, const double *gl_w, int degree) {
+    if (degree < 0 || degree > 32 || s->nb > 64) return -1;
+    s->contact_degree = degree;
+    for (int k = 0; k < degree; k++) {
+        s->contact_x[k] = gl_x[k];
+        s->contact_w[k] = gl_w[k];
+    }
+    return 0;
+}
+
+void ref_system_contact_stats(const ref_system *s, uint32_t out[2]) {
+    out[0] = s->contact_cells;
+    out[1] = s->contact_iters;
+}
+
+size_t ref_system_contact_cells(const ref_system *s, double *out, size_t cap) {
+    size_t n = s->contact_cells;
+    if (out) memcpy(out, s->contact_rec, (n < cap ? n : cap) * REF_CONTACT_REC * sizeof(double));
+    return n;
+}
+
 /* SPHSystem::step_forward_in_time   (sph/mod.rs:125-144) */
 int ref_system_step(ref_system *s, int nsteps) {
     for (int st = 0; st < nsteps; st++) {
@@ -1369,6 +2027,7 @@ int ref_system_step(ref_system *s, int nsteps) {
             t = s->x; s->x = s->ix; s->ix = t;
             t = s->v; s->v = s->iv; s->iv = t;
         }
+        contact_step(s, dt); /* VolumeMapBoundary::step_forward_in_time (:1440-1453) */
         for (int i = 0; i < s->nb; i++) { /* B::step_forward_in_time */
             ref_boundary *b = &s->b[i];
             if (!b->dynamic) continue;

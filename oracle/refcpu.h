@@ -152,6 +152,31 @@ int ref_system_add_static_samples(ref_system *s, const double *pos, size_t m,
  * com[3], q[4], v[3], omega[3]  (32 doubles). */
 int ref_system_add_volume_map(ref_system *s, const ref_lattice *sdf,
                               const ref_lattice *vm, const double *rigid_init);
+/* The body-frame bounding box of boundary b's mesh (LocalAABB,
+ * volume_map_boundary.rs:1137-1150, 1209-1222); a boundary without one takes
+ * no part in rigid-rigid contact. */
+int ref_system_set_aabb(ref_system *s, int b, const double center[3], const double half[3]);
+/* Rigid-rigid contact of a system whose boundaries are all volume maps
+ * (VolumeMapBoundary::step_forward_in_time, volume_map_boundary.rs:1440-1453):
+ * the Gauss-Legendre rule (nodes ascending on [-1, 1]) of the contact-cell
+ * quadrature, degree 8 in the reference (:57); degree 0 turns it off. */
+int ref_system_set_contact(ref_system *s, const double *gl_x, const double *gl_w, int degree);
+/* active contact cells and NNCG iterations of the last step */
+void ref_system_contact_stats(const ref_system *s, uint32_t out[2]);
+/* the last step's active contact cells in solve order, REF_CONTACT_REC
+ * doubles each: body a, body b, volume, center[3], f_a[3], tau_a[3], f_b[3],
+ * tau_b[3], a_cc pressure, a_cc friction, p, lambda[3]; returns the count */
+#define REF_CONTACT_REC 24
+size_t ref_system_contact_cells(const ref_system *s, double *out, size_t cap);
+/* solve_for_p_c (volume_map_boundary.rs:186-637) after compute_diagonal, on
+ * cell records as above (p and lambda are written) and REF_CONTACT_BODY
+ * doubles per body: dynamic flag, mass, com[3], linear velocity[3], angular
+ * momentum[3], force[3], torque[3], world inverse inertia[9] (row major).
+ * extra[6 * nb] receives each body's committed contact force and torque;
+ * returns the NNCG iteration count. */
+#define REF_CONTACT_BODY 26
+uint32_t ref_contact_solve(const double *bodies, int nb, double *cells, size_t nc, double dt,
+                           double *extra);
 /* BoundaryHandling::initialize + resize_slots + update() (System::new_boxed). */
 int ref_system_init(ref_system *s);
 int ref_system_step(ref_system *s, int nsteps);

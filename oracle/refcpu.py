@@ -102,6 +102,15 @@ def lib():
         L.ref_system_add_volume_map.restype = C.c_int
         L.ref_system_add_volume_map.argtypes = [C.c_void_p, C.POINTER(RefLattice),
                                                 C.POINTER(RefLattice), _dp]
+        L.ref_system_set_aabb.restype = C.c_int
+        L.ref_system_set_aabb.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.ref_system_set_contact.restype = C.c_int
+        L.ref_system_set_contact.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.ref_system_contact_stats.argtypes = [C.c_void_p, _u32p]
+        L.ref_system_contact_cells.restype = C.c_size_t
+        L.ref_system_contact_cells.argtypes = [C.c_void_p, _dp, C.c_size_t]
+        L.ref_contact_solve.restype = C.c_uint32
+        L.ref_contact_solve.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_double, _dp]
         L.ref_system_init.restype = C.c_int
         L.ref_system_init.argtypes = [C.c_void_p]
         L.ref_system_step.restype = C.c_int
@@ -251,6 +260,23 @@ def vm_build(sdf, h, x_min, n, dx, gl_x, gl_w, stats=None):
     return vals, pres
 
 
+# contact-cell record: body a, body b, volume, center[3], f_a[3], tau_a[3],
+# f_b[3], tau_b[3], a_cc pressure, a_cc friction, p, lambda[3]
+CONTACT_REC = 24
+# contact body record: dynamic, mass, com[3], v[3], L[3], F[3], tau[3], I^-1 world[9]
+CONTACT_BODY = 26
+
+
+def contact_solve(bodies, cells, dt):
+    """solve_for_p_c after compute_diagonal on cell and body records; returns
+    (cells with p and lambda, extra force/torque per body (nb, 6), iterations)."""
+    b = f64c(np.asarray(bodies, dtype=np.float64).reshape(-1, CONTACT_BODY))
+    c = f64c(np.array(cells, dtype=np.float64).reshape(-1, CONTACT_REC))
+    extra = np.zeros((len(b), 6))
+    it = lib().ref_contact_solve(dptr(b), len(b), dptr(c), len(c), float(dt), dptr(extra))
+    return c, extra, int(it)
+
+
 def params_struct(P):
     from_solver = {"SESPH": 0, "IISPH": 1, "IISPHwOST": 2}
     from_integ = {"EulerCromer": 0, "TakePredicted": 1}
@@ -304,6 +330,16 @@ class OracleSystem:
                 vm = lattice_struct(b.vm)
                 L.ref_system_add_volume_map(self.h, C.byref(sdf), C.byref(vm), rp)
             self.nb += 1
+        vms = [b for b in scene.boundaries if not hasattr(b, "positions")]
+        for i, b in enumerate(scene.boundaries):
+            if getattr(b, "aabb", None) is not None:
+                c, hw = f64c(b.aabb[0]), f64c(b.aabb[1])
+                L.ref_system_set_aabb(self.h, i, dptr(c), dptr(hw))
+        degree = int(getattr(scene, "contact_degree", 0) or 0)
+        if degree and len(vms) == len(scene.boundaries):
+            gx, gw = np.polynomial.legendre.leggauss(degree)
+            gx, gw = f64c(gx), f64c(gw)
+            L.ref_system_set_contact(self.h, dptr(gx), dptr(gw), degree)
         L.ref_system_init(self.h)
 
     def __del__(self):
@@ -324,6 +360,20 @@ class OracleSystem:
             out.ctypes.data_as(C.POINTER(C.c_float)))
         if st:
             raise ValueError(f"quantity {quantity} refused")
+        return out
+
+    def contact_stats(self):
+        """(active contact cells, NNCG iterations) of the last step."""
+        out = np.zeros(2, dtype=np.uint32)
+        lib().ref_system_contact_stats(self.h, out.ctypes.data_as(_u32p))
+        return int(out[0]), int(out[1])
+
+    def contact_cells(self):
+        """The last step's active contact cells, CONTACT_REC doubles each."""
+        n = int(lib().ref_system_contact_cells(self.h, None, 0))
+        out = np.zeros((n, CONTACT_REC))
+        if n:
+            lib().ref_system_contact_cells(self.h, dptr(out), n)
         return out
 
     def __len__(self):
