@@ -2028,10 +2028,13 @@ static void contact_step(Batch &B, Scene &S, double dt) {
         constexpr uint64_t kChunk = 1u << 20;  // workgroups per launch
         for (uint64_t base = 0; base < total; base += kChunk) {
             const uint64_t nblk = std::min<uint64_t>(kChunk, total - base);
+            Prof::Rec r{SPH_PROF_CONTACT, (size_t)nblk, nullptr, nullptr};
+            if (B.prof.on) B.prof.begin(B.s, r);
             hipLaunchKernelGGL(k_contact_cells, dim3((unsigned)nblk), dim3(kContactBlock), 0,
                                B.s, S.cpairs.p, (int)pairs.size(), S.cbodies.p, Q,
                                (uint32_t)base, S.cnact.p, (uint32_t)cap, S.cidx.p, S.csums.p);
             HIPCK(hipGetLastError());
+            if (B.prof.on) B.prof.end(B.s, r);
         }
         unsigned int *hn = reinterpret_cast<unsigned int *>(B.pinned);
         HIPCK(hipMemcpyAsync(hn, S.cnact.p, sizeof(unsigned int), hipMemcpyDeviceToHost, B.s));
@@ -3217,6 +3220,140 @@ int sph_volume_map_build(sph_ctx *c, const sph_lattice *sdf, double h, const dou
         }
         const size_t ncell = (size_t)n[0] * n[1] * n[2];
         LAUNCH(k_vm_cells, ncell, B.s, g, status.p, present.p);
+        LAUNCH(k_vm_finish, nodes, B.s, nodes, present.p, values.p);
+        values.download(values_out, nodes, B.s);
+        present.download(present_out, nodes, B.s);
+        B.sync();
+    });
+}
+
+int sph_mesh_sdf_build(sph_ctx *c, const double *vertices, size_t nv, const uint32_t *triangles,
+                       size_t nt, const double x_min[3], const int64_t n[3], double dx,
+                       double prune, double *values_out, uint8_t *present_out) {
+    return guarded([&] {
+        if (!vertices || !triangles || !x_min || !n || !values_out || !present_out)
+            throw Error(SPH_ERR_ARG, "null argument");
+        if (nt == 0 || nt >= (1u << 31)) throw Error(SPH_ERR_ARG, "triangle count out of range");
+        if (!(dx > 0) || !(prune >= 0)) throw Error(SPH_ERR_ARG, "dx must be > 0, prune >= 0");
+        for (size_t i = 0; i < 3 * nt; i++)
+            if (triangles[i] >= nv) throw Error(SPH_ERR_ARG, "triangle vertex index out of range");
+        VmGrid g{};
+        size_t nodes = 1;
+        for (int d = 0; d < 3; d++) {
+            if (n[d] <= 0 || n[d] > (1 << 20)) throw Error(SPH_ERR_ARG, "lattice n out of range");
+            g.x_min[d] = x_min[d];
+            g.n[d] = (int)n[d];
+            nodes *= 3 * (size_t)n[d] + 1;
+        }
+        if (nodes >= ((size_t)1 << 31)) throw Error(SPH_ERR_UNSUPPORTED, "lattice too large");
+        g.dx = dx;
+        // pseudo-normals (Baerentzen and Aanaes 2005), host binary64: unit face
+        // normals; vertices: incident faces in index order weighted by the
+        // face angle at the vertex; edges: the sum of the incident faces'
+        // normals over the unique vertex pairs; each normalized
+        auto cross = [](const double *a, const double *b, double *o) {
+            const double x = a[1] * b[2] - a[2] * b[1];
+            const double y = a[2] * b[0] - a[0] * b[2];
+            const double z = a[0] * b[1] - a[1] * b[0];
+            o[0] = x;
+            o[1] = y;
+            o[2] = z;
+        };
+        auto dot = [](const double *a, const double *b) {
+            return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+        };
+        auto unit = [&](double *a) {
+            const double l = std::sqrt(dot(a, a));
+            if (l > 0.0)
+                for (int k = 0; k < 3; k++) a[k] = a[k] / l;
+        };
+        std::vector<double> tri(9 * nt), fn(3 * nt, 0.0), vn(3 * nv, 0.0);
+        for (size_t f = 0; f < nt; f++) {
+            const double *a = vertices + 3 * triangles[3 * f], *b = vertices + 3 * triangles[3 * f + 1],
+                         *cc = vertices + 3 * triangles[3 * f + 2];
+            for (int k = 0; k < 3; k++) {
+                tri[9 * f + k] = a[k];
+                tri[9 * f + 3 + k] = b[k];
+                tri[9 * f + 6 + k] = cc[k];
+            }
+            double e1[3], e2[3];
+            for (int k = 0; k < 3; k++) {
+                e1[k] = b[k] - a[k];
+                e2[k] = cc[k] - a[k];
+            }
+            cross(e1, e2, &fn[3 * f]);
+            unit(&fn[3 * f]);
+        }
+        for (size_t f = 0; f < nt; f++)
+            for (int i = 0; i < 3; i++) {
+                const double *p = vertices + 3 * triangles[3 * f + i],
+                             *q = vertices + 3 * triangles[3 * f + (i + 1) % 3],
+                             *r = vertices + 3 * triangles[3 * f + (i + 2) % 3];
+                double e1[3], e2[3];
+                for (int k = 0; k < 3; k++) {
+                    e1[k] = q[k] - p[k];
+                    e2[k] = r[k] - p[k];
+                }
+                const double l = std::sqrt(dot(e1, e1)) * std::sqrt(dot(e2, e2));
+                double cs = l > 0.0 ? dot(e1, e2) / l : 1.0;
+                cs = cs < -1.0 ? -1.0 : (cs > 1.0 ? 1.0 : cs);
+                const double ang = std::acos(cs);
+                double *o = &vn[3 * triangles[3 * f + i]];
+                for (int k = 0; k < 3; k++) o[k] += ang * fn[3 * f + k];
+            }
+        for (size_t i = 0; i < nv; i++) unit(&vn[3 * i]);
+        struct EdgeRef {
+            uint32_t lo, hi, tri, local;
+        };
+        std::vector<EdgeRef> er(3 * nt);
+        for (size_t f = 0; f < nt; f++)
+            for (uint32_t i = 0; i < 3; i++) {
+                const uint32_t a = triangles[3 * f + i], b = triangles[3 * f + (i + 1) % 3];
+                er[3 * f + i] = {std::min(a, b), std::max(a, b), (uint32_t)f, i};
+            }
+        std::sort(er.begin(), er.end(), [](const EdgeRef &a, const EdgeRef &b) {
+            if (a.lo != b.lo) return a.lo < b.lo;
+            if (a.hi != b.hi) return a.hi < b.hi;
+            if (a.tri != b.tri) return a.tri < b.tri;
+            return a.local < b.local;
+        });
+        std::vector<uint32_t> te(3 * nt), tv(triangles, triangles + 3 * nt);
+        std::vector<double> en(9 * nt, 0.0);
+        size_t ne = 0;
+        for (size_t k = 0; k < er.size(); k++) {
+            if (k > 0 && (er[k].lo != er[k - 1].lo || er[k].hi != er[k - 1].hi)) ne++;
+            te[3 * er[k].tri + er[k].local] = (uint32_t)ne;
+            for (int d = 0; d < 3; d++) en[3 * ne + d] += fn[3 * er[k].tri + d];
+        }
+        ne++;
+        for (size_t e = 0; e < ne; e++) unit(&en[3 * e]);
+        Batch local;
+        if (!c) {
+            check_device();
+            local.init();
+        }
+        Batch &B = c ? c->own : local;
+        DBuf<double> dtri, dfn, dvn, den, values;
+        DBuf<uint32_t> dtv, dte;
+        DBuf<uint8_t> present;
+        dtri.upload(tri.data(), tri.size(), B.s);
+        dfn.upload(fn.data(), fn.size(), B.s);
+        dvn.upload(vn.data(), vn.size(), B.s);
+        den.upload(en.data(), 3 * ne, B.s);
+        dtv.upload(tv.data(), tv.size(), B.s);
+        dte.upload(te.data(), te.size(), B.s);
+        values.reserve(nodes);
+        present.reserve(nodes);
+        HIPCK(hipMemsetAsync(present.p, 0, nodes, B.s));
+        MeshDev M{dtri.p, dtv.p, dte.p, dfn.p, dvn.p, den.p, (uint32_t)nt};
+        {
+            Prof::Rec r{SPH_PROF_MESH_SDF, nodes, nullptr, nullptr};
+            if (B.prof.on) B.prof.begin(B.s, r);
+            LAUNCH(k_mesh_sdf, nodes, B.s, M, g, values.p);
+            if (B.prof.on) B.prof.end(B.s, r);
+        }
+        const size_t ncell = (size_t)n[0] * n[1] * n[2];
+        LAUNCH(k_sdf_cells, ncell, B.s, g, values.p, prune, present.p);
         LAUNCH(k_vm_finish, nodes, B.s, nodes, present.p, values.p);
         values.download(values_out, nodes, B.s);
         present.download(present_out, nodes, B.s);

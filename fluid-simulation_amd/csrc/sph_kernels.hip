@@ -674,5 +674,141 @@ __global__ __launch_bounds__(kContactBlock) void k_contact_cells(
     }
 }
 
+// ------------------------------------------------------------------------
+// Signed distance to an oriented triangle mesh at the lattice nodes
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void sub3v(const double *a, const double *b, double *o) {
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+__device__ __forceinline__ double dot3v(const double *a, const double *b) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// The closest point q of triangle (a, b, c) to p by Voronoi-region tests
+// (Ericson, Real-Time Collision Detection, 5.1.5) and its feature: 0..2 the
+// vertices a, b, c; 3 edge ab, 4 edge bc, 5 edge ca; 6 the face
+__device__ __forceinline__ int mesh_closest(const double *p, const double *a, const double *b,
+                                            const double *c, double *q) {
+    double ab[3], ac[3], ap[3], bp[3], cp[3];
+    sub3v(b, a, ab);
+    sub3v(c, a, ac);
+    sub3v(p, a, ap);
+    const double d1 = dot3v(ab, ap), d2 = dot3v(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        q[0] = a[0], q[1] = a[1], q[2] = a[2];
+        return 0;
+    }
+    sub3v(p, b, bp);
+    const double d3 = dot3v(ab, bp), d4 = dot3v(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) {
+        q[0] = b[0], q[1] = b[1], q[2] = b[2];
+        return 1;
+    }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double s = d1 / (d1 - d3);
+        for (int k = 0; k < 3; k++) q[k] = a[k] + s * ab[k];
+        return 3;
+    }
+    sub3v(p, c, cp);
+    const double d5 = dot3v(ab, cp), d6 = dot3v(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) {
+        q[0] = c[0], q[1] = c[1], q[2] = c[2];
+        return 2;
+    }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double s = d2 / (d2 - d6);
+        for (int k = 0; k < 3; k++) q[k] = a[k] + s * ac[k];
+        return 5;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double s = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int k = 0; k < 3; k++) q[k] = b[k] + s * (c[k] - b[k]);
+        return 4;
+    }
+    const double den = 1.0 / ((va + vb) + vc);
+    const double s = vb * den, w = vc * den;
+    for (int k = 0; k < 3; k++) q[k] = (a[k] + ab[k] * s) + ac[k] * w;
+    return 6;
+}
+
+// One thread per lattice position; serendipity nodes take the distance to
+// the closest point over all triangles (the first of least distance in index
+// order), signed by the pseudo-normal of its feature. Triangles are staged
+// through LDS in tiles shared by the workgroup.
+__global__ __launch_bounds__(256) void k_mesh_sdf(MeshDev M, VmGrid g, double *values) {
+    __shared__ double tile[kMeshTile * 9];
+    const size_t nx = 3 * (size_t)g.n[0] + 1, ny = 3 * (size_t)g.n[1] + 1,
+                 nz = 3 * (size_t)g.n[2] + 1;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = k < nx * ny * nz;
+    const int ix = in ? (int)(k % nx) : 0, iy = in ? (int)((k / nx) % ny) : 0,
+              iz = in ? (int)(k / (nx * ny)) : 0;
+    const bool node = in && vm_is_node(ix, iy, iz);
+    const double p[3] = {g.x_min[0] + (double)ix / 3.0 * g.dx, g.x_min[1] + (double)iy / 3.0 * g.dx,
+                         g.x_min[2] + (double)iz / 3.0 * g.dx};
+    double best = __builtin_inf(), bq[3] = {0.0, 0.0, 0.0};
+    uint32_t bt = 0;
+    int bf = -1;
+    for (uint32_t base = 0; base < M.nt; base += kMeshTile) {
+        const uint32_t cnt = min((uint32_t)kMeshTile, M.nt - base);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < 9 * cnt; i += blockDim.x)
+            tile[i] = M.tri[(size_t)base * 9 + i];
+        __syncthreads();
+        if (!node) continue;
+        for (uint32_t f = 0; f < cnt; f++) {
+            double q[3], d[3];
+            const int ft = mesh_closest(p, &tile[9 * f], &tile[9 * f + 3], &tile[9 * f + 6], q);
+            sub3v(p, q, d);
+            const double d2 = dot3v(d, d);
+            if (d2 < best) {
+                best = d2;
+                bt = base + f;
+                bf = ft;
+                bq[0] = q[0], bq[1] = q[1], bq[2] = q[2];
+            }
+        }
+    }
+    if (!in) return;
+    if (!node || bf < 0) {
+        values[k] = 0.0;
+        return;
+    }
+    double diff[3];
+    sub3v(p, bq, diff);
+    const double dist = sqrt(dot3v(diff, diff));
+    const double *n = bf < 3 ? M.vn + 3 * (size_t)M.tv[3 * (size_t)bt + bf]
+                     : (bf < 6 ? M.en + 3 * (size_t)M.te[3 * (size_t)bt + (bf - 3)]
+                               : M.fn + 3 * (size_t)bt);
+    values[k] = dot3v(diff, n) < 0.0 ? -dist : dist;
+}
+
+// CubicSerendipityDiscretization::new pruning (discretization.rs:208-236): a
+// cell is dropped when all 32 node values lie below -prune or all above
+// +prune; the nodes of kept cells are present
+__global__ void k_sdf_cells(VmGrid g, const double *values, double prune, uint8_t *present) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ncell = (size_t)g.n[0] * g.n[1] * g.n[2];
+    if (c >= ncell) return;
+    const size_t nx = 3 * (size_t)g.n[0] + 1, ny = 3 * (size_t)g.n[1] + 1;
+    const size_t cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / ((size_t)g.n[0] * g.n[1]);
+    const size_t base = (3 * cz * ny + 3 * cy) * nx + 3 * cx;
+    bool low = true, high = true;
+    for (int i = 0; i < 32; i++) {
+        const double v =
+            values[base + ((size_t)kNodeOff[i][2] * ny + kNodeOff[i][1]) * nx + kNodeOff[i][0]];
+        low = low && v < -prune;
+        high = high && v > prune;
+    }
+    if (low || high) return;
+    for (int i = 0; i < 32; i++)
+        present[base + ((size_t)kNodeOff[i][2] * ny + kNodeOff[i][1]) * nx + kNodeOff[i][0]] = 1;
+}
+
 }  // namespace sph
 

@@ -92,6 +92,21 @@ __global__ void k_vm_cells(VmGrid g, const uint8_t *status, uint8_t *present);
 __global__ void k_vm_finish(size_t n, const uint8_t *present, double *values);
 __global__ void k_widen_u32(size_t n, const uint32_t *in, uint64_t *out);
 
+// Signed distance to a triangle mesh at lattice nodes
+// (TriangleMeshWrapper::signed_distance, volume_map_boundary.rs:1655-1671)
+constexpr int kMeshTile = 128;  // triangles staged in LDS per round
+struct MeshDev {
+    const double *tri;     // [nt][9] vertex positions a, b, c
+    const uint32_t *tv;    // [nt][3] vertex ids
+    const uint32_t *te;    // [nt][3] edge ids of (a b), (b c), (c a)
+    const double *fn;      // [nt][3] unit face normals
+    const double *vn;      // [nv][3] vertex pseudo-normals
+    const double *en;      // [ne][3] edge pseudo-normals
+    uint32_t nt;
+};
+__global__ void k_mesh_sdf(MeshDev M, VmGrid g, double *values);
+__global__ void k_sdf_cells(VmGrid g, const double *values, double prune, uint8_t *present);
+
 // Rigid-rigid contact cells (VolumeMapBoundary::generate_contact_cells,
 // volume_map_boundary.rs:98-161, 702-768, 835-882): candidate sub-cells of
 // the overlapping world AABBs of a body pair, numbered pair by pair in

@@ -18,7 +18,8 @@ SPH_OK = 0
 # ScalarQuantity (render_info.rs:205-212)
 QUANTITIES = ["Speed", "Volume", "Density", "DensityError", "Pressure", "KineticEnergy"]
 PROF_KERNELS = ["neighbor_count", "pairs_volume", "boundary_sample", "nonpressure",
-                "source_diag", "pressure_accel", "jacobi", "vm_quadrature", "quantity"]
+                "source_diag", "pressure_accel", "jacobi", "vm_quadrature", "quantity",
+                "mesh_sdf", "contact"]
 STATUS = {-1: "SPH_ERR_ARG", -2: "SPH_ERR_HIP", -3: "SPH_ERR_STATE", -4: "SPH_ERR_CAPACITY",
           -5: "SPH_ERR_UNSUPPORTED", -6: "SPH_ERR_NO_DEVICE"}
 
@@ -116,6 +117,8 @@ SIGNATURES = [
     ("sph_world_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("sph_world_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
+    ("sph_mesh_sdf_build", C.c_int, [C.c_void_p, _dp, C.c_size_t, _u32p, C.c_size_t, _dp,
+                                     _i64p, C.c_double, C.c_double, _dp, _u8p]),
     ("sph_quantity_at_positions", C.c_int, [C.c_void_p, C.c_int, C.c_size_t,
                                             C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("sph_volume_map_build", C.c_int, [C.c_void_p, C.POINTER(SphLattice), C.c_double, _dp, _i64p,

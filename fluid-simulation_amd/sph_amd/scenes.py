@@ -310,6 +310,78 @@ def volume_map_quadrature(sd_fn, aabb_min, aabb_max, h, dx_particle, builder, de
 
 
 # --------------------------------------------------------------------------
+# triangle meshes (the reference builds its boundary fields from meshes)
+# --------------------------------------------------------------------------
+def box_mesh(lo, hi, inward=False):
+    """The 8 vertices and 12 triangles of the box [lo, hi], wound counter-
+    clockwise seen from outside (normals outward), or inward for a container
+    whose fluid side is the box's interior."""
+    lo, hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+    v = np.array([[hi[0] if i & 1 else lo[0], hi[1] if i & 2 else lo[1], hi[2] if i & 4 else lo[2]]
+                  for i in range(8)])
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    t = []
+    for a, b, c, d in quads:
+        t += [(a, b, c), (a, c, d)]
+    t = np.array(t, dtype=np.uint32)
+    return v, (t[:, ::-1].copy() if inward else t)
+
+
+def icosphere(radius, subdivisions=2, center=(0.0, 0.0, 0.0), inward=False):
+    """A geodesic sphere: the icosahedron with each triangle split in four
+    `subdivisions` times, new vertices pushed onto the sphere; outward winding
+    unless `inward`."""
+    g = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, g, 0), (1, g, 0), (-1, -g, 0), (1, -g, 0), (0, -1, g), (0, 1, g), (0, -1, -g),
+         (0, 1, -g), (g, 0, -1), (g, 0, 1), (-g, 0, -1), (-g, 0, 1)]
+    v = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    t = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def m(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = v[a] + v[b]
+                v.append(p / np.linalg.norm(p))
+                mid[key] = len(v) - 1
+            return mid[key]
+        t = [tri for a, b, c in t
+             for tri in ((a, m(a, b), m(c, a)), (b, m(b, c), m(a, b)), (c, m(c, a), m(b, c)),
+                         (m(a, b), m(b, c), m(c, a)))]
+    verts = np.asarray(center, dtype=np.float64) + radius * np.array(v)
+    tris = np.array(t, dtype=np.uint32)
+    return verts, (tris[:, ::-1].copy() if inward else tris)
+
+
+def mesh_fields(vertices, triangles, h, dx_particle, sdf_builder, vm_builder, degree=30):
+    """DiscretizedBoundaryFields::new (volume_map_boundary.rs:1547-1636) from a
+    triangle mesh: the signed-distance lattice over the mesh AABB padded by
+    3.1 h, pruned outside +-3.1 h, then the volume map integrated from it with
+    the degree-`degree` ball rule.
+
+    ``sdf_builder(vertices, triangles, x_min, n, dx, prune) -> (values,
+    present)`` evaluates the mesh distance (``sph_amd.system.build_mesh_sdf``
+    on the device); ``vm_builder`` is as for volume_map_quadrature."""
+    vertices = np.asarray(vertices, dtype=np.float64)
+    cell = 4.0 * dx_particle                        # :1120
+    lo, hi = vertices.min(axis=0), vertices.max(axis=0)
+    pad_sd = 3.1 * h
+    x_min, x_max, n = lattice_grid(lo, hi, pad_sd, cell)
+    vals, pres = sdf_builder(vertices, triangles, x_min, n, cell, pad_sd)
+    sdf = Lattice(x_min, x_max, float(cell), n, np.ascontiguousarray(vals, dtype=np.float64),
+                  np.ascontiguousarray(pres, dtype=np.uint8))
+    vx_min, vx_max, vn = lattice_grid(lo, hi, 2.0 * h + cell / 6.0, cell)
+    gl_x, gl_w = np.polynomial.legendre.leggauss(degree)
+    vvals, vpres = vm_builder(sdf, h, vx_min, vn, cell, gl_x, gl_w)
+    vm = Lattice(vx_min, vx_max, float(cell), vn, np.ascontiguousarray(vvals, dtype=np.float64),
+                 np.ascontiguousarray(vpres, dtype=np.uint8))
+    return sdf, vm
+
+
+# --------------------------------------------------------------------------
 # fluid blocks
 # --------------------------------------------------------------------------
 def lattice_block(lo, counts, dx, rng, jitter):

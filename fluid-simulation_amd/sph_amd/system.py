@@ -565,6 +565,29 @@ def build_volume_map(sdf, h: float, x_min, n, dx: float, gl_x, gl_w, system=None
     return vals, pres
 
 
+def build_mesh_sdf(vertices, triangles, x_min, n, dx: float, prune: float, system=None):
+    """Signed-distance node values and present flags on the lattice (x_min, n,
+    dx) from a triangle mesh (``DiscretizedBoundaryFields::new``,
+    ``volume_map_boundary.rs:1547-1591``): vertices [nv][3], triangles [nt][3]
+    with outward winding; cells whose node values all lie beyond +-prune are
+    pruned. Runs on the stream of `system` if given."""
+    v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1, 3)
+    xm = np.ascontiguousarray(x_min, dtype=np.float64)
+    nn = np.ascontiguousarray(n, dtype=np.int64)
+    if xm.shape != (3,) or nn.shape != (3,):
+        raise ValueError("x_min and n need 3 entries")
+    shape = (3 * int(nn[2]) + 1, 3 * int(nn[1]) + 1, 3 * int(nn[0]) + 1)
+    vals = np.empty(shape, dtype=np.float64)
+    pres = np.empty(shape, dtype=np.uint8)
+    handle = system._ctx.h if system is not None else None
+    N.check(N.lib().sph_mesh_sdf_build(handle, N.dptr(v), len(v), t.ctypes.data_as(N._u32p),
+                                       len(t), N.dptr(xm), nn.ctypes.data_as(N._i64p),
+                                       float(dx), float(prune), N.dptr(vals),
+                                       pres.ctypes.data_as(N._u8p)))
+    return vals, pres
+
+
 def device_memory():
     """(free, total) bytes of device memory."""
     f, t = C.c_size_t(), C.c_size_t()

@@ -223,7 +223,9 @@ enum {
     SPH_PROF_JACOBI = 6,           /* relaxed Jacobi update + error */
     SPH_PROF_VM_QUADRATURE = 7,    /* volume-map ball quadrature (units: band nodes) */
     SPH_PROF_QUANTITY = 8,         /* sensor queries (units: positions) */
-    SPH_PROF_KERNELS = 9
+    SPH_PROF_MESH_SDF = 9,         /* mesh signed distance at lattice nodes (units: nodes) */
+    SPH_PROF_CONTACT = 10,         /* rigid-rigid contact cells (units: candidate sub-cells) */
+    SPH_PROF_KERNELS = 11
 };
 int sph_profile(sph_ctx *ctx, int enable);
 int sph_profile_read(sph_ctx *ctx, int kernel, uint64_t *launches, uint64_t *units, double *ms);
@@ -325,6 +327,22 @@ int sph_volume_map_build(sph_ctx *ctx, const sph_lattice *sdf, double h, const d
                          const int64_t n[3], double dx, const double *gl_x, const double *gl_w,
                          int degree, double *values_out, uint8_t *present_out,
                          uint64_t *quadrature_nodes);
+/* The signed-distance half of DiscretizedBoundaryFields::new (volume_map_
+ * boundary.rs:1547-1591) over the cells (x_min, n, dx): at every serendipity
+ * node, TriangleMeshWrapper::signed_distance (:1655-1671) to the triangle
+ * mesh (vertices [nv][3], triangles [nt][3] of vertex indices, outward
+ * counter-clockwise winding, shared vertices merged): the distance to the
+ * closest point of the first closest triangle, negative where the offset
+ * points against the outward pseudo-normal of that point's feature (face;
+ * edge: the normalized sum of its faces' normals; vertex: the angle-weighted
+ * normalized sum). Cells whose 32 node values all lie below -prune or all
+ * above +prune are pruned (CubicSerendipityDiscretization::new,
+ * discretization.rs:208-236; the reference prunes at 3.1 h). Outputs as for
+ * sph_volume_map_build; `ctx` may be NULL. */
+int sph_mesh_sdf_build(sph_ctx *ctx, const double *vertices, size_t nv,
+                       const uint32_t *triangles, size_t nt, const double x_min[3],
+                       const int64_t n[3], double dx, double prune, double *values_out,
+                       uint8_t *present_out);
 
 /* free and total device memory (hipMemGetInfo), for admission control */
 int sph_device_memory(size_t *free_bytes, size_t *total_bytes);

@@ -2341,3 +2341,239 @@ uint64_t ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], c
     free(isnode);
     return quadratures;
 }
+
+/* ===================================================================== */
+/* signed distance to an oriented triangle mesh                            */
+/* (TriangleMeshWrapper::signed_distance, volume_map_boundary.rs:1655-1671) */
+/* ===================================================================== */
+/* The reference projects onto parry3d's oriented TriMesh (closest point and
+ * feature) and signs the distance with the feature's outward pseudo-normal.
+ * Restated with published methods: the closest point of each triangle by
+ * Voronoi-region tests (Ericson, Real-Time Collision Detection, 5.1.5), the
+ * first triangle of least distance in index order, and the pseudo-normals of
+ * Baerentzen and Aanaes (2005): the face normal, the normalized sum of the
+ * two face normals at an edge, and the angle-weighted normalized sum of the
+ * face normals at a vertex. */
+struct ref_mesh {
+    size_t nv, nt, ne;
+    double *v;   /* [nv][3] */
+    uint32_t *t; /* [nt][3] */
+    double *fn;  /* [nt][3] unit face normals */
+    double *vn;  /* [nv][3] vertex pseudo-normals */
+    double *en;  /* [ne][3] edge pseudo-normals */
+    uint32_t *te; /* [nt][3] edge ids of (v0 v1), (v1 v2), (v2 v0) */
+};
+
+static void normalize3(double *a) {
+    const double l = norm3(a);
+    if (l > 0.0)
+        for (int k = 0; k < 3; k++) a[k] = a[k] / l;
+}
+
+typedef struct mesh_edge_ref {
+    uint32_t lo, hi, tri, local;
+} mesh_edge_ref;
+
+static int mesh_edge_cmp(const void *x, const void *y) {
+    const mesh_edge_ref *a = (const mesh_edge_ref *)x, *b = (const mesh_edge_ref *)y;
+    if (a->lo != b->lo) return a->lo < b->lo ? -1 : 1;
+    if (a->hi != b->hi) return a->hi < b->hi ? -1 : 1;
+    if (a->tri != b->tri) return a->tri < b->tri ? -1 : 1;
+    return a->local < b->local ? -1 : (a->local > b->local);
+}
+
+ref_mesh *ref_mesh_create(const double *v, size_t nv, const uint32_t *t, size_t nt) {
+    for (size_t i = 0; i < 3 * nt; i++)
+        if (t[i] >= nv) return NULL;
+    ref_mesh *m = (ref_mesh *)calloc(1, sizeof(ref_mesh));
+    m->nv = nv;
+    m->nt = nt;
+    m->v = (double *)malloc(3 * nv * sizeof(double) + 1);
+    m->t = (uint32_t *)malloc(3 * nt * sizeof(uint32_t) + 1);
+    memcpy(m->v, v, 3 * nv * sizeof(double));
+    memcpy(m->t, t, 3 * nt * sizeof(uint32_t));
+    m->fn = (double *)calloc(3 * nt + 1, sizeof(double));
+    m->vn = (double *)calloc(3 * nv + 1, sizeof(double));
+    m->te = (uint32_t *)calloc(3 * nt + 1, sizeof(uint32_t));
+    for (size_t f = 0; f < nt; f++) {
+        const double *a = v + 3 * t[3 * f], *b = v + 3 * t[3 * f + 1], *c = v + 3 * t[3 * f + 2];
+        double e1[3], e2[3];
+        sub3(b, a, e1);
+        sub3(c, a, e2);
+        cross3(e1, e2, m->fn + 3 * f);
+        normalize3(m->fn + 3 * f);
+    }
+    /* vertex pseudo-normals: incident faces in index order, weighted by the
+     * face's angle at the vertex */
+    for (size_t f = 0; f < nt; f++)
+        for (int i = 0; i < 3; i++) {
+            const double *p = v + 3 * t[3 * f + i], *q = v + 3 * t[3 * f + (i + 1) % 3],
+                         *r = v + 3 * t[3 * f + (i + 2) % 3];
+            double e1[3], e2[3];
+            sub3(q, p, e1);
+            sub3(r, p, e2);
+            const double l = norm3(e1) * norm3(e2);
+            double cs = l > 0.0 ? dot3(e1, e2) / l : 1.0;
+            cs = cs < -1.0 ? -1.0 : (cs > 1.0 ? 1.0 : cs);
+            const double ang = acos(cs);
+            double *o = m->vn + 3 * t[3 * f + i];
+            for (int k = 0; k < 3; k++) o[k] += ang * m->fn[3 * f + k];
+        }
+    for (size_t i = 0; i < nv; i++) normalize3(m->vn + 3 * i);
+    /* edges: unique vertex pairs; the sum of the incident face normals in
+     * face order */
+    mesh_edge_ref *er = (mesh_edge_ref *)malloc(3 * nt * sizeof(mesh_edge_ref) + 1);
+    for (size_t f = 0; f < nt; f++)
+        for (uint32_t i = 0; i < 3; i++) {
+            const uint32_t a = t[3 * f + i], b = t[3 * f + (i + 1) % 3];
+            mesh_edge_ref e = {a < b ? a : b, a < b ? b : a, (uint32_t)f, i};
+            er[3 * f + i] = e;
+        }
+    qsort(er, 3 * nt, sizeof(mesh_edge_ref), mesh_edge_cmp);
+    m->en = (double *)calloc(3 * 3 * nt + 1, sizeof(double));
+    size_t ne = 0;
+    for (size_t k = 0; k < 3 * nt; k++) {
+        if (k > 0 && (er[k].lo != er[k - 1].lo || er[k].hi != er[k - 1].hi)) ne++;
+        m->te[3 * er[k].tri + er[k].local] = (uint32_t)ne;
+        for (int d = 0; d < 3; d++) m->en[3 * ne + d] += m->fn[3 * er[k].tri + d];
+    }
+    m->ne = nt ? ne + 1 : 0;
+    for (size_t e = 0; e < m->ne; e++) normalize3(m->en + 3 * e);
+    free(er);
+    return m;
+}
+
+void ref_mesh_destroy(ref_mesh *m) {
+    if (!m) return;
+    free(m->v); free(m->t); free(m->fn); free(m->vn); free(m->en); free(m->te);
+    free(m);
+}
+
+size_t ref_mesh_edges(const ref_mesh *m) { return m->ne; }
+
+/* the closest point of triangle (a, b, c) to p and its feature: 0, 1, 2 the
+ * vertices, 3 edge ab, 4 edge bc, 5 edge ca, 6 the face */
+static int mesh_closest(const double *p, const double *a, const double *b, const double *c,
+                        double q[3]) {
+    double ab[3], ac[3], ap[3], bp[3], cp[3];
+    sub3(b, a, ab);
+    sub3(c, a, ac);
+    sub3(p, a, ap);
+    const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        memcpy(q, a, 3 * sizeof(double));
+        return 0;
+    }
+    sub3(p, b, bp);
+    const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) {
+        memcpy(q, b, 3 * sizeof(double));
+        return 1;
+    }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double s = d1 / (d1 - d3);
+        for (int k = 0; k < 3; k++) q[k] = a[k] + s * ab[k];
+        return 3;
+    }
+    sub3(p, c, cp);
+    const double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) {
+        memcpy(q, c, 3 * sizeof(double));
+        return 2;
+    }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double s = d2 / (d2 - d6);
+        for (int k = 0; k < 3; k++) q[k] = a[k] + s * ac[k];
+        return 5;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double s = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int k = 0; k < 3; k++) q[k] = b[k] + s * (c[k] - b[k]);
+        return 4;
+    }
+    const double den = 1.0 / ((va + vb) + vc);
+    const double s = vb * den, w = vc * den;
+    for (int k = 0; k < 3; k++) q[k] = (a[k] + ab[k] * s) + ac[k] * w;
+    return 6;
+}
+
+double ref_mesh_signed_distance(const ref_mesh *m, const double p[3]) {
+    double best = INFINITY, bq[3] = {0., 0., 0.};
+    size_t bt = 0;
+    int bf = -1;
+    for (size_t f = 0; f < m->nt; f++) {
+        double q[3], d[3];
+        const int ft = mesh_closest(p, m->v + 3 * m->t[3 * f], m->v + 3 * m->t[3 * f + 1],
+                                    m->v + 3 * m->t[3 * f + 2], q);
+        sub3(p, q, d);
+        const double d2 = dot3(d, d);
+        if (d2 < best) {
+            best = d2;
+            bt = f;
+            bf = ft;
+            memcpy(bq, q, sizeof bq);
+        }
+    }
+    if (bf < 0) return INFINITY;
+    double diff[3];
+    sub3(p, bq, diff);
+    const double dist = norm3(diff);
+    const double *n;
+    if (bf < 3) n = m->vn + 3 * m->t[3 * bt + bf];
+    else if (bf < 6) n = m->en + 3 * m->te[3 * bt + (bf - 3)];
+    else n = m->fn + 3 * bt;
+    return dot3(diff, n) < 0.0 ? -dist : dist;
+}
+
+void ref_mesh_sdf(const ref_mesh *m, const double *p, size_t np, double *out) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t i = 0; i < np; i++) out[i] = ref_mesh_signed_distance(m, p + 3 * i);
+}
+
+void ref_mesh_sdf_build(const ref_mesh *m, const double x_min[3], const int64_t n[3], double dx,
+                        double prune, double *values, uint8_t *present) {
+    const int64_t nx = 3 * n[0] + 1, ny = 3 * n[1] + 1, nz = 3 * n[2] + 1;
+    const size_t nodes = (size_t)(nx * ny * nz);
+    uint8_t *isnode = (uint8_t *)calloc(nodes, 1);
+    for (int64_t cz = 0; cz < n[2]; cz++)
+        for (int64_t cy = 0; cy < n[1]; cy++)
+            for (int64_t cx = 0; cx < n[0]; cx++)
+                for (int i = 0; i < 32; i++)
+                    isnode[((3 * cz + NODE_OFFS[i][2]) * ny + (3 * cy + NODE_OFFS[i][1])) * nx +
+                           (3 * cx + NODE_OFFS[i][0])] = 1;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < (int64_t)nodes; k++) {
+        values[k] = 0.0;
+        present[k] = 0;
+        if (!isnode[k]) continue;
+        const int64_t ix = k % nx, iy = (k / nx) % ny, iz = k / (nx * ny);
+        const double p[3] = {x_min[0] + (double)ix / 3.0 * dx, x_min[1] + (double)iy / 3.0 * dx,
+                             x_min[2] + (double)iz / 3.0 * dx};
+        values[k] = ref_mesh_signed_distance(m, p);
+    }
+    /* CubicSerendipityDiscretization::new pruning (discretization.rs:208-236):
+     * a cell goes when all 32 node values lie below -prune or all above
+     * +prune; nodes of kept cells are present */
+    for (int64_t cz = 0; cz < n[2]; cz++)
+        for (int64_t cy = 0; cy < n[1]; cy++)
+            for (int64_t cx = 0; cx < n[0]; cx++) {
+                int low = 1, high = 1;
+                for (int i = 0; i < 32; i++) {
+                    const double v = values[((3 * cz + NODE_OFFS[i][2]) * ny +
+                                             (3 * cy + NODE_OFFS[i][1])) * nx +
+                                            (3 * cx + NODE_OFFS[i][0])];
+                    low &= v < -prune;
+                    high &= v > prune;
+                }
+                if (low || high) continue;
+                for (int i = 0; i < 32; i++)
+                    present[((3 * cz + NODE_OFFS[i][2]) * ny + (3 * cy + NODE_OFFS[i][1])) * nx +
+                            (3 * cx + NODE_OFFS[i][0])] = 1;
+            }
+    for (size_t k = 0; k < nodes; k++)
+        if (!present[k]) values[k] = 0.0;
+    free(isnode);
+}

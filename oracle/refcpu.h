@@ -228,6 +228,26 @@ uint64_t ref_vm_build(const ref_lattice *sdf, double h, const double x_min[3], c
 int ref_system_quantity_at_positions(ref_system *s, int quantity, size_t nq, const float *pos,
                                      float *out);
 
+/* ---- signed distance to an oriented triangle mesh ---------------------------
+ * TriangleMeshWrapper::signed_distance (volume_map_boundary.rs:1655-1671):
+ * the distance to the mesh's closest point, negative where the offset points
+ * against the outward pseudo-normal of the closest feature (face, edge or
+ * vertex). vertices [nv][3], triangles [nt][3] with outward (counter-
+ * clockwise) winding and shared vertices already merged; NULL on an index
+ * out of range. */
+typedef struct ref_mesh ref_mesh;
+ref_mesh *ref_mesh_create(const double *vertices, size_t nv, const uint32_t *triangles,
+                          size_t nt);
+void ref_mesh_destroy(ref_mesh *m);
+size_t ref_mesh_edges(const ref_mesh *m);
+double ref_mesh_signed_distance(const ref_mesh *m, const double p[3]);
+void ref_mesh_sdf(const ref_mesh *m, const double *p, size_t np, double *out);
+/* The signed-distance lattice of DiscretizedBoundaryFields::new (:1547-1591)
+ * over (x_min, n, dx), pruned as CubicSerendipityDiscretization::new does
+ * with bounds -prune / +prune (discretization.rs:208-236). */
+void ref_mesh_sdf_build(const ref_mesh *m, const double x_min[3], const int64_t n[3], double dx,
+                        double prune, double *values, uint8_t *present);
+
 #ifdef __cplusplus
 }
 #endif

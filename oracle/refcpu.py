@@ -111,6 +111,14 @@ def lib():
         L.ref_system_contact_cells.argtypes = [C.c_void_p, _dp, C.c_size_t]
         L.ref_contact_solve.restype = C.c_uint32
         L.ref_contact_solve.argtypes = [_dp, C.c_int, _dp, C.c_size_t, C.c_double, _dp]
+        L.ref_mesh_create.restype = C.c_void_p
+        L.ref_mesh_create.argtypes = [_dp, C.c_size_t, _u32p, C.c_size_t]
+        L.ref_mesh_destroy.argtypes = [C.c_void_p]
+        L.ref_mesh_edges.restype = C.c_size_t
+        L.ref_mesh_edges.argtypes = [C.c_void_p]
+        L.ref_mesh_sdf.argtypes = [C.c_void_p, _dp, C.c_size_t, _dp]
+        L.ref_mesh_sdf_build.argtypes = [C.c_void_p, _dp, _i64p, C.c_double, C.c_double, _dp,
+                                         _u8p]
         L.ref_system_init.restype = C.c_int
         L.ref_system_init.argtypes = [C.c_void_p]
         L.ref_system_step.restype = C.c_int
@@ -258,6 +266,45 @@ def vm_build(sdf, h, x_min, n, dx, gl_x, gl_w, stats=None):
     if stats is not None:
         stats["quadrature_nodes"] = int(q)
     return vals, pres
+
+
+class Mesh:
+    """An oriented triangle mesh with its pseudo-normals (ref_mesh)."""
+
+    def __init__(self, vertices, triangles):
+        self.v = f64c(np.asarray(vertices).reshape(-1, 3))
+        self.t = np.ascontiguousarray(np.asarray(triangles).reshape(-1, 3), dtype=np.uint32)
+        self.h = lib().ref_mesh_create(dptr(self.v), len(self.v),
+                                       self.t.ctypes.data_as(_u32p), len(self.t))
+        if not self.h:
+            raise ValueError("triangle index out of range")
+
+    def __del__(self):
+        try:
+            lib().ref_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def edges(self):
+        return int(lib().ref_mesh_edges(self.h))
+
+    def signed_distance(self, points):
+        p = f64c(np.asarray(points).reshape(-1, 3))
+        out = np.zeros(len(p))
+        lib().ref_mesh_sdf(self.h, dptr(p), len(p), dptr(out))
+        return out
+
+    def sdf_lattice(self, x_min, n, dx, prune):
+        """(values, present) of the dense (3n+1)^3 nodes, z-major."""
+        x_min = f64c(x_min)
+        n = np.ascontiguousarray(n, dtype=np.int64)
+        shape = tuple(int(3 * k + 1) for k in n[::-1])
+        values = np.zeros(shape)
+        present = np.zeros(shape, dtype=np.uint8)
+        lib().ref_mesh_sdf_build(self.h, dptr(x_min), n.ctypes.data_as(_i64p), float(dx),
+                                 float(prune), dptr(values), present.ctypes.data_as(_u8p))
+        return values, present
 
 
 # contact-cell record: body a, body b, volume, center[3], f_a[3], tau_a[3],
