@@ -31,6 +31,13 @@ instead: the six ScalarQuantity fields (one per step, in turn) on a
 field instead: the volume map of the radius-1 m sphere container (h = 0.02,
 cell 0.04) integrated from its SDF lattice with the degree-30 Gauss-Legendre
 ball rule, in ball quadratures per second; a step is one whole build.
+
+--workload mesh_sdf times the signed-distance lattice of that container built
+from a triangle mesh (a geodesic sphere, 5,120 triangles), in lattice nodes
+per second; a step is one whole build.
+
+--workload contact times rigid-rigid contact: six dynamic cubes pressed into
+a static slab, steps with contact and with it turned off.
 """
 import argparse
 import json
@@ -360,6 +367,175 @@ def vm_cpu_baseline(sdf, h, x_min, n, cell, gx, gw, budget_s, threads):
     }
 
 
+MESH_FLOPS_PER_PAIR = 78  # closest point on a triangle, face-region path, + squared distance
+
+
+def mesh_sdf_bench(args):
+    """Signed-distance lattice of the two-fluid ball's container built from a
+    triangle mesh: a geodesic sphere of radius 1 m (5,120 triangles), h = 0.02."""
+    from sph_amd import scenes, system
+    dx = 0.01
+    h = 2 * dx
+    v, t = scenes.icosphere(1.0, subdivisions=4, inward=True)
+    pad = 3.1 * h
+    x_min, _, n = scenes.lattice_grid(v.min(axis=0), v.max(axis=0), pad, 4 * dx)
+    cell = 4 * dx
+    sim = system.System(scenes.small_iisph_box())  # lends its stream and profiler
+    for _ in range(args.warmup):
+        system.build_mesh_sdf(v, t, x_min, n, cell, pad, system=sim)
+    sim.profile(True)
+    lat = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        vals, pres = system.build_mesh_sdf(v, t, x_min, n, cell, pad, system=sim)
+        lat.append(time.perf_counter() - t0)
+    prof = sim.profile_read()["mesh_sdf"]
+    sim.profile(False)
+    nx, ny, nz = (3 * int(k) + 1 for k in n)
+    ix, iy, iz = np.meshgrid(np.arange(nx) % 3, np.arange(ny) % 3, np.arange(nz) % 3,
+                             indexing="ij")
+    nodes = int((((ix != 0).astype(int) + (iy != 0) + (iz != 0)) <= 1).sum())
+    total = float(np.sum(lat))
+    avg_s = prof["ms"] / 1e3 / max(prof["launches"], 1)
+    flops = MESH_FLOPS_PER_PAIR * nodes * len(t)
+    achieved = flops / avg_s / 1e12 if avg_s > 0 else 0.0
+    out = {
+        "metric": "signed-distance lattice from a triangle mesh, lattice nodes/s",
+        "value": nodes * args.steps / total,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "p50_step_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_step_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (geodesic sphere mesh, radius 1 m, 4 subdivisions)",
+        "config": {"workload": "mesh_sdf_sphere_r1_h0.02", "triangles": int(len(t)),
+                   "cells": [int(k) for k in n], "nodes": nodes,
+                   "present_nodes": int(pres.sum())},
+        "roofline": {
+            "bound": "valu_fp64",
+            "kernel": "k_mesh_sdf",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "algorithmic_flops_per_launch": flops,
+            "avg_launch_ms": avg_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    sim._ctx.close()
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import refcpu
+            refcpu.lib().ref_system_set_threads(threads)
+            m = refcpu.Mesh(v, t)
+            rng = np.random.default_rng(0)
+            k = 2000
+            while True:
+                pts = x_min + rng.uniform(0, 1, (k, 3)) * (np.asarray(n) * cell)
+                t0 = time.perf_counter()
+                m.signed_distance(pts)
+                dt = time.perf_counter() - t0
+                if dt > 0.3 * args.cpu_seconds or k >= nodes:
+                    break
+                k = min(nodes, max(2 * k, int(k * 0.8 * args.cpu_seconds / max(dt, 1e-3))))
+            out["cpu_baseline"] = {
+                "value": k / dt, "unit": "nodes/s", "cores": threads, "kind": "port",
+                "sample": (f"oracle ref_mesh_sdf (binary64 C restatement, OpenMP) at {k} "
+                           f"random lattice-box positions in {dt:.1f} s")}
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline unavailable: {e}")
+    print(json.dumps(out))
+
+
+def contact_bench(args):
+    """Rigid-rigid contact: six dynamic cubes pressed into a static slab
+    (scenes.contact_pile); steps timed with contact and with it turned off."""
+    from sph_amd import scenes, system
+
+    def run(degree):
+        sc = scenes.contact_pile()
+        sc.contact_degree = degree
+        sim = system.System(sc)
+        sim.step_forward_in_time(args.warmup)
+        sim.profile(True)
+        lat, cells, iters = [], 0, 0
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            sim.step_forward_in_time(1)
+            lat.append(time.perf_counter() - t0)
+            c, it = sim.contact_stats()
+            cells += c
+            iters += it
+        prof = sim.profile_read()["contact"]
+        sim.profile(False)
+        sim._ctx.close()
+        return sc, lat, cells, iters, prof
+
+    sc, lat, cells, iters, prof = run(scenes.CONTACT_DEGREE)
+    _, lat_off, _, _, _ = run(0)
+    step_ms = float(np.mean(lat)) * 1e3
+    off_ms = float(np.mean(lat_off)) * 1e3
+    kern_ms = prof["ms"] / max(args.steps, 1)
+    cand = prof["units"] / max(args.steps, 1)
+    out = {
+        "metric": "rigid-rigid contact: active contact cells per second of contact time",
+        "value": cells / args.steps / max((step_ms - off_ms) / 1e3, 1e-9),
+        "unit": "active cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "ms_per_step_contact_off": off_ms,
+        "contact_ms_per_step": step_ms - off_ms,
+        "contact_kernel_ms_per_step": kern_ms,
+        "candidate_subcells_per_step": cand,
+        "active_cells_per_step": cells / args.steps,
+        "nncg_iterations_per_step": iters / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (box, slab and cube SDF lattices, seeded tilts)",
+        "config": {"workload": sc.name, "bodies": len(sc.boundaries), "particles": sc.n,
+                   "quadrature_degree": scenes.CONTACT_DEGREE},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        try:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import refcpu
+            refcpu.lib().ref_system_set_threads(threads)
+            o = refcpu.OracleSystem(scenes.contact_pile())
+            o.step(1)
+            k, t_on = 0, 0.0
+            c_on = 0
+            while t_on < 0.5 * args.cpu_seconds and k < 200:
+                t0 = time.perf_counter()
+                o.step(1)
+                t_on += time.perf_counter() - t0
+                c_on += o.contact_stats()[0]
+                k += 1
+            out["cpu_baseline"] = {
+                "value": c_on / t_on, "unit": "active cells/s (whole step)", "cores": threads,
+                "kind": "port",
+                "sample": (f"oracle steps of the same scene with contact ({k} steps, "
+                           f"{t_on:.1f} s; the fluid is 144 particles, so contact dominates)")}
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline unavailable: {e}")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,6 +570,12 @@ def main():
         return
     if args.workload == "volume_map":
         volume_map_bench(args)
+        return
+    if args.workload == "mesh_sdf":
+        mesh_sdf_bench(args)
+        return
+    if args.workload == "contact":
+        contact_bench(args)
         return
     if args.workload == "sensor_plane":
         sensor_plane_bench(args)

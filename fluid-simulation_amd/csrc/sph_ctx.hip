@@ -3333,10 +3333,67 @@ int sph_mesh_sdf_build(sph_ctx *c, const double *vertices, size_t nv, const uint
             local.init();
         }
         Batch &B = c ? c->own : local;
-        DBuf<double> dtri, dfn, dvn, den, values;
-        DBuf<uint32_t> dtv, dte;
+        // spatially compact tiles: triangles ordered by the Morton code of their
+        // centroid on a 1024^3 grid over the mesh bounds (ties by mesh index)
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        double scale = 0.0;
+        for (size_t i = 0; i < nv; i++)
+            for (int d = 0; d < 3; d++) {
+                lo[d] = std::fmin(lo[d], vertices[3 * i + d]);
+                hi[d] = std::fmax(hi[d], vertices[3 * i + d]);
+                scale = std::fmax(scale, std::fabs(vertices[3 * i + d]));
+            }
+        for (int d = 0; d < 3; d++) {
+            scale = std::fmax(scale, std::fabs(x_min[d]));
+            scale = std::fmax(scale, std::fabs(x_min[d] + (double)n[d] * dx));
+        }
+        auto spread = [](uint64_t x) {  // 10 bits -> every third bit
+            x &= 0x3ff;
+            x = (x | (x << 16)) & 0x30000ff;
+            x = (x | (x << 8)) & 0x300f00f;
+            x = (x | (x << 4)) & 0x30c30c3;
+            x = (x | (x << 2)) & 0x9249249;
+            return x;
+        };
+        std::vector<std::pair<uint64_t, uint32_t>> order(nt);
+        for (size_t f = 0; f < nt; f++) {
+            uint64_t code = 0;
+            for (int d = 0; d < 3; d++) {
+                const double cen = (tri[9 * f + d] + tri[9 * f + 3 + d] + tri[9 * f + 6 + d]) / 3.0;
+                const double ext = hi[d] - lo[d];
+                const double u = ext > 0.0 ? (cen - lo[d]) / ext : 0.0;
+                const uint64_t q = (uint64_t)std::min(1023.0, std::max(0.0, u * 1024.0));
+                code |= spread(q) << d;
+            }
+            order[f] = {code, (uint32_t)f};
+        }
+        std::sort(order.begin(), order.end());
+        const size_t tiles = (nt + kMeshTile - 1) / kMeshTile;
+        std::vector<double> stri(9 * nt), box(6 * tiles);
+        std::vector<uint32_t> orig(nt);
+        for (size_t i = 0; i < nt; i++) {
+            orig[i] = order[i].second;
+            for (int k = 0; k < 9; k++) stri[9 * i + k] = tri[9 * (size_t)order[i].second + k];
+        }
+        for (size_t tl = 0; tl < tiles; tl++) {
+            double *bx = &box[6 * tl];
+            for (int d = 0; d < 3; d++) {
+                bx[d] = INFINITY;
+                bx[3 + d] = -INFINITY;
+            }
+            for (size_t i = tl * kMeshTile; i < std::min(nt, (tl + 1) * kMeshTile); i++)
+                for (int v = 0; v < 3; v++)
+                    for (int d = 0; d < 3; d++) {
+                        bx[d] = std::fmin(bx[d], stri[9 * i + 3 * v + d]);
+                        bx[3 + d] = std::fmax(bx[3 + d], stri[9 * i + 3 * v + d]);
+                    }
+        }
+        DBuf<double> dtri, dbox, dfn, dvn, den, values;
+        DBuf<uint32_t> dtv, dte, dorig;
         DBuf<uint8_t> present;
-        dtri.upload(tri.data(), tri.size(), B.s);
+        dtri.upload(stri.data(), stri.size(), B.s);
+        dbox.upload(box.data(), box.size(), B.s);
+        dorig.upload(orig.data(), orig.size(), B.s);
         dfn.upload(fn.data(), fn.size(), B.s);
         dvn.upload(vn.data(), vn.size(), B.s);
         den.upload(en.data(), 3 * ne, B.s);
@@ -3345,7 +3402,11 @@ int sph_mesh_sdf_build(sph_ctx *c, const double *vertices, size_t nv, const uint
         values.reserve(nodes);
         present.reserve(nodes);
         HIPCK(hipMemsetAsync(present.p, 0, nodes, B.s));
-        MeshDev M{dtri.p, dtv.p, dte.p, dfn.p, dvn.p, den.p, (uint32_t)nt};
+        // distances are formed from coordinates of magnitude <= scale, so both the
+        // tile-box and the triangle distance carry absolute errors far below
+        // 1e-17 scale^2 (plus a relative 1e-6 in the test itself)
+        MeshDev M{dtri.p, dorig.p, dbox.p, dtv.p, dte.p, dfn.p, dvn.p, den.p, (uint32_t)nt,
+                  1e-17 * scale * scale};
         {
             Prof::Rec r{SPH_PROF_MESH_SDF, nodes, nullptr, nullptr};
             if (B.prof.on) B.prof.begin(B.s, r);

@@ -737,11 +737,16 @@ __device__ __forceinline__ int mesh_closest(const double *p, const double *a, co
 }
 
 // One thread per lattice position; serendipity nodes take the distance to
-// the closest point over all triangles (the first of least distance in index
-// order), signed by the pseudo-normal of its feature. Triangles are staged
-// through LDS in tiles shared by the workgroup.
+// the closest point over all triangles, signed by the pseudo-normal of its
+// feature. The least (distance, mesh index) wins, which is the first closest
+// triangle in mesh order whatever order the triangles are visited in.
+// Triangles come in spatially compact tiles staged through LDS; a node skips
+// a tile whose bounding box lies farther than its best distance so far (with
+// a margin over the rounding of both distances, so that no skipped triangle
+// could have won), and the workgroup skips the tile when all its nodes do.
 __global__ __launch_bounds__(256) void k_mesh_sdf(MeshDev M, VmGrid g, double *values) {
     __shared__ double tile[kMeshTile * 9];
+    __shared__ uint32_t tile_orig[kMeshTile];
     const size_t nx = 3 * (size_t)g.n[0] + 1, ny = 3 * (size_t)g.n[1] + 1,
                  nz = 3 * (size_t)g.n[2] + 1;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -752,23 +757,32 @@ __global__ __launch_bounds__(256) void k_mesh_sdf(MeshDev M, VmGrid g, double *v
     const double p[3] = {g.x_min[0] + (double)ix / 3.0 * g.dx, g.x_min[1] + (double)iy / 3.0 * g.dx,
                          g.x_min[2] + (double)iz / 3.0 * g.dx};
     double best = __builtin_inf(), bq[3] = {0.0, 0.0, 0.0};
-    uint32_t bt = 0;
+    uint32_t bt = 0xffffffffu;
     int bf = -1;
     for (uint32_t base = 0; base < M.nt; base += kMeshTile) {
         const uint32_t cnt = min((uint32_t)kMeshTile, M.nt - base);
-        __syncthreads();
+        const double *bx = M.box + 6 * (size_t)(base / kMeshTile);
+        double bd2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double gap = fmax(fmax(bx[d] - p[d], p[d] - bx[3 + d]), 0.0);
+            bd2 += gap * gap;
+        }
+        const bool need = node && !(bd2 > best + best * 1e-6 + M.slack);
+        if (!__syncthreads_or(need)) continue;
         for (uint32_t i = threadIdx.x; i < 9 * cnt; i += blockDim.x)
             tile[i] = M.tri[(size_t)base * 9 + i];
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) tile_orig[i] = M.orig[base + i];
         __syncthreads();
-        if (!node) continue;
+        if (!need) continue;
         for (uint32_t f = 0; f < cnt; f++) {
             double q[3], d[3];
             const int ft = mesh_closest(p, &tile[9 * f], &tile[9 * f + 3], &tile[9 * f + 6], q);
             sub3v(p, q, d);
             const double d2 = dot3v(d, d);
-            if (d2 < best) {
+            const uint32_t o = tile_orig[f];
+            if (d2 < best || (d2 == best && o < bt)) {
                 best = d2;
-                bt = base + f;
+                bt = o;
                 bf = ft;
                 bq[0] = q[0], bq[1] = q[1], bq[2] = q[2];
             }

@@ -96,13 +96,16 @@ __global__ void k_widen_u32(size_t n, const uint32_t *in, uint64_t *out);
 // (TriangleMeshWrapper::signed_distance, volume_map_boundary.rs:1655-1671)
 constexpr int kMeshTile = 128;  // triangles staged in LDS per round
 struct MeshDev {
-    const double *tri;     // [nt][9] vertex positions a, b, c
-    const uint32_t *tv;    // [nt][3] vertex ids
-    const uint32_t *te;    // [nt][3] edge ids of (a b), (b c), (c a)
-    const double *fn;      // [nt][3] unit face normals
+    const double *tri;     // [nt][9] vertex positions a, b, c, in tile order
+    const uint32_t *orig;  // [nt] mesh index of each triangle in tile order
+    const double *box;     // [tiles][6] bounding box (min, max) of each tile's vertices
+    const uint32_t *tv;    // [nt][3] vertex ids (mesh order)
+    const uint32_t *te;    // [nt][3] edge ids of (a b), (b c), (c a) (mesh order)
+    const double *fn;      // [nt][3] unit face normals (mesh order)
     const double *vn;      // [nv][3] vertex pseudo-normals
     const double *en;      // [ne][3] edge pseudo-normals
     uint32_t nt;
+    double slack;          // absolute slack of the tile-skip test (rounding of distances)
 };
 __global__ void k_mesh_sdf(MeshDev M, VmGrid g, double *values);
 __global__ void k_sdf_cells(VmGrid g, const double *values, double prune, uint8_t *present);

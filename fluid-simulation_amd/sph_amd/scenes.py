@@ -609,6 +609,43 @@ def contact_stack(seed=3, dx=0.02, depth=0.002, tilt=0.05, sphere=True,
     return Scene(f"contact_stack_{solver}", p, x, v, m, fid, bounds)
 
 
+def contact_pile(cubes=(2, 3), dx=0.01, depth=0.002, seed=5, solver="IISPH") -> Scene:
+    """Rigid-rigid contact: cubes[0] x cubes[1] dynamic cubes (edge 0.1 m,
+    density 600, seeded tilts up to 0.05 rad) on a 0.15 m grid, each sinking
+    about ``depth`` into a static slab that covers the floor of a box
+    container (height 0.5 m); a 6 x 6 x 4 fluid block falls from a corner
+    above the cubes. Two static boundaries and six cubes fill the eight
+    boundary slots of a system. IISPH; CFL dt."""
+    rng = np.random.default_rng(seed)
+    sx, sy = cubes[0] * 0.15 + 0.05, cubes[1] * 0.15 + 0.05
+    top = 0.05
+    pos = lattice_block([0.01, 0.01, 0.3], [6, 6, 4], dx, rng, 1e-4)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    integ = "TakePredicted" if solver == "IISPHwOST" else "EulerCromer"
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver=solver, integrator=integ)
+    lo_b, hi_b = np.zeros(3), np.array([sx, sy, 0.5])
+    sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
+    bounds = [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))]
+    lo_s, hi_s = np.zeros(3), np.array([sx, sy, top])
+    s_s, v_s = volume_map_fields(lambda q: sd_box_solid(q, lo_s, hi_s), lo_s, hi_s, h, dx)
+    bounds.append(VolumeMap(s_s, v_s, None, "slab", aabb_of(lo_s, hi_s)))
+    half = 0.05
+    lo_c, hi_c = np.full(3, -half), np.full(3, half)
+    s_c, v_c = volume_map_fields(lambda q: sd_box_solid(q, lo_c, hi_c), lo_c, hi_c, h, dx)
+    mass = 600.0 * (2 * half) ** 3
+    for i in range(cubes[0]):
+        for j in range(cubes[1]):
+            tilt = rng.uniform(-0.05, 0.05)
+            drop = half * (math.cos(tilt) + abs(math.sin(tilt)))
+            com = np.array([0.1 + 0.15 * i, 0.1 + 0.15 * j, top + drop - depth])
+            rb = Rigid(mass=mass, inertia_body=np.eye(3) * (mass * (2 * half) ** 2 / 6.0),
+                       com=com, q=_rot_x(tilt))
+            bounds.append(VolumeMap(s_c, v_c, rb, "cube", aabb_of(lo_c, hi_c)))
+    return Scene(f"contact_pile_{cubes[0] * cubes[1]}", p, x, v, m, fid, bounds)
+
+
 def halfspace_contact(depth, dx=0.02, half=0.1, offset=(0.013, -0.007), dt=1e-3,
                       floor_half=0.3) -> Scene:
     """Planar rigid-rigid contact: a static floor (solid below z = 0.5, AABB
