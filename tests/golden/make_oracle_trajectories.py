@@ -1,0 +1,97 @@
+"""Writes tests/golden/oracle_trajectories.json: seeded scenes stepped by the
+oracle (oracle/refcpu.c), recorded as SHA-256 digests of the exact binary64
+bits of the fluid state in the reference's particle order, with the solver
+iteration counts, per-iteration density errors and measured density errors,
+and a digest of each scene's generator inputs.
+
+Run from the repository root after building the oracle:
+    python tests/golden/make_oracle_trajectories.py
+The cases follow the survey's golden-fixture plan: config 1 (SESPH dam
+break, N = 8000) at steps 0 and 10 with the neighbor CSR of its initial
+positions, a small IISPH scene with its per-iteration errors, an IISPHwOST
+scene and a two-way coupled rigid scene.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "fluid-simulation_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from sph_amd import scenes  # noqa: E402
+
+OUT = os.path.join(HERE, "oracle_trajectories.json")
+
+CASES = {
+    "sesph_dam_break_8000": (lambda: scenes.dam_break_sesph(side=20), [0, 10]),
+    "iisph_box_2197": (lambda: scenes.small_iisph_box(side=13, jitter=2e-3), [0, 3, 6]),
+    "iisphwost_box_1728": (lambda: scenes.small_iisph_box(side=12, solver="IISPHwOST",
+                                                          jitter=2e-3), [0, 3, 6]),
+    "rigid_cube": (lambda: scenes.small_rigid_scene(), [0, 4]),
+}
+
+
+def digest(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        a = np.ascontiguousarray(a)
+        h.update(str(a.dtype).encode())
+        h.update(str(a.shape).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def inputs_digest(sc) -> str:
+    parts = [sc.x, sc.v, sc.m, sc.fluid_id,
+             np.frombuffer(repr(sorted(vars(sc.params).items())).encode(), dtype=np.uint8)]
+    for b in sc.boundaries:
+        if hasattr(b, "positions"):
+            parts.append(b.positions)
+        else:
+            for lat in (b.sdf, b.vm):
+                parts += [lat.x_min, lat.n, np.array([lat.dx]), lat.values, lat.present]
+        if getattr(b, "rigid", None) is not None:
+            parts.append(b.rigid.packed())
+    return digest(*parts)
+
+
+def record(state, m, hist):
+    return {
+        "x": digest(state["x"]), "v": digest(state["v"]), "a": digest(state["a"]),
+        "V": digest(state["V"]), "p": digest(state["p"]),
+        "solver_iterations": int(m["solver_iterations"]),
+        "density_error": float(m["density_error"]),
+        "error_history": [float(e) for e in hist],
+        "p_head": [float(v) for v in state["p"][:4]],
+    }
+
+
+def main():
+    import refcpu
+    out = {"_generator": "tests/golden/make_oracle_trajectories.py (oracle/refcpu.c)", "cases": {}}
+    for name, (make, steps) in CASES.items():
+        sc = make()
+        o = refcpu.OracleSystem(sc)
+        case = {"inputs_sha256": inputs_digest(sc), "particles": sc.n, "steps": {}}
+        off, idx = refcpu.find_samples(sc.params.kernel_support_radius,
+                                       sc.params.kernel_support_radius, sc.x, sc.x)
+        case["neighbors_step0"] = {"offsets": digest(off), "indices": digest(idx),
+                                   "pairs": int(len(idx))}
+        done = 0
+        for s in steps:
+            o.step(s - done)
+            done = s
+            case["steps"][str(s)] = record(o.state(), o.measure(), o.error_history())
+        out["cases"][name] = case
+        print(name, sc.n, "particles", flush=True)
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

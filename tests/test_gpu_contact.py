@@ -85,3 +85,25 @@ def test_contact_off_leaves_bodies_alone(native):
     o.step(2)
     assert g.contact_stats() == (0, 0)
     compare(g, o, sc)
+
+
+def test_contact_in_a_world_matches_solo(native):
+    """Two contact scenes stepped together in one world equal each stepped
+    alone, bit for bit (contact runs per scene before its bodies step)."""
+    scs = [scenes.contact_stack(), scenes.contact_stack(seed=4, tilt=-0.03)]
+    solo = [native.System(s) for s in scs]
+    world = native.World([native.System(s) for s in scs])
+    for s in solo:
+        s.step_forward_in_time(3)
+    world.step(3)
+    for k, s in enumerate(solo):
+        w = world.members[k]
+        assert w.contact_stats() == s.contact_stats()
+        np.testing.assert_array_equal(bits(w.contact_cells()), bits(s.contact_cells()))
+        for b, bd in enumerate(scs[k].boundaries):
+            if bd.rigid is not None:
+                np.testing.assert_array_equal(bits(w.body_state(b)["com"]),
+                                              bits(s.body_state(b)["com"]))
+        ws, ss = w.state(), s.state()
+        for f in ("x", "v", "p"):
+            np.testing.assert_array_equal(bits(ws[f]), bits(ss[f]), err_msg=f)
