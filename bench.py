@@ -89,6 +89,10 @@ def build_scene(workload, side):
         return scenes.two_fluid_ball(side=side, solver="IISPH")
     if workload == "block_drop":
         return scenes.block_drop_iisph()
+    if workload == "rigid_coupling":
+        return scenes.rigid_coupling()
+    if workload == "dam_break":
+        return scenes.dam_break_sesph(side=20)
     raise SystemExit(f"unknown workload {workload}")
 
 
@@ -649,7 +653,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(dom),
+            "traffic": (traffic_from_profiles(dom)
+                        if args.workload == "two_fluid_ball" and args.side == 100 else None),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_s * 1e3,
         },
