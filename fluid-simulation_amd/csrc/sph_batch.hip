@@ -228,18 +228,40 @@ __device__ __forceinline__ uint32_t local_key(const GridDev &g, int cx, int cy, 
            (uint32_t)(cz - g.org[2]);
 }
 
+// ctr[key] += (active lanes holding key), one atomic per distinct key of the
+// wavefront; returns the counter's value before the group's addition plus
+// the lane's rank among the group's lanes. Particles arrive in the previous
+// step's cell order, so a wavefront holds few distinct cells.
+__device__ __forceinline__ uint32_t wave_group_add(uint32_t *ctr, uint32_t key) {
+    uint64_t todo = __ballot(1);
+    const int lane = __lane_id();
+    uint32_t mine = 0;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const uint64_t grp = __ballot(key == lk);
+        uint32_t old = 0;
+        if (lane == leader) old = atomicAdd(&ctr[lk], (uint32_t)__popcll(grp));
+        old = __shfl(old, leader);
+        if (key == lk) mine = old + (uint32_t)__popcll(grp & __lanemask_lt());
+        todo &= ~grp;
+    }
+    return mine;
+}
+
 __global__ void kb_cell_count(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
     uint32_t k = v.cell_base + local_key(v.grid, gm(B.ci)[3 * gi], gm(B.ci)[3 * gi + 1], gm(B.ci)[3 * gi + 2]);
-    atomicAdd(&B.cell_cnt[k], 1u);
+    (void)wave_group_add(B.cell_cnt, k);
 }
 
+// any order within a cell: kb_cell_order sorts each cell by reference index
 __global__ void kb_cell_scatter(BatchDev B) {
     PARTICLE_THREAD(B);
     if (!live) return;
     uint32_t k = v.cell_base + local_key(v.grid, gm(B.ci)[3 * gi], gm(B.ci)[3 * gi + 1], gm(B.ci)[3 * gi + 2]);
-    uint32_t slot = gm(B.cell_start)[k] + atomicAdd(&B.cursor[k], 1u);  // batch position
+    uint32_t slot = gm(B.cell_start)[k] + wave_group_add(B.cursor, k);  // batch position
     gm(B.perm)[slot] = (uint32_t)i;                                      // scene-local source
 }
 

@@ -525,8 +525,19 @@ struct Batch {
     DBuf<int> ci, ipart;
     DBuf<uint32_t> perm, cnt, slice_len, slice_off, ell_j, cell_cnt, cell_start, cursor;
     DBuf<uint32_t> pblk, cblk, scan_a, scan_b, scan_c;
-    DBuf<double> ell_gx, ell_gy, ell_gz, partials, red, chunk_err;
-    DBuf<uint32_t> d_active, d_iters, chunk_ran, ablk, amap, max_cnt;
+    DBuf<double> ell_gx, ell_gy, ell_gz, partials, red;
+    DBuf<uint32_t> ablk, amap, max_cnt;
+    // pressure-solve state in one allocation, so that a chunk's results come
+    // back in one copy: chunk errors [kMaxChunk][ns] (f64), then as u32 the
+    // chunk's ran flags [kMaxChunk][ns], the active flags [ns] and iteration
+    // counts [ns]
+    DBuf<double> solve_io;
+    double *chunk_err() const { return solve_io.p; }
+    uint32_t *chunk_ran() const {
+        return reinterpret_cast<uint32_t *>(solve_io.p + (size_t)kMaxChunk * sc.size());
+    }
+    uint32_t *d_active() const { return chunk_ran() + (size_t)kMaxChunk * sc.size(); }
+    uint32_t *d_iters() const { return d_active() + sc.size(); }
     uint32_t iter_hint = 3;  // iterations of the last target-terminated solve
     // Jacobi chunks captured as HIP graphs, keyed by their launch arguments (on
     // by default for a context, off for a world, whose layout changes with
@@ -1008,10 +1019,10 @@ static BatchDev batch_dev(Batch &B) {
     d.cursor = B.cursor.p;
     d.bodies = B.bodies.p;
     d.partials = B.partials.p;
-    d.active = B.d_active.p;
-    d.iters = B.d_iters.p;
-    d.chunk_err = B.chunk_err.p;
-    d.chunk_ran = B.chunk_ran.p;
+    d.active = B.d_active();
+    d.iters = B.d_iters();
+    d.chunk_err = B.chunk_err();
+    d.chunk_ran = B.chunk_ran();
     return d;
 }
 
@@ -1048,10 +1059,7 @@ static void layout(Batch &B) {
     B.partials.reserve((blocks + 1) * kMaxBodiesPerScene * 6 + 64);
     B.red.reserve(std::max<size_t>(ns, 1) * kMaxBodiesPerScene * 6 + 64);
     B.ensure_pinned(std::max<size_t>(ns, 1) * (kMaxBodiesPerScene * 6 + 2 * kMaxChunk + 2) + 64);
-    B.d_active.reserve(std::max<size_t>(ns, 1));
-    B.d_iters.reserve(std::max<size_t>(ns, 1));
-    B.chunk_err.reserve(std::max<size_t>(ns, 1) * kMaxChunk);
-    B.chunk_ran.reserve(std::max<size_t>(ns, 1) * kMaxChunk);
+    B.solve_io.reserve(std::max<size_t>(ns, 1) * (kMaxChunk + (kMaxChunk + 2) / 2 + 1));
     upload_bodies(B);
     upload_views(B);
 }
@@ -1416,8 +1424,8 @@ static void resolve_pressure(Batch &B, int vp, int after) {
         act[s] = c.active ? 1u : 0u;
         any = any || c.active;
     }
-    B.put(B.d_active.p, act.data(), ns * sizeof(uint32_t));
-    B.put(B.d_iters.p, zero.data(), ns * sizeof(uint32_t));
+    act.insert(act.end(), zero.begin(), zero.end());  // active flags, then iteration counts
+    B.put(B.d_active(), act.data(), 2 * ns * sizeof(uint32_t));
     int chunk = after >= 0 ? after + 1 : std::max<int>(3, (int)B.iter_hint);
     uint32_t most = 0;
     std::vector<uint32_t> ablk, amap;
@@ -1442,15 +1450,14 @@ static void resolve_pressure(Batch &B, int vp, int after) {
             da.ns = (int)amap.size();
         }
         jacobi_chunk(B, da, ablk.back(), after, chunk);
+        // errors, ran flags and active flags in one copy, in the device layout
         double *errs = B.pinned;
-        uint32_t *ran = reinterpret_cast<uint32_t *>(errs + (size_t)chunk * ns);
-        uint32_t *alive = ran + (size_t)chunk * ns;
-        HIPCK(hipMemcpyAsync(errs, B.chunk_err.p, (size_t)chunk * ns * sizeof(double),
+        uint32_t *ran = reinterpret_cast<uint32_t *>(errs + (size_t)kMaxChunk * ns);
+        uint32_t *alive = ran + (size_t)kMaxChunk * ns;
+        HIPCK(hipMemcpyAsync(errs, B.solve_io.p,
+                             (size_t)kMaxChunk * ns * (sizeof(double) + sizeof(uint32_t)) +
+                                 ns * sizeof(uint32_t),
                              hipMemcpyDeviceToHost, B.s));
-        HIPCK(hipMemcpyAsync(ran, B.chunk_ran.p, (size_t)chunk * ns * sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, B.s));
-        HIPCK(hipMemcpyAsync(alive, B.d_active.p, ns * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                             B.s));
         B.sync();
         if (B.prof.on) B.prof.drain();
         any = false;
