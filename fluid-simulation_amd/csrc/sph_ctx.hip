@@ -2051,6 +2051,7 @@ static void contact_step(Batch &B, Scene &S, double dt) {
         if (nc <= cap) break;
         cap = nc;
     }
+    if (nc > cap) throw Error(SPH_ERR_HIP, "contact cell count changed between passes");
     if (nc == 0) return;
     std::vector<uint32_t> idx(nc);
     std::vector<double> sums(nc * kContactSums);
