@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -633,6 +634,24 @@ struct sph_ctx {
 struct sph_world {
     Batch batch;
     std::vector<sph_ctx *> members;
+};
+
+// continuous batching: FIFO admission under slots and a byte budget, admitted
+// systems resident in one world, finished systems queued for the caller
+struct sph_server {
+    struct Req {
+        uint64_t ticket, bytes, steps, done;
+        sph_ctx *ctx;
+    };
+    uint64_t budget = 0;
+    int max_slots = 0;
+    uint64_t next_ticket = 1, used = 0, world_steps = 0;
+    int slots = 0;
+    std::deque<Req> queued;          // submitted, in submission order
+    std::vector<Req> admitted;       // admitted, context not yet attached
+    std::vector<Req> resident;       // attached to the world
+    std::deque<Req> finished;        // out of the world, not yet polled
+    sph_world *world = nullptr;
 };
 
 namespace sph {
@@ -2930,6 +2949,127 @@ int sph_world_step(sph_world *w, int nsteps) {
         if (B.sc.empty()) return;
         for (int k = 0; k < nsteps; k++) step_batch(B);
         B.sync();
+    });
+}
+
+// ---- serving -------------------------------------------------------------------
+static void server_stats(const sph_server *s, sph_server_stats *o) {
+    o->queued = s->queued.size();
+    o->admitted = (uint64_t)s->slots;
+    o->resident = s->resident.size();
+    o->finished = s->finished.size();
+    o->bytes = s->used;
+    o->world_steps = s->world_steps;
+}
+
+int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out) {
+    return guarded([&] {
+        if (!out || max_slots < 1) throw Error(SPH_ERR_ARG, "need an output and at least one slot");
+        std::unique_ptr<sph_server> s(new sph_server());
+        s->budget = budget_bytes;
+        s->max_slots = max_slots;
+        *out = s.release();
+    });
+}
+
+int sph_server_destroy(sph_server *s) {
+    if (!s) return SPH_OK;
+    return guarded([&] {
+        if (s->world) (void)sph_world_destroy(s->world);
+        delete s;
+    });
+}
+
+int sph_server_submit(sph_server *s, uint64_t device_bytes, uint64_t steps, uint64_t *ticket) {
+    return guarded([&] {
+        if (!s || !ticket) throw Error(SPH_ERR_ARG, "null argument");
+        if (steps == 0) throw Error(SPH_ERR_ARG, "a request needs at least one step");
+        if (device_bytes > s->budget)
+            throw Error(SPH_ERR_ARG, "request needs more device memory than the whole budget");
+        s->queued.push_back({s->next_ticket, device_bytes, steps, 0, nullptr});
+        *ticket = s->next_ticket++;
+    });
+}
+
+int sph_server_admit(sph_server *s, uint64_t *ticket) {
+    int got = 0;
+    int st = guarded([&] {
+        if (!s || !ticket) throw Error(SPH_ERR_ARG, "null argument");
+        if (s->queued.empty()) return;
+        const sph_server::Req &r = s->queued.front();
+        if (s->slots >= s->max_slots || s->used + r.bytes > s->budget) return;
+        s->admitted.push_back(r);
+        s->queued.pop_front();
+        s->slots++;
+        s->used += r.bytes;
+        *ticket = s->admitted.back().ticket;
+        got = 1;
+    });
+    return st ? st : got;
+}
+
+int sph_server_attach(sph_server *s, uint64_t ticket, sph_ctx *ctx) {
+    return guarded([&] {
+        if (!s || !ctx) throw Error(SPH_ERR_ARG, "null argument");
+        auto it = std::find_if(s->admitted.begin(), s->admitted.end(),
+                               [&](const sph_server::Req &r) { return r.ticket == ticket; });
+        if (it == s->admitted.end()) throw Error(SPH_ERR_STATE, "ticket is not awaiting a system");
+        if (!s->world) {
+            int e = sph_world_create(&s->world);
+            if (e) throw Error(e, sph_last_error());
+        }
+        int e = sph_world_attach(s->world, ctx);
+        if (e) throw Error(e, sph_last_error());
+        sph_server::Req r = *it;
+        r.ctx = ctx;
+        s->admitted.erase(it);
+        s->resident.push_back(r);
+    });
+}
+
+int sph_server_step(sph_server *s, sph_server_stats *out) {
+    return guarded([&] {
+        if (!s) throw Error(SPH_ERR_ARG, "null argument");
+        if (!s->resident.empty()) {
+            int e = sph_world_step(s->world, 1);
+            if (e) throw Error(e, sph_last_error());
+            s->world_steps++;
+            std::vector<sph_server::Req> keep;
+            for (sph_server::Req &r : s->resident) {
+                if (++r.done < r.steps) {
+                    keep.push_back(r);
+                    continue;
+                }
+                e = sph_world_detach(s->world, r.ctx);
+                if (e) throw Error(e, sph_last_error());
+                s->finished.push_back(r);
+            }
+            s->resident.swap(keep);
+        }
+        if (out) server_stats(s, out);
+    });
+}
+
+int sph_server_poll(sph_server *s, uint64_t *ticket, sph_ctx **ctx) {
+    int got = 0;
+    int st = guarded([&] {
+        if (!s || !ticket || !ctx) throw Error(SPH_ERR_ARG, "null argument");
+        if (s->finished.empty()) return;
+        const sph_server::Req r = s->finished.front();
+        s->finished.pop_front();
+        s->slots--;
+        s->used -= r.bytes;
+        *ticket = r.ticket;
+        *ctx = r.ctx;
+        got = 1;
+    });
+    return st ? st : got;
+}
+
+int sph_server_read_stats(const sph_server *s, sph_server_stats *out) {
+    return guarded([&] {
+        if (!s || !out) throw Error(SPH_ERR_ARG, "null argument");
+        server_stats(s, out);
     });
 }
 

@@ -62,6 +62,14 @@ class SphRigidInit(C.Structure):
                 ("q", C.c_double * 4), ("lin_vel", C.c_double * 3), ("ang_vel", C.c_double * 3)]
 
 
+class SphServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("queued", "admitted", "resident", "finished",
+                                           "bytes", "world_steps")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class SphMeasurement(C.Structure):
     _fields_ = [("time", C.c_double), ("time_step_size", C.c_double), ("density", C.c_double),
                 ("density_error", C.c_double), ("kinetic_energy", C.c_double),
@@ -116,6 +124,14 @@ SIGNATURES = [
     ("sph_world_size", C.c_int, [C.c_void_p, _szp, _szp]),
     ("sph_world_profile", C.c_int, [C.c_void_p, C.c_int]),
     ("sph_world_profile_read", C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p, _dp]),
+    ("sph_server_create", C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    ("sph_server_destroy", C.c_int, [C.c_void_p]),
+    ("sph_server_submit", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, _u64p]),
+    ("sph_server_admit", C.c_int, [C.c_void_p, _u64p]),
+    ("sph_server_attach", C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("sph_server_step", C.c_int, [C.c_void_p, C.POINTER(SphServerStats)]),
+    ("sph_server_poll", C.c_int, [C.c_void_p, _u64p, C.POINTER(C.c_void_p)]),
+    ("sph_server_read_stats", C.c_int, [C.c_void_p, C.POINTER(SphServerStats)]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
     ("sph_mesh_sdf_build", C.c_int, [C.c_void_p, _dp, C.c_size_t, _u32p, C.c_size_t, _dp,
                                      _i64p, C.c_double, C.c_double, _dp, _u8p]),

@@ -141,30 +141,6 @@ def estimate_device_bytes(scene: S.Scene) -> int:
     return int(b)
 
 
-class Admission:
-    """Slots plus an HBM budget in bytes."""
-
-    def __init__(self, budget_bytes: int, max_slots: int):
-        self.budget = int(budget_bytes)
-        self.max_slots = int(max_slots)
-        self.used = 0
-        self.slots = 0
-
-    def fits(self, nbytes: int) -> bool:
-        return self.slots < self.max_slots and self.used + nbytes <= self.budget
-
-    def take(self, nbytes: int):
-        if not self.fits(nbytes):
-            raise RuntimeError("admission over budget")
-        self.used += nbytes
-        self.slots += 1
-
-    def give(self, nbytes: int):
-        self.used -= nbytes
-        self.slots -= 1
-        assert self.used >= 0 and self.slots >= 0
-
-
 def _digest(state: dict) -> str:
     h = hashlib.sha256()
     h.update(np.ascontiguousarray(state["x"]).tobytes())
@@ -178,78 +154,77 @@ def _finish(req: Request, sim, admitted: float, now: float, keep_state: bool) ->
                   sim.take_measurement(), _digest(st), st if keep_state else None)
 
 
-def _check_fits(req: Request, adm: Admission):
+def _check_fits(req: Request, budget: int):
     need = estimate_device_bytes(req.scene)
-    if need > adm.budget:
+    if need > budget:
         raise ValueError(f"request {req.rid} needs {need} B, more than the whole budget")
     return need
 
 
 class BatchedServer:
-    """Continuous batching: admitted scenes are stepped together in a World."""
+    """Continuous batching: the library's server (sph_server_*) admits requests
+    in arrival order under slots and the HBM budget and steps the admitted
+    scenes together in one world; this loop feeds it arrivals, builds each
+    admitted request's System and collects finished ones."""
 
     def __init__(self, hbm_budget_bytes: int, max_slots: int = 64,
                  clock: Callable[[], float] = time.perf_counter, keep_state: bool = False):
         from . import system  # loads the HIP library
         self._system = system
-        self.adm = Admission(hbm_budget_bytes, max_slots)
+        self.budget = int(hbm_budget_bytes)
+        self.max_slots = int(max_slots)
         self.clock = clock
         self.keep_state = keep_state
 
     def run(self, stream: List[Request]) -> Report:
         system = self._system
         pending = deque(sorted(stream, key=lambda r: r.arrival))
-        queue: deque = deque()
-        active = []   # [req, sim, steps_done, bytes, admitted_at]
+        waiting: Dict[int, Request] = {}     # ticket -> submitted request
+        admitted: Dict[int, list] = {}       # ticket -> [request, admitted at]
         results: Dict[int, Result] = {}
         step_ms: List[float] = []
-        world = system.World()
+        srv = system.Server(self.budget, self.max_slots)
         busy = admit = 0.0
         peak_n = peak_b = 0
         t0 = self.clock()
         try:
-            while pending or queue or active:
+            while pending or waiting or admitted:
                 now = self.clock() - t0
                 while pending and pending[0].arrival <= now:
-                    queue.append(pending.popleft())
-                # admit in arrival order while a slot and the budget allow
-                while queue:
-                    need = _check_fits(queue[0], self.adm)
-                    if not self.adm.fits(need):
+                    req = pending.popleft()
+                    waiting[srv.submit(_check_fits(req, self.budget), req.steps)] = req
+                while True:
+                    t = srv.admit()
+                    if t is None:
                         break
-                    req = queue.popleft()
+                    req = waiting.pop(t)
                     a0 = self.clock()
-                    sim = system.System(req.scene)
-                    world.attach(sim)
+                    srv.attach(t, system.System(req.scene))
                     admit += self.clock() - a0
-                    self.adm.take(need)
-                    active.append([req, sim, 0, need, self.clock() - t0])
-                peak_n = max(peak_n, len(active))
-                peak_b = max(peak_b, self.adm.used)
-                if not active:
+                    admitted[t] = [req, self.clock() - t0]
+                st = srv.stats()
+                peak_n = max(peak_n, st["resident"])
+                peak_b = max(peak_b, st["bytes"])
+                if st["resident"] == 0:
                     if pending:  # idle until the next arrival
                         time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
                     continue
                 s0 = self.clock()
-                world.step(1)
+                srv.step()
                 dt = self.clock() - s0
                 busy += dt
-                step_ms.extend([dt * 1e3] * len(active))
-                done = []
-                for a in active:
-                    a[2] += 1
-                    if a[2] >= a[0].steps:
-                        done.append(a)
-                for a in done:
-                    req, sim, _, need, t_adm = a
-                    world.detach(sim)
+                step_ms.extend([dt * 1e3] * st["resident"])
+                while True:
+                    got = srv.poll()
+                    if got is None:
+                        break
+                    t, sim = got
+                    req, t_adm = admitted.pop(t)
                     results[req.rid] = _finish(req, sim, t_adm, self.clock() - t0,
                                                self.keep_state)
                     sim._ctx.close()
-                    self.adm.give(need)
-                    active.remove(a)
         finally:
-            world.close()
+            srv.close()
         makespan = max(r.finished for r in results.values()) - min(r.arrival for r in stream)
         return Report("batched", results, step_ms, makespan, busy, admit, peak_n, peak_b)
 

@@ -565,6 +565,66 @@ def build_volume_map(sdf, h: float, x_min, n, dx: float, gl_x, gl_w, system=None
     return vals, pres
 
 
+class Server:
+    """Continuous batching in the library (sph_server_*): FIFO admission under
+    slots and a device-memory budget, admitted systems stepped together in one
+    world, finished systems handed back. Admission and its bookkeeping touch
+    no device; a request's System is built after it is admitted."""
+
+    def __init__(self, budget_bytes: int, max_slots: int):
+        h = C.c_void_p()
+        N.check(N.lib().sph_server_create(int(budget_bytes), int(max_slots), C.byref(h)))
+        self.h = h
+        self._sims = {}
+
+    def submit(self, device_bytes: int, steps: int) -> int:
+        t = C.c_uint64()
+        N.check(N.lib().sph_server_submit(self.h, int(device_bytes), int(steps), C.byref(t)))
+        return int(t.value)
+
+    def admit(self):
+        """The next admitted ticket, or None while the queue's head does not fit."""
+        t = C.c_uint64()
+        st = N.lib().sph_server_admit(self.h, C.byref(t))
+        N.check(st if st < 0 else 0)
+        return int(t.value) if st == 1 else None
+
+    def attach(self, ticket: int, sim: "System"):
+        N.check(N.lib().sph_server_attach(self.h, int(ticket), sim.handle))
+        self._sims[int(ticket)] = sim
+
+    def step(self) -> dict:
+        out = N.SphServerStats()
+        N.check(N.lib().sph_server_step(self.h, C.byref(out)))
+        return out.as_dict()
+
+    def poll(self):
+        """(ticket, System) of the oldest finished request, or None."""
+        t, c = C.c_uint64(), C.c_void_p()
+        st = N.lib().sph_server_poll(self.h, C.byref(t), C.byref(c))
+        N.check(st if st < 0 else 0)
+        if st != 1:
+            return None
+        sim = self._sims.pop(int(t.value), None)
+        return int(t.value), sim
+
+    def stats(self) -> dict:
+        out = N.SphServerStats()
+        N.check(N.lib().sph_server_read_stats(self.h, C.byref(out)))
+        return out.as_dict()
+
+    def close(self):
+        if self.h:
+            N.lib().sph_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def build_mesh_sdf(vertices, triangles, x_min, n, dx: float, prune: float, system=None):
     """Signed-distance node values and present flags on the lattice (x_min, n,
     dx) from a triangle mesh (``DiscretizedBoundaryFields::new``,

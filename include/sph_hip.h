@@ -285,6 +285,48 @@ int sph_world_size(const sph_world *world, size_t *systems, size_t *particles);
 int sph_world_profile(sph_world *world, int enable);
 int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
                            uint64_t *units, double *ms);
+/* ---- serving: continuous batching over a world ----------------------------------
+ * No counterpart in simulation_lib, whose worker runs one system per job
+ * (simulation_worker/src/lib.rs). A server admits requests in submission
+ * order while a slot and the HBM budget allow, keeps admitted systems resident
+ * in one world and steps them together, and hands back each system once it has
+ * taken its requested steps.
+ * Protocol, per scheduling round: sph_server_submit for arrived requests
+ * (a device-memory estimate and a step count; nothing is allocated);
+ * sph_server_admit while it returns 1, building each admitted request's
+ * context only then (so allocations follow admission) and handing it over
+ * with sph_server_attach; sph_server_step (one world step, then finished
+ * systems leave the world); sph_server_poll while it returns 1, which returns
+ * a finished context to the caller and releases its slot and budget.
+ * Creating a server and the submit/admit bookkeeping touch no device; the
+ * world is created at the first attach. One thread at a time. */
+typedef struct sph_server sph_server;
+typedef struct sph_server_stats {
+    uint64_t queued;          /* submitted, not yet admitted */
+    uint64_t admitted;        /* admitted and not yet polled (slots in use) */
+    uint64_t resident;        /* attached to the world */
+    uint64_t finished;        /* finished, waiting for sph_server_poll */
+    uint64_t bytes;           /* estimates of the admitted requests */
+    uint64_t world_steps;     /* world steps taken */
+} sph_server_stats;
+int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out);
+/* frees the world; contexts still attached are detached, not destroyed */
+int sph_server_destroy(sph_server *server);
+/* SPH_ERR_ARG if device_bytes exceeds the whole budget or steps is 0 */
+int sph_server_submit(sph_server *server, uint64_t device_bytes, uint64_t steps,
+                      uint64_t *ticket);
+/* returns 1 and the oldest queued ticket if it fits (taking its slot and
+ * bytes), 0 if the queue is empty or its head does not fit */
+int sph_server_admit(sph_server *server, uint64_t *ticket);
+/* ctx (after sph_system_init) serves an admitted ticket; it joins the world */
+int sph_server_attach(sph_server *server, uint64_t ticket, sph_ctx *ctx);
+/* one world step if any system is resident; systems that have taken their
+ * steps leave the world for the finished queue. *out may be NULL. */
+int sph_server_step(sph_server *server, sph_server_stats *out);
+/* returns 1 with the oldest finished ticket and its context (detached, now
+ * the caller's), releasing its slot and bytes; 0 if none has finished */
+int sph_server_poll(sph_server *server, uint64_t *ticket, sph_ctx **ctx);
+int sph_server_read_stats(const sph_server *server, sph_server_stats *out);
 /* ---- sensor queries -------------------------------------------------------------
  * SPHSystem::get_quantity_at_positions (sph/mod.rs:241-341): a scalar field
  * of the current fluid state at nq positions ([nq][3] f32, widened to f64),

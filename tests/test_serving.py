@@ -65,20 +65,27 @@ def test_device_estimate_grows_with_particles():
     assert eb - es >= (big.n - small.n) * serving.BYTES_PER_PARTICLE
 
 
-def test_admission_slots_and_budget():
-    adm = serving.Admission(budget_bytes=1000, max_slots=2)
-    assert adm.fits(600)
-    adm.take(600)
-    assert not adm.fits(500)      # over budget
-    assert adm.fits(400)
-    adm.take(400)
-    assert not adm.fits(0)        # out of slots
+def test_native_admission_slots_budget_and_order():
+    """The library's server admits in submission order under slots and the
+    byte budget (no device is touched before a system is attached)."""
+    from sph_amd import system
+    srv = system.Server(budget_bytes=1000, max_slots=2)
     with pytest.raises(RuntimeError):
-        adm.take(0)
-    adm.give(600)
-    assert adm.fits(600) and not adm.fits(601)
-    adm.give(400)
-    assert adm.used == 0 and adm.slots == 0
+        srv.submit(1001, 5)            # never admissible
+    with pytest.raises(RuntimeError):
+        srv.submit(10, 0)              # no steps
+    a, b, c = srv.submit(600, 5), srv.submit(500, 5), srv.submit(100, 5)
+    assert srv.admit() == a
+    assert srv.admit() is None         # b does not fit beside a; c waits behind b
+    st = srv.stats()
+    assert (st["queued"], st["admitted"], st["bytes"]) == (2, 1, 600)
+    assert srv.poll() is None and srv.step()["world_steps"] == 0   # nothing resident
+    srv.close()
+    srv = system.Server(budget_bytes=1000, max_slots=2)
+    t = [srv.submit(100, 1) for _ in range(3)]
+    assert [srv.admit(), srv.admit(), srv.admit()] == [t[0], t[1], None]   # out of slots
+    assert srv.stats()["admitted"] == 2
+    srv.close()
 
 
 def test_report_summary_percentiles():
