@@ -6,10 +6,10 @@ and a digest of each scene's generator inputs.
 
 Run from the repository root after building the oracle:
     python tests/golden/make_oracle_trajectories.py
-The cases follow the survey's golden-fixture plan: config 1 (SESPH dam
-break, N = 8000) at steps 0 and 10 with the neighbor CSR of its initial
-positions, a small IISPH scene with its per-iteration errors, an IISPHwOST
-scene and a two-way coupled rigid scene.
+The cases: the SESPH dam break (N = 8000, two static-sample layers) at steps
+0 and 10 with the neighbor CSR of its initial positions, a small IISPH scene
+with its per-iteration errors, an IISPHwOST scene and a two-way coupled
+rigid scene.
 """
 import hashlib
 import json
