@@ -38,8 +38,12 @@ __device__ __forceinline__ int scene_of(const uint32_t *blk, int ns, uint32_t lb
     (void)gi;                                                              \
     (void)live
 
-// pair-cache slot of particle i's first neighbor
-#define ELL_BASE(B, v, i) ((size_t)(B).slice_off[(v).slice_base + ((i) >> 6)] * 64 + ((i)&63))
+// Pair-cache slots: a slice's neighbor rows are stored in pairs, so that
+// neighbors 2r and 2r+1 of a lane are adjacent (one 8-byte load reads both):
+// neighbor k of lane l of slice s sits at 64 slice_off[s] + 128 (k / 2) +
+// 2 l + k % 2. Slices hold an even number of rows.
+#define ELL_BASE(B, v, i) ((size_t)(B).slice_off[(v).slice_base + ((i) >> 6)] * 64 + 2 * ((i)&63))
+#define ELL_AT(e, k) ((e) + 128 * ((size_t)(k) >> 1) + ((k)&1))
 
 // The field pointers of the scene and body tables are read from device
 // memory, so the compiler cannot tell their address space and would emit FLAT
@@ -56,6 +60,7 @@ __device__ __forceinline__ GLOBAL_AS T *gm(T *p) {
 // pa4[i] = {pressure acceleration x, y, z, 0}
 typedef double rec4 __attribute__((ext_vector_type(4)));
 typedef double rec2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ rec4 ld4(const double *a, size_t i) {
     return ((const GLOBAL_AS rec4 *)a)[i];
 }
@@ -82,44 +87,44 @@ __device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &
 
 // The neighbor loop of a pair pass: body(k, r) for k = 0 .. c-1 in order,
 // where r = load(j) gathers the records of slot k's neighbor j (ej points at
-// the particle's first slot; slot k is ej[64 k]). Three forms, all calling
-// body in the same order with the same records:
+// the particle's first slot; slot k is ej[ELL_AT(0, k)]). Three forms, all
+// calling body in the same order with the same records:
 //   PIPE = 0: the plain loop (index, then records, then the arithmetic);
 //   PIPE = 1: software-pipelined: slot k+1's records and slot k+2's index are
 //     in flight while slot k is evaluated;
-//   PIPE = 2: only the next index is prefetched, issued after the current
-//     records so that waiting for the records does not wait for it.
-// Prefetch slots clamp to c-1, so every load stays in bounds.
+//   PIPE = 2: two neighbors per step: one 8-byte load reads both indices and
+//     both records are gathered before either is evaluated (for an odd count
+//     the second index slot is read but not used: the record load repeats
+//     the first neighbor).
+// Prefetch slots clamp to c-1 and a pair's second slot lies inside the
+// slice's even row count, so every load stays in bounds.
 template <int PIPE, int UNROLL, class Load, class Body>
 __device__ __forceinline__ void neighbor_loop(const GLOBAL_AS uint32_t *ej, uint32_t c, Load load,
                                               Body body) {
     if (c == 0) return;
     if (PIPE == 1) {
-        uint32_t j1 = ej[64 * (size_t)min(1u, c - 1)];
+        uint32_t j1 = ej[ELL_AT(0, min(1u, c - 1))];
         auto r0 = load(ej[0]);
 #pragma unroll UNROLL
         for (uint32_t k = 0; k < c; k++) {
-            const uint32_t j2 = ej[64 * (size_t)min(k + 2, c - 1)];
+            const uint32_t j2 = ej[ELL_AT(0, min(k + 2, c - 1))];
             const auto r1 = load(j1);
             body(k, r0);
             j1 = j2;
             r0 = r1;
         }
     } else if (PIPE == 2) {
-        uint32_t jn = ej[0];
-#pragma unroll UNROLL
-        for (uint32_t k = 0; k < c; k++) {
-            // the index load stays after the record loads (scheduling
-            // barriers), so that waiting for the records leaves it in flight
-            const auto r = load(jn);
-            __builtin_amdgcn_sched_barrier(0);
-            jn = ej[64 * (size_t)min(k + 1, c - 1)];
-            __builtin_amdgcn_sched_barrier(0);
-            body(k, r);
+        for (uint32_t k = 0; k < c; k += 2) {
+            const u32x2 jj = *(const GLOBAL_AS u32x2 *)(ej + 128 * (size_t)(k >> 1));
+            const uint32_t j1 = k + 1 < c ? jj.y : jj.x;
+            const auto r0 = load(jj.x);
+            const auto r1 = load(j1);
+            body(k, r0);
+            if (k + 1 < c) body(k + 1, r1);
         }
     } else {
 #pragma unroll UNROLL
-        for (uint32_t k = 0; k < c; k++) body(k, load(ej[64 * (size_t)k]));
+        for (uint32_t k = 0; k < c; k++) body(k, load(ej[ELL_AT(0, k)]));
     }
 }
 
@@ -350,7 +355,7 @@ __global__ void kb_count_ell(BatchDev B) {
     }
     uint32_t mx = cnt;
     for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-    if ((threadIdx.x & 63) == 0 && live) gm(B.slice_len)[v.slice_base + (i >> 6)] = mx;
+    if ((threadIdx.x & 63) == 0 && live) gm(B.slice_len)[v.slice_base + (i >> 6)] = (mx + 1) & ~1u;
 }
 
 __global__ void kb_fill_ell(BatchDev B) {
@@ -361,7 +366,8 @@ __global__ void kb_fill_ell(BatchDev B) {
     const GridDev &g = v.grid;
     const int cx = sat_i32(floor(px / v.cs)), cy = sat_i32(floor(py / v.cs)),
               cz = sat_i32(floor(pz / v.cs));
-    size_t e = ELL_BASE(B, v, i);
+    const size_t e0 = ELL_BASE(B, v, i);
+    uint32_t kk = 0;
     int zlo = cz - v.reach - g.org[2], zhi = cz + v.reach - g.org[2];
     if (zlo < 0) zlo = 0;
     if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
@@ -379,8 +385,8 @@ __global__ void kb_fill_ell(BatchDev B) {
                 double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
                 double d2 = (rx * rx + ry * ry) + rz * rz;
                 if (d2 < v.dthr) {
-                    gm(B.ell_j)[e] = j;
-                    e += 64;
+                    gm(B.ell_j)[ELL_AT(e0, kk)] = j;
+                    kk++;
                 }
             }
         }
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
     uint32_t cnt = 0;
     // slot of lane l's first neighbor; for lanes past the scene's end it is
     // the slot the lane would have, inside the slice's allocation
-    const size_t e0 = ((size_t)(v.slice_base + (i >> 6)) * cap) * 64 + (i & 63);
+    const size_t e0 = ((size_t)(v.slice_base + (i >> 6)) * cap) * 64 + 2 * (i & 63);
     if (live) {
         const KernelConst K = v.K;
         const double rh = refined_rcp(K.h);
@@ -450,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
                             const double d2 = (rx * rx + ry * ry) + rz * rz;
                             if (j < b && d2 < v.dthr) {
                                 if (cnt < kSweepRows) rows[w][cnt][l] = j;
-                                else if (cnt < cap) gm(B.ell_j)[e0 + 64 * (size_t)cnt] = j;
+                                else if (cnt < cap) gm(B.ell_j)[ELL_AT(e0, cnt)] = j;
                                 const double q = div_rcp(sqrt(d2), K.h, rh);
                                 accu += v.V0 * (K.wfac * kernel_w(q));
                                 cnt++;
@@ -488,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
     // write a value no pass reads)
     const uint32_t nrows = min(min(mx, (uint32_t)kSweepRows), cap);
     if (live)
-        for (uint32_t k = 0; k < nrows; k++) gm(B.ell_j)[e0 + 64 * (size_t)k] = rows[w][k][l];
+        for (uint32_t k = 0; k < nrows; k++) gm(B.ell_j)[ELL_AT(e0, k)] = rows[w][k][l];
 }
 
 // slice_off[g] = g * cap: the fixed-width layout in the form ELL_BASE reads
@@ -540,10 +546,11 @@ __global__ PAIR_PASS_BOUNDS void kb_pairs_volume(BatchDev B) {
     const rec4 qi = ld4(v.q4, i);
     const double px = qi.x, py = qi.y, pz = qi.z;
     const uint32_t c = gm(B.cnt)[gi];
-    size_t e = ELL_BASE(B, v, i);
+    const size_t e0 = ELL_BASE(B, v, i);
     double accu = 0.;
 #pragma unroll UNROLL
-    for (uint32_t k = 0; k < c; k++, e += 64) {
+    for (uint32_t k = 0; k < c; k++) {
+        const size_t e = ELL_AT(e0, k);
         uint32_t j = gm(B.ell_j)[e];
         const rec4 qj = ld4(v.q4, j);  // .w is being written by this pass; unused here
         double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
@@ -623,7 +630,7 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
                             dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, rx, ry, rz) /
                             (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
                 double wx, wy, wz;
-                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, rx, ry, rz, wx, wy, wz);
+                pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), rx, ry, rz, wx, wy, wz);
                 acx += cf * wx;
                 acy += cf * wy;
                 acz += cf * wz;
@@ -724,7 +731,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
             [&](uint32_t k, const QP &r) {
                 double cf = vm * r.q.w * (pi + r.p);
                 double wx, wy, wz;
-                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y,
+                pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y,
                                   pz - r.q.z, wx, wy, wz);
                 acx -= cf * wx;
                 acy -= cf * wy;
@@ -824,7 +831,7 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
         gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.sv4, j)}; },
         [&](uint32_t k, const QU &r) {
             double wx, wy, wz;
-            pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y, pz - r.q.z,
+            pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y, pz - r.q.z,
                               wx, wy, wz);
             const double Vj = r.q.w;
             src -= dt * Vj * dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, wx, wy, wz);
@@ -889,7 +896,7 @@ __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
             gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.pa4, j)}; },
             [&](uint32_t k, const QU &r) {
                 double wx, wy, wz;
-                pair_grad<CACHED>(B, K, rh, e0 + 64 * (size_t)k, px - r.q.x, py - r.q.y,
+                pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y,
                                   pz - r.q.z, wx, wy, wz);
                 adp += dt2 * r.q.w * dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, wx, wy, wz);
             });

@@ -60,9 +60,10 @@ struct BatchDev {
     uint32_t *perm;
     uint32_t *cnt;
     // wavefront-sliced ELL pair cache: slice s holds 64 consecutive sorted particles of
-    // one scene, and the k-th neighbor of lane l sits at slot 64 (slice_off[s] + k) + l,
-    // so the k-th accesses of a wavefront are one contiguous run (256 B of indices,
-    // 512 B per gradient component); slice_len[s] = the slice's largest neighbor count
+    // one scene, and the k-th neighbor of lane l sits at slot
+    // 64 slice_off[s] + 128 (k / 2) + 2 l + k % 2 (rows stored in pairs), so the
+    // accesses of neighbors k and k+1 of a wavefront are one contiguous run (512 B of
+    // indices); slice_len[s] = the slice's largest neighbor count rounded up to even
     uint32_t *slice_len;
     const uint32_t *slice_off;
     uint32_t *ell_j;                   // neighbor's scene-local sorted index

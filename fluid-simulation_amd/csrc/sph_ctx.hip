@@ -3120,7 +3120,7 @@ static void set_option(Batch &B, int option, int value) {
     } else if (option == SPH_OPT_SWEEP) {
         if (value < 0) throw Error(SPH_ERR_ARG, "sweep option must be >= 0");
         B.sweep = value != 0;
-        if (value > 1) B.ell_cap = (uint32_t)value;
+        if (value > 1) B.ell_cap = (uint32_t)((value + 1) & ~1);  // rows come in pairs
     } else if (option == SPH_OPT_SHAPE) {
         if (value < 0 || value >= kPairVariants) throw Error(SPH_ERR_ARG, "no such code shape");
         for (int p = 0; p < kPairPasses; p++) B.variant[p] = value;

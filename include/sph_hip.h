@@ -396,7 +396,7 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *   SPH_OPT_AUTOTUNE  steps per code shape of the online tuner (default 1;
  *       0 = off): the first steps of a context or world, and the steps after
  *       its particle count changed 2x, time each code shape (unroll, waves per
- *       SIMD, loop form: plain, software-pipelined, next index prefetched) of
+ *       SIMD, loop form: plain, software-pipelined, two neighbors per step) of
  *       the five neighbor passes, which then keep their fastest.
  *   SPH_OPT_SHAPE  (set) run every neighbor pass in this code shape
  *       (0 .. SPH_SHAPES-1) and turn the tuner off.
