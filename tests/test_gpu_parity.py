@@ -236,6 +236,29 @@ def test_trajectories_bit_identical(native, refcpu, case):
     assert len(he) == len(hg) and np.allclose(hg, he, rtol=1e-12, atol=1e-18)
 
 
+@pytest.mark.parametrize("solver", ["IISPHwOST", "IISPH"])
+def test_headline_scene_bit_identical_at_full_size(native, refcpu, solver):
+    """The measured workload itself: the 1,000,000-particle two-fluid ball
+    (both rest densities, the sphere volume map, CFL dt) equals the oracle bit
+    for bit over two steps, with equal iteration counts. The GPU path runs
+    the sorted, paired pair cache, the tuned code shapes' first steps and
+    graph replay; the oracle runs its own hash grid on the host's threads."""
+    sc = scenes.two_fluid_ball(side=100, solver=solver)
+    print(f"full-size {solver}: {sc.n} particles", flush=True)
+    o, g = refcpu.OracleSystem(sc), native.System(sc)
+    for step in range(2):
+        o.step(1)
+        g.step_forward_in_time(1)
+        so, sg = o.state(), g.state()
+        for k in ("x", "v", "a", "V", "p"):
+            assert np.array_equal(sg[k], so[k]), (step, k)
+        mo, mg = o.measure(), g.take_measurement()
+        assert mg["solver_iterations"] == mo["solver_iterations"]
+        assert mg["eqs1_iterations"] == mo["eqs1_iterations"]
+        print(f"  step {step}: iterations {mo['eqs1_iterations']} + {mo['solver_iterations']}",
+              flush=True)
+
+
 def test_per_trait_calls_bit_identical(native, refcpu):
     for solver in ("SESPH", "IISPH", "IISPHwOST"):
         sc = scenes.small_iisph_box(side=10, solver=solver, jitter=2e-3)
