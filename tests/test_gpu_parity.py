@@ -236,15 +236,26 @@ def test_trajectories_bit_identical(native, refcpu, case):
     assert len(he) == len(hg) and np.allclose(hg, he, rtol=1e-12, atol=1e-18)
 
 
-@pytest.mark.parametrize("solver", ["IISPHwOST", "IISPH"])
-def test_headline_scene_bit_identical_at_full_size(native, refcpu, solver):
-    """The measured workload itself: the 1,000,000-particle two-fluid ball
-    (both rest densities, the sphere volume map, CFL dt) equals the oracle bit
-    for bit over two steps, with equal iteration counts. The GPU path runs
-    the sorted, paired pair cache, the tuned code shapes' first steps and
-    graph replay; the oracle runs its own hash grid on the host's threads."""
-    sc = scenes.two_fluid_ball(side=100, solver=solver)
-    print(f"full-size {solver}: {sc.n} particles", flush=True)
+FULL_SIZE = {
+    "ball_IISPHwOST": lambda: scenes.two_fluid_ball(side=100, solver="IISPHwOST"),
+    "ball_IISPH": lambda: scenes.two_fluid_ball(side=100, solver="IISPH"),
+    "block_drop": scenes.block_drop_iisph,
+    "rigid_coupling": scenes.rigid_coupling,
+}
+
+
+@pytest.mark.parametrize("case", list(FULL_SIZE))
+def test_headline_scene_bit_identical_at_full_size(native, refcpu, case):
+    """The measured workloads at their full sizes: the 1,000,000-particle
+    two-fluid ball (both rest densities, the sphere volume map, CFL dt) with
+    either solver, the 125,000-particle block drop and the 492,960-particle
+    rigid coupling (three two-way coupled bodies, contact on) equal the oracle
+    bit for bit over two steps, with equal iteration counts and rigid states.
+    The GPU path runs the sorted, paired pair cache, the tuner's first shapes
+    and graph replay; the oracle runs its own hash grid on the host's
+    threads."""
+    sc = FULL_SIZE[case]()
+    print(f"full-size {case}: {sc.n} particles", flush=True)
     o, g = refcpu.OracleSystem(sc), native.System(sc)
     for step in range(2):
         o.step(1)
@@ -255,6 +266,11 @@ def test_headline_scene_bit_identical_at_full_size(native, refcpu, solver):
         mo, mg = o.measure(), g.take_measurement()
         assert mg["solver_iterations"] == mo["solver_iterations"]
         assert mg["eqs1_iterations"] == mo["eqs1_iterations"]
+        for b, bd in enumerate(sc.boundaries):
+            if getattr(bd, "rigid", None) is not None:
+                gb, ob = g.body_state(b), o.body_state(b)
+                for k in ("com", "q", "lin_vel", "ang_mom"):
+                    assert np.array_equal(gb[k], ob[k]), (step, b, k)
         print(f"  step {step}: iterations {mo['eqs1_iterations']} + {mo['solver_iterations']}",
               flush=True)
 
