@@ -99,8 +99,9 @@ __global__ void kb_cell_scatter(BatchDev B);
 __global__ void kb_cell_order(BatchDev B);
 __global__ void kb_permute(BatchDev B);
 __global__ void kb_count_ell(BatchDev B);
-// neighbor rows a wavefront of kb_sweep_ell buffers in LDS (48 KB per workgroup)
-constexpr int kSweepRows = 48;
+// neighbor rows a wavefront of kb_sweep_ell buffers in LDS (24 KB per
+// workgroup, which leaves room for six workgroups per CU)
+constexpr int kSweepRows = 24;
 // candidates whose positions kb_sweep_ell loads together
 constexpr int kSweepBatch = 4;
 __global__ void kb_sweep_ell(BatchDev B, uint32_t cap, uint32_t *maxc);
