@@ -236,6 +236,48 @@ def test_trajectories_bit_identical(native, refcpu, case):
     assert len(he) == len(hg) and np.allclose(hg, he, rtol=1e-12, atol=1e-18)
 
 
+def with_twins(sc, dup):
+    """The scene with copies of particles `dup` at exactly their positions."""
+    sc.x = np.concatenate([sc.x, sc.x[dup]])
+    sc.v = np.concatenate([sc.v, sc.v[dup]])
+    sc.m = np.concatenate([sc.m, sc.m[dup]])
+    sc.fluid_id = np.concatenate([sc.fluid_id, sc.fluid_id[dup]])
+    return sc
+
+
+TWINS = np.array([3, 77, 200, 200, 411])  # particle 200 three times over
+
+
+def test_coincident_particles_bit_identical(native, refcpu):
+    """Distinct particles at exactly the same position (distance 0), SESPH:
+    grad W(0) is the zero vector (kernel/mod.rs:37-45) and the viscosity
+    denominator's 0.01 dx^2 term (non_pressure_accelerations.rs:106-185) keeps
+    every sum finite; both sides list the twins as neighbors and agree bit for
+    bit."""
+    sc = with_twins(scenes.dam_break_sesph(side=8), TWINS)
+    print(f"coincident particles: {sc.n} particles, {len(TWINS)} twins", flush=True)
+    o, g = refcpu.OracleSystem(sc), native.System(sc)
+    for _ in range(4):
+        o.step(1)
+        g.step_forward_in_time(1)
+        so, sg = o.state(), g.state()
+        for k in ("x", "v", "a", "V", "p"):
+            assert np.all(np.isfinite(sg[k])), k
+            assert np.array_equal(sg[k], so[k]), k
+
+
+def test_coincident_particles_stall_iisph_loudly(native):
+    """Twins never separate under IISPH: pressure gradients between them are
+    zero, and the Jacobi error stays above the target. The reference's loop
+    has no cap (iisph.rs:387-390) and would not return; the library stops at
+    its 100,000-iteration cap with SPH_ERR_UNSUPPORTED instead of hanging."""
+    from sph_amd import _native as N
+    sc = with_twins(scenes.small_iisph_box(side=8, jitter=2e-3), TWINS)
+    print(f"coincident particles under IISPH: {sc.n} particles", flush=True)
+    with pytest.raises(N.SphError, match="did not converge"):
+        native.System(sc)  # System::new already runs an update with a solve
+
+
 FULL_SIZE = {
     "ball_IISPHwOST": lambda: scenes.two_fluid_ball(side=100, solver="IISPHwOST"),
     "ball_IISPH": lambda: scenes.two_fluid_ball(side=100, solver="IISPH"),
