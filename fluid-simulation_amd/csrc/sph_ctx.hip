@@ -19,6 +19,7 @@
 #include <deque>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -561,7 +562,10 @@ struct Batch {
     std::vector<uint32_t> hp_blk, hc_blk;
     size_t total_particles = 0, total_pblocks = 0, total_cells = 0, total_slices = 0;
 
+    int device = -1;  // the GPU the stream belongs to
+
     void init() {
+        HIPCK(hipGetDevice(&device));
         HIPCK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         ensure_pinned(64);
     }
@@ -657,6 +661,86 @@ struct sph_server {
 namespace sph {
 
 static thread_local std::string g_last_error;
+
+// ------------------------------------------------------------------------
+// Static scratch of the allocation-free neighbor search
+// (sph_neighbors_find_noalloc): every buffer beyond the caller's inputs and
+// outputs lives in one device array that each GPU holds from the first load
+// of this library until the process ends. Calls that overlap on one GPU
+// share it and race.
+// ------------------------------------------------------------------------
+constexpr size_t kNaMaxPoints = (size_t)1 << 18;  // queries, and samples, per call
+constexpr size_t kNaMaxCells = (size_t)1 << 20;   // cells of the samples' occupied box
+constexpr size_t na_round(size_t b) { return (b + 255) & ~(size_t)255; }
+// byte offsets of the scratch regions
+struct NaLayout {
+    size_t bbox_s, bbox_q, flag, last, part, sci, perm, ssx, ssy, ssz, ucnt, ucur, start, scan_a,
+        scan_b, scan_c, qcnt, off, bytes;
+};
+constexpr NaLayout na_layout() {
+    NaLayout L{};
+    const size_t nb1 = (kNaMaxCells + 1023) / 1024, nb2 = (nb1 + 1023) / 1024;
+    size_t o = 0;
+    L.bbox_s = o;  // 6 int: the samples' occupied-cell box
+    L.bbox_q = o + 32;  // 6 int: the queries' box
+    L.flag = o + 64;  // uint32: hash-collision flag
+    o += 256;
+    L.part = o; o += na_round((kNaMaxPoints + kBlock - 1) / kBlock * 6 * sizeof(int));
+    L.sci = o; o += na_round(3 * kNaMaxPoints * sizeof(int));
+    L.perm = o; o += na_round(kNaMaxPoints * sizeof(uint32_t));
+    L.ssx = o; o += na_round(kNaMaxPoints * sizeof(double));
+    L.ssy = o; o += na_round(kNaMaxPoints * sizeof(double));
+    L.ssz = o; o += na_round(kNaMaxPoints * sizeof(double));
+    L.ucnt = o; o += na_round(kNaMaxCells * sizeof(uint32_t));
+    L.ucur = o; o += na_round(kNaMaxCells * sizeof(uint32_t));
+    L.start = o; o += na_round((kNaMaxCells + 1) * sizeof(uint32_t));
+    L.scan_a = o; o += na_round((nb1 + 1) * sizeof(uint32_t));
+    L.scan_b = o; o += na_round((nb1 + 1) * sizeof(uint32_t));
+    L.scan_c = o; o += na_round((nb2 + 2) * sizeof(uint32_t));
+    L.qcnt = o; o += na_round(kNaMaxPoints * sizeof(uint32_t));
+    L.off = o; o += na_round((kNaMaxPoints + 1) * sizeof(uint32_t));
+    L.last = 0;
+    L.bytes = o;
+    return L;
+}
+constexpr NaLayout kNa = na_layout();
+static_assert(kNa.bytes <= (size_t)25 << 20, "no-alloc scratch larger than stated (25 MiB)");
+
+static __device__ __align__(256) unsigned char scratch[kNa.bytes];
+
+// The array's name is a host-side stand-in: its device address on each GPU
+// comes from hipGetSymbolAddress, resolved once per device at set-up
+// (sph_ctx_create, with that device current) and kept here.
+struct ScratchSlot {
+    bool resolved = false;
+    hipError_t status = hipSuccess;
+    unsigned char *ptr = nullptr;
+};
+static std::mutex g_scratch_mu;
+static std::vector<ScratchSlot> g_scratch;  // by device ordinal
+
+static void scratch_setup(int dev) {
+    if (dev < 0) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)dev >= g_scratch.size()) g_scratch.resize((size_t)dev + 1);
+    ScratchSlot &sl = g_scratch[(size_t)dev];
+    if (sl.resolved) return;
+    void *p = nullptr;
+    sl.status = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    sl.ptr = sl.status == hipSuccess ? static_cast<unsigned char *>(p) : nullptr;
+    sl.resolved = true;
+}
+
+static unsigned char *scratch_device(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0 || (size_t)dev >= g_scratch.size() || !g_scratch[(size_t)dev].resolved)
+        throw Error(SPH_ERR_STATE, "static scratch not set up on this context's device");
+    const ScratchSlot &sl = g_scratch[(size_t)dev];
+    if (sl.status != hipSuccess || !sl.ptr)
+        throw Error(SPH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(sl.status));
+    return sl.ptr;
+}
 
 static void set_error(const char *what) { g_last_error = what ? what : ""; }
 
@@ -786,7 +870,32 @@ static void tune_end_step(Batch &B, size_t particles) {
     T.tuned_particles = particles;
 }
 
-// exclusive scan of n uint32 into out[0..n]; out[n] = total (returned)
+// Launches of an exclusive scan of n > 0 counts into out[0..n) on stream st,
+// with block sums in a (>= nb1 + 1 entries), b (>= nb1 + 1) and c (>= nb2 + 2),
+// nb1 = ceil(n / 1024), nb2 = ceil(nb1 / 1024); out[n] is not written.
+static void scan_launch(hipStream_t st, const uint32_t *in, size_t n, uint32_t *out, uint32_t *a,
+                        uint32_t *b, uint32_t *c) {
+    const size_t BS = 1024;
+    size_t nb1 = (n + BS - 1) / BS;
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb1), dim3(512), 0, st, in, n, out, a);
+    HIPCK(hipGetLastError());
+    if (nb1 > 1) {
+        size_t nb2 = (nb1 + BS - 1) / BS;
+        hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb2), dim3(512), 0, st, a, nb1, b, c);
+        HIPCK(hipGetLastError());
+        if (nb2 > 1) {
+            if (nb2 > BS) throw Error(SPH_ERR_UNSUPPORTED, "scan longer than 2^30 elements");
+            // third level in a single workgroup; its block sum goes past the scanned entries
+            hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(512), 0, st, c, nb2, c, c + nb2 + 1);
+            HIPCK(hipGetLastError());
+            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb2), dim3(512), 0, st, b, nb1, c);
+            HIPCK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb1), dim3(512), 0, st, out, n, b);
+        HIPCK(hipGetLastError());
+    }
+}
+
 // exclusive scan of n counts into out[0..n]; out[n] = the total. A caller that
 // knows the total passes it (known_total >= 0): no read-back, no host wait.
 static uint32_t scan_u32(Batch &B, const uint32_t *in, size_t n, uint32_t *out,
@@ -797,30 +906,11 @@ static uint32_t scan_u32(Batch &B, const uint32_t *in, size_t n, uint32_t *out,
         B.put(out, &z, sizeof z);
         return 0;
     }
-    size_t nb1 = (n + BS - 1) / BS;
+    size_t nb1 = (n + BS - 1) / BS, nb2 = (nb1 + BS - 1) / BS;
     B.scan_a.reserve(nb1 + 1);
-    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb1), dim3(512), 0, B.s, in, n, out, B.scan_a.p);
-    HIPCK(hipGetLastError());
-    if (nb1 > 1) {
-        size_t nb2 = (nb1 + BS - 1) / BS;
-        B.scan_b.reserve(nb1 + 1);
-        B.scan_c.reserve(nb2 + 2);
-        hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb2), dim3(512), 0, B.s, B.scan_a.p, nb1,
-                           B.scan_b.p, B.scan_c.p);
-        HIPCK(hipGetLastError());
-        if (nb2 > 1) {
-            if (nb2 > BS) throw Error(SPH_ERR_UNSUPPORTED, "scan longer than 2^30 elements");
-            // third level in a single workgroup; its block sum goes past the scanned entries
-            hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(512), 0, B.s, B.scan_c.p, nb2,
-                               B.scan_c.p, B.scan_c.p + nb2 + 1);
-            HIPCK(hipGetLastError());
-            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb2), dim3(512), 0, B.s, B.scan_b.p,
-                               nb1, B.scan_c.p);
-            HIPCK(hipGetLastError());
-        }
-        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb1), dim3(512), 0, B.s, out, n, B.scan_b.p);
-        HIPCK(hipGetLastError());
-    }
+    B.scan_b.reserve(nb1 + 1);
+    B.scan_c.reserve(nb2 + 2);
+    scan_launch(B.s, in, n, out, B.scan_a.p, B.scan_b.p, B.scan_c.p);
     if (known_total >= 0) {
         const uint32_t t32 = (uint32_t)known_total;
         B.put(out + n, &t32, sizeof t32);
@@ -2287,6 +2377,7 @@ int sph_ctx_create(const sph_params *params, sph_ctx **out) {
         S.reach = (int)std::ceil(h / h);
         S.dt = params->cfl ? 0.0 : params->time_increment;
         c->own.init();
+        scratch_setup(c->own.device);
         reserve_scene(S, 1);
         *out = c.release();
     });
@@ -2457,6 +2548,125 @@ int sph_neighbors_find(sph_ctx *c, double cell_size, double range, const double 
         idx64.reserve(tot);
         LAUNCH(k_map_indices, tot, B.s, (size_t)tot, perm.p, idx.p, idx64.p);
         idx64.download(indices, tot, B.s);
+        B.sync();
+    });
+}
+
+int sph_neighbors_find_noalloc(sph_ctx *c, double cell_size, double range, const double *pos,
+                               size_t n, const double *samples, size_t m, uint64_t *offsets,
+                               uint64_t *indices, size_t cap, size_t *total) {
+    return guarded([&] {
+        if (!c || (!pos && n) || (!samples && m) || !offsets || !total)
+            throw Error(SPH_ERR_ARG, "null argument");
+        if (!(cell_size > 0)) throw Error(SPH_ERR_ARG, "cell_size must be > 0");
+        if (n > kNaMaxPoints || m > kNaMaxPoints)
+            throw Error(SPH_ERR_UNSUPPORTED,
+                        "more than 262144 queries or samples (the static scratch limit)");
+        check_solo(c);
+        Batch &B = c->own;
+        unsigned char *S = scratch_device(B.device);
+        const int reach = (int)std::ceil(range / cell_size);
+        if (reach < 0 || reach > 64) throw Error(SPH_ERR_UNSUPPORTED, "range / cell_size out of range");
+        const double dthr = distance_threshold(range);
+        int *bbox_s = reinterpret_cast<int *>(S + kNa.bbox_s);
+        int *bbox_q = reinterpret_cast<int *>(S + kNa.bbox_q);
+        unsigned int *flag = reinterpret_cast<unsigned int *>(S + kNa.flag);
+        int *part = reinterpret_cast<int *>(S + kNa.part);
+        int *sci = reinterpret_cast<int *>(S + kNa.sci);
+        uint32_t *perm = reinterpret_cast<uint32_t *>(S + kNa.perm);
+        double *ssx = reinterpret_cast<double *>(S + kNa.ssx);
+        double *ssy = reinterpret_cast<double *>(S + kNa.ssy);
+        double *ssz = reinterpret_cast<double *>(S + kNa.ssz);
+        uint32_t *ucnt = reinterpret_cast<uint32_t *>(S + kNa.ucnt);
+        uint32_t *ucur = reinterpret_cast<uint32_t *>(S + kNa.ucur);
+        uint32_t *start = reinterpret_cast<uint32_t *>(S + kNa.start);
+        uint32_t *sa = reinterpret_cast<uint32_t *>(S + kNa.scan_a);
+        uint32_t *sb = reinterpret_cast<uint32_t *>(S + kNa.scan_b);
+        uint32_t *sc = reinterpret_cast<uint32_t *>(S + kNa.scan_c);
+        uint32_t *qcnt = reinterpret_cast<uint32_t *>(S + kNa.qcnt);
+        uint32_t *off = reinterpret_cast<uint32_t *>(S + kNa.off);
+        int *h = reinterpret_cast<int *>(B.pinned);  // 64 doubles of pinned host memory
+        *total = 0;
+        // occupied-cell boxes of the samples (with their cells) and of the queries
+        auto box = [&](size_t cnt, const double *p, int *ci, int *bbox) {
+            hipLaunchKernelGGL(k_init_bbox, dim3(1), dim3(64), 0, B.s, bbox);
+            HIPCK(hipGetLastError());
+            if (cnt == 0) return;
+            LAUNCH(k_cells_aos, cnt, B.s, cnt, cell_size, p, ci, part);
+            hipLaunchKernelGGL(k_fold_bbox, dim3(1), dim3(kBlock), 0, B.s, part,
+                               (int)nblocks(cnt), bbox);
+            HIPCK(hipGetLastError());
+        };
+        box(m, samples, sci, bbox_s);
+        box(n, pos, nullptr, bbox_q);
+        HIPCK(hipMemcpyAsync(h, bbox_s, 64, hipMemcpyDeviceToHost, B.s));  // both boxes
+        B.sync();
+        int b1[6], b2[6];
+        std::memcpy(b1, h, sizeof b1);
+        std::memcpy(b2, h + 8, sizeof b2);  // bbox_q sits 32 bytes after bbox_s
+        if (n == 0 || m == 0) {  // offsets all zero
+            LAUNCH(k_fill_u32, n + 1, B.s, off, n + 1, 0u);
+            LAUNCH(k_widen_u32, n + 1, B.s, n + 1, off, offsets);
+            B.sync();
+            return;
+        }
+        // refuse boxes where two cells of one stencil share a hash bucket
+        int lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(b1[k], b2[k]) - reach - 4;
+            hi[k] = std::max(b1[3 + k], b2[3 + k]) + reach + 4;
+        }
+        GridDev hg = make_grid(lo, hi);
+        LAUNCH(k_fill_u32, 1, B.s, (uint32_t *)flag, (size_t)1, 0u);
+        LAUNCH(k_hash_check, (size_t)hg.dim[0] * hg.dim[1] * hg.dim[2], B.s, hg, 2 * reach, flag);
+        // counting sort of the samples by cell, members in index order
+        GridDev g = make_grid(b1, b1 + 3);
+        const size_t ncell = (size_t)g.dim[0] * g.dim[1] * g.dim[2];
+        if (ncell > kNaMaxCells)
+            throw Error(SPH_ERR_UNSUPPORTED,
+                        "occupied cell box of the samples exceeds 2^20 cells (the static "
+                        "scratch limit)");
+        LAUNCH(k_fill_u32, ncell, B.s, ucnt, ncell, 0u);
+        LAUNCH(k_fill_u32, ncell, B.s, ucur, ncell, 0u);
+        LAUNCH(k_cell_count, m, B.s, m, sci, g, ucnt);
+        scan_launch(B.s, ucnt, ncell, start, sa, sb, sc);
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, B.s, ucnt, ncell, start);
+        HIPCK(hipGetLastError());
+        LAUNCH(k_cell_scatter, m, B.s, m, sci, g, start, ucur, perm);
+        LAUNCH(k_cell_order, ncell, B.s, ncell, start, perm, (const uint32_t *)nullptr);
+        g.start = start;
+        LAUNCH(k_gather_aos, m, B.s, m, perm, samples, ssx, ssy, ssz);
+        // counts, offsets, then the indices straight into the caller's buffer
+        LAUNCH(k_neighbors_aos<false>, n, B.s, n, pos, cell_size, reach, dthr, g, ssx, ssy, ssz,
+               (const uint32_t *)nullptr, qcnt, (const uint32_t *)nullptr, (uint64_t *)nullptr);
+        scan_launch(B.s, qcnt, n, off, sa, sb, sc);
+        uint32_t *last = reinterpret_cast<uint32_t *>(B.pinned);
+        HIPCK(hipMemcpyAsync(&last[0], flag, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
+        HIPCK(hipMemcpyAsync(&last[1], off + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
+        HIPCK(hipMemcpyAsync(&last[2], qcnt + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, B.s));
+        B.sync();
+        if (last[0])
+            throw Error(SPH_ERR_UNSUPPORTED,
+                        "two cells of one neighbor stencil share a spatial-hash bucket; the "
+                        "reference would report duplicate neighbors there");
+        const uint64_t tot = (uint64_t)last[1] + last[2];
+        if (tot > 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "more than 2^32 entries");
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, B.s, qcnt, n, off);
+        HIPCK(hipGetLastError());
+        LAUNCH(k_widen_u32, n + 1, B.s, n + 1, off, offsets);
+        *total = (size_t)tot;
+        if (tot > cap) {
+            B.sync();
+            throw Error(SPH_ERR_CAPACITY, "indices buffer too small");
+        }
+        if (tot > 0) {
+            if (!indices) {
+                B.sync();
+                throw Error(SPH_ERR_ARG, "null indices");
+            }
+            LAUNCH(k_neighbors_aos<true>, n, B.s, n, pos, cell_size, reach, dthr, g, ssx, ssy,
+                   ssz, off, (uint32_t *)nullptr, perm, indices);
+        }
         B.sync();
     });
 }

@@ -29,18 +29,12 @@ __global__ void k_fill_f64(double *p, size_t n, double v) {
 // ------------------------------------------------------------------------
 // per-block partials of the occupied-cell box (min x,y,z; max x,y,z); a
 // single-block pass folds them (no contended atomics on six words)
-__global__ void k_cells(size_t n, double cs, const double *x, const double *y, const double *z,
-                        int *ci, int *bbox_partials) {
+__device__ __forceinline__ void bbox_block_partial(bool valid, int a, int b, int c,
+                                                   int *bbox_partials) {
     __shared__ int lds[6][kBlock / 64];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int v[6] = {2147483647, 2147483647, 2147483647, -2147483647 - 1, -2147483647 - 1,
                 -2147483647 - 1};
-    if (i < n) {
-        int a = sat_i32(floor(x[i] / cs)), b = sat_i32(floor(y[i] / cs)),
-            c = sat_i32(floor(z[i] / cs));
-        ci[3 * i] = a;
-        ci[3 * i + 1] = b;
-        ci[3 * i + 2] = c;
+    if (valid) {
         v[0] = v[3] = a;
         v[1] = v[4] = b;
         v[2] = v[5] = c;
@@ -62,6 +56,38 @@ __global__ void k_cells(size_t n, double cs, const double *x, const double *y, c
         }
         for (int k = 0; k < 6; k++) bbox_partials[(size_t)blockIdx.x * 6 + k] = v[k];
     }
+}
+
+__global__ void k_cells(size_t n, double cs, const double *x, const double *y, const double *z,
+                        int *ci, int *bbox_partials) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a = 0, b = 0, c = 0;
+    if (i < n) {
+        a = sat_i32(floor(x[i] / cs));
+        b = sat_i32(floor(y[i] / cs));
+        c = sat_i32(floor(z[i] / cs));
+        ci[3 * i] = a;
+        ci[3 * i + 1] = b;
+        ci[3 * i + 2] = c;
+    }
+    bbox_block_partial(i < n, a, b, c, bbox_partials);
+}
+
+// the same for [n][3] positions; ci may be null (box only)
+__global__ void k_cells_aos(size_t n, double cs, const double *p, int *ci, int *bbox_partials) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a = 0, b = 0, c = 0;
+    if (i < n) {
+        a = sat_i32(floor(p[3 * i] / cs));
+        b = sat_i32(floor(p[3 * i + 1] / cs));
+        c = sat_i32(floor(p[3 * i + 2] / cs));
+        if (ci) {
+            ci[3 * i] = a;
+            ci[3 * i + 1] = b;
+            ci[3 * i + 2] = c;
+        }
+    }
+    bbox_block_partial(i < n, a, b, c, bbox_partials);
 }
 
 __global__ void k_fold_bbox(const int *partials, int nparts, int *bbox) {
@@ -172,6 +198,17 @@ template __global__ void k_gather<uint32_t>(size_t, const uint32_t *, const uint
                                              uint32_t *);
 template __global__ void k_gather<int>(size_t, const uint32_t *, const int *, int *);
 
+// sorted structure-of-arrays copy of [n][3] positions: dst[i] = src[perm[i]]
+__global__ void k_gather_aos(size_t n, const uint32_t *perm, const double *p, double *x,
+                             double *y, double *z) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t j = perm[i];
+    x[i] = p[3 * j];
+    y[i] = p[3 * j + 1];
+    z[i] = p[3 * j + 2];
+}
+
 // ------------------------------------------------------------------------
 // neighbor search (spatial_hashing.rs:75-103): for every query, the samples
 // of the (2c+1)^3 stencil with |x_q - x_s| < range. The stencil is walked in
@@ -181,6 +218,41 @@ template __global__ void k_gather<int>(size_t, const uint32_t *, const int *, in
 // reference's. The predicate d2 < dthr, with dthr the least double whose
 // correctly rounded sqrt is >= range, is exactly sqrt(d2) < range.
 // ------------------------------------------------------------------------
+// One query's stencil walk; emit(cnt, j) receives the cnt-th accepted sample
+// (sorted index j). Returns the number accepted.
+template <class Emit>
+__device__ __forceinline__ uint32_t stencil_walk(double px, double py, double pz, double cs,
+                                                 int reach, double dthr, const GridDev &g,
+                                                 const double *sx, const double *sy,
+                                                 const double *sz, Emit emit) {
+    const int cx = sat_i32(floor(px / cs)), cy = sat_i32(floor(py / cs)),
+              cz = sat_i32(floor(pz / cs));
+    uint32_t cnt = 0;
+    int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
+    if (zlo < 0) zlo = 0;
+    if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
+    if (zlo > zhi) return 0;
+    for (int dx = -reach; dx <= reach; dx++) {
+        int gx = cx + dx - g.org[0];
+        if (gx < 0 || gx >= g.dim[0]) continue;
+        for (int dy = -reach; dy <= reach; dy++) {
+            int gy = cy + dy - g.org[1];
+            if (gy < 0 || gy >= g.dim[1]) continue;
+            uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
+            uint32_t a = g.start[row + zlo], b = g.start[row + zhi + 1];
+            for (uint32_t j = a; j < b; j++) {
+                double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+                double d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                if (d2 < dthr) {
+                    emit(cnt, j);
+                    cnt++;
+                }
+            }
+        }
+    }
+    return cnt;
+}
+
 template <bool FILL>
 __global__ void k_neighbors(size_t nq, const double *qx, const double *qy, const double *qz,
                             double cs, int reach, double dthr, GridDev g, const double *sx,
@@ -188,34 +260,11 @@ __global__ void k_neighbors(size_t nq, const double *qx, const double *qy, const
                             uint32_t *out_count, uint32_t *out_idx) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
-    const double px = qx[i], py = qy[i], pz = qz[i];
-    const int cx = sat_i32(floor(px / cs)), cy = sat_i32(floor(py / cs)),
-              cz = sat_i32(floor(pz / cs));
-    uint32_t cnt = 0;
-    uint32_t o = FILL ? off[i] : 0;
-    int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
-    if (zlo < 0) zlo = 0;
-    if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
-    if (zlo <= zhi) {
-        for (int dx = -reach; dx <= reach; dx++) {
-            int gx = cx + dx - g.org[0];
-            if (gx < 0 || gx >= g.dim[0]) continue;
-            for (int dy = -reach; dy <= reach; dy++) {
-                int gy = cy + dy - g.org[1];
-                if (gy < 0 || gy >= g.dim[1]) continue;
-                uint32_t row = ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2];
-                uint32_t a = g.start[row + zlo], b = g.start[row + zhi + 1];
-                for (uint32_t j = a; j < b; j++) {
-                    double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
-                    double d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                    if (d2 < dthr) {
-                        if (FILL) out_idx[o + cnt] = j;
-                        cnt++;
-                    }
-                }
-            }
-        }
-    }
+    const uint32_t o = FILL ? off[i] : 0;
+    const uint32_t cnt = stencil_walk(qx[i], qy[i], qz[i], cs, reach, dthr, g, sx, sy, sz,
+                                      [&](uint32_t k, uint32_t j) {
+                                          if (FILL) out_idx[o + k] = j;
+                                      });
     if (!FILL) out_count[i] = cnt;
 }
 template __global__ void k_neighbors<false>(size_t, const double *, const double *,
@@ -226,6 +275,30 @@ template __global__ void k_neighbors<true>(size_t, const double *, const double 
                                            const double *, double, int, double, GridDev,
                                            const double *, const double *, const double *,
                                            const uint32_t *, uint32_t *, uint32_t *);
+
+// [nq][3] queries; FILL writes the caller's sample index perm[j] as uint64_t
+template <bool FILL>
+__global__ void k_neighbors_aos(size_t nq, const double *q, double cs, int reach, double dthr,
+                                GridDev g, const double *sx, const double *sy, const double *sz,
+                                const uint32_t *off, uint32_t *out_count, const uint32_t *perm,
+                                uint64_t *out_idx) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint32_t o = FILL ? off[i] : 0;
+    const uint32_t cnt = stencil_walk(q[3 * i], q[3 * i + 1], q[3 * i + 2], cs, reach, dthr, g,
+                                      sx, sy, sz, [&](uint32_t k, uint32_t j) {
+                                          if (FILL) out_idx[o + k] = (uint64_t)perm[j];
+                                      });
+    if (!FILL) out_count[i] = cnt;
+}
+template __global__ void k_neighbors_aos<false>(size_t, const double *, double, int, double,
+                                                GridDev, const double *, const double *,
+                                                const double *, const uint32_t *, uint32_t *,
+                                                const uint32_t *, uint64_t *);
+template __global__ void k_neighbors_aos<true>(size_t, const double *, double, int, double,
+                                               GridDev, const double *, const double *,
+                                               const double *, const uint32_t *, uint32_t *,
+                                               const uint32_t *, uint64_t *);
 
 // ------------------------------------------------------------------------
 // exclusive scan of uint32 (three-phase, 1024 elements per block)
@@ -288,6 +361,11 @@ __global__ void k_static_volume(size_t n, KernelConst K, double gamma_v, const d
 __global__ void k_map_indices(size_t n, const uint32_t *perm, const uint32_t *in, uint64_t *out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = perm ? (uint64_t)perm[in[i]] : (uint64_t)in[i];
+}
+
+// out[n] = out[n-1] + in[n-1]: the total after an exclusive scan of n > 0 counts
+__global__ void k_scan_total(const uint32_t *in, size_t n, uint32_t *out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = out[n - 1] + in[n - 1];
 }
 
 __global__ void k_widen_u32(size_t n, const uint32_t *in, uint64_t *out) {

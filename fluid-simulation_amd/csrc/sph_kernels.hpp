@@ -29,6 +29,7 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v);
 __global__ void k_fill_f64(double *p, size_t n, double v);
 __global__ void k_cells(size_t n, double cs, const double *x, const double *y, const double *z,
                         int *ci, int *bbox_partials);
+__global__ void k_cells_aos(size_t n, double cs, const double *p, int *ci, int *bbox_partials);
 __global__ void k_fold_bbox(const int *partials, int nparts, int *bbox);
 __global__ void k_init_bbox(int *bbox);
 __global__ void k_hash_check(GridDev g, int reach, unsigned int *flag);
@@ -39,6 +40,13 @@ __global__ void k_cell_order(size_t ncell, const uint32_t *start, uint32_t *perm
                              const uint32_t *ref);
 template <class T>
 __global__ void k_gather(size_t n, const uint32_t *perm, const T *src, T *dst);
+__global__ void k_gather_aos(size_t n, const uint32_t *perm, const double *p, double *x,
+                             double *y, double *z);
+template <bool FILL>
+__global__ void k_neighbors_aos(size_t nq, const double *q, double cs, int reach, double dthr,
+                                GridDev g, const double *sx, const double *sy, const double *sz,
+                                const uint32_t *off, uint32_t *out_count, const uint32_t *perm,
+                                uint64_t *out_idx);
 template <bool FILL>
 __global__ void k_neighbors(size_t nq, const double *qx, const double *qy, const double *qz,
                             double cs, int reach, double dthr, GridDev g, const double *sx,
@@ -90,6 +98,7 @@ __global__ void k_vm_quadrature(LatticeDev sdf, VmGrid g, VmQuad Q, const uint32
                                 double *values, uint8_t *status);
 __global__ void k_vm_cells(VmGrid g, const uint8_t *status, uint8_t *present);
 __global__ void k_vm_finish(size_t n, const uint8_t *present, double *values);
+__global__ void k_scan_total(const uint32_t *in, size_t n, uint32_t *out);
 __global__ void k_widen_u32(size_t n, const uint32_t *in, uint64_t *out);
 
 // Signed distance to a triangle mesh at lattice nodes

@@ -89,6 +89,9 @@ SIGNATURES = [
     ("sph_ctx_destroy", C.c_int, [C.c_void_p]),
     ("sph_neighbors_find", C.c_int, [C.c_void_p, C.c_double, C.c_double, _dp, C.c_size_t, _dp,
                                      C.c_size_t, _u64p, _u64p, C.c_size_t, _szp]),
+    ("sph_neighbors_find_noalloc", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                             C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                                             C.c_void_p, C.c_size_t, _szp]),
     ("sph_boundary_add_samples", C.c_int, [C.c_void_p, _dp, C.c_size_t,
                                            C.POINTER(SphRigidInit)]),
     ("sph_boundary_add_volume_map", C.c_int, [C.c_void_p, C.POINTER(SphLattice),

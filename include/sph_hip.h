@@ -130,6 +130,24 @@ int sph_neighbors_find(sph_ctx *ctx, double cell_size, double range, const doubl
                        size_t n, const double *samples, size_t m, uint64_t *offsets,
                        uint64_t *indices, size_t cap, size_t *total);
 
+/* Allocation-free variant of sph_neighbors_find. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * pos, samples, offsets and indices are DEVICE pointers ([n][3] and [m][3]
+ * doubles, n+1 and cap uint64_t); *total is host memory. The results equal
+ * sph_neighbors_find's. It calls no hipMalloc/hipFree: all intermediate
+ * buffers live in a static device array of 25 MiB per GPU, held from the
+ * library's load until the process ends. Limits: n, m <= 262144 and an
+ * occupied cell box of the samples of at most 2^20 cells, else
+ * SPH_ERR_UNSUPPORTED before any launch that writes the outputs. Every call on
+ * one GPU shares that array: calls that overlap on one GPU (several threads,
+ * several contexts) race. The call returns after its work on the context's
+ * stream has finished; the caller makes the inputs visible to that stream
+ * first (e.g. a device synchronization). */
+int sph_neighbors_find_noalloc(sph_ctx *ctx, double cell_size, double range, const double *pos,
+                               size_t n, const double *samples, size_t m, uint64_t *offsets,
+                               uint64_t *indices, size_t cap, size_t *total);
+
 /* ---- BoundaryHandling::add_static_boundary / add_dynamic_boundary ----------
  * (boundary_handling/mod.rs:44-60). StaticSampleBoundary: sample positions
  * (world frame when rigid == NULL, body frame otherwise).
