@@ -2215,6 +2215,78 @@ double ref_system_static_volume(const ref_system *s, int bi, size_t k) {
     return s->b[bi].svol[k];
 }
 
+/* ===================================================================== */
+/* test hooks: one reference function at a time on the current state, so  */
+/* the reference's own per-function tests (given volumes, pressures,      */
+/* velocities) can be restated against the oracle                          */
+/* ===================================================================== */
+int ref_system_call(ref_system *s, int fn) {
+    switch (fn) {
+        case REF_FN_NEIGHBORS: { /* the search part of update() (sph/mod.rs:701-712) */
+            double h = s->P.kernel_support_radius;
+            ref_nlist_free(&s->nl);
+            ref_find_samples(h, h, s->x, s->n, s->x, s->n, &s->nl);
+            for (int i = 0; i < s->nb; i++) memset(s->b[i].applied, 0, sizeof s->b[i].applied);
+            find_boundary_samples(s);
+            return 0;
+        }
+        case REF_FN_VOLUME: get_volume(s); return 0;
+        case REF_FN_RESET_ACCELERATION: memset(s->a, 0, 3 * s->n * sizeof(double)); return 0;
+        case REF_FN_GRAVITY: add_gravity(s); return 0;
+        case REF_FN_VISCOSITY: add_viscosity(s); return 0;
+        case REF_FN_PRESSURE_ACCELERATION: add_pressure_acceleration(s, s->a, 0, 1); return 0;
+        case REF_FN_PRESSURE_ACCELERATION_OVERWRITE:
+            add_pressure_acceleration(s, s->a, 1, 1);
+            return 0;
+        case REF_FN_PRESSURE_ACCELERATION_SCRATCH:
+            add_pressure_acceleration(s, s->pacc, 1, 0);
+            return 0;
+        case REF_FN_SESPH: sesph_solve(s); return 0;
+        case REF_FN_PRED_VEL: set_pred_vel(s, 0); return 0;
+        case REF_FN_SOURCE_VDE: set_source_term(s, 0); return 0;
+        case REF_FN_SOURCE_VP: set_source_term(s, 1); return 0;
+        case REF_FN_DIAGONAL: set_diagonal(s); return 0;
+        case REF_FN_INITIALIZE: iisph_initialize(s); return 0;
+        case REF_FN_RESOLVE: return (int)resolve_pressure(s, -1);
+        default: return -1;
+    }
+}
+
+int ref_system_field(ref_system *s, int field, int write, double *buf) {
+    double *f = NULL;
+    size_t k = 0;
+    switch (field) {
+        case REF_F_X: f = s->x; k = 3 * s->n; break;
+        case REF_F_VEL: f = s->v; k = 3 * s->n; break;
+        case REF_F_ACC: f = s->a; k = 3 * s->n; break;
+        case REF_F_MASS: f = s->m; k = s->n; break;
+        case REF_F_VOLUME: f = s->V; k = s->n; break;
+        case REF_F_PRESSURE: f = s->p; k = s->n; break;
+        case REF_F_PRED_VEL: f = s->sv; k = 3 * s->n; break;
+        case REF_F_PRED_POS: f = s->sx; k = 3 * s->n; break;
+        case REF_F_SOURCE: f = s->s_f; k = s->n; break;
+        case REF_F_DIAGONAL: f = s->a_ff; k = s->n; break;
+        case REF_F_PRESSURE_ACC: f = s->pacc; k = 3 * s->n; break;
+        case REF_F_TIME_STEP: f = &s->dt; k = 1; break;
+        default: return -1;
+    }
+    if (write)
+        memcpy(f, buf, k * sizeof(double));
+    else
+        memcpy(buf, f, k * sizeof(double));
+    return (int)k;
+}
+
+int ref_system_sample_volumes(ref_system *s, int bi, int write, double *buf) {
+    if (bi < 0 || bi >= s->nb || s->b[bi].kind != 0) return -1;
+    ref_boundary *b = &s->b[bi];
+    if (write)
+        memcpy(b->svol, buf, b->ns * sizeof(double));
+    else
+        memcpy(buf, b->svol, b->ns * sizeof(double));
+    return (int)b->ns;
+}
+
 void ref_system_set_threads(int nthreads) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);

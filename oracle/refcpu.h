@@ -248,6 +248,42 @@ void ref_mesh_sdf(const ref_mesh *m, const double *p, size_t np, double *out);
 void ref_mesh_sdf_build(const ref_mesh *m, const double x_min[3], const int64_t n[3], double dx,
                         double prune, double *values, uint8_t *present);
 
+/* ---- test hooks ---------------------------------------------------------
+ * One reference function on the system's current state (after
+ * ref_system_init), so per-function tests can set volumes, pressures and
+ * predicted velocities directly:
+ *   NEIGHBORS            fluid neighbor list + boundary samples (update())
+ *   VOLUME               get_volume (quantities.rs:14-61)
+ *   RESET_ACCELERATION   reset_acceleration (non_pressure_accelerations.rs:19-27)
+ *   GRAVITY, VISCOSITY   (non_pressure_accelerations.rs:64-185)
+ *   PRESSURE_ACCELERATION[_OVERWRITE]  add_pressure_acceleration into a, with
+ *                        reaction forces (pressure_solver/mod.rs:121-195);
+ *   PRESSURE_ACCELERATION_SCRATCH      the same into the IISPH scratch, no reaction
+ *   SESPH                SESPH::solve_and_add_acceleration (sesph.rs:31-67)
+ *   PRED_VEL             set_pred_vel_by_applying_acc(v) (pressure_solver/mod.rs:90-108)
+ *   SOURCE_VDE/VP        IISPH source terms (iisph.rs:117-251)
+ *   DIAGONAL, INITIALIZE (iisph.rs:254-376)
+ *   RESOLVE              resolve_pressure_globally with the target density
+ *                        error; returns the iteration count */
+enum {
+    REF_FN_NEIGHBORS = 0, REF_FN_VOLUME, REF_FN_RESET_ACCELERATION, REF_FN_GRAVITY,
+    REF_FN_VISCOSITY, REF_FN_PRESSURE_ACCELERATION, REF_FN_PRESSURE_ACCELERATION_OVERWRITE,
+    REF_FN_PRESSURE_ACCELERATION_SCRATCH, REF_FN_SESPH, REF_FN_PRED_VEL, REF_FN_SOURCE_VDE,
+    REF_FN_SOURCE_VP, REF_FN_DIAGONAL, REF_FN_INITIALIZE, REF_FN_RESOLVE
+};
+int ref_system_call(ref_system *s, int fn);
+/* read (write = 0) or overwrite (write = 1) one field; returns its length in
+ * doubles: x, v, a [n][3]; m, V, p [n]; predicted velocity / position [n][3];
+ * source term s_f, diagonal a_ii [n]; pressure-acceleration scratch [n][3];
+ * time step (1) */
+enum {
+    REF_F_X = 0, REF_F_VEL, REF_F_ACC, REF_F_MASS, REF_F_VOLUME, REF_F_PRESSURE, REF_F_PRED_VEL,
+    REF_F_PRED_POS, REF_F_SOURCE, REF_F_DIAGONAL, REF_F_PRESSURE_ACC, REF_F_TIME_STEP
+};
+int ref_system_field(ref_system *s, int field, int write, double *buf);
+/* pseudo volumes of static-sample boundary b (read or overwrite) */
+int ref_system_sample_volumes(ref_system *s, int b, int write, double *buf);
+
 #ifdef __cplusplus
 }
 #endif

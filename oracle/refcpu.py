@@ -52,6 +52,23 @@ class RefMeasurement(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RefRigid(C.Structure):
+    _fields_ = [("mass", C.c_double), ("inertia_inv_body", C.c_double * 9),
+                ("com", C.c_double * 3), ("q", C.c_double * 4), ("lin_vel", C.c_double * 3),
+                ("ang_mom", C.c_double * 3), ("force", C.c_double * 3),
+                ("torque", C.c_double * 3), ("inertia_inv_world", C.c_double * 9),
+                ("ang_vel", C.c_double * 3)]
+
+
+# single reference functions and fields of the test hooks (refcpu.h)
+FUNCTIONS = ["neighbors", "volume", "reset_acceleration", "gravity", "viscosity",
+             "pressure_acceleration", "pressure_acceleration_overwrite",
+             "pressure_acceleration_scratch", "sesph", "pred_vel", "source_vde", "source_vp",
+             "diagonal", "initialize", "resolve"]
+FIELDS = {"x": 3, "v": 3, "a": 3, "m": 1, "V": 1, "p": 1, "pred_v": 3, "pred_x": 3, "s": 1,
+          "a_ii": 1, "pacc": 3, "dt": 0}
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -138,6 +155,17 @@ def lib():
         L.ref_system_boundary_samples.restype = C.c_size_t
         L.ref_system_boundary_samples.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_int,
                                                   _dp, C.c_size_t]
+        L.ref_system_call.restype = C.c_int
+        L.ref_system_call.argtypes = [C.c_void_p, C.c_int]
+        L.ref_system_field.restype = C.c_int
+        L.ref_system_field.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.ref_system_sample_volumes.restype = C.c_int
+        L.ref_system_sample_volumes.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.ref_rigid_init.argtypes = [C.POINTER(RefRigid), C.c_double, _dp, _dp, _dp, _dp, _dp,
+                                     _dp]
+        L.ref_rigid_add_force.argtypes = [C.POINTER(RefRigid), _dp, _dp]
+        L.ref_rigid_step.argtypes = [C.POINTER(RefRigid), C.c_double]
+        L.ref_rigid_velocity_at_point.argtypes = [C.POINTER(RefRigid), _dp, _dp]
         L.ref_system_static_volume.restype = C.c_double
         L.ref_system_static_volume.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.ref_system_set_threads.argtypes = [C.c_int]
@@ -474,5 +502,61 @@ class OracleSystem:
         k = lib().ref_system_boundary_samples(self.h, b, i, mode, dptr(buf), 64)
         return buf[:7 * min(k, 64)].reshape(-1, 7)
 
+    def call(self, fn):
+        """Run one reference function on the current state (refcpu.h test hooks)."""
+        return int(lib().ref_system_call(self.h, FUNCTIONS.index(fn)))
+
+    def field(self, name):
+        w = FIELDS[name]
+        k = 1 if w == 0 else w * len(self)
+        out = np.zeros(k)
+        lib().ref_system_field(self.h, list(FIELDS).index(name), 0, dptr(out))
+        return float(out[0]) if w == 0 else out.reshape(-1, 3) if w == 3 else out
+
+    def set_field(self, name, values):
+        w = FIELDS[name]
+        buf = f64c(np.asarray(values, dtype=np.float64).reshape(-1))
+        k = 1 if w == 0 else w * len(self)
+        if len(buf) != k:
+            raise ValueError(f"{name}: {len(buf)} values, need {k}")
+        lib().ref_system_field(self.h, list(FIELDS).index(name), 1, dptr(buf))
+
+    def sample_volumes(self, b, values=None):
+        if values is None:
+            out = np.zeros(4096)
+            k = lib().ref_system_sample_volumes(self.h, b, 0, dptr(out))
+            return out[:k].copy()
+        buf = f64c(values)
+        lib().ref_system_sample_volumes(self.h, b, 1, dptr(buf))
+
     def static_volume(self, b, k):
         return lib().ref_system_static_volume(self.h, b, k)
+
+
+class Rigid:
+    """RigidBodyMotion of the oracle (ref_rigid), for the rigid-body pins."""
+
+    def __init__(self, mass, inertia, com, q=(1.0, 0.0, 0.0, 0.0), lin_vel=(0, 0, 0),
+                 ang_vel=(0, 0, 0)):
+        I = f64c(np.asarray(inertia, dtype=np.float64).reshape(9))
+        Ii = f64c(np.linalg.inv(I.reshape(3, 3)).reshape(9))
+        self.s = RefRigid()
+        args = [f64c(a) for a in (com, q, lin_vel, ang_vel)]
+        lib().ref_rigid_init(C.byref(self.s), float(mass), dptr(I), dptr(Ii),
+                             *[dptr(a) for a in args])
+        self._keep = (I, Ii, args)
+
+    def add_force(self, f, loc):
+        f, loc = f64c(f), f64c(loc)
+        lib().ref_rigid_add_force(C.byref(self.s), dptr(f), dptr(loc))
+
+    def step(self, dt):
+        lib().ref_rigid_step(C.byref(self.s), float(dt))
+
+    def velocity_at_point(self, p):
+        p, out = f64c(p), np.zeros(3)
+        lib().ref_rigid_velocity_at_point(C.byref(self.s), dptr(p), dptr(out))
+        return out
+
+    def __getattr__(self, k):
+        return np.array(getattr(self.s, k))
