@@ -1,10 +1,11 @@
 """The allocation-free neighbor search (sph_neighbors_find_noalloc) against
 the allocating one (sph_neighbors_find), through the C ABI.
 
-Device buffers for inputs and outputs come from torch (plumbing only); the
-entry point itself allocates nothing. Bar: offsets and indices bit-identical,
-on two calls in a row, the second with a smaller input (the static scratch
-keeps the first call's contents).
+Device buffers for the inputs and outputs are allocated here with hipMalloc
+from the HIP runtime the library itself links (ctypes); the entry point
+allocates nothing. Bar: offsets and indices bit-identical, on two calls in a
+row, the second with a smaller input (the static scratch keeps the first
+call's contents).
 """
 import ctypes as C
 
@@ -17,37 +18,61 @@ from sph_amd import scenes, system
 pytestmark = pytest.mark.gpu
 
 MAX_POINTS = 1 << 18
+H2D, D2H = 1, 2  # hipMemcpyHostToDevice, hipMemcpyDeviceToHost
 
 
 @pytest.fixture(scope="module")
-def torch_cuda():
-    torch = pytest.importorskip("torch")
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
-    return torch
+def hip():
+    N.lib()  # loads libsphhip.so and with it the HIP runtime it links
+    rt = C.CDLL("libamdhip64.so.7")
+    rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    rt.hipFree.argtypes = [C.c_void_p]
+    rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rt.hipDeviceSynchronize.argtypes = []
+    return rt
 
 
-def noalloc(torch, sh, search_range, pos, smp, cap=None):
+class DeviceArray:
+    """A device copy of a numpy array (or an uninitialised one)."""
+
+    def __init__(self, rt, host=None, nbytes=None):
+        self.rt = rt
+        self.nbytes = int(host.nbytes if host is not None else nbytes)
+        self.p = C.c_void_p()
+        assert rt.hipMalloc(C.byref(self.p), max(self.nbytes, 8)) == 0
+        if host is not None and self.nbytes:
+            h = np.ascontiguousarray(host)
+            assert rt.hipMemcpy(self.p, h.ctypes.data, self.nbytes, H2D) == 0
+
+    def get(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if count:
+            assert self.rt.hipMemcpy(out.ctypes.data, self.p, out.nbytes, D2H) == 0
+        return out
+
+    def __del__(self):
+        self.rt.hipFree(self.p)
+
+
+def noalloc(rt, sh, search_range, pos, smp, cap=None):
     """One call of the no-alloc entry point on device copies of pos / smp."""
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
     smp = np.ascontiguousarray(smp, dtype=np.float64).reshape(-1, 3)
     n, m = len(pos), len(smp)
-    dpos = torch.from_numpy(pos).to("cuda")
-    dsmp = torch.from_numpy(smp).to("cuda")
     if cap is None:
         cap = max(64 * n, 1)
-    doff = torch.full((n + 1,), -1, dtype=torch.int64, device="cuda")
-    didx = torch.full((max(cap, 1),), -1, dtype=torch.int64, device="cuda")
-    torch.cuda.synchronize()
+    dpos, dsmp = DeviceArray(rt, pos), DeviceArray(rt, smp)
+    doff = DeviceArray(rt, np.full(n + 1, 0xffffffffffffffff, dtype=np.uint64))
+    didx = DeviceArray(rt, np.full(max(cap, 1), 0xffffffffffffffff, dtype=np.uint64))
+    assert rt.hipDeviceSynchronize() == 0
     total = C.c_size_t(0)
     st = N.lib().sph_neighbors_find_noalloc(sh._ctx.h, sh.cell_size, float(search_range),
-                                            dpos.data_ptr() if n else None, n,
-                                            dsmp.data_ptr() if m else None, m,
-                                            doff.data_ptr(), didx.data_ptr(), cap,
-                                            C.byref(total))
-    torch.cuda.synchronize()
-    off = doff.cpu().numpy().view(np.uint64)
-    idx = didx.cpu().numpy().view(np.uint64)[:int(total.value)]
+                                            dpos.p if n else None, n,
+                                            dsmp.p if m else None, m,
+                                            doff.p, didx.p, cap, C.byref(total))
+    assert rt.hipDeviceSynchronize() == 0
+    off = doff.get(np.uint64, n + 1)
+    idx = didx.get(np.uint64, min(int(total.value), cap))
     return st, off, idx, int(total.value)
 
 
@@ -68,51 +93,49 @@ def case_inputs(name):
     raise KeyError(name)
 
 
-def compare(torch, name):
+def compare(rt, name):
     cs, rng_, pos, smp = case_inputs(name)
     print(f"no-alloc case {name}: n={len(pos)} m={len(smp)} cell={cs} range={rng_}", flush=True)
     sh = system.SpatialHashing(cs)
     ref = sh.find_samples(rng_, pos, smp)
-    st, off, idx, total = noalloc(torch, sh, rng_, pos, smp)
+    st, off, idx, total = noalloc(rt, sh, rng_, pos, smp)
     assert st == 0, N.lib().sph_last_error()
     assert total == len(ref.indices) and total > 0
     assert np.array_equal(off, ref.offsets)
     assert np.array_equal(idx, ref.indices)
 
 
-def test_two_calls_in_a_row_second_smaller(torch_cuda):
-    compare(torch_cuda, "random_large")
-    compare(torch_cuda, "lattice_boundary")
-    compare(torch_cuda, "wide_range_small")
+def test_two_calls_in_a_row_second_smaller(hip):
+    compare(hip, "random_large")
+    compare(hip, "lattice_boundary")
+    compare(hip, "wide_range_small")
 
 
-def test_empty_inputs(torch_cuda):
+def test_empty_inputs(hip):
     sh = system.SpatialHashing(0.1)
     print("no-alloc case empty inputs", flush=True)
-    st, off, idx, total = noalloc(torch_cuda, sh, 0.1, np.zeros((0, 3)), np.zeros((5, 3)))
+    st, off, idx, total = noalloc(hip, sh, 0.1, np.zeros((0, 3)), np.zeros((5, 3)))
     assert st == 0 and total == 0 and np.array_equal(off, np.zeros(1, np.uint64))
-    st, off, idx, total = noalloc(torch_cuda, sh, 0.1, np.zeros((4, 3)), np.zeros((0, 3)))
+    st, off, idx, total = noalloc(hip, sh, 0.1, np.zeros((4, 3)), np.zeros((0, 3)))
     assert st == 0 and total == 0 and np.array_equal(off, np.zeros(5, np.uint64))
 
 
-def test_capacity_error_reports_total(torch_cuda):
+def test_capacity_error_reports_total(hip):
     cs, rng_, pos, smp = case_inputs("wide_range_small")
     print("no-alloc case capacity", flush=True)
     sh = system.SpatialHashing(cs)
     ref = sh.find_samples(rng_, pos, smp)
-    st, off, _, total = noalloc(torch_cuda, sh, rng_, pos, smp, cap=10)
+    st, off, _, total = noalloc(hip, sh, rng_, pos, smp, cap=10)
     assert st == -4 and total == len(ref.indices)
     assert np.array_equal(off, ref.offsets)
 
 
-def test_sizes_beyond_the_static_limit_are_refused(torch_cuda):
+def test_sizes_beyond_the_static_limit_are_refused(hip):
     sh = system.SpatialHashing(0.1)
     print("no-alloc case over the limit", flush=True)
     total = C.c_size_t(7)
     # refused before any device access: the pointers are never dereferenced
-    buf = torch_cuda.zeros(8, dtype=torch_cuda.float64, device="cuda")
-    torch_cuda.cuda.synchronize()
-    st = N.lib().sph_neighbors_find_noalloc(sh._ctx.h, 0.1, 0.1, buf.data_ptr(), MAX_POINTS + 1,
-                                            buf.data_ptr(), 3, buf.data_ptr(), buf.data_ptr(),
-                                            8, C.byref(total))
+    buf = DeviceArray(hip, nbytes=64)
+    st = N.lib().sph_neighbors_find_noalloc(sh._ctx.h, 0.1, 0.1, buf.p, MAX_POINTS + 1,
+                                            buf.p, 3, buf.p, buf.p, 8, C.byref(total))
     assert st == -5 and b"262144" in N.lib().sph_last_error()
