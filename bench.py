@@ -1,21 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the SPH particle hot path on one MI355X.
 
-Workload: the two-fluid large ball, 1,000,000 particles (a 100^3 lattice at
-dx = 0.01 m split at x = 0 into rest densities 1000 and 1500) in a radius-1 m
-spherical volume-map boundary, IISPHwOST + TakePredicted, CFL dt
-(lambda 0.4, max 1e-3), target average density error 0.1 %.
+Workload: the two-fluid large ball, 1,000,000 particles (the lattice at
+dx = 0.0127 m filling a radius-1 m spherical volume-map boundary from the
+bottom to just below the equator, split at x = 0 into rest densities 1000 and
+1500), IISPHwOST + TakePredicted, CFL dt (lambda 0.4, max 1e-3), target
+average density error 0.1 %. The fluid rests on the wall from the first step.
 
 A step is one SPHSystem::step_forward_in_time on the device-resident state.
-W warm-up steps run untimed, then exactly K steps are timed between device
-synchronizations. value = particles x K / wall time (particle-steps/s).
+--settle untimed steps (default 60) let the fluid compress until every solve
+carries its weight (EQS2 then takes ~18 Jacobi iterations per step) and let
+the code-shape tuner finish; W warm-up steps follow, then exactly K steps are
+timed between device synchronizations. value = particles x K / wall time
+(particle-steps/s).
 
 The JSON line also carries
-  roofline      the dominant kernel's algorithmic HBM bytes per launch divided
-                by its mean launch duration (HIP events on the library's own
-                stream, over the timed steps), against 8 TB/s;
+  roofline      the dominant kernel's compulsory HBM bytes per launch (single
+                precision fields, F32_BYTES) divided by its mean launch
+                duration (HIP events on the library's own stream, over the
+                timed steps), against 8 TB/s; the same with this build's
+                binary64 fields beside it;
+  step_roofline the whole update's compulsory bytes per particle-step at the
+                measured iteration count, times the particle-step rate;
   cpu_baseline  the CPU oracle (binary64 restatement of the reference) timed
-                on this host over a bounded sample of the same workload.
+                on this host over a bounded sample of the same workload,
+                started from the GPU's state at the timed window's start.
 
 --workload serving runs the serving stream instead: seeded Poisson arrivals
 of IISPH scene requests of mixed sizes served by continuous batching (one
@@ -68,6 +77,29 @@ ALGO_BYTES = {
 }
 
 
+# Compulsory HBM bytes per particle per launch with single-precision SoA
+# fields (each field read once, outputs written once; neighbor data from cache
+# after the cell sort; neighbor indices not counted). This is the roofline
+# count; the binary64 count above (this build's storage) is reported beside it.
+F32_BYTES = {
+    "neighbor_count": 32,   # volume pass
+    "pairs_volume": 32,
+    "boundary_sample": 28,
+    "nonpressure": 56,      # viscosity (gravity fused)
+    "source_diag": 72,      # predicted velocity + source + diagonal + init
+    "pressure_accel": 52,   # per Jacobi iteration
+    "jacobi": 60,           # Ap + update + error
+}
+
+
+def step_bytes_f32(solver, iterations):
+    """Compulsory bytes per particle-step of the whole update (single
+    precision, static boundary): 384 fixed + 112 per Jacobi iteration, plus
+    112 for IISPHwOST's second pressure acceleration and predicted x/v."""
+    extra = 112 if solver == "IISPHwOST" else 0
+    return 384 + 112 * iterations + extra
+
+
 # Algorithmic binary64 operations per SDF evaluation in the volume-map ball
 # quadrature (+, -, *, /, sqrt each one; common factors counted once): the
 # serendipity interpolation 247 (cell and local coordinates 29, shape-function
@@ -81,12 +113,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_scene(workload, side):
+def build_scene(workload, particles):
     from sph_amd import scenes
     if workload == "two_fluid_ball":
-        return scenes.two_fluid_ball(side=side, solver="IISPHwOST")
+        return scenes.two_fluid_ball(n=particles, solver="IISPHwOST")
     if workload == "two_fluid_ball_iisph":
-        return scenes.two_fluid_ball(side=side, solver="IISPH")
+        return scenes.two_fluid_ball(n=particles, solver="IISPH")
     if workload == "block_drop":
         return scenes.block_drop_iisph()
     if workload == "rigid_coupling":
@@ -107,30 +139,34 @@ def traffic_from_profiles(kernel):
         return None
 
 
-def cpu_baseline(scene, budget_s, threads):
-    """The oracle on this host's cores over a bounded sample of the workload."""
+def cpu_baseline(scene, budget_s, threads, start_step=0):
+    """The oracle on this host's cores over a bounded sample of the workload,
+    started from `scene` (the state the GPU's timed window starts from)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcpu
     t0 = time.perf_counter()
     o = refcpu.OracleSystem(scene, threads=threads)
     setup = time.perf_counter() - t0
-    steps, elapsed = 0, 0.0
+    steps, elapsed, iters, errs = 0, 0.0, [], []
     while steps < 50 and (steps == 0 or elapsed < budget_s):
         t = time.perf_counter()
         o.step(1)
         elapsed += time.perf_counter() - t
         steps += 1
-    m = o.measure()
+        m = o.measure()
+        iters.append(m["solver_iterations"] + m["eqs1_iterations"])
+        errs.append(m["density_error"])
+    where = (f"the GPU's state at step {start_step} (the timed window's start), restarted "
+             f"as continue_from_checkpoint does" if start_step else "the initial state")
     return {
         "value": scene.n * steps / elapsed,
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle (binary64 C restatement of the reference CPU path, OpenMP) on the "
-                   f"same {scene.n}-particle initial state: the first {steps} step(s) after "
-                   f"System::new, {elapsed:.1f} s of stepping ({setup:.1f} s setup untimed); "
-                   f"IISPH iterations {m['solver_iterations']}, density error "
-                   f"{m['density_error']:.4f} %"),
+        "sample": (f"oracle (binary64 C restatement of the reference CPU path, OpenMP) on "
+                   f"{where}, {scene.n} particles: {steps} step(s), {elapsed:.1f} s of stepping "
+                   f"({setup:.1f} s setup untimed); Jacobi iterations per step "
+                   f"{np.mean(iters):.1f}, density error max {max(errs):.4f} %"),
     }
 
 
@@ -191,7 +227,7 @@ def sensor_plane_bench(args):
     """get_quantity_at_positions on a 1024^2 plane through the 1M-particle ball."""
     from sph_amd import system
     from sph_amd._native import QUANTITIES
-    scene = build_scene("two_fluid_ball", args.side)
+    scene = build_scene("two_fluid_ball", args.particles)
     sim = system.System(scene)
     for _ in range(args.warmup):
         sim.step_forward_in_time(1)
@@ -546,7 +582,11 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="two_fluid_ball")
-    ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M particles)")
+    ap.add_argument("--particles", type=int, default=1_000_000,
+                    help="two-fluid ball: particles")
+    ap.add_argument("--settle", type=int, default=60,
+                    help="untimed steps before the warm-up (the fluid settles into a loaded "
+                         "state; the code-shape tuner finishes)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -585,29 +625,49 @@ def main():
         sensor_plane_bench(args)
         return
 
+    import dataclasses
     from sph_amd import system
-    scene = build_scene(args.workload, args.side)
+    scene = build_scene(args.workload, args.particles)
     log(f"[bench] scene {scene.name}: {scene.n} particles")
     sim = system.System(scene)
+    # settle: the first steps let the resting fluid compress until every
+    # solve carries its weight, and the code-shape tuner finishes (one timed
+    # step per shape); neither is part of the timed window, whatever --warmup is
+    settled = 0
+    while settled < args.settle or sim.tuning()["state"] != 2:
+        sim.step_forward_in_time(1)
+        settled += 1
+        if settled > args.settle + 100:
+            raise SystemExit("the code-shape tuner did not finish")
     for _ in range(args.warmup):
         sim.step_forward_in_time(1)
+    start_step = settled + args.warmup
+    start_state = sim.state() if not args.no_cpu else None
     sim.take_measurement()  # drains the stream
-    sim.profile(True)
-    lat = []
-    iters = []
-    errs = []
+
+    def run(k):
+        lat, iters, errs = [], [], []
+        for _ in range(k):
+            t = time.perf_counter()
+            sim.step_forward_in_time(1)   # returns after the step's last device read-back
+            lat.append(time.perf_counter() - t)
+            m = sim.take_measurement()
+            iters.append(m["solver_iterations"] + m["eqs1_iterations"])
+            errs.append(m["density_error"])
+        return lat, iters, errs
+
+    # the timed window: the production path (Jacobi chunks replayed from HIP
+    # graphs, no per-kernel events). Each step call returns after the step's
+    # final device read-back, so the sum of the step latencies is the
+    # device-synchronized time of the K steps.
     t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        sim.step_forward_in_time(1)   # returns after the step's last device read-back
-        lat.append(time.perf_counter() - t)
-        m = sim.take_measurement()
-        iters.append(m["solver_iterations"] + m["eqs1_iterations"])
-        errs.append(m["density_error"])
-    # each step call returns after the step's final device read-back, so the
-    # sum of the step latencies is the device-synchronized time of the K steps
+    lat, iters, errs = run(args.steps)
     total = float(np.sum(lat))
     wall = time.perf_counter() - t_start
+    # per-kernel durations: the next K steps again with HIP events around every
+    # kernel launch (which launches the Jacobi chunks directly)
+    sim.profile(True)
+    plat, piters, _ = run(args.steps)
     prof = sim.profile_read()
     sim.profile(False)
     n = len(sim)
@@ -616,10 +676,14 @@ def main():
     nb = len(scene.boundaries)
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]
-    per_unit = ALGO_BYTES[dom]["base"] + ALGO_BYTES[dom]["per_body"] * nb
-    bytes_per_launch = per_unit * d["units"] / max(d["launches"], 1)
+    units = d["units"] / max(d["launches"], 1)
     avg_s = d["ms"] / 1e3 / max(d["launches"], 1)
+    bytes_per_launch = F32_BYTES[dom] * units
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    per_unit64 = ALGO_BYTES[dom]["base"] + ALGO_BYTES[dom]["per_body"] * nb
+    achieved64 = per_unit64 * units / avg_s / 1e9 if avg_s > 0 else 0.0
+    mean_iters = float(np.mean(iters))
+    step_bytes = step_bytes_f32(scene.params.solver, mean_iters)
 
     out = {
         "metric": "particle-steps/sec (IISPH, 1M particles) at <=0.1% density error; p50/p99 step latency",
@@ -640,12 +704,21 @@ def main():
         "config": {"workload": scene.name, "particles": n, "solver": scene.params.solver,
                    "integrator": scene.params.integrator, "dt": "CFL 0.4, max 1e-3",
                    "target_density_error_pct": scene.params.target_density_error,
+                   "timed_steps": [start_step + 1, start_step + args.steps],
                    "parallelism": "single GPU"},
         "density_error_pct_max": float(max(errs)),
         "density_error_pct_last": float(errs[-1]),
-        "jacobi_iterations_per_step_mean": float(np.mean(iters)),
+        "jacobi_iterations_per_step_mean": mean_iters,
+        "jacobi_iterations_per_step_range": [int(min(iters)), int(max(iters))],
+        "step_roofline": {"bytes_per_particle_step": step_bytes,
+                          "achieved": step_bytes * value / 1e9, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": step_bytes * value / 1e9 / HBM_PEAK_GBS},
         "code_shapes": sim.tuning(),
         "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()},
+        "profiled_window": {"ms_per_step": float(np.mean(plat)) * 1e3,
+                            "jacobi_iterations_per_step_mean": float(np.mean(piters)),
+                            "note": "the K steps after the timed window, with HIP events "
+                                    "around every kernel (Jacobi chunks launched directly)"},
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -654,9 +727,14 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": (traffic_from_profiles(dom)
-                        if args.workload == "two_fluid_ball" and args.side == 100 else None),
+                        if args.workload == "two_fluid_ball" and args.particles == 1_000_000
+                        else None),
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "bytes_per_particle": F32_BYTES[dom],
             "avg_launch_ms": avg_s * 1e3,
+            "binary64_bytes_per_particle": per_unit64,
+            "achieved_binary64": achieved64,
+            "frac_binary64": achieved64 / HBM_PEAK_GBS,
         },
         "cpu_baseline": None,
     }
@@ -669,7 +747,13 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         try:
-            out["cpu_baseline"] = cpu_baseline(scene, args.cpu_seconds, threads)
+            base = scene
+            if not any(getattr(b, "rigid", None) is not None for b in scene.boundaries):
+                base = dataclasses.replace(scene, x=start_state["x"], v=start_state["v"],
+                                           m=start_state["m"], fluid_id=start_state["fluid_id"])
+            else:
+                start_step = 0
+            out["cpu_baseline"] = cpu_baseline(base, args.cpu_seconds, threads, start_step)
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline unavailable: {e}")
     print(json.dumps(out))

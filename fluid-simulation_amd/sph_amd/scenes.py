@@ -440,12 +440,23 @@ def dam_break_sesph(side=20, seed=0, f32_state=True) -> Scene:
     return Scene("sesph_dam_break", p, x, v, m, fid, [StaticSamples(samples)])
 
 
-def block_drop_iisph(side=50, seed=0, dx=0.01, box=1.0) -> Scene:
-    """IISPH block drop: side^3 particles at dx in a box-shaped VolumeMapBoundary;
-    relaxed Jacobi omega 0.5, target 0.1 %; CFL dt (0.4, max 1e-3)."""
+# Clearance (in particle spacings) between the fluid and a volume-map wall at
+# which the wall's volume-map contribution leaves a lattice particle at about
+# the rest volume: the cubic extension of the signed distance counts the band
+# within h of the wall as boundary (volume_map_boundary.rs:1716-1726), so
+# particles closer than ~1.2 dx start compressed (V0 / V = 2.1 at dx / 2).
+WALL_CLEARANCE = 1.2
+
+
+def block_drop_iisph(side=50, seed=0, dx=0.01, box=1.0, lift=0.0) -> Scene:
+    """IISPH block drop: side^3 particles at dx in a box-shaped VolumeMapBoundary,
+    the block centred in x and y and standing on the floor (``lift`` = 0: its
+    lowest layer WALL_CLEARANCE spacings above it), so that the pressure solve
+    carries the block's weight from the first steps while the block collapses
+    sideways; relaxed Jacobi omega 0.5, target 0.1 %; CFL dt (0.4, max 1e-3)."""
     rng = np.random.default_rng(seed)
     edge = side * dx
-    lo = [0.5 * (box - edge), 0.5 * (box - edge), 0.3 * box]
+    lo = [0.5 * (box - edge), 0.5 * (box - edge), lift + (WALL_CLEARANCE - 0.5) * dx]
     pos = lattice_block(lo, [side] * 3, dx, rng, 1e-4)
     x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
     h = 2 * dx
@@ -458,15 +469,35 @@ def block_drop_iisph(side=50, seed=0, dx=0.01, box=1.0) -> Scene:
                  [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))])
 
 
-def two_fluid_ball(side=100, seed=0, dx=0.01, radius=1.0, solver="IISPHwOST") -> Scene:
-    """Two-fluid large ball: a side^3 cube (1M particles at side=100) centred in
-    a radius-1 m spherical VolumeMapBoundary, split at x = 0 into rest
-    densities 1000 and 1500; IISPHwOST + TakePredicted (or IISPH + EulerCromer);
-    CFL dt (0.4, max 1e-3); uniform gravity."""
+def lower_ball_fill(n, dx, radius, rng, jitter, clearance=WALL_CLEARANCE):
+    """The n lowest points of the lattice ((i + 1/2) dx per axis, centred on
+    the origin) lying at least clearance * dx inside the sphere of ``radius``:
+    the sphere filled from the bottom, its top layer a disc around the
+    vertical axis. Seeded uniform jitter of +-jitter per coordinate."""
+    k = np.arange(-int(radius / dx) - 1, int(radius / dx) + 1, dtype=np.float64)
+    c = (k + 0.5) * dx
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=-1)
+    pos = pos[np.linalg.norm(pos, axis=-1) <= radius - clearance * dx]
+    order = np.lexsort((np.hypot(pos[:, 0], pos[:, 1]), pos[:, 2]))
+    if n > len(order):
+        raise ValueError(f"{n} particles do not fit in the sphere at dx = {dx}")
+    pos = pos[np.sort(order[:n])]
+    if jitter > 0:
+        pos = pos + rng.uniform(-jitter, jitter, size=pos.shape)
+    return pos
+
+
+def two_fluid_ball(n=1_000_000, seed=0, dx=0.0127, radius=1.0, solver="IISPHwOST") -> Scene:
+    """Two-fluid large ball: n particles (1M by default) filling a radius-1 m
+    spherical VolumeMapBoundary from the bottom (dx = 0.0127 m puts 1M
+    particles just below the equator), split at x = 0 into rest densities 1000
+    and 1500; at rest on the container wall from the first step, so every
+    pressure solve carries the fluid's weight while the heavier half slides
+    under the lighter one. IISPHwOST + TakePredicted (or IISPH + EulerCromer);
+    CFL dt (0.4, max 1e-3); uniform gravity; seeded +-1e-4 m jitter."""
     rng = np.random.default_rng(seed)
-    edge = side * dx
-    lo = np.array([-0.5 * edge, -0.5 * edge, -0.5 * edge])
-    pos = lattice_block(lo, [side] * 3, dx, rng, 1e-4)
+    pos = lower_ball_fill(n, dx, radius, rng, 1e-4)
     left = pos[:, 0] < 0.0
     x, v, m, fid = _fluid([(pos[left], 1000.0, 0), (pos[~left], 1500.0, 1)], dx)
     h = 2 * dx

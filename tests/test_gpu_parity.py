@@ -279,8 +279,8 @@ def test_coincident_particles_stall_iisph_loudly(native):
 
 
 FULL_SIZE = {
-    "ball_IISPHwOST": lambda: scenes.two_fluid_ball(side=100, solver="IISPHwOST"),
-    "ball_IISPH": lambda: scenes.two_fluid_ball(side=100, solver="IISPH"),
+    "ball_IISPHwOST": lambda: scenes.two_fluid_ball(solver="IISPHwOST"),
+    "ball_IISPH": lambda: scenes.two_fluid_ball(solver="IISPH"),
     "block_drop": scenes.block_drop_iisph,
     "rigid_coupling": scenes.rigid_coupling,
 }
