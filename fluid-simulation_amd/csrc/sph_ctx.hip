@@ -411,6 +411,11 @@ struct Scene {
     bool in_world = false;
     double dt = 0, time = 0;
     uint64_t steps = 0;
+    // fixed dt: time = time_offset + (steps - offset_steps) * dt, the
+    // reference's time() without the cfl_time_step feature (sph/mod.rs:107-116);
+    // CFL dt: time accumulates (sph/mod.rs:134-139)
+    double time_offset = 0;
+    uint64_t offset_steps = 0;
     uint32_t last_iter = 0, eqs1_iter = 0;
     double last_err = 0, avg_density = 0;
     std::vector<double> err_hist;
@@ -2213,7 +2218,10 @@ static void step_batch(Batch &B) {
         for (auto &b : c->bodies)
             if (b->dynamic) b->rb.step(dt);
         c->steps++;
-        c->time += dt;
+        if (c->P.cfl)
+            c->time += dt;
+        else
+            c->time = c->time_offset + (double)(c->steps - c->offset_steps) * c->P.time_increment;
     }
     update(B);
     tune_end_step(B, B.total_particles);
@@ -2435,6 +2443,8 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
         init_boundaries(c->own, d);
         d.dt = s.dt;
         d.time = s.time;
+        d.time_offset = s.time_offset;
+        d.offset_steps = s.offset_steps;
         d.steps = s.steps;
         d.last_iter = s.last_iter;
         d.eqs1_iter = s.eqs1_iter;
@@ -2783,6 +2793,8 @@ int sph_system_init(sph_ctx *c) {
         S.dt = S.P.cfl ? 0.0 : S.P.time_increment;
         S.time = 0;
         S.steps = 0;
+        S.time_offset = 0;
+        S.offset_steps = 0;
         update(solo(c));
         S.initialized = true;
     });
@@ -2948,6 +2960,10 @@ int sph_system_restore(sph_ctx *c, uint64_t time_steps_propagated, double curren
         S.dt = S.P.cfl ? 0.0 : S.P.time_increment;
         S.time = current_time;
         S.steps = time_steps_propagated;
+        // a system built from a checkpoint takes its time as the offset
+        // (new_boxed, sph/mod.rs:408-417)
+        S.time_offset = current_time;
+        S.offset_steps = time_steps_propagated;
         update(solo(c));
         S.initialized = true;
     });

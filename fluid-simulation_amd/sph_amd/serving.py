@@ -1,8 +1,9 @@
 """Continuous-batching service for many SPH scenes on one GPU.
 
-The reference runs one simulation per worker process and steps it alone
-(``sci-phi-backend`` -> ``simulation_worker``: one ``Box<dyn SPHSystem>`` per
-job, ``System::step_forward_in_time`` in a loop). Here a stream of scene
+The reference runs one simulation at a time on one worker thread and steps
+it alone (``worker_loop``, sci-phi-backend/src/lib.rs:507, a thread spawned by
+the UI, sci-phi/src/app/mod.rs:236: one ``Box<dyn SPHSystem>`` whose
+``step_forward_in_time`` runs in a loop). Here a stream of scene
 requests is served by one process and one device:
 
 * a request is admitted when a slot and its device-memory estimate fit the

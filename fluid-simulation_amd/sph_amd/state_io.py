@@ -28,10 +28,10 @@ RON is the text serde produces through ron 0.12: structs as
 pretty-printed form (whitespace, trailing commas, comments) and the
 ``Name(...)`` struct prefix.
 
-The fluid checkpoint holds the particle arrays in the system's own particle
-order (the reference keeps them in its storage order; this library keeps them
-sorted by cell), so a restored system carries the same particles, not the
-same array order.
+The fluid checkpoint holds the particle arrays in the reference's particle
+order: the library keeps its particles sorted by cell on the device and
+scatters them back to the reference's index order when it downloads them, so
+a checkpoint written here lists the particles as the reference's would.
 """
 from __future__ import annotations
 

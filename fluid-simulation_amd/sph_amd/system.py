@@ -473,8 +473,9 @@ def _tuning(get) -> dict:
 class World:
     """Many systems advanced together by shared multi-system launches.
 
-    The reference runs one ``System`` per simulation job
-    (simulation_worker/src/lib.rs). A world steps every attached system by
+    The reference steps one ``System`` at a time on its worker thread
+    (``worker_loop``, sci-phi-backend/src/lib.rs:507, spawned by
+    sci-phi/src/app/mod.rs:236). A world steps every attached system by
     whole time steps in one sequence of launches; each system's results are
     identical to stepping it alone. Attached systems are stepped only through
     the world (their own ``step_forward_in_time`` raises until detached).

@@ -276,8 +276,9 @@ int sph_pressure_solve(sph_ctx *ctx, size_t n, const double *x, const double *v,
                        double *vpred_out, sph_measurement *info_out);
 
 /* ---- worlds: many systems advanced in shared launches ------------------------
- * The reference runs one System per simulation worker
- * (simulation_worker/src/lib.rs: one Box<dyn SPHSystem> per job). A world
+ * The reference steps one System at a time on its worker thread
+ * (worker_loop, sci-phi-backend/src/lib.rs:507, a thread spawned by
+ * sci-phi/src/app/mod.rs:236). A world
  * advances every attached context by whole time steps in one sequence of
  * multi-system launches: each launch covers all systems' particles, and every
  * per-system result (state, step size, iteration counts, error history,
@@ -304,8 +305,8 @@ int sph_world_profile(sph_world *world, int enable);
 int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
                            uint64_t *units, double *ms);
 /* ---- serving: continuous batching over a world ----------------------------------
- * No counterpart in simulation_lib, whose worker runs one system per job
- * (simulation_worker/src/lib.rs). A server admits requests in submission
+ * No counterpart in the reference, whose worker thread steps one system at a
+ * time (worker_loop, sci-phi-backend/src/lib.rs:507). A server admits requests in submission
  * order while a slot and the HBM budget allow, keeps admitted systems resident
  * in one world and steps them together, and hands back each system once it has
  * taken its requested steps.

@@ -583,6 +583,9 @@ struct ref_system {
     double dt;
     double time;
     uint64_t steps;
+    /* fixed dt: time = time_offset + (steps - offset_steps) * dt (sph/mod.rs:107-116) */
+    double time_offset;
+    uint64_t offset_steps;
     size_t n, cap;
     double *x, *v, *a, *m, *V, *p;
     uint32_t *fid;
@@ -2035,7 +2038,10 @@ int ref_system_step(ref_system *s, int nsteps) {
             if (b->kind == 0) ss_update_positions(b);
         }
         s->steps++;
-        s->time += dt;
+        if (s->P.cfl) /* cfl_time_step: current_time += dt (sph/mod.rs:134-139) */
+            s->time += dt;
+        else
+            s->time = s->time_offset + (double)(s->steps - s->offset_steps) * s->P.time_increment;
         update(s);
     }
     return 0;

@@ -332,3 +332,16 @@ def test_per_trait_calls_bit_identical(native, refcpu):
         res = g.solve_and_add_acceleration(st["x"], st["v"], st["m"], V, a_np,
                                            m["time_step_size"])
         assert np.array_equal(res["p"], st["p"]) and np.array_equal(res["a"], st["a"])
+
+
+def test_dynamic_static_sample_bodies_are_refused(native):
+    """Moving StaticSampleBoundary bodies (static_sample_boundary.rs:79-130) are
+    not built on the device: adding one fails with SPH_ERR_UNSUPPORTED and a
+    message naming the alternative, instead of simulating it as static."""
+    from sph_amd import _native as N
+    sc = scenes.small_iisph_box(side=6, jitter=2e-3)
+    rb = scenes.Rigid(mass=1.0, inertia_body=np.eye(3), com=np.array([0.3, 0.3, 0.3]))
+    sc.boundaries.append(scenes.StaticSamples(positions=np.zeros((4, 3)), rigid=rb))
+    with pytest.raises(N.SphError, match="dynamic StaticSampleBoundary") as e:
+        native.System(sc)
+    assert e.value.status == -5

@@ -138,3 +138,24 @@ def test_recording_of_a_run(native, tmp_path):
     dens = steps[1]["fluid"]["coloring"]["data"]
     assert dens.dtype == np.float32 and np.all(dens > 0)
     assert os.path.getsize(path) > 0 and st["x"].shape == (sc.n, 3)
+
+
+def test_fixed_step_time_is_offset_plus_steps_times_dt(native, tmp_path):
+    """Without CFL the reference reports time_offset + (steps - offset_steps) *
+    time_increment (sph/mod.rs:107-116), not a running sum: after 10 steps of
+    5e-4 the time is exactly 10 * 5e-4; a system built from a checkpoint takes
+    the saved time as its offset (sph/mod.rs:408-417). The oracle agrees."""
+    sc = scenes.dam_break_sesph(side=8)
+    dt = sc.params.time_increment
+    a = native.System(sc)
+    a.step_forward_in_time(10)
+    assert a.take_measurement()["time"] == 10 * dt
+    o = refcpu.OracleSystem(sc)
+    o.step(10)
+    assert o.measure()["time"] == 10 * dt
+    path = str(tmp_path / "state.ron")
+    state_io.save_checkpoint(a.get_checkpoint(), path, "ron")
+    b = native.System.from_checkpoint(sc, state_io.load_checkpoint(path))
+    b.step_forward_in_time(7)
+    t0 = state_io.load_checkpoint(path).current_time
+    assert b.take_measurement()["time"] == t0 + 7 * dt

@@ -515,3 +515,13 @@ def test_rigid_step_integrates_momenta():
     assert np.allclose(r.ang_mom, [0.0, 0.0, 1.0])
     assert np.allclose(r.ang_vel, [0.0, 0.0, 0.25])  # I_zz = 4 at the start orientation
     assert np.all(r.force == 0) and np.all(r.torque == 0)
+
+
+def test_fixed_step_time_is_steps_times_dt():
+    """sph/mod.rs:107-116 without cfl_time_step: time = offset + steps * dt, so
+    after 10 steps of 1e-3 the time is 10 * 1e-3 = 0.01 exactly (a running sum
+    gives 0.010000000000000002)."""
+    o = make([[0.0, 0.0, 0.0]], time_increment=1e-3)
+    o.step(10)
+    t = o.measure()["time"]
+    assert t == 10 * 1e-3 and t == 0.01
