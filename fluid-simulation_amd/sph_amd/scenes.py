@@ -567,6 +567,28 @@ def small_iisph_box(side=12, seed=1, dx=0.02, box=0.6, solver="IISPH", jitter=1e
                  [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))])
 
 
+def pool(width=0.24, layers=40, dx=0.01, seed=4, solver="IISPH", jitter=1e-4) -> Scene:
+    """A pool at rest: a box container of the given width filled wall to wall
+    (WALL_CLEARANCE spacings from every wall) with ``layers`` lattice layers,
+    so that every pressure solve carries the column's weight from the first
+    steps and nothing collapses sideways (CPU-oracle sized for long runs)."""
+    rng = np.random.default_rng(seed)
+    c = WALL_CLEARANCE * dx
+    nxy = int(math.floor((width - 2 * c) / dx + 1e-9))
+    off = 0.5 * (width - nxy * dx)
+    pos = lattice_block([off, off, c - 0.5 * dx], [nxy, nxy, layers], dx, rng, jitter)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    integ = "TakePredicted" if solver == "IISPHwOST" else "EulerCromer"
+    p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+               max_time_increment=1e-3, cfl_number=0.4, solver=solver, integrator=integ)
+    height = max(width, 2.0 * layers * dx)
+    lo_b, hi_b = np.zeros(3), np.array([width, width, height])
+    sdf, vm = volume_map_fields(lambda q: sd_box_container(q, lo_b, hi_b), lo_b, hi_b, h, dx)
+    return Scene(f"pool_{solver}", p, x, v, m, fid,
+                 [VolumeMap(sdf, vm, None, "box", aabb_of(lo_b, hi_b))])
+
+
 def small_rigid_scene(side=(14, 14, 8), seed=2, dx=0.02, solver="IISPH") -> Scene:
     """Small two-way coupling case: fluid layer + one dynamic cube."""
     rng = np.random.default_rng(seed)
