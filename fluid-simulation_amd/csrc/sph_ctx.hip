@@ -416,6 +416,13 @@ struct Scene {
     // CFL dt: time accumulates (sph/mod.rs:134-139)
     double time_offset = 0;
     uint64_t offset_steps = 0;
+    // a scene whose step failed (in a world, the others continue): the status
+    // and message; its state is not valid after the failing step
+    int failed = 0;
+    std::string fail_msg;
+    // Jacobi iterations after which a solve that has not met its termination
+    // test fails (the reference has no cap, iisph.rs:387-390)
+    uint32_t max_iter = 100000u;
     uint32_t last_iter = 0, eqs1_iter = 0;
     double last_err = 0, avg_density = 0;
     std::vector<double> err_hist;
@@ -660,6 +667,11 @@ struct sph_server {
     std::vector<Req> admitted;       // admitted, context not yet attached
     std::vector<Req> resident;       // attached to the world
     std::deque<Req> finished;        // out of the world, not yet polled
+    struct Failed {
+        Req req;
+        int status;
+    };
+    std::deque<Failed> failed;       // out of the world after a failed step
     sph_world *world = nullptr;
 };
 
@@ -1545,6 +1557,8 @@ static void resolve_pressure(Batch &B, int vp, int after) {
     std::vector<uint32_t> ablk, amap;
     while (any) {
         chunk = std::min(std::max(chunk, 1), kMaxChunk);
+        for (Scene *c : B.sc)  // no scene iterates past its cap
+            if (c->active) chunk = std::min<int>(chunk, (int)(c->max_iter - c->iter));
         // launch over the scenes still iterating only
         ablk.assign(1, 0u);
         amap.clear();
@@ -1585,8 +1599,15 @@ static void resolve_pressure(Batch &B, int vp, int after) {
                 }
             c.active = alive[s] != 0;
             most = std::max(most, c.iter);
-            if (c.active && c.iter >= 100000u)
-                throw Error(SPH_ERR_UNSUPPORTED, "pressure solve did not converge");
+            if (c.active && c.iter >= c.max_iter) {
+                // this scene fails; the others keep iterating
+                c.failed = SPH_ERR_UNSUPPORTED;
+                c.fail_msg = "pressure solve did not converge within " +
+                             std::to_string(c.max_iter) + " iterations";
+                c.active = false;
+                const uint32_t zero = 0;
+                B.put(B.d_active() + s, &zero, sizeof zero);
+            }
             any = any || c.active;
         }
         chunk = 2;
@@ -1692,7 +1713,18 @@ static void prepare(Batch &B) {
 }
 
 // System::update (sph/mod.rs:674-731) for every scene of the batch
+// the first failure of a scene in the last step, or nullptr
+static const Scene *first_failed(const Batch &B) {
+    for (const Scene *c : B.sc)
+        if (c->failed) return c;
+    return nullptr;
+}
+
 static void update(Batch &B) {
+    for (Scene *c : B.sc) {
+        c->failed = 0;
+        c->fail_msg.clear();
+    }
     layout(B);
     std::vector<double> vmax, below;
     speed_below(B, vmax, below);
@@ -2199,6 +2231,11 @@ static void contact_step(Batch &B, Scene &S, double dt) {
 }
 
 // integrator + rigid bodies of every scene, then update (sph/mod.rs:125-144)
+// a context steps alone: its failure is the call's error
+static void throw_if_failed(const Batch &B) {
+    if (const Scene *c = first_failed(B)) throw Error(c->failed, c->fail_msg);
+}
+
 static void step_batch(Batch &B) {
     layout(B);
     tune_begin_step(B, B.total_particles);
@@ -2444,6 +2481,7 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
         d.dt = s.dt;
         d.time = s.time;
         d.time_offset = s.time_offset;
+        d.max_iter = s.max_iter;
         d.offset_steps = s.offset_steps;
         d.steps = s.steps;
         d.last_iter = s.last_iter;
@@ -2796,6 +2834,7 @@ int sph_system_init(sph_ctx *c) {
         S.time_offset = 0;
         S.offset_steps = 0;
         update(solo(c));
+        throw_if_failed(c->own);
         S.initialized = true;
     });
 }
@@ -2806,7 +2845,10 @@ int sph_step(sph_ctx *c, int nsteps, sph_measurement *out) {
         check_solo(c);
         if (!c->scene.initialized) throw Error(SPH_ERR_STATE, "sph_system_init has not run");
         Batch &B = solo(c);
-        for (int k = 0; k < nsteps; k++) step_batch(B);
+        for (int k = 0; k < nsteps; k++) {
+            step_batch(B);
+            throw_if_failed(B);
+        }
         if (out) fill_measurement(c, out);
     });
 }
@@ -2965,7 +3007,16 @@ int sph_system_restore(sph_ctx *c, uint64_t time_steps_propagated, double curren
         S.time_offset = current_time;
         S.offset_steps = time_steps_propagated;
         update(solo(c));
+        throw_if_failed(c->own);
         S.initialized = true;
+    });
+}
+
+int sph_system_status(const sph_ctx *c, int *status) {
+    return guarded([&] {
+        if (!c || !status) throw Error(SPH_ERR_ARG, "null argument");
+        *status = c->scene.failed;
+        if (c->scene.failed) set_error(c->scene.fail_msg.c_str());
     });
 }
 
@@ -3081,8 +3132,11 @@ int sph_pressure_solve(sph_ctx *c, size_t n, const double *x, const double *v, c
         const double keep_dt = S.dt;
         S.dt = dt;
         upload_views(c->own);
+        S.failed = 0;
+        S.fail_msg.clear();
         solve_pressure(c->own);
         S.dt = keep_dt;
+        throw_if_failed(c->own);
         std::vector<double> A, B_, C_;
         download_field(c, S.ax, A);
         download_field(c, S.ay, B_);
@@ -3173,7 +3227,19 @@ int sph_world_step(sph_world *w, int nsteps) {
         B.sc.clear();
         for (sph_ctx *c : w->members) B.sc.push_back(&c->scene);
         if (B.sc.empty()) return;
-        for (int k = 0; k < nsteps; k++) step_batch(B);
+        for (int k = 0; k < nsteps; k++) {
+            step_batch(B);
+            // a failing member stops the call after this step: the others
+            // have completed it and stay valid (sph_system_status names the
+            // failed ones)
+            for (size_t m = 0; m < w->members.size(); m++)
+                if (w->members[m]->scene.failed) {
+                    B.sync();
+                    throw Error(w->members[m]->scene.failed,
+                                "system " + std::to_string(m) + " of the world: " +
+                                    w->members[m]->scene.fail_msg);
+                }
+        }
         B.sync();
     });
 }
@@ -3184,6 +3250,7 @@ static void server_stats(const sph_server *s, sph_server_stats *o) {
     o->admitted = (uint64_t)s->slots;
     o->resident = s->resident.size();
     o->finished = s->finished.size();
+    o->failed = s->failed.size();
     o->bytes = s->used;
     o->world_steps = s->world_steps;
 }
@@ -3253,27 +3320,80 @@ int sph_server_attach(sph_server *s, uint64_t ticket, sph_ctx *ctx) {
     });
 }
 
+// One world step of every resident request. A request whose step fails (its
+// pressure solve did not converge within its iteration cap) leaves the world
+// for the failed queue and the others count the step as done; an error that
+// is not confined to one request (a HIP error) fails every resident request.
 int sph_server_step(sph_server *s, sph_server_stats *out) {
     return guarded([&] {
         if (!s) throw Error(SPH_ERR_ARG, "null argument");
         if (!s->resident.empty()) {
             int e = sph_world_step(s->world, 1);
-            if (e) throw Error(e, sph_last_error());
+            bool confined = e == SPH_OK;
+            if (e) {
+                std::string msg = sph_last_error();
+                for (const sph_server::Req &r : s->resident)
+                    if (r.ctx->scene.failed) confined = true;
+                if (!confined) {  // every resident request fails with the world's error
+                    (void)hipStreamSynchronize(s->world->batch.s);
+                    for (sph_server::Req &r : s->resident) {
+                        auto &m = s->world->members;
+                        m.erase(std::remove(m.begin(), m.end(), r.ctx), m.end());
+                        r.ctx->scene.in_world = false;
+                        s->failed.push_back({r, e});
+                    }
+                    s->resident.clear();
+                    throw Error(e, msg);
+                }
+            }
             s->world_steps++;
             std::vector<sph_server::Req> keep;
             for (sph_server::Req &r : s->resident) {
-                if (++r.done < r.steps) {
+                const int f = r.ctx->scene.failed;
+                if (!f && ++r.done < r.steps) {
                     keep.push_back(r);
                     continue;
                 }
                 e = sph_world_detach(s->world, r.ctx);
                 if (e) throw Error(e, sph_last_error());
-                s->finished.push_back(r);
+                if (f)
+                    s->failed.push_back({r, f});
+                else
+                    s->finished.push_back(r);
             }
             s->resident.swap(keep);
         }
         if (out) server_stats(s, out);
     });
+}
+
+int sph_server_cancel(sph_server *s, uint64_t ticket) {
+    return guarded([&] {
+        if (!s) throw Error(SPH_ERR_ARG, "null argument");
+        auto it = std::find_if(s->admitted.begin(), s->admitted.end(),
+                               [&](const sph_server::Req &r) { return r.ticket == ticket; });
+        if (it == s->admitted.end()) throw Error(SPH_ERR_STATE, "ticket is not awaiting a system");
+        s->slots--;
+        s->used -= it->bytes;
+        s->admitted.erase(it);
+    });
+}
+
+int sph_server_poll_failed(sph_server *s, uint64_t *ticket, sph_ctx **ctx, int *status) {
+    int got = 0;
+    int st = guarded([&] {
+        if (!s || !ticket || !ctx || !status) throw Error(SPH_ERR_ARG, "null argument");
+        if (s->failed.empty()) return;
+        const sph_server::Failed f = s->failed.front();
+        s->failed.pop_front();
+        s->slots--;
+        s->used -= f.req.bytes;
+        *ticket = f.req.ticket;
+        *ctx = f.req.ctx;
+        *status = f.status;
+        got = 1;
+    });
+    return st ? st : got;
 }
 
 int sph_server_poll(sph_server *s, uint64_t *ticket, sph_ctx **ctx) {
@@ -3371,6 +3491,10 @@ static int get_option(const Batch &B, int option) {
 int sph_get_option(const sph_ctx *c, int option, int *value) {
     return guarded([&] {
         if (!c || !value) throw Error(SPH_ERR_ARG, "null argument");
+        if (option == SPH_OPT_MAX_ITERATIONS) {
+            *value = (int)c->scene.max_iter;
+            return;
+        }
         *value = get_option(c->own, option);
     });
 }
@@ -3385,6 +3509,11 @@ int sph_world_get_option(const sph_world *w, int option, int *value) {
 int sph_set_option(sph_ctx *c, int option, int value) {
     return guarded([&] {
         if (!c) throw Error(SPH_ERR_ARG, "null argument");
+        if (option == SPH_OPT_MAX_ITERATIONS) {
+            if (value < 1) throw Error(SPH_ERR_ARG, "the iteration cap must be >= 1");
+            c->scene.max_iter = (uint32_t)value;
+            return;
+        }
         // every step and per-trait call rebuilds the pair cache before using it
         set_option(c->own, option, value);
     });

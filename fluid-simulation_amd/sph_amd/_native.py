@@ -64,7 +64,7 @@ class SphRigidInit(C.Structure):
 
 class SphServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("queued", "admitted", "resident", "finished",
-                                           "bytes", "world_steps")]
+                                           "bytes", "world_steps", "failed")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -134,6 +134,10 @@ SIGNATURES = [
     ("sph_server_attach", C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     ("sph_server_step", C.c_int, [C.c_void_p, C.POINTER(SphServerStats)]),
     ("sph_server_poll", C.c_int, [C.c_void_p, _u64p, C.POINTER(C.c_void_p)]),
+    ("sph_server_cancel", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("sph_server_poll_failed", C.c_int, [C.c_void_p, _u64p, C.POINTER(C.c_void_p),
+                                         C.POINTER(C.c_int)]),
+    ("sph_system_status", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("sph_server_read_stats", C.c_int, [C.c_void_p, C.POINTER(SphServerStats)]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
     ("sph_mesh_sdf_build", C.c_int, [C.c_void_p, _dp, C.c_size_t, _u32p, C.c_size_t, _dp,
@@ -155,6 +159,7 @@ OPT_TUNE_STATE = 3
 OPT_SHAPE_VOLUME = 4
 OPT_GRAPHS = 9
 OPT_SWEEP = 10
+OPT_MAX_ITERATIONS = 11
 SHAPES = 9
 PAIR_PASSES = ["volume", "nonpressure", "source_diag", "pressure_accel", "jacobi"]
 

@@ -50,6 +50,9 @@ class Request:
     arrival: float                  # seconds after the stream starts
     scene: S.Scene
     steps: int
+    # Jacobi iterations after which its pressure solve fails (None: the
+    # library's default); a failing request is reported, the others go on
+    max_iterations: Optional[int] = None
 
 
 @dataclasses.dataclass
@@ -63,6 +66,7 @@ class Result:
     measurement: dict
     digest: str                     # sha256 of the final x, v (reference order)
     state: Optional[dict] = None
+    error: Optional[str] = None     # set for a failed request (no valid state)
 
 
 @dataclasses.dataclass
@@ -78,12 +82,17 @@ class Report:
 
     @property
     def particle_steps(self) -> int:
-        return sum(r.n * r.steps for r in self.results.values())
+        return sum(r.n * r.steps for r in self.results.values() if r.error is None)
+
+    @property
+    def failed(self) -> Dict[int, Result]:
+        return {k: r for k, r in self.results.items() if r.error is not None}
 
     def summary(self) -> dict:
         lat = np.asarray(self.step_ms) if self.step_ms else np.zeros(1)
-        req = np.asarray([r.finished - r.arrival for r in self.results.values()] or [0.0])
-        derr = [r.measurement["density_error"] for r in self.results.values()]
+        ok = [r for r in self.results.values() if r.error is None]
+        req = np.asarray([r.finished - r.arrival for r in ok] or [0.0])
+        derr = [r.measurement["density_error"] for r in ok]
         return {
             "mode": self.mode,
             "requests": len(self.results),
@@ -100,6 +109,7 @@ class Report:
             "peak_resident_scenes": self.peak_resident,
             "peak_resident_bytes_est": self.peak_bytes,
             "density_error_pct_max": float(max(derr)) if derr else 0.0,
+            "failed_requests": len(self.results) - len(ok),
         }
 
 
@@ -155,6 +165,10 @@ def _finish(req: Request, sim, admitted: float, now: float, keep_state: bool) ->
                   sim.take_measurement(), _digest(st), st if keep_state else None)
 
 
+def _failed(req: Request, sim, admitted: float, now: float, error: str) -> Result:
+    return Result(req.rid, len(sim), req.steps, req.arrival, admitted, now, {}, "", None, error)
+
+
 def _check_fits(req: Request, budget: int):
     need = estimate_device_bytes(req.scene)
     if need > budget:
@@ -200,7 +214,18 @@ class BatchedServer:
                         break
                     req = waiting.pop(t)
                     a0 = self.clock()
-                    srv.attach(t, system.System(req.scene))
+                    sim = system.System(req.scene, init=False)
+                    if req.max_iterations is not None:
+                        sim.set_max_iterations(req.max_iterations)
+                    try:
+                        sim.init()
+                    except system.N.SphError as e:  # System::new's update failed
+                        srv.cancel(t)
+                        now = self.clock() - t0
+                        results[req.rid] = _failed(req, sim, now, now, str(e))
+                        sim._ctx.close()
+                        continue
+                    srv.attach(t, sim)
                     admit += self.clock() - a0
                     admitted[t] = [req, self.clock() - t0]
                 st = srv.stats()
@@ -223,6 +248,15 @@ class BatchedServer:
                     req, t_adm = admitted.pop(t)
                     results[req.rid] = _finish(req, sim, t_adm, self.clock() - t0,
                                                self.keep_state)
+                    sim._ctx.close()
+                while True:
+                    got = srv.poll_failed()
+                    if got is None:
+                        break
+                    t, sim, status = got
+                    req, t_adm = admitted.pop(t)
+                    results[req.rid] = _failed(req, sim, t_adm, self.clock() - t0,
+                                               f"status {status}: {sim.status()[1]}")
                     sim._ctx.close()
         finally:
             srv.close()
@@ -250,15 +284,24 @@ class SequentialServer:
             if wait > 0:
                 time.sleep(wait)
             a0 = self.clock()
-            sim = system.System(req.scene)
-            admit += self.clock() - a0
-            t_adm = self.clock() - t0
-            for _ in range(req.steps):
-                s0 = self.clock()
-                sim.step_forward_in_time(1)
-                dt = self.clock() - s0
-                busy += dt
-                step_ms.append(dt * 1e3)
+            sim = system.System(req.scene, init=False)
+            if req.max_iterations is not None:
+                sim.set_max_iterations(req.max_iterations)
+            try:
+                sim.init()
+                admit += self.clock() - a0
+                t_adm = self.clock() - t0
+                for _ in range(req.steps):
+                    s0 = self.clock()
+                    sim.step_forward_in_time(1)
+                    dt = self.clock() - s0
+                    busy += dt
+                    step_ms.append(dt * 1e3)
+            except system.N.SphError as e:  # this request fails, the stream goes on
+                now = self.clock() - t0
+                results[req.rid] = _failed(req, sim, now, now, str(e))
+                sim._ctx.close()
+                continue
             results[req.rid] = _finish(req, sim, t_adm, self.clock() - t0, self.keep_state)
             sim._ctx.close()
         makespan = max(r.finished for r in results.values()) - min(r.arrival for r in stream)

@@ -178,6 +178,10 @@ class System:
         if init:
             N.check(L.sph_system_init(self._ctx.h))
 
+    def init(self):
+        """``System::new_boxed``'s update (after ``init=False``)."""
+        N.check(N.lib().sph_system_init(self._ctx.h))
+
     @property
     def handle(self):
         return self._ctx.h
@@ -276,6 +280,18 @@ class System:
     def tuning(self) -> dict:
         """Tuner state and the code shape of each neighbor pass."""
         return _tuning(self.get_option)
+
+    def set_max_iterations(self, cap: int):
+        """Jacobi iterations after which a pressure solve that has not met its
+        termination test fails (the reference has no cap)."""
+        self.set_option(N.OPT_MAX_ITERATIONS, int(cap))
+
+    def status(self):
+        """(0, "") or (status, message) of the last step's failure."""
+        v = C.c_int()
+        N.check(N.lib().sph_system_status(self._ctx.h, C.byref(v)))
+        msg = N.lib().sph_last_error() if v.value else b""
+        return int(v.value), (msg.decode() if msg else "")
 
     # ---- checkpoints (sph/mod.rs:362-378, 884-915) -------------------------
     def get_checkpoint(self):
@@ -594,6 +610,10 @@ class Server:
         N.check(N.lib().sph_server_attach(self.h, int(ticket), sim.handle))
         self._sims[int(ticket)] = sim
 
+    def cancel(self, ticket: int):
+        """Give up an admitted ticket whose System could not be built."""
+        N.check(N.lib().sph_server_cancel(self.h, int(ticket)))
+
     def step(self) -> dict:
         out = N.SphServerStats()
         N.check(N.lib().sph_server_step(self.h, C.byref(out)))
@@ -608,6 +628,17 @@ class Server:
             return None
         sim = self._sims.pop(int(t.value), None)
         return int(t.value), sim
+
+    def poll_failed(self):
+        """(ticket, System, status) of the oldest failed request, or None; the
+        System's state is not valid."""
+        t, c, f = C.c_uint64(), C.c_void_p(), C.c_int()
+        st = N.lib().sph_server_poll_failed(self.h, C.byref(t), C.byref(c), C.byref(f))
+        N.check(st if st < 0 else 0)
+        if st != 1:
+            return None
+        sim = self._sims.pop(int(t.value), None)
+        return int(t.value), sim, int(f.value)
 
     def stats(self) -> dict:
         out = N.SphServerStats()

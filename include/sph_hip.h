@@ -297,8 +297,14 @@ int sph_world_destroy(sph_world *world);
 /* ctx must have run sph_system_init */
 int sph_world_attach(sph_world *world, sph_ctx *ctx);
 int sph_world_detach(sph_world *world, sph_ctx *ctx);
-/* SPHSystem::step_forward_in_time for every attached system, nsteps times */
+/* SPHSystem::step_forward_in_time for every attached system, nsteps times.
+ * If a system's step fails (see sph_system_status), the call returns its
+ * status after completing that step for every other system, which stays
+ * valid; the failed system should be detached. */
 int sph_world_step(sph_world *world, int nsteps);
+/* 0, or the status with which the context's last step failed (the message
+ * becomes sph_last_error()); its state is then not valid. */
+int sph_system_status(const sph_ctx *ctx, int *status);
 int sph_world_size(const sph_world *world, size_t *systems, size_t *particles);
 /* kernel timing of the world's launches (as sph_profile / sph_profile_read) */
 int sph_world_profile(sph_world *world, int enable);
@@ -327,6 +333,7 @@ typedef struct sph_server_stats {
     uint64_t finished;        /* finished, waiting for sph_server_poll */
     uint64_t bytes;           /* estimates of the admitted requests */
     uint64_t world_steps;     /* world steps taken */
+    uint64_t failed;          /* failed, waiting for sph_server_poll_failed */
 } sph_server_stats;
 int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out);
 /* frees the world; contexts still attached are detached, not destroyed */
@@ -339,9 +346,22 @@ int sph_server_submit(sph_server *server, uint64_t device_bytes, uint64_t steps,
 int sph_server_admit(sph_server *server, uint64_t *ticket);
 /* ctx (after sph_system_init) serves an admitted ticket; it joins the world */
 int sph_server_attach(sph_server *server, uint64_t ticket, sph_ctx *ctx);
+/* gives up an admitted ticket whose system could not be built (e.g. its
+ * System::new update failed), releasing its slot and bytes */
+int sph_server_cancel(sph_server *server, uint64_t ticket);
 /* one world step if any system is resident; systems that have taken their
- * steps leave the world for the finished queue. *out may be NULL. */
+ * steps leave the world for the finished queue. A system whose step fails
+ * (its pressure solve does not meet the termination test within its
+ * SPH_OPT_MAX_ITERATIONS cap) leaves for the failed queue and the others
+ * complete the step (the call returns SPH_OK); an error not confined to one
+ * system (a HIP error) moves every resident system to the failed queue and is
+ * returned. *out may be NULL. */
 int sph_server_step(sph_server *server, sph_server_stats *out);
+/* returns 1 with the oldest failed ticket, its context (detached, now the
+ * caller's, its state not valid) and the failure status, releasing its slot
+ * and bytes; 0 if none has failed. No counterpart in the reference, whose
+ * solve loops without a cap. */
+int sph_server_poll_failed(sph_server *server, uint64_t *ticket, sph_ctx **ctx, int *status);
 /* returns 1 with the oldest finished ticket and its context (detached, now
  * the caller's), releasing its slot and bytes; 0 if none has finished */
 int sph_server_poll(sph_server *server, uint64_t *ticket, sph_ctx **ctx);
@@ -426,6 +446,10 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *   SPH_OPT_GRAPHS  1 = replay repeated Jacobi iteration chunks from HIP
  *       graphs (default for a context), 0 = launch every kernel directly
  *       (default for a world).
+ *   SPH_OPT_MAX_ITERATIONS  (context) Jacobi iterations after which a pressure
+ *       solve that has not met its termination test fails with
+ *       SPH_ERR_UNSUPPORTED (default 100000; the reference has no cap,
+ *       iisph.rs:387-390).
  *   SPH_OPT_SWEEP  1 = neighbor list and volume from one cell sweep into a
  *       fixed-width pair cache (default), 0 = separate count, fill and
  *       volume passes into a slice-width pair cache. The pair cache with
@@ -441,7 +465,8 @@ enum {
     SPH_OPT_TUNE_STATE = 3,
     SPH_OPT_SHAPE_VOLUME = 4,
     SPH_OPT_GRAPHS = 9,
-    SPH_OPT_SWEEP = 10
+    SPH_OPT_SWEEP = 10,
+    SPH_OPT_MAX_ITERATIONS = 11
 };
 enum { SPH_SHAPES = 9 };
 int sph_set_option(sph_ctx *ctx, int option, int value);

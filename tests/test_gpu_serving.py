@@ -44,3 +44,79 @@ def test_sequential_baseline_matches_batched():
     for rid in seq.results:
         assert seq.results[rid].digest == bat.results[rid].digest
         assert seq.results[rid].measurement == bat.results[rid].measurement
+
+
+TWINS = np.array([3, 77, 200, 200, 411])
+
+
+def twin_scene():
+    """Particles at exactly the same positions never separate under IISPH,
+    so every pressure solve stalls above its target."""
+    from sph_amd import scenes
+    sc = scenes.small_iisph_box(side=8, jitter=2e-3)
+    for k in ("x", "v", "m", "fluid_id"):
+        setattr(sc, k, np.concatenate([getattr(sc, k), getattr(sc, k)[TWINS]]))
+    return sc
+
+
+def test_world_member_failure_is_confined():
+    """A world member whose solve exceeds its iteration cap fails that world
+    step alone: the call reports it (sph_system_status names it), the other
+    member has completed the step and equals its solo run bit for bit, and
+    the world steps on after the failed member is detached."""
+    from sph_amd import _native as N
+    from sph_amd import scenes, system
+    pool = scenes.pool(solver="IISPH")  # 3 iterations at first, more once loaded
+    other = scenes.small_iisph_box(side=10, jitter=2e-3)
+    a = system.System(pool, init=False)
+    a.set_max_iterations(3)
+    a.init()
+    b = system.System(other)
+    w = system.World([a, b])
+    steps = 0
+    print("world with a capped member: stepping until it fails", flush=True)
+    with pytest.raises(N.SphError, match="did not converge within 3 iterations"):
+        while steps < 400:
+            w.step(1)
+            steps += 1
+    steps += 1  # the failing step completed for the other member
+    assert a.status()[0] == -5 and b.status()[0] == 0
+    print(f"  member failed in step {steps}", flush=True)
+    w.detach(a)
+    w.step(5)
+    w.detach(b)
+    solo = system.System(other)
+    solo.step_forward_in_time(steps + 5)
+    sb, ss = b.state(), solo.state()
+    for k in ("x", "v", "a", "V", "p"):
+        assert np.array_equal(sb[k], ss[k]), k
+    w.close()
+
+
+def test_served_stream_isolates_failing_requests():
+    """One request fails in System::new (twin particles, cap 50), one fails
+    mid-run (a pool capped at 3 iterations once it is loaded); both are
+    reported failed with their status and the other requests finish
+    bit-identical to their solo runs, in the batched and in the sequential
+    server."""
+    from sph_amd import scenes
+    good = serving.make_stream(n_requests=2, rate=1000.0, n_min=1500, n_max=4000, steps=30,
+                               seed=21)
+    bad = [serving.Request(10, 0.0, twin_scene(), 30, max_iterations=50),
+           serving.Request(11, 0.0, scenes.pool(solver="IISPH"), 150, max_iterations=3)]
+    stream = good + bad
+    print(f"stream with failing requests: {[r.scene.n for r in stream]}", flush=True)
+    for server in (serving.BatchedServer(1 << 34, max_slots=8, keep_state=True),
+                   serving.SequentialServer(keep_state=True)):
+        rep = server.run(stream)
+        assert set(rep.results) == {r.rid for r in stream}
+        assert set(rep.failed) == {10, 11}
+        for rid in (10, 11):
+            assert "did not converge" in rep.results[rid].error
+        for req in good:
+            st, meas = solo_result(req)
+            got = rep.results[req.rid]
+            assert got.error is None
+            for k in ("x", "v", "a", "V", "p"):
+                assert np.array_equal(got.state[k], st[k]), (req.rid, k)
+        assert rep.summary()["failed_requests"] == 2
