@@ -17,7 +17,8 @@ namespace sph {
                    amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1)))
 
 // scene of workgroup lb: the largest s with blk[s] <= lb (block-uniform)
-__device__ __forceinline__ int scene_of(const uint32_t *blk, int ns, uint32_t lb) {
+template <class P>
+__device__ __forceinline__ int scene_of(P blk, int ns, uint32_t lb) {
     int lo = 0, hi = ns;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
@@ -26,6 +27,24 @@ __device__ __forceinline__ int scene_of(const uint32_t *blk, int ns, uint32_t lb
     }
     return lo;
 }
+
+// DESC: the launch reads B from device memory over a grid rounded up from
+// the batch's B.pblk[B.ns] workgroups (the form a captured graph replays);
+// workgroups past them return at once
+// (the pointers of a B read from memory are marked global: the compiler can
+// infer that only for kernel arguments)
+#define PARTICLE_THREAD_G(B, DESC)                                         \
+    bool ok_ = true;                                                       \
+    const unsigned lb = DESC ? xcd_block_of(gm((B).pblk)[(B).ns], ok_) : xcd_block(); \
+    if (!ok_) return;                                                      \
+    const int sk_ = scene_of(gm((B).pblk), (B).ns, lb);                    \
+    const int sid = (B).smap ? (int)gm((B).smap)[sk_] : sk_;               \
+    const auto &v = gm((B).views)[sid];                                    \
+    const size_t i = (size_t)(lb - gm((B).pblk)[sk_]) * kBlock + threadIdx.x;  \
+    const bool live = i < v.n;                                             \
+    const size_t gi = (size_t)v.part_base + i;                             \
+    (void)gi;                                                              \
+    (void)live
 
 #define PARTICLE_THREAD(B)                                                 \
     const unsigned lb = xcd_block();                                       \
@@ -42,7 +61,7 @@ __device__ __forceinline__ int scene_of(const uint32_t *blk, int ns, uint32_t lb
 // neighbors 2r and 2r+1 of a lane are adjacent (one 8-byte load reads both):
 // neighbor k of lane l of slice s sits at 64 slice_off[s] + 128 (k / 2) +
 // 2 l + k % 2. Slices hold an even number of rows.
-#define ELL_BASE(B, v, i) ((size_t)(B).slice_off[(v).slice_base + ((i) >> 6)] * 64 + 2 * ((i)&63))
+#define ELL_BASE(B, v, i) ((size_t)gm((B).slice_off)[(v).slice_base + ((i) >> 6)] * 64 + 2 * ((i)&63))
 #define ELL_AT(e, k) ((e) + 128 * ((size_t)(k) >> 1) + ((k)&1))
 
 // The field pointers of the scene and body tables are read from device
@@ -709,12 +728,13 @@ __global__ void kb_sesph_pressure(BatchDev B) {
 // acceleration or the solver's scratch (IISPH pressure_acc_f). Scenes whose
 // solve has converged are skipped when the target is the scratch.
 // ------------------------------------------------------------------------
-template <bool CACHED, int UNROLL, int WAVES, int PIPE>
-__global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror) {
+template <bool CACHED, int UNROLL, int WAVES, int PIPE, bool DESC>
+__device__ __forceinline__ void pressure_accel_impl(const BatchDev &B, int target_scratch,
+                                                    int overwrite, int mirror) {
     __shared__ double lds[6 * kBlock];
-    PARTICLE_THREAD(B);
-    if (target_scratch && !B.active[sid]) return;  // uniform per workgroup
-    const KernelConst K = v.K;
+    PARTICLE_THREAD_G(B, DESC);
+    if (target_scratch && !gm(B.active)[sid]) return;  // uniform per workgroup
+    const KernelConst K = *(const KernelConst *)&v.K;
     const double rh = refined_rcp(K.h);  // 1/h refined once per thread
     const double gamma = v.S.gamma;
     double px = 0, py = 0, pz = 0, Vi = 0, mi = 1, pi = 0;
@@ -739,7 +759,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
             });
     }
     for (uint32_t bi = 0; bi < v.nb; bi++) {
-        const BodyDev &b = B.bodies[v.body_base + bi];
+        const auto &b = gm(B.bodies)[v.body_base + bi];
         double f[6] = {0., 0., 0., 0., 0., 0.};
         if (live) {
             if (b.kind == 0) {
@@ -773,7 +793,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
         if (mirror && b.dynamic) {
             block_sum<6>(f, lds);
             if (threadIdx.x == 0) {
-                double *dst = B.partials + ((size_t)lb * kMaxBodiesPerScene + bi) * 6;
+                auto dst = gm(B.partials) + ((size_t)lb * kMaxBodiesPerScene + bi) * 6;
                 for (int k = 0; k < 6; k++) dst[k] = f[k];
             }
         }
@@ -792,6 +812,19 @@ __global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratc
             tz_[i] += acz;
         }
     }
+}
+
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
+__global__ PAIR_PASS_BOUNDS void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite,
+                                                   int mirror) {
+    pressure_accel_impl<CACHED, UNROLL, WAVES, PIPE, false>(B, target_scratch, overwrite, mirror);
+}
+
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
+__global__ PAIR_PASS_BOUNDS void kb_pressure_accel_d(const BatchDev *__restrict__ B, int target_scratch,
+                                                     int overwrite, int mirror) {
+    const BatchDev Bl = *B;  // loaded before any store: its pointers stay global, uniform
+    pressure_accel_impl<CACHED, UNROLL, WAVES, PIPE, true>(Bl, target_scratch, overwrite, mirror);
 }
 
 // v* = (v or v*) + dt a   (pressure_solver/mod.rs:90-108)
@@ -874,14 +907,14 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
 
 // relaxed Jacobi update and per-workgroup predicted density error
 // (iisph.rs:440-531); converged scenes are skipped
-template <bool CACHED, int UNROLL, int WAVES, int PIPE>
-__global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
+template <bool CACHED, int UNROLL, int WAVES, int PIPE, bool DESC>
+__device__ __forceinline__ void jacobi_impl(const BatchDev &B) {
     __shared__ double lds[kBlock];
-    PARTICLE_THREAD(B);
-    if (!B.active[sid]) return;  // uniform per workgroup
-    const KernelConst K = v.K;
+    PARTICLE_THREAD_G(B, DESC);
+    if (!gm(B.active)[sid]) return;  // uniform per workgroup
+    const KernelConst K = *(const KernelConst *)&v.K;
     const double rh = refined_rcp(K.h);  // 1/h refined once per thread
-    const SolveConst S = v.S;
+    const SolveConst S = *(const SolveConst *)&v.S;
     double err = 0.;
     if (live) {
         const rec4 ai = ld4(v.pa4, i);
@@ -901,7 +934,7 @@ __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
                 adp += dt2 * r.q.w * dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, wx, wy, wz);
             });
         for (uint32_t bi = 0; bi < v.nb; bi++) {
-            const BodyDev &b = B.bodies[v.body_base + bi];
+            const auto &b = gm(B.bodies)[v.body_base + bi];
             if (b.kind == 0) {
                 for (uint32_t k = gm(b.soff)[i]; k < gm(b.soff)[i + 1]; k++) {
                     uint32_t s = gm(b.sidx)[k];
@@ -925,6 +958,17 @@ __global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
     double r[1] = {err};
     block_sum<1>(r, lds);
     if (threadIdx.x == 0) gm(B.partials)[lb] = r[0];
+}
+
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
+__global__ PAIR_PASS_BOUNDS void kb_jacobi(BatchDev B) {
+    jacobi_impl<CACHED, UNROLL, WAVES, PIPE, false>(B);
+}
+
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
+__global__ PAIR_PASS_BOUNDS void kb_jacobi_d(const BatchDev *__restrict__ B) {
+    const BatchDev Bl = *B;  // loaded before any store: its pointers stay global, uniform
+    jacobi_impl<CACHED, UNROLL, WAVES, PIPE, true>(Bl);
 }
 
 // IISPHwOST: x* = x + dt v*   (iisph_optimized_source_term.rs:103-113)
@@ -981,30 +1025,41 @@ __global__ void kb_fold_sum(const double *partials, const uint32_t *blk, int ns,
     if (threadIdx.x == 0) out[(size_t)s * rows + r] = v[0];
 }
 
-__global__ void kb_fold_converge(BatchDev B, int after, int k) {
+template <bool DESC>
+__device__ __forceinline__ void fold_converge_impl(const BatchDev &B, int after, int k) {
     __shared__ double lds[kBlock];
     const int sk = blockIdx.x;
-    const int s = B.smap ? (int)B.smap[sk] : sk;
+    if (DESC && sk >= B.ns) return;  // grid rounded up (uniform per workgroup)
+    const int s = B.smap ? (int)gm(B.smap)[sk] : sk;
     double acc = 0.0;  // the shape of kb_fold_sum with one row
-    for (uint32_t q = B.pblk[sk] + threadIdx.x; q < B.pblk[sk + 1]; q += kBlock)
-        acc += B.partials[q];
+    for (uint32_t q = gm(B.pblk)[sk] + threadIdx.x; q < gm(B.pblk)[sk + 1]; q += kBlock)
+        acc += gm(B.partials)[q];
     double r[1] = {acc};
     block_sum<1>(r, lds);
     if (threadIdx.x == 0) {
         const size_t slot = (size_t)k * B.ns_all + s;
-        if (B.active[s]) {
-            const SceneView &v = B.views[s];
+        if (gm(B.active)[s]) {
+            const auto &v = gm(B.views)[s];
             const double err = r[0] / (double)v.n * 100.0;
-            const uint32_t it = B.iters[s] + 1;
-            B.iters[s] = it;
-            B.chunk_err[slot] = err;
-            B.chunk_ran[slot] = 1;
+            const uint32_t it = gm(B.iters)[s] + 1;
+            gm(B.iters)[s] = it;
+            gm(B.chunk_err)[slot] = err;
+            gm(B.chunk_ran)[slot] = 1;
             const bool done = after >= 0 ? it > (uint32_t)after : (it > 2 && err < v.target_err);
-            if (done) B.active[s] = 0;
+            if (done) gm(B.active)[s] = 0;
         } else {
-            B.chunk_ran[slot] = 0;
+            gm(B.chunk_ran)[slot] = 0;
         }
     }
+}
+
+__global__ void kb_fold_converge(BatchDev B, int after, int k) {
+    fold_converge_impl<false>(B, after, k);
+}
+
+__global__ void kb_fold_converge_d(const BatchDev *__restrict__ B, int after, int k) {
+    const BatchDev Bl = *B;
+    fold_converge_impl<true>(Bl, after, k);
 }
 
 __global__ void kb_fold_speed(const double *partials, const uint32_t *blk, int ns, double *out) {
@@ -1057,7 +1112,9 @@ __global__ void kb_fold_bbox(const int *partials, const uint32_t *blk, int ns, i
     template __global__ void kb_nonpressure<C, U, W, P>(BatchDev);                      \
     template __global__ void kb_pressure_accel<C, U, W, P>(BatchDev, int, int, int);    \
     template __global__ void kb_source_diag<C, U, W, P>(BatchDev, int);                 \
-    template __global__ void kb_jacobi<C, U, W, P>(BatchDev);
+    template __global__ void kb_jacobi<C, U, W, P>(BatchDev);                            \
+    template __global__ void kb_pressure_accel_d<C, U, W, P>(const BatchDev *, int, int, int); \
+    template __global__ void kb_jacobi_d<C, U, W, P>(const BatchDev *);
 #define INSTANTIATE_BOTH(U, W, P) \
     INSTANTIATE_PAIR_PASSES(true, U, W, P) INSTANTIATE_PAIR_PASSES(false, U, W, P)
 SPH_PAIR_VARIANTS(INSTANTIATE_BOTH)

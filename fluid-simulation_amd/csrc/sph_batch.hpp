@@ -120,6 +120,15 @@ template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_source_diag(BatchDev B, int vp);
 template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_jacobi(BatchDev B);
+// the Jacobi chunk's kernels reading the batch from device memory over a
+// grid rounded up from B->pblk[B->ns] workgroups (B->ns for the fold): the
+// launches a captured graph replays whatever the batch's layout
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
+__global__ void kb_pressure_accel_d(const BatchDev *B, int target_scratch, int overwrite,
+                                    int mirror);
+template <bool CACHED, int UNROLL, int WAVES, int PIPE>
+__global__ void kb_jacobi_d(const BatchDev *B);
+__global__ void kb_fold_converge_d(const BatchDev *B, int after, int k);
 __global__ void kb_pred_pos(BatchDev B);
 __global__ void kb_ost_commit(BatchDev B);
 __global__ void kb_measure(BatchDev B);

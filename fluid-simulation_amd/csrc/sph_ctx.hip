@@ -489,7 +489,7 @@ struct Scene {
 };
 
 constexpr int kMaxChunk = 16;      // Jacobi iterations launched between host waits
-constexpr size_t kMaxGraphs = 16;  // executable Jacobi-chunk graphs per batch
+constexpr size_t kMaxGraphs = 64;  // executable Jacobi-chunk graphs per batch
 
 // ------------------------------------------------------------------------
 // online tuning of the pair passes' code shapes
@@ -553,9 +553,8 @@ struct Batch {
     uint32_t *d_active() const { return chunk_ran() + (size_t)kMaxChunk * sc.size(); }
     uint32_t *d_iters() const { return d_active() + sc.size(); }
     uint32_t iter_hint = 3;  // iterations of the last target-terminated solve
-    // Jacobi chunks captured as HIP graphs, keyed by their launch arguments (on
-    // by default for a context, off for a world, whose layout changes with
-    // every admission)
+    // Jacobi chunks replayed from HIP graphs (on by default for contexts and
+    // worlds)
     bool use_graphs = true;
     struct ChunkGraph {
         std::string key;
@@ -565,6 +564,9 @@ struct Batch {
     std::vector<std::string> seen;   // keys launched once without a graph
     DBuf<SceneView> views;
     DBuf<BodyDev> bodies;
+    // the batch descriptor that graph-replayed Jacobi chunks read (never
+    // reallocated while graphs exist: graphs capture its address)
+    DBuf<BatchDev> desc;
     // utility scratch (standalone queries and static samples)
     DBuf<int> bbox;
     DBuf<uint32_t> ucnt, ucursor;
@@ -1479,51 +1481,92 @@ static void jacobi_chunk_launches(Batch &B, const BatchDev &d, size_t grid, int 
     }
 }
 
-// A chunk whose launch arguments repeat (the same layout, buffers, code shapes
-// and chunk length) is replayed from a HIP graph: one submission instead of
-// three per iteration. A key is captured on its second use; timing and tuning
-// runs launch directly. Executable graphs live until the batch ends or graphs
-// are switched off (none is destroyed while stepping); once kMaxGraphs exist,
-// new keys launch directly.
+// The chunk's launches in the form a graph replays: the kernels read the batch
+// descriptor from B.desc (uploaded before each chunk) and run over grids
+// rounded up to powers of two (gp workgroups for the pair passes, gs for the
+// per-scene fold), so their arguments do not depend on the layout.
+typedef void (*KernDIII)(const BatchDev *, int, int, int);
+typedef void (*KernD)(const BatchDev *);
+#define PAD_F(U, W, P) &kb_pressure_accel_d<false, U, W, P>,
+#define PAD_T(U, W, P) &kb_pressure_accel_d<true, U, W, P>,
+#define JAD_F(U, W, P) &kb_jacobi_d<false, U, W, P>,
+#define JAD_T(U, W, P) &kb_jacobi_d<true, U, W, P>,
+static const KernDIII k_pressure_accel_d_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(PAD_F)},
+                                                                  {SPH_PAIR_VARIANTS(PAD_T)}};
+static const KernD k_jacobi_d_tab[2][kPairVariants] = {{SPH_PAIR_VARIANTS(JAD_F)},
+                                                       {SPH_PAIR_VARIANTS(JAD_T)}};
+
+static unsigned grid_bucket(size_t n) {
+    unsigned g = 8;
+    while (g < n) g <<= 1;
+    return g;
+}
+
+static void jacobi_chunk_desc_launches(Batch &B, unsigned gp, unsigned gs, int after, int chunk) {
+    const int gc = B.grad_cache ? 1 : 0;
+    const KernDIII pa = k_pressure_accel_d_tab[gc][B.variant[PP_PRESSURE_ACCEL]];
+    const KernD ja = k_jacobi_d_tab[gc][B.variant[PP_JACOBI]];
+    const BatchDev *dd = B.desc.p;
+    for (int k = 0; k < chunk; k++) {
+        hipLaunchKernelGGL(pa, dim3(gp), dim3(kBlock), 0, B.s, dd, 1, 1, 0);
+        HIPCK(hipGetLastError());
+        hipLaunchKernelGGL(ja, dim3(gp), dim3(kBlock), 0, B.s, dd);
+        HIPCK(hipGetLastError());
+        hipLaunchKernelGGL(kb_fold_converge_d, dim3(gs), dim3(kBlock), 0, B.s, dd, after, k);
+        HIPCK(hipGetLastError());
+    }
+}
+
+// A chunk is replayed from a HIP graph of its descriptor-form launches: one
+// submission instead of three per iteration. The graph's arguments are the
+// descriptor's address, the termination rule, the chunk length, the grid
+// buckets and the code shapes, so a world reuses its graphs across
+// admissions and departures. A key is captured on its second use; timing and
+// tuning runs launch directly. Executable graphs live until the batch ends or
+// graphs are switched off (none is destroyed while stepping); once
+// kMaxGraphs exist, new keys launch directly.
 static void jacobi_chunk(Batch &B, const BatchDev &d, size_t grid, int after, int chunk) {
     if (!B.use_graphs || B.prof.on || B.tuner.measuring()) {
         jacobi_chunk_launches(B, d, grid, after, chunk);
         return;
     }
-    std::string key(reinterpret_cast<const char *>(&d), sizeof d);
-    const int meta[6] = {after, chunk, (int)grid, B.grad_cache ? 1 : 0,
+    const unsigned gp = grid_bucket(kXcds * ((grid + kXcds - 1) / kXcds));
+    const unsigned gs = grid_bucket((size_t)d.ns);
+    const int meta[7] = {after, chunk, (int)gp, (int)gs, B.grad_cache ? 1 : 0,
                          B.variant[PP_PRESSURE_ACCEL], B.variant[PP_JACOBI]};
-    key.append(reinterpret_cast<const char *>(meta), sizeof meta);
-    for (size_t g = 0; g < B.graphs.size(); g++)
-        if (B.graphs[g].key == key) {
-            Batch::ChunkGraph hit = B.graphs[g];
-            B.graphs.erase(B.graphs.begin() + g);
-            B.graphs.push_back(hit);
-            HIPCK(hipGraphLaunch(hit.exec, B.s));
-            return;
+    const std::string key(reinterpret_cast<const char *>(meta), sizeof meta);
+    hipGraphExec_t exec = nullptr;
+    for (const Batch::ChunkGraph &g : B.graphs)
+        if (g.key == key) exec = g.exec;
+    const bool capture = !exec && B.graphs.size() < kMaxGraphs &&
+                         std::find(B.seen.begin(), B.seen.end(), key) != B.seen.end();
+    if (!exec && !capture) {
+        if (std::find(B.seen.begin(), B.seen.end(), key) == B.seen.end()) {
+            B.seen.push_back(key);
+            if (B.seen.size() > 16) B.seen.erase(B.seen.begin());
         }
-    if (B.graphs.size() >= kMaxGraphs ||
-        std::find(B.seen.begin(), B.seen.end(), key) == B.seen.end()) {
-        B.seen.push_back(key);
-        if (B.seen.size() > 16) B.seen.erase(B.seen.begin());
         jacobi_chunk_launches(B, d, grid, after, chunk);
         return;
     }
-    hipGraph_t graph = nullptr;
-    HIPCK(hipStreamBeginCapture(B.s, hipStreamCaptureModeThreadLocal));
-    try {
-        jacobi_chunk_launches(B, d, grid, after, chunk);
-    } catch (...) {
-        (void)hipStreamEndCapture(B.s, &graph);
-        if (graph) (void)hipGraphDestroy(graph);
-        throw;
+    // the descriptor the replayed kernels read, ahead of them in stream order
+    B.desc.reserve(1);
+    B.put(B.desc.p, &d, sizeof d);
+    if (capture) {
+        hipGraph_t graph = nullptr;
+        HIPCK(hipStreamBeginCapture(B.s, hipStreamCaptureModeThreadLocal));
+        try {
+            jacobi_chunk_desc_launches(B, gp, gs, after, chunk);
+        } catch (...) {
+            (void)hipStreamEndCapture(B.s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPCK(hipStreamEndCapture(B.s, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCK(e);
+        B.graphs.push_back({key, exec});
     }
-    HIPCK(hipStreamEndCapture(B.s, &graph));
-    hipGraphExec_t exec = nullptr;
-    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    HIPCK(e);
-    B.graphs.push_back({key, exec});
     HIPCK(hipGraphLaunch(exec, B.s));
 }
 
@@ -3183,7 +3226,6 @@ int sph_world_create(sph_world **out) {
         check_device();
         std::unique_ptr<sph_world> w(new sph_world());
         w->batch.init();
-        w->batch.use_graphs = false;
         *out = w.release();
     });
 }
@@ -3482,6 +3524,7 @@ static int get_option(const Batch &B, int option) {
     if (option == SPH_OPT_AUTOTUNE) return B.tuner.steps_per_shape;
     if (option == SPH_OPT_TUNE_STATE) return B.tuner.state;
     if (option == SPH_OPT_GRAPHS) return B.use_graphs ? 1 : 0;
+    if (option == SPH_OPT_GRAPH_COUNT) return (int)B.graphs.size();
     if (option == SPH_OPT_SWEEP) return B.sweep ? 1 : 0;
     if (option >= SPH_OPT_SHAPE_VOLUME && option < SPH_OPT_SHAPE_VOLUME + kPairPasses)
         return B.variant[option - SPH_OPT_SHAPE_VOLUME];

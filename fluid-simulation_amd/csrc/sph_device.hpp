@@ -63,6 +63,19 @@ __device__ __forceinline__ unsigned xcd_block() {
     return x * q + (x < r ? x : r) + k;
 }
 
+// The same order for a grid of at least 8 ceil(t / 8) workgroups serving t
+// logical ones: workgroup b (on XCD b % 8) takes logical block xcd_block(t)
+// when valid; the others hold no work. With gridDim.x == t it equals
+// xcd_block(), so a launch over a rounded-up grid computes what a launch over
+// exactly t workgroups does.
+__device__ __forceinline__ unsigned xcd_block_of(unsigned t, bool &valid) {
+    const unsigned b = blockIdx.x;
+    const unsigned x = b % kXcds, k = b / kXcds;
+    const unsigned q = t / kXcds, r = t % kXcds;
+    valid = k < q + (x < r ? 1u : 0u);
+    return x * q + (x < r ? x : r) + k;
+}
+
 __device__ __forceinline__ double powi2(double x) { return x * x; }
 __device__ __forceinline__ double powi3(double x) { return (x * x) * x; }
 

@@ -160,6 +160,7 @@ OPT_SHAPE_VOLUME = 4
 OPT_GRAPHS = 9
 OPT_SWEEP = 10
 OPT_MAX_ITERATIONS = 11
+OPT_GRAPH_COUNT = 12
 SHAPES = 9
 PAIR_PASSES = ["volume", "nonpressure", "source_diag", "pressure_accel", "jacobi"]
 

@@ -444,8 +444,10 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *       volume, non-pressure, source/diagonal, pressure-acceleration and
  *       Jacobi passes.
  *   SPH_OPT_GRAPHS  1 = replay repeated Jacobi iteration chunks from HIP
- *       graphs (default for a context), 0 = launch every kernel directly
- *       (default for a world).
+ *       graphs (default), 0 = launch every kernel directly. The replayed
+ *       kernels read the batch layout from device memory, so a world's graphs
+ *       serve every admission and departure.
+ *   SPH_OPT_GRAPH_COUNT  (get) executable graphs the context or world holds.
  *   SPH_OPT_MAX_ITERATIONS  (context) Jacobi iterations after which a pressure
  *       solve that has not met its termination test fails with
  *       SPH_ERR_UNSUPPORTED (default 100000; the reference has no cap,
@@ -466,7 +468,8 @@ enum {
     SPH_OPT_SHAPE_VOLUME = 4,
     SPH_OPT_GRAPHS = 9,
     SPH_OPT_SWEEP = 10,
-    SPH_OPT_MAX_ITERATIONS = 11
+    SPH_OPT_MAX_ITERATIONS = 11,
+    SPH_OPT_GRAPH_COUNT = 12
 };
 enum { SPH_SHAPES = 9 };
 int sph_set_option(sph_ctx *ctx, int option, int value);

@@ -209,7 +209,7 @@ def test_graph_replay_bit_identical(native):
             assert_same(eager, graph)
     w = native.World([native.System(scenes.small_iisph_box(side=9, seed=k, jitter=2e-3))
                       for k in (1, 2)])
-    assert w.get_option(N.OPT_GRAPHS) == 0   # worlds launch directly by default
+    assert w.get_option(N.OPT_GRAPHS) == 1   # worlds replay graphs too
     solo = [native.System(scenes.small_iisph_box(side=9, seed=k, jitter=2e-3)) for k in (1, 2)]
     for s in solo:
         s.set_option(N.OPT_GRAPHS, 0)
@@ -219,3 +219,45 @@ def test_graph_replay_bit_identical(native):
             s.step_forward_in_time(1)
     for a, b in zip(solo, w.members):
         assert_same(a, b)
+
+
+def test_world_graphs_survive_admissions(native):
+    """A world that replays its Jacobi chunks from graphs while systems join
+    and leave (every join or departure changes the launch layout) keeps each
+    system bit-identical to its eager solo run, and its graphs serve every
+    layout: their number stays at the few (termination rule, chunk length,
+    grid bucket) combinations instead of growing with each layout."""
+    from sph_amd import _native as N
+    mk = [lambda k=k: scenes.small_iisph_box(side=7 + k, seed=20 + k, jitter=2e-3)
+          for k in range(5)]
+    world_sys = [native.System(f()) for f in mk]
+    solo = [native.System(f()) for f in mk]
+    for s in world_sys + solo:
+        s.set_option(N.OPT_SHAPE, 1)
+    for s in solo:
+        s.set_option(N.OPT_GRAPHS, 0)
+    w = native.World([world_sys[0], world_sys[1]])
+    w.set_option(N.OPT_SHAPE, 1)
+    steps = [0] * 5
+    plan = [("step", 3), ("attach", 2), ("step", 3), ("attach", 3), ("step", 2),
+            ("detach", 0), ("step", 3), ("attach", 4), ("detach", 2), ("step", 4),
+            ("detach", 1), ("step", 3)]
+    members = [0, 1]
+    for op, arg in plan:
+        if op == "attach":
+            w.attach(world_sys[arg])
+            members.append(arg)
+        elif op == "detach":
+            w.detach(world_sys[arg])
+            members.remove(arg)
+        else:
+            w.step(arg)
+            for m in members:
+                steps[m] += arg
+    counts = w.get_option(N.OPT_GRAPH_COUNT)
+    print(f"world graphs after {len(plan)} layout changes: {counts}", flush=True)
+    assert 0 < counts <= 12
+    for k in range(5):
+        solo[k].step_forward_in_time(steps[k]) if steps[k] else None
+        assert_same(solo[k], world_sys[k])
+    w.close()
