@@ -83,6 +83,16 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ rec4 ld4(const double *a, size_t i) {
     return ((const GLOBAL_AS rec4 *)a)[i];
 }
+// Neighbor gathers address a record by a 32-bit byte offset from the
+// workgroup-uniform array base, which the compiler issues with the base in
+// SGPRs and one 32-bit offset per lane (no 64-bit address arithmetic); the
+// host keeps every scene below 2^27 particles so that j << 5 fits.
+__device__ __forceinline__ rec4 ldj4(const double *a, uint32_t j) {
+    return *(const GLOBAL_AS rec4 *)((const GLOBAL_AS char *)a + (j << 5));
+}
+__device__ __forceinline__ double ldj(const double *a, uint32_t j) {
+    return *(const GLOBAL_AS double *)((const GLOBAL_AS char *)a + (j << 3));
+}
 __device__ __forceinline__ void st4(double *a, size_t i, double x, double y, double z,
                                     double w) {
     rec4 r = {x, y, z, w};
@@ -463,10 +473,11 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
 #pragma unroll
                         for (int t = 0; t < kSweepBatch; t++) {
                             const uint32_t jj = min(j0 + (uint32_t)t, b - 1);
-                            const rec2 xy = ((const GLOBAL_AS rec2 *)v.q4)[2 * (size_t)jj];
+                            const GLOBAL_AS char *rec = (const GLOBAL_AS char *)v.q4 + (jj << 5);
+                            const rec2 xy = *(const GLOBAL_AS rec2 *)rec;
                             cx_[t] = xy.x;
                             cy_[t] = xy.y;
-                            cz_[t] = gm(v.q4)[4 * (size_t)jj + 2];  // .w is being written; unused
+                            cz_[t] = *(const GLOBAL_AS double *)(rec + 16);  // .w is being written; unused
                         }
 #pragma unroll
                         for (int t = 0; t < kSweepBatch; t++) {
@@ -571,7 +582,7 @@ __global__ PAIR_PASS_BOUNDS void kb_pairs_volume(BatchDev B) {
     for (uint32_t k = 0; k < c; k++) {
         const size_t e = ELL_AT(e0, k);
         uint32_t j = gm(B.ell_j)[e];
-        const rec4 qj = ld4(v.q4, j);  // .w is being written by this pass; unused here
+        const rec4 qj = ldj4(v.q4, j);  // .w is being written by this pass; unused here
         double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
         double rn = sqrt((rx * rx + ry * ry) + rz * rz);
         double q = div_rcp(rn, K.h, rh);
@@ -642,7 +653,7 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
         const uint32_t c = gm(B.cnt)[gi];
         const size_t e0 = ELL_BASE(B, v, i);
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.v4, j)}; },
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(v.q4, j), ldj4(v.v4, j)}; },
             [&](uint32_t k, const QU &r) {
                 double rx = px - r.q.x, ry = py - r.q.y, rz = pz - r.q.z;
                 double cf = v.nu_f * 2. * (3. + 2.) * r.q.w *
@@ -747,7 +758,7 @@ __device__ __forceinline__ void pressure_accel_impl(const BatchDev &B, int targe
         const size_t e0 = ELL_BASE(B, v, i);
         const double vm = Vi / mi;
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QP{ld4(v.q4, j), gm(v.p)[j]}; },
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QP{ldj4(v.q4, j), ldj(v.p, j)}; },
             [&](uint32_t k, const QP &r) {
                 double cf = vm * r.q.w * (pi + r.p);
                 double wx, wy, wz;
@@ -861,7 +872,7 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
     const uint32_t c = gm(B.cnt)[gi];
     const size_t e0 = ELL_BASE(B, v, i);
     neighbor_loop<PIPE, UNROLL>(
-        gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.sv4, j)}; },
+        gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(v.q4, j), ldj4(v.sv4, j)}; },
         [&](uint32_t k, const QU &r) {
             double wx, wy, wz;
             pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y, pz - r.q.z,
@@ -926,7 +937,7 @@ __device__ __forceinline__ void jacobi_impl(const BatchDev &B) {
         const uint32_t c = gm(B.cnt)[gi];
         const size_t e0 = ELL_BASE(B, v, i);
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ld4(v.q4, j), ld4(v.pa4, j)}; },
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(v.q4, j), ldj4(v.pa4, j)}; },
             [&](uint32_t k, const QU &r) {
                 double wx, wy, wz;
                 pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y,

@@ -2347,7 +2347,9 @@ static void set_host_fluid(sph_ctx *c, size_t n, const double *xyz, const double
                            const double *m, const uint32_t *fid) {
     Scene &S = c->scene;
     Batch &B = c->own;
-    if (n >= 0xffffffffull) throw Error(SPH_ERR_UNSUPPORTED, "more than 2^32-1 particles");
+    // neighbor gathers address a 32-byte record by a 32-bit byte offset
+    if (n >= (1ull << 27))
+        throw Error(SPH_ERR_UNSUPPORTED, "more than 2^27 particles in one system");
     reserve_scene(S, n);
     std::vector<double> X(n), Y(n), Z(n), U(n), W(n), T(n);
     for (size_t i = 0; i < n; i++) {
