@@ -9,12 +9,13 @@
 
 namespace sph {
 
-// launch bounds of the pair passes: at least WAVES waves per SIMD, with
-// 256-thread workgroups; WAVES = 0 keeps the compiler's default bounds
+// launch bounds of the pair passes: WAVES waves per SIMD exactly (registers
+// for WAVES waves, and no fewer registers to reach more), with 256-thread
+// workgroups; WAVES = 0 keeps the compiler's default bounds
 #define PAIR_PASS_BOUNDS                                                       \
     __attribute__((amdgpu_flat_work_group_size(WAVES > 0 ? kBlock : 1,         \
                                                WAVES > 0 ? kBlock : 1024),     \
-                   amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1)))
+                   amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES > 0 ? WAVES : 8)))
 
 // scene of workgroup lb: the largest s with blk[s] <= lb (block-uniform)
 template <class P>
@@ -101,7 +102,7 @@ __device__ __forceinline__ void st4(double *a, size_t i, double x, double y, dou
 
 // grad W of pair slot e: read from the pair cache (CACHED) or recomputed from
 // r = x_i - x_j with the same expression (bit-identical either way)
-template <bool CACHED>
+template <bool CACHED, bool ZS = false>
 __device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &K, double rh,
                                           size_t e, double rx, double ry, double rz, double &wx,
                                           double &wy, double &wz) {
@@ -110,7 +111,7 @@ __device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &
         wy = gm(B.ell_gy)[e];
         wz = gm(B.ell_gz)[e];
     } else {
-        kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
+        kernel_grad_rh<ZS>(K, rh, rx, ry, rz, wx, wy, wz);
     }
 }
 
@@ -124,7 +125,11 @@ __device__ __forceinline__ void pair_grad(const BatchDev &B, const KernelConst &
 //   PIPE = 2: two neighbors per step: one 8-byte load reads both indices and
 //     both records are gathered before either is evaluated (for an odd count
 //     the second index slot is read but not used: the record load repeats
-//     the first neighbor).
+//     the first neighbor);
+//   PIPE = 3: UNROLL neighbors per step while UNROLL slots remain, all
+//     their records gathered before the first is evaluated (a scheduling
+//     barrier keeps the compiler from sinking the gathers between the
+//     evaluations to save registers), then the remaining slots one by one.
 // Prefetch slots clamp to c-1 and a pair's second slot lies inside the
 // slice's even row count, so every load stays in bounds.
 template <int PIPE, int UNROLL, class Load, class Body>
@@ -142,6 +147,18 @@ __device__ __forceinline__ void neighbor_loop(const GLOBAL_AS uint32_t *ej, uint
             j1 = j2;
             r0 = r1;
         }
+    } else if (PIPE == 3) {
+        uint32_t k = 0;
+        for (; k + UNROLL <= c; k += UNROLL) {
+            decltype(load(0u)) r[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) r[u] = load(ej[ELL_AT(0, k + u)]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) body(k + u, r[u]);
+        }
+#pragma nounroll
+        for (; k < c; k++) body(k, load(ej[ELL_AT(0, k)]));
     } else if (PIPE == 2) {
         for (uint32_t k = 0; k < c; k += 2) {
             const u32x2 jj = *(const GLOBAL_AS u32x2 *)(ej + 128 * (size_t)(k >> 1));
@@ -487,7 +504,7 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
                             if (j < b && d2 < v.dthr) {
                                 if (cnt < kSweepRows) rows[w][cnt][l] = j;
                                 else if (cnt < cap) gm(B.ell_j)[ELL_AT(e0, cnt)] = j;
-                                const double q = div_rcp(sqrt(d2), K.h, rh);
+                                const double q = div_rcp_sum(sqrt(d2), K.h, rh);
                                 accu += v.V0 * (K.wfac * kernel_w(q));
                                 cnt++;
                             }
@@ -585,15 +602,15 @@ __global__ PAIR_PASS_BOUNDS void kb_pairs_volume(BatchDev B) {
         const rec4 qj = ldj4(v.q4, j);  // .w is being written by this pass; unused here
         double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
         double rn = sqrt((rx * rx + ry * ry) + rz * rz);
-        double q = div_rcp(rn, K.h, rh);
+        double q = div_rcp_sum(rn, K.h, rh);
         accu += v.V0 * (K.wfac * kernel_w(q));
         if (CACHED) {  // the expression of kernel_grad_rh
             const double d = kernel_dqw(q);
             const double rs = rn != 0.0 ? rn : 1.0;
             const double yr = refined_rcp(rs);
-            const double wx = div_rcp(K.gfac * rx, rs, yr) * d;
-            const double wy = div_rcp(K.gfac * ry, rs, yr) * d;
-            const double wz = div_rcp(K.gfac * rz, rs, yr) * d;
+            const double wx = div_rcp_sum(K.gfac * rx, rs, yr) * d;
+            const double wy = div_rcp_sum(K.gfac * ry, rs, yr) * d;
+            const double wz = div_rcp_sum(K.gfac * rz, rs, yr) * d;
             gm(B.ell_gx)[e] = wx;
             gm(B.ell_gy)[e] = wy;
             gm(B.ell_gz)[e] = wz;
@@ -652,8 +669,9 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
         }
         const uint32_t c = gm(B.cnt)[gi];
         const size_t e0 = ELL_BASE(B, v, i);
+        const double *q4 = v.q4, *v4 = v.v4;  // bases read once, ahead of the loop
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(v.q4, j), ldj4(v.v4, j)}; },
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(q4, j), ldj4(v4, j)}; },
             [&](uint32_t k, const QU &r) {
                 double rx = px - r.q.x, ry = py - r.q.y, rz = pz - r.q.z;
                 double cf = v.nu_f * 2. * (3. + 2.) * r.q.w *
@@ -757,8 +775,9 @@ __device__ __forceinline__ void pressure_accel_impl(const BatchDev &B, int targe
         const uint32_t c = gm(B.cnt)[gi];
         const size_t e0 = ELL_BASE(B, v, i);
         const double vm = Vi / mi;
+        const double *q4 = v.q4, *pj = v.p;  // bases read once, ahead of the loop
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QP{ldj4(v.q4, j), ldj(v.p, j)}; },
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QP{ldj4(q4, j), ldj(pj, j)}; },
             [&](uint32_t k, const QP &r) {
                 double cf = vm * r.q.w * (pi + r.p);
                 double wx, wy, wz;
@@ -871,8 +890,9 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
     double sfx = 0., sfy = 0., sfz = 0., sf2 = 0., sbx = 0., sby = 0., sbz = 0.;
     const uint32_t c = gm(B.cnt)[gi];
     const size_t e0 = ELL_BASE(B, v, i);
+    const double *q4 = v.q4, *sv4 = v.sv4;  // bases read once, ahead of the loop
     neighbor_loop<PIPE, UNROLL>(
-        gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(v.q4, j), ldj4(v.sv4, j)}; },
+        gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(q4, j), ldj4(sv4, j)}; },
         [&](uint32_t k, const QU &r) {
             double wx, wy, wz;
             pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y, pz - r.q.z,
@@ -936,12 +956,14 @@ __device__ __forceinline__ void jacobi_impl(const BatchDev &B) {
         const double px = qi.x, py = qi.y, pz = qi.z;
         const uint32_t c = gm(B.cnt)[gi];
         const size_t e0 = ELL_BASE(B, v, i);
+        const double *q4 = v.q4, *pa4 = v.pa4;  // bases read once, ahead of the loop
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(v.q4, j), ldj4(v.pa4, j)}; },
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(q4, j), ldj4(pa4, j)}; },
             [&](uint32_t k, const QU &r) {
                 double wx, wy, wz;
-                pair_grad<CACHED>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y,
-                                  pz - r.q.z, wx, wy, wz);
+                // signed-zero form: faster here, same sums (kernel_grad_rh)
+                pair_grad<CACHED, true>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y,
+                                        pz - r.q.z, wx, wy, wz);
                 adp += dt2 * r.q.w * dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, wx, wy, wz);
             });
         for (uint32_t bi = 0; bi < v.nb; bi++) {

@@ -88,8 +88,8 @@ struct BatchDev {
 // shape evaluates the same expressions in the same order, so results do not
 // depend on the shape.
 #define SPH_PAIR_VARIANTS(X)                                                      \
-    X(4, 0, 1) X(2, 0, 1) X(4, 4, 1) X(1, 0, 0) X(2, 0, 0) X(4, 0, 0) X(1, 0, 2)  \
-    X(2, 0, 2) X(1, 8, 2)
+    X(4, 0, 1) X(2, 0, 1) X(4, 4, 1) X(1, 0, 0) X(2, 0, 0) X(4, 0, 0) X(4, 0, 3)  \
+    X(2, 0, 3) X(4, 3, 0)
 constexpr int kPairVariants = SPH_SHAPES;
 __global__ void kb_integrate_ec(BatchDev B);
 __global__ void kb_speed_below(BatchDev B);

@@ -106,18 +106,20 @@ struct DBuf {
 // ------------------------------------------------------------------------
 // per-kernel timing with HIP events on a batch's stream
 // ------------------------------------------------------------------------
-struct Prof {
+struct ProfRec {
+    int id;
+    size_t units;
+    hipEvent_t a, b;
+};
+
+template <int N> struct ProfT {
     bool on = false;
-    struct Rec {
-        int id;
-        size_t units;
-        hipEvent_t a, b;
-    };
+    using Rec = ProfRec;
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
-    uint64_t launches[SPH_PROF_KERNELS] = {0};
-    uint64_t units[SPH_PROF_KERNELS] = {0};
-    double ms[SPH_PROF_KERNELS] = {0};
+    uint64_t launches[N] = {0};
+    uint64_t units[N] = {0};
+    double ms[N] = {0};
 
     hipEvent_t get() {
         if (!pool.empty()) {
@@ -151,12 +153,12 @@ struct Prof {
         pending.clear();
     }
     void reset() {
-        for (int k = 0; k < SPH_PROF_KERNELS; k++) {
+        for (int k = 0; k < N; k++) {
             launches[k] = units[k] = 0;
             ms[k] = 0;
         }
     }
-    ~Prof() {
+    ~ProfT() {
         for (auto &r : pending) {
             (void)hipEventDestroy(r.a);
             (void)hipEventDestroy(r.b);
@@ -164,6 +166,8 @@ struct Prof {
         for (auto e : pool) (void)hipEventDestroy(e);
     }
 };
+
+using Prof = ProfT<SPH_PROF_KERNELS>;
 
 // ------------------------------------------------------------------------
 // host rigid body (boundary_handling/rigid_body_motion.rs), binary64
@@ -494,19 +498,24 @@ constexpr size_t kMaxGraphs = 64;  // executable Jacobi-chunk graphs per batch
 // ------------------------------------------------------------------------
 // online tuning of the pair passes' code shapes
 // ------------------------------------------------------------------------
-// Every shape computes the same bits, so shapes are measured on live steps:
-// step k of a tuning round runs every pair pass in shape k / steps_per_shape,
-// and after all shapes each pass keeps the one with the least time per
-// particle. A batch is tuned on its first steps and again when its particle
-// count has changed by 2x since.
+// Every shape computes the same bits, so shapes are measured on live steps.
+// A tuning round lasts kPairVariants * steps_per_shape steps: step k runs the
+// once-per-step passes in shape k / steps_per_shape, and the Jacobi
+// iterations rotate through the shapes one iteration at a time (launched one
+// per host wait, so no launch finds its solve converged), which measures every
+// shape of the two iterated passes on the same fluid states. After the round
+// each pass keeps the shape with the least time per particle. A batch is tuned
+// on its first steps and again when its particle count has changed by 2x since.
 enum { PP_VOLUME, PP_NONPRESSURE, PP_SOURCE_DIAG, PP_PRESSURE_ACCEL, PP_JACOBI, kPairPasses };
 
 struct Tuner {
     int steps_per_shape = 1;  // 0: off
     int state = 0;            // 0 untuned, 1 measuring, 2 tuned
     int shape = 0, step_in = 0;
+    unsigned rotation = 0;    // the next Jacobi iteration's shape while measuring
     size_t tuned_particles = 0;
-    double cost[kPairPasses][kPairVariants];
+    double ms[kPairPasses][kPairVariants];
+    double units[kPairPasses][kPairVariants];
     bool measuring() const { return state == 1; }
 };
 
@@ -534,7 +543,7 @@ struct Batch {
     // online tuner that picks them from timed steps
     int variant[kPairPasses] = {0, 0, 0, 0, 0};
     Tuner tuner;
-    Prof tprof;
+    ProfT<kPairPasses * kPairVariants> tprof;  // tuning samples by pass and shape
 
     DBuf<int> ci, ipart;
     DBuf<uint32_t> perm, cnt, slice_len, slice_off, ell_j, cell_cnt, cell_start, cursor;
@@ -834,7 +843,7 @@ static void pair_pass_grid(Batch &B, size_t grid, int pass, int pid, const Tab &
     if (grid == 0) return;
     auto kern = tab[B.grad_cache ? 1 : 0][B.variant[pass]];
     Prof::Rec r{pid, B.total_particles, nullptr, nullptr};
-    Prof::Rec t{pass, B.total_particles, nullptr, nullptr};
+    Prof::Rec t{pass * kPairVariants + B.variant[pass], B.total_particles, nullptr, nullptr};
     const bool timed = B.prof.on && pid < SPH_PROF_KERNELS, tuning = B.tuner.measuring();
     if (timed) B.prof.begin(B.s, r);
     if (tuning) B.tprof.begin(B.s, t);
@@ -860,7 +869,7 @@ static void tune_begin_step(Batch &B, size_t particles) {
         T.shape = 0;
         T.step_in = 0;
         for (int p = 0; p < kPairPasses; p++)
-            for (int v = 0; v < kPairVariants; v++) T.cost[p][v] = 0.0;
+            for (int v = 0; v < kPairVariants; v++) T.ms[p][v] = T.units[p][v] = 0.0;
         B.tprof.reset();
     }
     if (T.state == 1)
@@ -872,17 +881,24 @@ static void tune_end_step(Batch &B, size_t particles) {
     if (T.state != 1) return;
     B.sync();
     B.tprof.drain();
-    for (int p = 0; p < kPairPasses; p++) {
-        T.cost[p][T.shape] += B.tprof.units[p] ? B.tprof.ms[p] / (double)B.tprof.units[p] : 0.0;
-    }
+    for (int p = 0; p < kPairPasses; p++)
+        for (int v = 0; v < kPairVariants; v++) {
+            T.ms[p][v] += B.tprof.ms[p * kPairVariants + v];
+            T.units[p][v] += (double)B.tprof.units[p * kPairVariants + v];
+        }
     B.tprof.reset();
     if (++T.step_in < T.steps_per_shape) return;
     T.step_in = 0;
     if (++T.shape < kPairVariants) return;
     for (int p = 0; p < kPairPasses; p++) {
+        // a shape a pass never ran in (a pass the steps did not use) is not chosen
         int best = 0;
-        for (int v = 1; v < kPairVariants; v++)
-            if (T.cost[p][v] < T.cost[p][best]) best = v;
+        double cost = INFINITY;
+        for (int v = 0; v < kPairVariants; v++)
+            if (T.units[p][v] > 0 && T.ms[p][v] / T.units[p][v] < cost) {
+                cost = T.ms[p][v] / T.units[p][v];
+                best = v;
+            }
         B.variant[p] = best;
     }
     T.state = 2;
@@ -1472,6 +1488,9 @@ static void pressure_accel(Batch &B, int overwrite) {
 static void jacobi_chunk_launches(Batch &B, const BatchDev &d, size_t grid, int after,
                                   int chunk) {
     for (int k = 0; k < chunk; k++) {
+        if (B.tuner.measuring())
+            B.variant[PP_PRESSURE_ACCEL] = B.variant[PP_JACOBI] =
+                (int)(B.tuner.rotation++ % kPairVariants);
         pair_pass_grid(B, grid, PP_PRESSURE_ACCEL, SPH_PROF_PRESSURE_ACCEL, k_pressure_accel_tab,
                        d, 1, 1, 0);
         pair_pass_grid(B, grid, PP_JACOBI, SPH_PROF_JACOBI, k_jacobi_tab, d);
@@ -1600,6 +1619,7 @@ static void resolve_pressure(Batch &B, int vp, int after) {
     std::vector<uint32_t> ablk, amap;
     while (any) {
         chunk = std::min(std::max(chunk, 1), kMaxChunk);
+        if (B.tuner.measuring()) chunk = 1;  // every timed iteration does work
         for (Scene *c : B.sc)  // no scene iterates past its cap
             if (c->active) chunk = std::min<int>(chunk, (int)(c->max_iter - c->iter));
         // launch over the scenes still iterating only

@@ -122,18 +122,43 @@ __device__ __forceinline__ double div_rcp(double a, double b, double y) {
     return a == 0.0 ? q : res;
 }
 
+// div_rcp without the zero-quotient sign: a zero numerator gives a zero of
+// either sign, every nonzero quotient is the correctly rounded one. For the
+// quotients of the kernel and its gradient in the neighbor sums: |r| / h is
+// >= +0 either way, and each gradient component only enters sums that start
+// at +0, which no zero term changes (x + (+-0) = x for x != 0, and
+// +0 + (+-0) = +0 - (+-0) = x - x = +0 in round to nearest), so the sign of a
+// zero never reaches a result.
+__device__ __forceinline__ double div_rcp_sum(double a, double b, double y) {
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+}
+
 // grad W with 1/h refined once per thread (rh = refined_rcp(K.h)): the
-// expression of kernel_grad below, bit for bit
+// expression of kernel_grad below, bit for bit up to the sign of zero
+// components (div_rcp_sum). ZS = true keeps the signs of zeros too (div_rcp):
+// the same sums, and on gfx950 the form the compiler schedules the Jacobi
+// pass best in (its gathers issued ahead of the arithmetic: 154 against
+// 196 us per launch at 1M particles in the 4-neighbor shape).
+template <bool ZS = false>
 __device__ __forceinline__ void kernel_grad_rh(const KernelConst &K, double rh, double rx,
                                                double ry, double rz, double &gx, double &gy,
                                                double &gz) {
     const double rn = sqrt(dot3(rx, ry, rz, rx, ry, rz));
-    const double d = kernel_dqw(div_rcp(rn, K.h, rh));
     const double rs = rn != 0.0 ? rn : 1.0;
     const double yr = refined_rcp(rs);
-    gx = div_rcp(K.gfac * rx, rs, yr) * d;
-    gy = div_rcp(K.gfac * ry, rs, yr) * d;
-    gz = div_rcp(K.gfac * rz, rs, yr) * d;
+    if (ZS) {
+        const double d = kernel_dqw(div_rcp(rn, K.h, rh));
+        gx = div_rcp(K.gfac * rx, rs, yr) * d;
+        gy = div_rcp(K.gfac * ry, rs, yr) * d;
+        gz = div_rcp(K.gfac * rz, rs, yr) * d;
+    } else {
+        const double d = kernel_dqw(div_rcp_sum(rn, K.h, rh));
+        gx = div_rcp_sum(K.gfac * rx, rs, yr) * d;
+        gy = div_rcp_sum(K.gfac * ry, rs, yr) * d;
+        gz = div_rcp_sum(K.gfac * rz, rs, yr) * d;
+    }
 }
 
 // grad W(r, h) (kernel/mod.rs:37-45): ((gfac r) / |r|) w'(|r|/h), zero at
