@@ -591,6 +591,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
+    ap.add_argument("--shapes", default="",
+                    help="code shapes of the volume, non-pressure, source/diagonal, "
+                         "pressure-acceleration and Jacobi passes (five comma-separated indices: "
+                         "the tuner is off; counter runs repeat a timed run's choice)")
     ap.add_argument("--requests", type=int, default=256, help="serving: requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: arrivals per second")
     ap.add_argument("--n-min", type=int, default=4096, help="serving: smallest scene")
@@ -630,11 +634,19 @@ def main():
     scene = build_scene(args.workload, args.particles)
     log(f"[bench] scene {scene.name}: {scene.n} particles")
     sim = system.System(scene)
+    pinned = bool(args.shapes)
+    if pinned:
+        from sph_amd import _native as N
+        shapes = [int(x) for x in args.shapes.split(",")]
+        if len(shapes) != len(N.PAIR_PASSES):
+            raise SystemExit("--shapes takes one code shape per neighbor pass")
+        for k, v in enumerate(shapes):
+            sim.set_option(N.OPT_SHAPE_VOLUME + k, v)
     # settle: the first steps let the resting fluid compress until every
     # solve carries its weight, and the code-shape tuner finishes (one timed
     # step per shape); neither is part of the timed window, whatever --warmup is
     settled = 0
-    while settled < args.settle or sim.tuning()["state"] != 2:
+    while settled < args.settle or (not pinned and sim.tuning()["state"] != 2):
         sim.step_forward_in_time(1)
         settled += 1
         if settled > args.settle + 100:

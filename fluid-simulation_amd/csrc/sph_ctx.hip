@@ -3536,6 +3536,11 @@ static void set_option(Batch &B, int option, int value) {
         for (int p = 0; p < kPairPasses; p++) B.variant[p] = value;
         B.tuner.steps_per_shape = 0;
         B.tuner.state = 0;
+    } else if (option >= SPH_OPT_SHAPE_VOLUME && option < SPH_OPT_SHAPE_VOLUME + kPairPasses) {
+        if (value < 0 || value >= kPairVariants) throw Error(SPH_ERR_ARG, "no such code shape");
+        B.variant[option - SPH_OPT_SHAPE_VOLUME] = value;
+        B.tuner.steps_per_shape = 0;
+        B.tuner.state = 0;
     } else {
         throw Error(SPH_ERR_ARG, "unknown option");
     }

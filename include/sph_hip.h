@@ -435,14 +435,16 @@ int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
  *   SPH_OPT_AUTOTUNE  steps per code shape of the online tuner (default 1;
  *       0 = off): the first steps of a context or world, and the steps after
  *       its particle count changed 2x, time each code shape (unroll, waves per
- *       SIMD, loop form: plain, software-pipelined, two neighbors per step) of
- *       the five neighbor passes, which then keep their fastest.
+ *       SIMD, loop form: plain, software-pipelined, two neighbors per step,
+ *       grouped gathers) of the five neighbor passes, which then keep their
+ *       fastest.
  *   SPH_OPT_SHAPE  (set) run every neighbor pass in this code shape
  *       (0 .. SPH_SHAPES-1) and turn the tuner off.
  *   SPH_OPT_TUNE_STATE  (get) 0 untuned, 1 measuring, 2 tuned.
  *   SPH_OPT_SHAPE_VOLUME .. SPH_OPT_SHAPE_VOLUME+4  (get) the code shape of the
  *       volume, non-pressure, source/diagonal, pressure-acceleration and
- *       Jacobi passes.
+ *       Jacobi passes; (set) run that pass in this shape and turn the tuner
+ *       off (the other passes keep theirs).
  *   SPH_OPT_GRAPHS  1 = replay repeated Jacobi iteration chunks from HIP
  *       graphs (default), 0 = launch every kernel directly. The replayed
  *       kernels read the batch layout from device memory, so a world's graphs

@@ -189,6 +189,14 @@ def test_code_shapes_bit_identical_and_tuner(native):
     assert all(0 <= t[p] < N.SHAPES for p in N.PAIR_PASSES)
     tuned.step_forward_in_time(2)
     assert_same(ref, tuned)
+    # one pass pinned: that pass takes the shape, the others keep theirs, the tuner is off
+    mixed = native.System(sc)
+    mixed.set_option(N.OPT_SHAPE, 0)
+    mixed.set_option(N.OPT_SHAPE_VOLUME + 4, N.SHAPES - 1)
+    t = mixed.tuning()
+    assert t["jacobi"] == N.SHAPES - 1 and t["pressure_accel"] == 0 and t["steps_per_shape"] == 0
+    mixed.step_forward_in_time(N.SHAPES + 2)
+    assert_same(ref, mixed)
 
 
 def test_graph_replay_bit_identical(native):
