@@ -188,12 +188,17 @@ def serving_bench(args):
     bat = serving.BatchedServer(budget, max_slots=args.slots).run(stream).summary()
     log(f"[bench] batched: {bat['throughput_particle_steps_per_s']:.3e} particle-steps/s, "
         f"makespan {bat['makespan_s']:.1f} s")
-    seq = None
+    seq = bound = None
     if not args.no_sequential:
         log("[bench] sequential run")
-        seq = serving.SequentialServer().run(stream).summary()
+        seq_rep = serving.SequentialServer().run(stream)
+        seq = seq_rep.summary()
+        bound = serving.arrival_bound(stream, seq_rep)
+        bound["batched_fraction_of_ceiling"] = (bat["throughput_particle_steps_per_s"] /
+                                                bound["throughput_ceiling_particle_steps_per_s"])
         log(f"[bench] sequential: {seq['throughput_particle_steps_per_s']:.3e} "
-            f"particle-steps/s, makespan {seq['makespan_s']:.1f} s")
+            f"particle-steps/s, makespan {seq['makespan_s']:.1f} s; arrival bound "
+            f"{bound['makespan_lower_bound_s']:.1f} s")
     out = {
         "metric": "serving stream throughput (particle-steps/s, continuous batching); "
                   "p50/p99 step latency",
@@ -219,6 +224,9 @@ def serving_bench(args):
         "busy_speedup_vs_sequential": (bat["busy_throughput_particle_steps_per_s"] /
                                        seq["busy_throughput_particle_steps_per_s"])
         if seq else None,
+        # the arrivals' ceiling: no server finishes a request before its
+        # arrival plus its service time alone
+        "arrival_bound": bound,
     }
     print(json.dumps(out))
 

@@ -1654,6 +1654,12 @@ static void resolve_pressure(Batch &B, int vp, int after) {
         any = false;
         for (size_t s = 0; s < ns; s++) {
             Scene &c = *B.sc[s];
+            // a scene outside this chunk's launch left the loop earlier: its
+            // ran and error slots still hold the chunk it last ran in
+            if (!c.active) {
+                most = std::max(most, c.iter);
+                continue;
+            }
             for (int k = 0; k < chunk; k++)
                 if (ran[(size_t)k * ns + s]) {
                     c.err = errs[(size_t)k * ns + s];

@@ -61,7 +61,7 @@ class Result:
     n: int
     steps: int
     arrival: float
-    admitted: float
+    admitted: float                 # service started (before the scene's init)
     finished: float
     measurement: dict
     digest: str                     # sha256 of the final x, v (reference order)
@@ -93,6 +93,7 @@ class Report:
         ok = [r for r in self.results.values() if r.error is None]
         req = np.asarray([r.finished - r.arrival for r in ok] or [0.0])
         derr = [r.measurement["density_error"] for r in ok]
+        perr = [r.measurement["predicted_density_error"] for r in ok]
         return {
             "mode": self.mode,
             "requests": len(self.results),
@@ -108,9 +109,33 @@ class Report:
             "p99_request_s": float(np.percentile(req, 99)),
             "peak_resident_scenes": self.peak_resident,
             "peak_resident_bytes_est": self.peak_bytes,
+            # the last step's average density error after the position update
+            # (measured) and the pressure solve's last predicted error
             "density_error_pct_max": float(max(derr)) if derr else 0.0,
+            "predicted_density_error_pct_max": float(max(perr)) if perr else 0.0,
             "failed_requests": len(self.results) - len(ok),
         }
+
+
+def arrival_bound(stream: List[Request], solo: Report) -> dict:
+    """What the arrivals allow any server on this device: request r finishes
+    no earlier than its arrival plus its service time alone (from the
+    sequential run `solo`, system creation and init included), so the makespan
+    is at least max_r(arrival_r + service_r) - first arrival; the throughput
+    ceiling is the stream's particle-steps over that bound."""
+    svc = {rid: r.finished - r.admitted for rid, r in solo.results.items() if r.error is None}
+    served = [q for q in stream if q.rid in svc]
+    if not served:
+        return {"makespan_lower_bound_s": None, "throughput_ceiling_particle_steps_per_s": None}
+    t0 = min(q.arrival for q in stream)
+    last = max(served, key=lambda q: q.arrival + svc[q.rid])
+    bound = last.arrival + svc[last.rid] - t0
+    ps = sum(q.scene.n * q.steps for q in served)
+    return {"makespan_lower_bound_s": bound,
+            "last_arrival_s": max(q.arrival for q in stream) - t0,
+            "binding_request": last.rid,
+            "throughput_ceiling_particle_steps_per_s": ps / bound,
+            "speedup_ceiling_vs_sequential": solo.makespan_s / bound}
 
 
 def make_stream(n_requests: int = 256, rate: float = 20.0, n_min: int = 4096,
@@ -284,13 +309,13 @@ class SequentialServer:
             if wait > 0:
                 time.sleep(wait)
             a0 = self.clock()
+            t_adm = a0 - t0
             sim = system.System(req.scene, init=False)
             if req.max_iterations is not None:
                 sim.set_max_iterations(req.max_iterations)
             try:
                 sim.init()
                 admit += self.clock() - a0
-                t_adm = self.clock() - t0
                 for _ in range(req.steps):
                     s0 = self.clock()
                     sim.step_forward_in_time(1)

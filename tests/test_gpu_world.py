@@ -62,6 +62,30 @@ def test_world_iisph_mixed_sizes_and_bodies(native):
     assert len(dts) > 1
 
 
+def test_world_scenes_leaving_the_solve_at_different_chunks(native):
+    """A pool solved to a 0.02 % target after 40 steps alone (6-10 Jacobi
+    iterations per step) beside a small box (3): the box leaves the solve in
+    the world's first chunk while the pool runs further chunks; the box's
+    iteration count, predicted error and error history stay its own (those of
+    the box stepped alone), not the slots of the chunk it last ran in."""
+    pool = scenes.pool()
+    pool.params.target_density_error = 0.02
+    scs = [pool, scenes.small_iisph_box(side=7, seed=5, jitter=2e-3)]
+    solo = [native.System(sc) for sc in scs]
+    mates = [native.System(sc) for sc in scs]
+    solo[0].step_forward_in_time(40)
+    mates[0].step_forward_in_time(40)
+    w = native.World(mates)
+    for _ in range(5):
+        for s in solo:
+            s.step_forward_in_time(1)
+        w.step(1)
+        for a, b in zip(solo, mates):
+            assert_same(a, b)
+        its = [s.take_measurement()["solver_iterations"] for s in solo]
+        assert its[0] > its[1] + 2, its
+
+
 def test_world_iisphwost(native):
     scs = [scenes.small_iisph_box(side=9, seed=2, solver="IISPHwOST", jitter=2e-3),
            scenes.small_iisph_box(side=12, seed=3, solver="IISPHwOST", jitter=2e-3)]

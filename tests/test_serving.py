@@ -90,7 +90,8 @@ def test_native_admission_slots_budget_and_order():
 
 def test_report_summary_percentiles():
     res = {k: serving.Result(k, 100 * (k + 1), 10, 0.1 * k, 0.1 * k, 0.1 * k + 1.0,
-                             {"density_error": 0.01 * k}, "") for k in range(4)}
+                             {"density_error": 0.01 * k, "predicted_density_error": 0.02 * k}, "")
+           for k in range(4)}
     rep = serving.Report("batched", res, [1.0] * 99 + [5.0], 2.0, 1.0, 0.1, 4, 123)
     s = rep.summary()
     assert s["particle_steps"] == 10 * 100 * (1 + 2 + 3 + 4)
@@ -98,3 +99,28 @@ def test_report_summary_percentiles():
     assert s["p50_step_ms"] == 1.0 and 1.0 < s["p99_step_ms"] <= 5.0
     assert math.isclose(s["p50_request_s"], 1.0)
     assert math.isclose(s["density_error_pct_max"], 0.03)
+    assert math.isclose(s["predicted_density_error_pct_max"], 0.06)
+
+
+def test_arrival_bound_is_the_latest_arrival_plus_service():
+    """The arrival bound: request r cannot finish before arrival_r plus its
+    service time alone; the binding request is the one maximising that sum,
+    and the ceiling is the stream's particle-steps over the bound."""
+    from types import SimpleNamespace as NS
+    reqs = [serving.Request(k, a, NS(n=n), s) for k, (a, n, s) in
+            enumerate([(0.0, 100, 10), (1.0, 200, 5), (2.0, 50, 20)])]
+    # sequential service times 3.0, 0.5, 0.25 s: r0 ends at 3.0, r1 at 3.5, r2 at 3.75
+    times = [(0.0, 3.0), (3.0, 3.5), (3.5, 3.75)]
+    res = {k: serving.Result(k, reqs[k].scene.n, reqs[k].steps, reqs[k].arrival, a, f, {}, "")
+           for k, (a, f) in enumerate(times)}
+    solo = serving.Report("sequential", res, [], 3.75, 3.75, 0.0, 1, 0)
+    b = serving.arrival_bound(reqs, solo)
+    # max(0 + 3.0, 1 + 0.5, 2 + 0.25) = 3.0, set by request 0
+    assert b["makespan_lower_bound_s"] == 3.0 and b["binding_request"] == 0
+    assert b["last_arrival_s"] == 2.0
+    assert math.isclose(b["throughput_ceiling_particle_steps_per_s"], (1000 + 1000 + 1000) / 3.0)
+    assert math.isclose(b["speedup_ceiling_vs_sequential"], 3.75 / 3.0)
+    # a failed request does not count
+    res[0].error = "solve did not converge"
+    b = serving.arrival_bound(reqs, solo)
+    assert b["makespan_lower_bound_s"] == 2.25 and b["binding_request"] == 2
