@@ -112,6 +112,8 @@ class Report:
             # the last step's average density error after the position update
             # (measured) and the pressure solve's last predicted error
             "density_error_pct_max": float(max(derr)) if derr else 0.0,
+            "density_error_pct_p50": float(np.percentile(derr, 50)) if derr else 0.0,
+            "requests_measured_error_above_0p1_pct": int(sum(e > 0.1 for e in derr)),
             "predicted_density_error_pct_max": float(max(perr)) if perr else 0.0,
             "failed_requests": len(self.results) - len(ok),
         }
