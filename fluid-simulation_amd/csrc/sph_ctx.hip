@@ -10,6 +10,7 @@
 // reference's control flow depends on, once per phase for all scenes (max
 // speed for the CFL step, the occupied-cell box, the Jacobi error per
 // iteration, rigid-body forces).
+#include <atomic>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,6 +62,9 @@ static unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock)
 // ------------------------------------------------------------------------
 // device buffer
 // ------------------------------------------------------------------------
+// device bytes the library holds in its buffers (every context, world and server)
+static std::atomic<uint64_t> g_device_bytes{0};
+
 template <class T>
 struct DBuf {
     T *p = nullptr;
@@ -68,17 +72,16 @@ struct DBuf {
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() {
-        if (p) (void)hipFree(p);
-    }
+    ~DBuf() { release(); }
     void reserve(size_t n) {
         if (n <= cap) return;
         size_t c = std::max<size_t>(n, cap + cap / 4);
         T *q = nullptr;
         HIPCK(hipMalloc((void **)&q, std::max<size_t>(c, 1) * sizeof(T)));
-        if (p) (void)hipFree(p);
+        release();
         p = q;
         cap = c;
+        g_device_bytes += (uint64_t)std::max<size_t>(c, 1) * sizeof(T);
     }
     // pageable host source: the caller synchronizes before the source dies
     void upload(const T *h, size_t n, hipStream_t s) {
@@ -97,7 +100,10 @@ struct DBuf {
         std::swap(cap, o.cap);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            g_device_bytes -= (uint64_t)std::max<size_t>(cap, 1) * sizeof(T);
+        }
         p = nullptr;
         cap = 0;
     }
@@ -3323,6 +3329,7 @@ static void server_stats(const sph_server *s, sph_server_stats *o) {
     o->failed = s->failed.size();
     o->bytes = s->used;
     o->world_steps = s->world_steps;
+    o->device_bytes = g_device_bytes.load();
 }
 
 int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out) {
@@ -3432,6 +3439,13 @@ int sph_server_step(sph_server *s, sph_server_stats *out) {
                     s->finished.push_back(r);
             }
             s->resident.swap(keep);
+            // the world's launch buffers only grow; once the resident
+            // particles need less than a quarter of them (a large request
+            // left), free them, and the next step reserves what it needs
+            size_t n = 0;
+            for (sph_server::Req &r : s->resident) n += r.ctx->scene.n;
+            Batch &B = s->world->batch;
+            if (B.perm.cap > 4 * std::max<size_t>(n, (size_t)1 << 16)) B.release_scratch();
         }
         if (out) server_stats(s, out);
     });
@@ -3998,6 +4012,13 @@ int sph_mesh_sdf_build(sph_ctx *c, const double *vertices, size_t nv, const uint
         values.download(values_out, nodes, B.s);
         present.download(present_out, nodes, B.s);
         B.sync();
+    });
+}
+
+int sph_device_bytes_held(uint64_t *bytes) {
+    return guarded([&] {
+        if (!bytes) throw Error(SPH_ERR_ARG, "null argument");
+        *bytes = g_device_bytes.load();
     });
 }
 

@@ -64,7 +64,7 @@ class SphRigidInit(C.Structure):
 
 class SphServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("queued", "admitted", "resident", "finished",
-                                           "bytes", "world_steps", "failed")]
+                                           "bytes", "world_steps", "failed", "device_bytes")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -140,6 +140,7 @@ SIGNATURES = [
     ("sph_system_status", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("sph_server_read_stats", C.c_int, [C.c_void_p, C.POINTER(SphServerStats)]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
+    ("sph_device_bytes_held", C.c_int, [C.POINTER(C.c_uint64)]),
     ("sph_mesh_sdf_build", C.c_int, [C.c_void_p, _dp, C.c_size_t, _u32p, C.c_size_t, _dp,
                                      _i64p, C.c_double, C.c_double, _dp, _u8p]),
     ("sph_quantity_at_positions", C.c_int, [C.c_void_p, C.c_int, C.c_size_t,

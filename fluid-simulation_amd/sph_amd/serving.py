@@ -37,8 +37,12 @@ from . import scenes as S
 
 # device bytes per particle while resident in a world: 49 binary64 and 4 u32
 # per-particle fields of the scene, the world's per-particle sort scratch and
-# neighbor slots (about 45 pair slots of 4 B in the default recompute mode) and
-# one volume-map slot (flag + 7 binary64) per boundary
+# 48 pair slots of 4 B (the world gives every member the same slot width, its
+# largest neighbor count + 8: ~48 for the stream's scenes, which share one
+# h / dx ratio) and one volume-map slot (flag + 7 binary64) per boundary. The
+# world's launch buffers only grow, and the library frees them once its
+# residents need less than a quarter of them; servers report the bytes the
+# library really holds (device_bytes) beside these estimates.
 BYTES_PER_PARTICLE = 49 * 8 + 4 * 4 + 20 + 48 * 4
 BYTES_PER_PARTICLE_BODY = 1 + 7 * 8
 BYTES_PER_SCENE = 1 << 20          # stream, pinned staging, small tables
@@ -79,6 +83,7 @@ class Report:
     admit_s: float                  # time spent creating and initializing systems
     peak_resident: int
     peak_bytes: int
+    peak_device_bytes: int = 0      # measured: what the library's buffers held
 
     @property
     def particle_steps(self) -> int:
@@ -109,6 +114,7 @@ class Report:
             "p99_request_s": float(np.percentile(req, 99)),
             "peak_resident_scenes": self.peak_resident,
             "peak_resident_bytes_est": self.peak_bytes,
+            "peak_device_bytes_held": self.peak_device_bytes,
             # the last step's average density error after the position update
             # (measured) and the pressure solve's last predicted error
             "density_error_pct_max": float(max(derr)) if derr else 0.0,
@@ -227,7 +233,7 @@ class BatchedServer:
         step_ms: List[float] = []
         srv = system.Server(self.budget, self.max_slots)
         busy = admit = 0.0
-        peak_n = peak_b = 0
+        peak_n = peak_b = peak_d = 0
         t0 = self.clock()
         try:
             while pending or waiting or admitted:
@@ -263,7 +269,7 @@ class BatchedServer:
                         time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
                     continue
                 s0 = self.clock()
-                srv.step()
+                peak_d = max(peak_d, srv.step()["device_bytes"])
                 dt = self.clock() - s0
                 busy += dt
                 step_ms.extend([dt * 1e3] * st["resident"])
@@ -288,7 +294,7 @@ class BatchedServer:
         finally:
             srv.close()
         makespan = max(r.finished for r in results.values()) - min(r.arrival for r in stream)
-        return Report("batched", results, step_ms, makespan, busy, admit, peak_n, peak_b)
+        return Report("batched", results, step_ms, makespan, busy, admit, peak_n, peak_b, peak_d)
 
 
 class SequentialServer:
@@ -305,6 +311,7 @@ class SequentialServer:
         results: Dict[int, Result] = {}
         step_ms: List[float] = []
         busy = admit = 0.0
+        peak_d = 0
         t0 = self.clock()
         for req in sorted(stream, key=lambda r: r.arrival):
             wait = req.arrival - (self.clock() - t0)
@@ -317,6 +324,7 @@ class SequentialServer:
                 sim.set_max_iterations(req.max_iterations)
             try:
                 sim.init()
+                peak_d = max(peak_d, system.device_bytes_held())
                 admit += self.clock() - a0
                 for _ in range(req.steps):
                     s0 = self.clock()
@@ -333,4 +341,4 @@ class SequentialServer:
             sim._ctx.close()
         makespan = max(r.finished for r in results.values()) - min(r.arrival for r in stream)
         return Report("sequential", results, step_ms, makespan, busy, admit, 1,
-                      max(estimate_device_bytes(r.scene) for r in stream))
+                      max(estimate_device_bytes(r.scene) for r in stream), peak_d)

@@ -680,6 +680,13 @@ def build_mesh_sdf(vertices, triangles, x_min, n, dx: float, prune: float, syste
     return vals, pres
 
 
+def device_bytes_held() -> int:
+    """Device bytes the library's buffers hold now (every context, world, server)."""
+    b = C.c_uint64()
+    N.check(N.lib().sph_device_bytes_held(C.byref(b)))
+    return int(b.value)
+
+
 def device_memory():
     """(free, total) bytes of device memory."""
     f, t = C.c_size_t(), C.c_size_t()

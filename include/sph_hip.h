@@ -334,6 +334,7 @@ typedef struct sph_server_stats {
     uint64_t bytes;           /* estimates of the admitted requests */
     uint64_t world_steps;     /* world steps taken */
     uint64_t failed;          /* failed, waiting for sph_server_poll_failed */
+    uint64_t device_bytes;    /* device memory the library holds (sph_device_bytes_held) */
 } sph_server_stats;
 int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out);
 /* frees the world; contexts still attached are detached, not destroyed */
@@ -427,6 +428,10 @@ int sph_mesh_sdf_build(sph_ctx *ctx, const double *vertices, size_t nv,
 
 /* free and total device memory (hipMemGetInfo), for admission control */
 int sph_device_memory(size_t *free_bytes, size_t *total_bytes);
+/* device memory the library's buffers hold right now, over every context,
+ * world and server (no counterpart in the reference: for admission control,
+ * beside the per-request estimates a server is given) */
+int sph_device_bytes_held(uint64_t *bytes);
 
 /* ---- implementation options (no counterpart in the reference) ----------------
  * None of them changes a result bit; they trade memory and speed.

@@ -120,3 +120,41 @@ def test_served_stream_isolates_failing_requests():
             for k in ("x", "v", "a", "V", "p"):
                 assert np.array_equal(got.state[k], st[k]), (req.rid, k)
         assert rep.summary()["failed_requests"] == 2
+
+
+def test_device_bytes_held_and_world_scratch_released():
+    """The library counts the device bytes its buffers hold: a system's
+    creation raises the count and closing it returns it. A world's launch
+    buffers, which only grow, are freed once a large request has left and
+    the residents need less than a quarter of them; the estimates admission
+    uses stay above what a resident request holds."""
+    import gc
+    from sph_amd import scenes, system
+    gc.collect()                     # systems of earlier tests free their buffers first
+    b0 = system.device_bytes_held()
+    small = scenes.serving_scene(4096, 1000.0, 0.01, "box", 1)
+    sim = system.System(small)
+    assert system.device_bytes_held() > b0
+    sim._ctx.close()
+    assert system.device_bytes_held() == b0
+    big = scenes.serving_scene(300000, 1000.0, 0.01, "box", 2)
+    srv = system.Server(1 << 36, 4)
+    sims = {}
+    for sc, steps in ((big, 2), (small, 8)):
+        t = srv.submit(serving.estimate_device_bytes(sc), steps)
+        assert srv.admit() == t
+        sims[t] = system.System(sc)
+        srv.attach(t, sims[t])
+    peak = srv.step()["device_bytes"]
+    srv.step()                       # the large request finishes and leaves
+    got = srv.poll()
+    assert got is not None and len(got[1]) == big.n
+    got[1]._ctx.close()
+    after = srv.step()["device_bytes"]
+    held_small = after - b0
+    assert after < 0.5 * peak, (after, peak)
+    assert held_small <= serving.estimate_device_bytes(small) + (8 << 20), held_small
+    srv.close()
+    for s in sims.values():
+        if s._ctx.h:
+            s._ctx.close()
