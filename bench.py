@@ -185,7 +185,8 @@ def serving_bench(args):
     serving.BatchedServer(budget, max_slots=2).run(serving.make_stream(
         n_requests=2, rate=1000.0, n_min=2000, n_max=3000, steps=3, seed=99))
     log("[bench] batched run")
-    bat = serving.BatchedServer(budget, max_slots=args.slots).run(stream).summary()
+    bat = serving.BatchedServer(budget, max_slots=args.slots,
+                                graphs=not args.no_graphs).run(stream).summary()
     log(f"[bench] batched: {bat['throughput_particle_steps_per_s']:.3e} particle-steps/s, "
         f"makespan {bat['makespan_s']:.1f} s")
     seq = bound = None
@@ -214,6 +215,7 @@ def serving_bench(args):
                    "arrival_rate_per_s": args.rate, "particles": [args.n_min, args.n_max],
                    "steps_per_request": args.serve_steps, "solver": "IISPH",
                    "hbm_budget_bytes": budget, "max_slots": args.slots,
+                   "world_graphs": not args.no_graphs,
                    "device_memory_bytes": total},
         "p50_step_ms": bat["p50_step_ms"],
         "p99_step_ms": bat["p99_step_ms"],
@@ -613,6 +615,8 @@ def main():
                     help="serving: HBM budget (default 80%% of free device memory)")
     ap.add_argument("--no-sequential", action="store_true",
                     help="serving: skip the sequential baseline")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="serving: launch the world's Jacobi chunks directly")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -849,7 +849,9 @@ static void pair_pass_grid(Batch &B, size_t grid, int pass, int pid, const Tab &
     if (grid == 0) return;
     auto kern = tab[B.grad_cache ? 1 : 0][B.variant[pass]];
     Prof::Rec r{pid, B.total_particles, nullptr, nullptr};
-    Prof::Rec t{pass * kPairVariants + B.variant[pass], B.total_particles, nullptr, nullptr};
+    // tuning cost per particle slot of this launch (a chunk over the scenes
+    // still iterating covers fewer than the batch's particles)
+    Prof::Rec t{pass * kPairVariants + B.variant[pass], grid * kBlock, nullptr, nullptr};
     const bool timed = B.prof.on && pid < SPH_PROF_KERNELS, tuning = B.tuner.measuring();
     if (timed) B.prof.begin(B.s, r);
     if (tuning) B.tprof.begin(B.s, t);
@@ -3332,6 +3334,15 @@ static void server_stats(const sph_server *s, sph_server_stats *o) {
     o->device_bytes = g_device_bytes.load();
 }
 
+// the server's world, created on first use
+static sph_world *server_world(sph_server *s) {
+    if (!s->world) {
+        int e = sph_world_create(&s->world);
+        if (e) throw Error(e, sph_last_error());
+    }
+    return s->world;
+}
+
 int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out) {
     return guarded([&] {
         if (!out || max_slots < 1) throw Error(SPH_ERR_ARG, "need an output and at least one slot");
@@ -3384,11 +3395,7 @@ int sph_server_attach(sph_server *s, uint64_t ticket, sph_ctx *ctx) {
         auto it = std::find_if(s->admitted.begin(), s->admitted.end(),
                                [&](const sph_server::Req &r) { return r.ticket == ticket; });
         if (it == s->admitted.end()) throw Error(SPH_ERR_STATE, "ticket is not awaiting a system");
-        if (!s->world) {
-            int e = sph_world_create(&s->world);
-            if (e) throw Error(e, sph_last_error());
-        }
-        int e = sph_world_attach(s->world, ctx);
+        int e = sph_world_attach(server_world(s), ctx);
         if (e) throw Error(e, sph_last_error());
         sph_server::Req r = *it;
         r.ctx = ctx;
@@ -3613,6 +3620,13 @@ int sph_world_set_option(sph_world *w, int option, int value) {
     return guarded([&] {
         if (!w) throw Error(SPH_ERR_ARG, "null argument");
         set_option(w->batch, option, value);
+    });
+}
+
+int sph_server_set_option(sph_server *s, int option, int value) {
+    return guarded([&] {
+        if (!s) throw Error(SPH_ERR_ARG, "null argument");
+        set_option(server_world(s)->batch, option, value);
     });
 }
 

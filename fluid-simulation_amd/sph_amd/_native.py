@@ -139,6 +139,7 @@ SIGNATURES = [
                                          C.POINTER(C.c_int)]),
     ("sph_system_status", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("sph_server_read_stats", C.c_int, [C.c_void_p, C.POINTER(SphServerStats)]),
+    ("sph_server_set_option", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("sph_device_memory", C.c_int, [_szp, _szp]),
     ("sph_device_bytes_held", C.c_int, [C.POINTER(C.c_uint64)]),
     ("sph_mesh_sdf_build", C.c_int, [C.c_void_p, _dp, C.c_size_t, _u32p, C.c_size_t, _dp,

@@ -39,11 +39,13 @@ from . import scenes as S
 # per-particle fields of the scene, the world's per-particle sort scratch and
 # 48 pair slots of 4 B (the world gives every member the same slot width, its
 # largest neighbor count + 8: ~48 for the stream's scenes, which share one
-# h / dx ratio) and one volume-map slot (flag + 7 binary64) per boundary. The
-# world's launch buffers only grow, and the library frees them once its
+# h / dx ratio) and one volume-map slot (flag + 7 binary64) per boundary, plus
+# 64 B for the world's shared buffers, which grow in steps of 1.25x (the
+# serving stream held 2.68 GB at its peak against 2.59 GB of estimates
+# without it). The world's launch buffers only grow, and the library frees them once its
 # residents need less than a quarter of them; servers report the bytes the
 # library really holds (device_bytes) beside these estimates.
-BYTES_PER_PARTICLE = 49 * 8 + 4 * 4 + 20 + 48 * 4
+BYTES_PER_PARTICLE = 49 * 8 + 4 * 4 + 20 + 48 * 4 + 64   # + 64: the shared buffers' 1.25x growth
 BYTES_PER_PARTICLE_BODY = 1 + 7 * 8
 BYTES_PER_SCENE = 1 << 20          # stream, pinned staging, small tables
 
@@ -216,13 +218,15 @@ class BatchedServer:
     admitted request's System and collects finished ones."""
 
     def __init__(self, hbm_budget_bytes: int, max_slots: int = 64,
-                 clock: Callable[[], float] = time.perf_counter, keep_state: bool = False):
+                 clock: Callable[[], float] = time.perf_counter, keep_state: bool = False,
+                 graphs: bool = True):
         from . import system  # loads the HIP library
         self._system = system
         self.budget = int(hbm_budget_bytes)
         self.max_slots = int(max_slots)
         self.clock = clock
         self.keep_state = keep_state
+        self.graphs = graphs      # replay the world's Jacobi chunks from HIP graphs
 
     def run(self, stream: List[Request]) -> Report:
         system = self._system
@@ -232,6 +236,8 @@ class BatchedServer:
         results: Dict[int, Result] = {}
         step_ms: List[float] = []
         srv = system.Server(self.budget, self.max_slots)
+        if not self.graphs:
+            srv.set_option(system.N.OPT_GRAPHS, 0)
         busy = admit = 0.0
         peak_n = peak_b = peak_d = 0
         t0 = self.clock()

@@ -619,6 +619,10 @@ class Server:
         N.check(N.lib().sph_server_step(self.h, C.byref(out)))
         return out.as_dict()
 
+    def set_option(self, option: int, value: int):
+        """An implementation option (N.OPT_*) of the server's world."""
+        N.check(N.lib().sph_server_set_option(self.h, int(option), int(value)))
+
     def poll(self):
         """(ticket, System) of the oldest finished request, or None."""
         t, c = C.c_uint64(), C.c_void_p()

@@ -367,6 +367,8 @@ int sph_server_poll_failed(sph_server *server, uint64_t *ticket, sph_ctx **ctx, 
  * the caller's), releasing its slot and bytes; 0 if none has finished */
 int sph_server_poll(sph_server *server, uint64_t *ticket, sph_ctx **ctx);
 int sph_server_read_stats(const sph_server *server, sph_server_stats *out);
+/* an implementation option (SPH_OPT_*) of the server's world */
+int sph_server_set_option(sph_server *server, int option, int value);
 /* ---- sensor queries -------------------------------------------------------------
  * SPHSystem::get_quantity_at_positions (sph/mod.rs:241-341): a scalar field
  * of the current fluid state at nq positions ([nq][3] f32, widened to f64),

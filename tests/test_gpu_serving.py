@@ -158,3 +158,16 @@ def test_device_bytes_held_and_world_scratch_released():
     for s in sims.values():
         if s._ctx.h:
             s._ctx.close()
+
+
+def test_server_options_before_the_first_request():
+    """A server's world options can be set before any request is attached
+    (the world is created on first use); with graph replay off the stream's
+    results are those of the default server, bit for bit."""
+    stream = serving.make_stream(n_requests=3, rate=1000.0, n_min=800, n_max=3000, steps=6,
+                                 seed=5)
+    a = serving.BatchedServer(1 << 34, max_slots=8, keep_state=True).run(stream)
+    b = serving.BatchedServer(1 << 34, max_slots=8, keep_state=True, graphs=False).run(stream)
+    for req in stream:
+        ra, rb = a.results[req.rid], b.results[req.rid]
+        assert ra.digest == rb.digest and ra.measurement == rb.measurement, req.rid
