@@ -9,9 +9,9 @@ average density error 0.1 %. The fluid rests on the wall from the first step.
 
 A step is one SPHSystem::step_forward_in_time on the device-resident state.
 --settle untimed steps (default 60) let the fluid compress until every solve
-carries its weight (EQS2 then takes ~18 Jacobi iterations per step) and let
-the code-shape tuner finish; W warm-up steps follow, then exactly K steps are
-timed between device synchronizations. value = particles x K / wall time
+carries its weight (about 22 Jacobi iterations per step, EQS1 + EQS2) and
+let the code-shape tuner finish; W warm-up steps follow, then exactly K steps
+are timed between device synchronizations. value = particles x K / wall time
 (particle-steps/s).
 
 The JSON line also carries
@@ -24,7 +24,8 @@ The JSON line also carries
                 measured iteration count, times the particle-step rate;
   cpu_baseline  the CPU oracle (binary64 restatement of the reference) timed
                 on this host over a bounded sample of the same workload,
-                started from the GPU's state at the timed window's start.
+                started from the GPU's state at the timed window's start, on
+                all host threads, and one step of it on one thread beside.
 
 --workload serving runs the serving stream instead: seeded Poisson arrivals
 of IISPH scene requests of mixed sizes served by continuous batching (one
@@ -599,6 +600,8 @@ def main():
                          "state; the code-shape tuner finishes)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-one-thread", action="store_true",
+                    help="skip the one-core step of the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain device copy of known size (counter calibration)")
     ap.add_argument("--shapes", default="",
@@ -778,6 +781,11 @@ def main():
             else:
                 start_step = 0
             out["cpu_baseline"] = cpu_baseline(base, args.cpu_seconds, threads, start_step)
+            if threads > 1 and not args.no_one_thread:
+                # the same state on one core: one step (a bounded sample)
+                one = cpu_baseline(base, 0.0, 1, start_step)
+                out["cpu_baseline"]["one_thread"] = {"value": one["value"], "cores": 1,
+                                                     "sample": one["sample"]}
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline unavailable: {e}")
     print(json.dumps(out))
