@@ -22,6 +22,9 @@ The JSON line also carries
                 binary64 fields beside it;
   step_roofline the whole update's compulsory bytes per particle-step at the
                 measured iteration count, times the particle-step rate;
+  roofline_fp64 the dominant kernel's algorithmic binary64 operations per
+                launch (FLOPS_PER_PAIR x neighbor pairs) over its mean launch
+                duration, against the 78.6 TFLOP/s FP64 peak;
   cpu_baseline  the CPU oracle (binary64 restatement of the reference) timed
                 on this host over a bounded sample of the same workload,
                 started from the GPU's state at the timed window's start, on
@@ -61,6 +64,14 @@ sys.path.insert(0, os.path.join(ROOT, "fluid-simulation_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic binary64 operations per neighbor pair (+, -, x, / and sqrt one
+# each; maxima and selects not counted). grad W: r 3, |r|^2 5, sqrt 1, |r|/h 1,
+# w'(q) 7, three components (gfac r_k / |r|) w' 9 = 26. Jacobi adds
+# (a_i - a_j) . grad W 8 and dt^2 V_j and the sum 3; pressure acceleration adds
+# V_j (p_i + p_j) c_i 3 and the three component updates 6.
+FLOPS_PER_PAIR = {"jacobi": 37, "pressure_accel": 35}
 
 # Algorithmic (compulsory) HBM bytes per particle per launch, binary64 SoA:
 # the particle's own fields read once and its outputs written once; neighbor
@@ -765,6 +776,23 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if dom in FLOPS_PER_PAIR:
+        # the same kernel against the FP64 vector peak: neighbor pairs per
+        # particle estimated from 20,000 particles of the final state
+        from scipy.spatial import cKDTree
+        xs = sim.state()["x"]
+        pick = np.random.default_rng(0).choice(len(xs), size=min(20000, len(xs)), replace=False)
+        pairs = float(np.mean(cKDTree(xs).query_ball_point(
+            xs[pick], scene.params.kernel_support_radius, return_length=True)))
+        flops = FLOPS_PER_PAIR[dom] * pairs * units
+        out["roofline_fp64"] = {
+            "bound": "fp64_valu", "kernel": dom,
+            "achieved": flops / avg_s / 1e12 if avg_s > 0 else 0.0,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": flops / avg_s / 1e12 / FP64_PEAK_TFLOPS if avg_s > 0 else 0.0,
+            "flops_per_pair": FLOPS_PER_PAIR[dom], "pairs_per_particle": pairs,
+            "note": "divisions and square roots count one operation each but issue as "
+                    "~10 VALU instructions; the counter profile gives VALU busy"}
     if args.copy_probe:
         # cloning the system copies each binary64 particle array device-to-device
         # (one runtime copy kernel per array, 8 bytes x particles each); those
