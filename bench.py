@@ -635,7 +635,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the hot path's "
-                          "multi-GPU decomposition is described in DESIGN.md and not built",
+                          "multi-GPU decomposition (spatial slabs with halo exchanges) is designed, not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
