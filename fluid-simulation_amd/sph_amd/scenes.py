@@ -23,7 +23,7 @@ That setup path is outside the hot path, so scenes here are generated directly:
 * rigid-body mass properties in closed form for boxes and spheres.
 
 Both the HIP path and the CPU oracle consume the identical arrays, so per-step
-parity is judged on the hot path alone.
+parity concerns the hot path alone.
 """
 from __future__ import annotations
 
