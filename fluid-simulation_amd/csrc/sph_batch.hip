@@ -75,11 +75,16 @@ __device__ __forceinline__ GLOBAL_AS T *gm(T *p) {
     return (GLOBAL_AS T *)p;
 }
 
-// packed per-particle records gathered by the neighbor loops (one 32-byte
-// record instead of separate 8-byte loads): q4[i] = {x, y, z, V},
-// pa4[i] = {pressure acceleration x, y, z, 0}
+// packed per-particle records gathered by the neighbor loops (one record
+// instead of separate 8-byte loads): q4[i] = {x, y, z, V} (32 bytes),
+// pa3[i] = {pressure acceleration x, y, z} and v3[i] = {velocity x, y, z}
+// (24 bytes, unpadded: the gathers' cost grows with the bytes per neighbor)
 typedef double rec4 __attribute__((ext_vector_type(4)));
 typedef double rec2 __attribute__((ext_vector_type(2)));
+typedef double rec2u __attribute__((ext_vector_type(2), aligned(8)));  // 8-byte aligned
+struct rec3 {
+    double x, y, z;
+};
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ rec4 ld4(const double *a, size_t i) {
     return ((const GLOBAL_AS rec4 *)a)[i];
@@ -93,6 +98,23 @@ __device__ __forceinline__ rec4 ldj4(const double *a, uint32_t j) {
 }
 __device__ __forceinline__ double ldj(const double *a, uint32_t j) {
     return *(const GLOBAL_AS double *)((const GLOBAL_AS char *)a + (j << 3));
+}
+// 24-byte record j: one 16-byte and one 8-byte load (the host keeps every
+// scene below 2^27 particles, so 24 j fits 32 bits)
+__device__ __forceinline__ rec3 ldj3(const double *a, uint32_t j) {
+    const GLOBAL_AS char *r = (const GLOBAL_AS char *)a + j * 24u;
+    const rec2u xy = *(const GLOBAL_AS rec2u *)r;
+    return rec3{xy.x, xy.y, *(const GLOBAL_AS double *)(r + 16)};
+}
+__device__ __forceinline__ rec3 ld3(const double *a, size_t i) {
+    const GLOBAL_AS double *r = (const GLOBAL_AS double *)a + 3 * i;
+    const rec2u xy = *(const GLOBAL_AS rec2u *)r;
+    return rec3{xy.x, xy.y, r[2]};
+}
+__device__ __forceinline__ void st3(double *a, size_t i, double x, double y, double z) {
+    GLOBAL_AS double *r = (GLOBAL_AS double *)a + 3 * i;
+    *(GLOBAL_AS rec2u *)r = rec2u{x, y};
+    r[2] = z;
 }
 __device__ __forceinline__ void st4(double *a, size_t i, double x, double y, double z,
                                     double w) {
@@ -177,6 +199,10 @@ __device__ __forceinline__ void neighbor_loop(const GLOBAL_AS uint32_t *ej, uint
 // gathered records of one neighbor
 struct QU {  // {x, y, z, V} and a second 32-byte record
     rec4 q, u;
+};
+struct QV {  // {x, y, z, V} and a 24-byte record
+    rec4 q;
+    rec3 u;
 };
 struct QP {  // {x, y, z, V} and the pressure
     rec4 q;
@@ -355,7 +381,7 @@ __global__ void kb_permute(BatchDev B) {
     gm(v.tvx)[i] = ux;
     gm(v.tvy)[i] = uy;
     gm(v.tvz)[i] = uz;
-    st4(v.v4, i, ux, uy, uz, 0.0);
+    st3(v.v3, i, ux, uy, uz);
     gm(v.tm)[i] = gm(v.m)[src];
     gm(v.tfid)[i] = gm(v.fid)[src];
     gm(v.tref)[i] = gm(v.ref)[src];
@@ -669,10 +695,10 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
         }
         const uint32_t c = gm(B.cnt)[gi];
         const size_t e0 = ELL_BASE(B, v, i);
-        const double *q4 = v.q4, *v4 = v.v4;  // bases read once, ahead of the loop
+        const double *q4 = v.q4, *v3 = v.v3;  // bases read once, ahead of the loop
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(q4, j), ldj4(v4, j)}; },
-            [&](uint32_t k, const QU &r) {
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QV{ldj4(q4, j), ldj3(v3, j)}; },
+            [&](uint32_t k, const QV &r) {
                 double rx = px - r.q.x, ry = py - r.q.y, rz = pz - r.q.z;
                 double cf = v.nu_f * 2. * (3. + 2.) * r.q.w *
                             dot3(ux - r.u.x, uy - r.u.y, uz - r.u.z, rx, ry, rz) /
@@ -829,7 +855,7 @@ __device__ __forceinline__ void pressure_accel_impl(const BatchDev &B, int targe
         }
     }
     if (live && target_scratch) {  // the solver's scratch is always overwritten
-        st4(v.pa4, i, acx, acy, acz, 0.0);
+        st3(v.pa3, i, acx, acy, acz);
     } else if (live) {
         double *tx_ = v.ax, *ty_ = v.ay, *tz_ = v.az;
         if (overwrite) {
@@ -948,7 +974,7 @@ __device__ __forceinline__ void jacobi_impl(const BatchDev &B) {
     const SolveConst S = *(const SolveConst *)&v.S;
     double err = 0.;
     if (live) {
-        const rec4 ai = ld4(v.pa4, i);
+        const rec3 ai = ld3(v.pa3, i);
         const double ux = ai.x, uy = ai.y, uz = ai.z;
         const double dt2 = S.dt * S.dt;
         double adp = 0.;
@@ -956,10 +982,10 @@ __device__ __forceinline__ void jacobi_impl(const BatchDev &B) {
         const double px = qi.x, py = qi.y, pz = qi.z;
         const uint32_t c = gm(B.cnt)[gi];
         const size_t e0 = ELL_BASE(B, v, i);
-        const double *q4 = v.q4, *pa4 = v.pa4;  // bases read once, ahead of the loop
+        const double *q4 = v.q4, *pa3 = v.pa3;  // bases read once, ahead of the loop
         neighbor_loop<PIPE, UNROLL>(
-            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QU{ldj4(q4, j), ldj4(pa4, j)}; },
-            [&](uint32_t k, const QU &r) {
+            gm(B.ell_j) + e0, c, [&](uint32_t j) { return QV{ldj4(q4, j), ldj3(pa3, j)}; },
+            [&](uint32_t k, const QV &r) {
                 double wx, wy, wz;
                 // signed-zero form: faster here, same sums (kernel_grad_rh)
                 pair_grad<CACHED, true>(B, K, rh, ELL_AT(e0, k), px - r.q.x, py - r.q.y,

@@ -39,9 +39,9 @@ struct SceneView {
     GridDev grid;         // occupied-cell box; start = batch table + cell_base
     double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *m, *V, *p;
     double *q4;   // {x, y, z, V} per particle: the record neighbor loops gather
-    double *pa4;  // {pressure acceleration, 0} per particle: the Jacobi scratch
+    double *pa3;  // {pressure acceleration} per particle (24 bytes): the Jacobi scratch
     double *sv4;  // {predicted velocity, m} per particle: gathered by the source pass
-    double *v4;   // {velocity, 0} per particle: gathered by the viscosity pass
+    double *v3;   // {velocity} per particle (24 bytes): gathered by the viscosity pass
     double *s_f, *a_ff, *svx, *svy, *svz, *sxp, *syp, *szp;
     double *tx, *ty, *tz, *tvx, *tvy, *tvz, *tm;  // permutation targets
     uint32_t *fid, *ref, *tfid, *tref;

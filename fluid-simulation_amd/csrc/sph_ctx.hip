@@ -397,9 +397,9 @@ struct Scene {
     DBuf<uint32_t> fid, ref;
     // solver and integrator slots
     DBuf<double> s_f, a_ff, svx, svy, svz, sxp, syp, szp, ix, iy, iz, ivx, ivy, ivz;
-    // packed gather records: {x, y, z, V}, {pressure acceleration, 0},
-    // {predicted velocity, m}, {velocity, 0}
-    DBuf<double> q4, pa4, sv4, v4;
+    // packed gather records: {x, y, z, V}, {pressure acceleration},
+    // {predicted velocity, m}, {velocity}
+    DBuf<double> q4, pa3, sv4, v3;
     // permutation targets
     DBuf<double> tx, ty, tz, tvx, tvy, tvz, tm;
     DBuf<uint32_t> tfid, tref;
@@ -488,7 +488,7 @@ struct Scene {
         v.ax = ax.p; v.ay = ay.p; v.az = az.p;
         v.m = m.p; v.V = V.p; v.p = p.p;
         v.s_f = s_f.p; v.a_ff = a_ff.p;
-        v.q4 = q4.p; v.pa4 = pa4.p; v.sv4 = sv4.p; v.v4 = v4.p;
+        v.q4 = q4.p; v.pa3 = pa3.p; v.sv4 = sv4.p; v.v3 = v3.p;
         v.svx = svx.p; v.svy = svy.p; v.svz = svz.p;
         v.sxp = sxp.p; v.syp = syp.p; v.szp = szp.p;
         v.tx = tx.p; v.ty = ty.p; v.tz = tz.p;
@@ -1065,9 +1065,9 @@ static void reserve_scene(Scene &c, size_t n) {
                             &c.tm};
     for (auto *b : f64s) b->reserve(k);
     c.q4.reserve(4 * k);
-    c.pa4.reserve(4 * k);
+    c.pa3.reserve(3 * k);
     c.sv4.reserve(4 * k);
-    c.v4.reserve(4 * k);
+    c.v3.reserve(3 * k);
     c.fid.reserve(k);
     c.ref.reserve(k);
     c.tfid.reserve(k);
