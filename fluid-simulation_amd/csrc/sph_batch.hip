@@ -476,6 +476,12 @@ __global__ void kb_fill_ell(BatchDev B) {
 // takes the neighbors in the order they are found, which is the order every
 // other pass reads them in, with kb_pairs_volume's expressions; the boundary
 // terms follow as in kb_pairs_volume, so boundary samples must be current.
+// The particle itself is a neighbor of the reference's list (r = 0); its
+// volume term W(0) is summed in its place, but it is not stored: in every
+// pass that reads the cache its term is grad W(0) = +0 times finite factors,
+// a zero that leaves the pass's sums unchanged (each sum starts at +0 and a
+// round-to-nearest sum is never -0, so adding a zero of either sign is the
+// identity), so one slot per particle is saved in every such pass.
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap, uint32_t *maxc) {
     __shared__ uint32_t rows[kBlock / 64][kSweepRows][64];
@@ -528,11 +534,13 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
                             const double rx = px - cx_[t], ry = py - cy_[t], rz = pz - cz_[t];
                             const double d2 = (rx * rx + ry * ry) + rz * rz;
                             if (j < b && d2 < v.dthr) {
-                                if (cnt < kSweepRows) rows[w][cnt][l] = j;
-                                else if (cnt < cap) gm(B.ell_j)[ELL_AT(e0, cnt)] = j;
+                                if (j != i) {  // the self pair is summed, not stored
+                                    if (cnt < kSweepRows) rows[w][cnt][l] = j;
+                                    else if (cnt < cap) gm(B.ell_j)[ELL_AT(e0, cnt)] = j;
+                                    cnt++;
+                                }
                                 const double q = div_rcp_sum(sqrt(d2), K.h, rh);
                                 accu += v.V0 * (K.wfac * kernel_w(q));
-                                cnt++;
                             }
                         }
                     }
