@@ -195,6 +195,22 @@ def test_compaction_below_threshold(native, refcpu):
     assert np.array_equal(g.state()["fluid_id"], o.state()["fluid_id"])
 
 
+@pytest.mark.parametrize("solver", ["IISPH", "IISPHwOST", "SESPH"])
+def test_compaction_to_no_particles(native, refcpu, solver):
+    """Every particle below disable_particles_below: System::new's update
+    disables them all (fluid.rs:159-210) and the empty system keeps stepping,
+    as the oracle's does (no iterations, zero errors, time advancing)."""
+    sc = scenes.small_iisph_box(side=6, jitter=2e-3, solver=solver)
+    sc.params.disable_particles_below = float(np.max(sc.x[:, 2])) + 1.0
+    print("compaction to no particles:", solver, flush=True)
+    o, g, _ = run_pair(native, refcpu, sc, 3)
+    assert len(g) == len(o) == 0
+    mg, mo = g.take_measurement(), o.measure()
+    assert mg["num_particles"] == mo["num_particles"] == 0
+    for k in ("density_error", "solver_iterations", "time_step_size", "time"):
+        assert mg[k] == mo[k], (k, mg[k], mo[k])
+
+
 def test_clone_is_independent(native, refcpu):
     sc = scenes.small_iisph_box(side=10, jitter=2e-3)
     g = native.System(sc)

@@ -105,6 +105,17 @@ def test_world_compaction(native):
     assert len(mates[0]) < a.n and len(mates[1]) == b.n
 
 
+def test_world_member_without_particles(native):
+    """A member compacted to no particles steps beside a live one: both stay
+    bit-identical to their solo runs."""
+    a = scenes.small_iisph_box(side=6, seed=1, jitter=2e-3)
+    a.params.disable_particles_below = float(np.max(a.x[:, 2])) + 1.0
+    b = scenes.small_iisph_box(side=9, seed=2, jitter=2e-3)
+    print("world member without particles", flush=True)
+    solo, mates, _ = run_world_vs_solo(native, [a, b], 4)
+    assert len(mates[0]) == 0 and len(mates[1]) == b.n
+
+
 def test_world_attach_detach(native):
     sa = scenes.small_iisph_box(side=9, seed=1, jitter=2e-3)
     sb = scenes.small_iisph_box(side=10, seed=2, jitter=2e-3)
