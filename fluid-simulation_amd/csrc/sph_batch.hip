@@ -404,7 +404,14 @@ __global__ void kb_count_ell(BatchDev B) {
         int zlo = cz - v.reach - g.org[2], zhi = cz + v.reach - g.org[2];
         if (zlo < 0) zlo = 0;
         if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
-        if (zlo <= zhi) {
+        if (g.part) {  // shared hash buckets: the reference's bucket visits
+            bucket_walk(g, cx, cy, cz, v.reach, [&](uint32_t jg) {
+                const rec4 qj = ld4(v.q4, jg - v.part_base);
+                double ddx = px - qj.x, ddy = py - qj.y, ddz = pz - qj.z;
+                double d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                cnt += d2 < v.dthr ? 1u : 0u;
+            });
+        } else if (zlo <= zhi) {
             for (int dx = -v.reach; dx <= v.reach; dx++) {
                 int gx = cx + dx - g.org[0];
                 if (gx < 0 || gx >= g.dim[0]) continue;
@@ -440,6 +447,19 @@ __global__ void kb_fill_ell(BatchDev B) {
               cz = sat_i32(floor(pz / v.cs));
     const size_t e0 = ELL_BASE(B, v, i);
     uint32_t kk = 0;
+    if (g.part) {  // shared hash buckets: the reference's bucket visits
+        bucket_walk(g, cx, cy, cz, v.reach, [&](uint32_t jg) {
+            const uint32_t j = jg - v.part_base;
+            const rec4 qj = ld4(v.q4, j);
+            double rx = px - qj.x, ry = py - qj.y, rz = pz - qj.z;
+            double d2 = (rx * rx + ry * ry) + rz * rz;
+            if (d2 < v.dthr) {
+                gm(B.ell_j)[ELL_AT(e0, kk)] = j;
+                kk++;
+            }
+        });
+        return;
+    }
     int zlo = cz - v.reach - g.org[2], zhi = cz + v.reach - g.org[2];
     if (zlo < 0) zlo = 0;
     if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
@@ -500,10 +520,31 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
         const int cx = sat_i32(floor(px / v.cs)), cy = sat_i32(floor(py / v.cs)),
                   cz = sat_i32(floor(pz / v.cs));
         double accu = 0.;
+        // a neighbor (d2 < dthr) in the order found: stored unless it is the
+        // particle itself, and its volume term summed
+        auto take = [&](uint32_t j, double d2) {
+            if (j != i) {  // the self pair is summed, not stored
+                if (cnt < kSweepRows) rows[w][cnt][l] = j;
+                else if (cnt < cap) gm(B.ell_j)[ELL_AT(e0, cnt)] = j;
+                cnt++;
+            }
+            const double q = div_rcp_sum(sqrt(d2), K.h, rh);
+            accu += v.V0 * (K.wfac * kernel_w(q));
+        };
         int zlo = cz - v.reach - g.org[2], zhi = cz + v.reach - g.org[2];
         if (zlo < 0) zlo = 0;
         if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
-        if (zlo <= zhi) {
+        if (g.part) {  // shared hash buckets: the reference's bucket visits
+            bucket_walk(g, cx, cy, cz, v.reach, [&](uint32_t jg) {
+                const uint32_t j = jg - v.part_base;
+                const GLOBAL_AS char *rec = (const GLOBAL_AS char *)v.q4 + (j << 5);
+                const rec2 xy = *(const GLOBAL_AS rec2 *)rec;  // .w is being written; unused
+                const double rx = px - xy.x, ry = py - xy.y,
+                             rz = pz - *(const GLOBAL_AS double *)(rec + 16);
+                const double d2 = (rx * rx + ry * ry) + rz * rz;
+                if (d2 < v.dthr) take(j, d2);
+            });
+        } else if (zlo <= zhi) {
             for (int dx = -v.reach; dx <= v.reach; dx++) {
                 const int gx = cx + dx - g.org[0];
                 if (gx < 0 || gx >= g.dim[0]) continue;
@@ -533,15 +574,7 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
                             const uint32_t j = j0 + (uint32_t)t;
                             const double rx = px - cx_[t], ry = py - cy_[t], rz = pz - cz_[t];
                             const double d2 = (rx * rx + ry * ry) + rz * rz;
-                            if (j < b && d2 < v.dthr) {
-                                if (j != i) {  // the self pair is summed, not stored
-                                    if (cnt < kSweepRows) rows[w][cnt][l] = j;
-                                    else if (cnt < cap) gm(B.ell_j)[ELL_AT(e0, cnt)] = j;
-                                    cnt++;
-                                }
-                                const double q = div_rcp_sum(sqrt(d2), K.h, rh);
-                                accu += v.V0 * (K.wfac * kernel_w(q));
-                            }
+                            if (j < b && d2 < v.dthr) take(j, d2);
                         }
                     }
                 }

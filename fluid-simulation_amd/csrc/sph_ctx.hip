@@ -340,6 +340,7 @@ struct Body {
     DBuf<double> sx, sy, sz, svol;       // sorted by (cell, index)
     DBuf<uint32_t> sperm;                // sorted -> caller sample index
     DBuf<uint32_t> sgrid;                // cell start
+    DBuf<uint32_t> spart;                // bucket partners of the sample cells (GridDev::part)
     GridDev sg{};
     DBuf<uint32_t> soff, sidx, scnt;     // per fluid particle CSR
     // volume map
@@ -409,6 +410,8 @@ struct Scene {
     int box[6] = {0, 0, 0, 0, 0, 0};
     int checked_box[6] = {1, 1, 1, 0, 0, 0};
     bool hash_box_valid = false;
+    bool hash_collides = false;  // the checked box holds cells that share hash buckets
+    DBuf<uint32_t> hpart;        // their bucket partners on `grid` (GridDev::part)
     // pressure-solve state of the running update
     bool active = false;
     uint32_t iter = 0;
@@ -483,6 +486,8 @@ struct Scene {
         for (int k = 0; k < 3; k++) v.gcenter[k] = P.gravity_center[k];
         v.grid = grid;
         v.grid.start = cell_start ? cell_start + cell_base : nullptr;
+        v.grid.mkey = ref.p;  // the reference index of each sorted particle
+        v.grid.mbase = part_base;
         v.x = x.p; v.y = y.p; v.z = z.p;
         v.vx = vx.p; v.vy = vy.p; v.vz = vz.p;
         v.ax = ax.p; v.ay = ay.p; v.az = az.p;
@@ -986,10 +991,12 @@ static GridDev make_grid(const int lo[3], const int hi[3]) {
     return g;
 }
 
-// refuse cell boxes where two cells that can share a stencil share a hash bucket
-static void hash_guard(Batch &B, int *checked, bool &valid, const int lo[3], const int hi[3],
-                       int reach) {
-    int elo[3], ehi[3];
+// Whether the cell box [lo, hi] widened by the reach holds two cells that can
+// share a stencil and share a hash bucket. The answer is kept for a checked
+// box (with a margin) so that a system re-checks only when it leaves it.
+static bool hash_guard(Batch &B, int *checked, bool &valid, bool &collides, const int lo[3],
+                       const int hi[3], int reach) {
+    int elo[3], ehi[3];  // the stencil cells of the box's particles
     for (int k = 0; k < 3; k++) {
         elo[k] = lo[k] - reach;
         ehi[k] = hi[k] + reach;
@@ -998,7 +1005,7 @@ static void hash_guard(Batch &B, int *checked, bool &valid, const int lo[3], con
         bool inside = true;
         for (int k = 0; k < 3; k++)
             inside = inside && elo[k] >= checked[k] && ehi[k] <= checked[3 + k];
-        if (inside) return;
+        if (inside) return collides;
     }
     for (int k = 0; k < 3; k++) {  // margin so that small drifts do not re-check
         elo[k] -= 4;
@@ -1012,15 +1019,34 @@ static void hash_guard(Batch &B, int *checked, bool &valid, const int lo[3], con
     unsigned int *f = reinterpret_cast<unsigned int *>(B.pinned);
     HIPCK(hipMemcpyAsync(f, B.flag.p, sizeof *f, hipMemcpyDeviceToHost, B.s));
     B.sync();
-    if (*f)
-        throw Error(SPH_ERR_UNSUPPORTED,
-                    "two cells of one neighbor stencil share a spatial-hash bucket; the "
-                    "reference would report duplicate neighbors there");
+    collides = *f != 0;
     for (int k = 0; k < 3; k++) {
         checked[k] = elo[k];
         checked[3 + k] = ehi[k];
     }
     valid = true;
+    return collides;
+}
+
+// The bucket partners of g's cells (GridDev::part) into `part`; g.part stays
+// null when no two cells of g that can share a stencil share a bucket.
+static void hash_partners(Batch &B, GridDev &g, DBuf<uint32_t> &part, int reach) {
+    g.part = nullptr;
+    g.pm = 2 * reach;  // a stencil cell sharing a bucket with a cell of the box lies this near it
+    const size_t nc = ((size_t)g.dim[0] + 2 * g.pm) * ((size_t)g.dim[1] + 2 * g.pm) *
+                      ((size_t)g.dim[2] + 2 * g.pm);
+    part.reserve(2 * nc);
+    B.flag.reserve(4);
+    LAUNCH(k_fill_u32, 4, B.s, (uint32_t *)B.flag.p, (size_t)4, 0u);
+    LAUNCH(k_hash_partners, nc, B.s, g, 2 * reach, part.p, B.flag.p);
+    unsigned int *f = reinterpret_cast<unsigned int *>(B.pinned);
+    HIPCK(hipMemcpyAsync(f, B.flag.p, 2 * sizeof *f, hipMemcpyDeviceToHost, B.s));
+    B.sync();
+    if (f[1])
+        throw Error(SPH_ERR_UNSUPPORTED,
+                    "more than two other cells within two stencil reaches share one cell's "
+                    "spatial-hash bucket");
+    if (f[0]) g.part = part.p;
 }
 
 // ---- single point-set helpers: standalone queries and static samples ----
@@ -1101,6 +1127,8 @@ static void prepare_samples(Batch &B, Scene &c, Body &b) {
     b.sg = make_grid(bb, bb + 3);
     cell_sort(B, m, sci.p, b.sg, b.sgrid, b.sperm);
     b.sg.start = b.sgrid.p;
+    b.sg.mkey = b.sperm.p;  // members in caller sample order (the bucket order)
+    if (m) hash_partners(B, b.sg, b.spart, c.reach);
     b.sx.reserve(std::max<size_t>(m, 1));
     b.sy.reserve(std::max<size_t>(m, 1));
     b.sz.reserve(std::max<size_t>(m, 1));
@@ -1316,8 +1344,10 @@ static void cells_and_grids(Batch &B) {
         if (c.n == 0)
             for (int k = 0; k < 6; k++) bb[k] = 0;
         memcpy(c.box, bb, sizeof c.box);
-        if (c.n) hash_guard(B, c.checked_box, c.hash_box_valid, bb, bb + 3, c.reach);
+        const bool shared = c.n && hash_guard(B, c.checked_box, c.hash_box_valid,
+                                              c.hash_collides, bb, bb + 3, c.reach);
         c.grid = make_grid(bb, bb + 3);
+        if (shared) hash_partners(B, c.grid, c.hpart, c.reach);
         c.cell_base = (uint32_t)cells;
         B.hc_blk[s] = (uint32_t)cblocks;
         cells += c.ncells();
@@ -1386,12 +1416,7 @@ static void boundary_samples(Batch &B) {
                 // per-scene CSR over the static samples
                 b.soff.reserve(n + 1);
                 b.scnt.reserve(std::max<size_t>(n, 1));
-                int lo[3], hi[3];  // the query cells may lie outside the sample grid
-                for (int k = 0; k < 3; k++) {
-                    lo[k] = std::min(c->box[k], b.sg.org[k]);
-                    hi[k] = std::max(c->box[3 + k], b.sg.org[k] + b.sg.dim[k] - 1);
-                }
-                if (n) hash_guard(B, c->checked_box, c->hash_box_valid, lo, hi, c->reach);
+                // shared buckets among the sample cells were found at set-up (b.sg.part)
                 const double range = c->P.kernel_support_radius;
                 LAUNCH(k_neighbors<false>, n, B.s, n, c->x.p, c->y.p, c->z.p, range, c->reach,
                        c->dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, (const uint32_t *)nullptr, b.scnt.p,
@@ -2635,18 +2660,12 @@ int sph_neighbors_find(sph_ctx *c, double cell_size, double range, const double 
             *total = 0;
             return;
         }
-        int lo[3], hi[3];
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(b1[k], b2[k]);
-            hi[k] = std::max(b1[3 + k], b2[3 + k]);
-        }
-        int checked[6] = {1, 1, 1, 0, 0, 0};
-        bool valid = false;
-        hash_guard(B, checked, valid, lo, hi, reach);
         GridDev g = make_grid(b1, b1 + 3);
-        DBuf<uint32_t> start, perm;
+        DBuf<uint32_t> start, perm, part;
         cell_sort(B, m, sci.p, g, start, perm);
         g.start = start.p;
+        g.mkey = perm.p;  // members in caller sample order (the bucket order)
+        hash_partners(B, g, part, reach);
         sx.reserve(m);
         sy.reserve(m);
         sz.reserve(m);
@@ -3662,41 +3681,29 @@ int sph_quantity_at_positions(sph_ctx *c, int quantity, size_t nq, const float *
         int b1[6];
         HIPCK(hipMemcpyAsync(b1, bbf.p, sizeof b1, hipMemcpyDeviceToHost, B.s));
         B.sync();
-        // Duplicate neighbors in the reference need two cells of one stencil
-        // in one hash bucket with occupied members in range, so the guard
-        // covers the occupied cells (fluid and static samples) widened by the
-        // stencil reach; a query elsewhere only meets members out of range.
-        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-        bool occupied = n > 0;
-        if (n)
-            for (int k = 0; k < 3; k++) {
-                lo[k] = b1[k];
-                hi[k] = b1[3 + k];
-            }
+        // Duplicate neighbors in the reference come from two cells of one
+        // stencil in one hash bucket; the static-sample grids carry their
+        // bucket partners from set-up, the fluid's grid gets them here.
         std::vector<BodyDev> bv;
         std::vector<GridDev> sg;
         for (auto &b : S.bodies) {
             bv.push_back(b->view());
             sg.push_back(b->sg);
-            if (b->kind == 0 && b->ns)
-                for (int k = 0; k < 3; k++) {
-                    const int a = b->sg.org[k], z = b->sg.org[k] + b->sg.dim[k] - 1;
-                    lo[k] = occupied ? std::min(lo[k], a) : a;
-                    hi[k] = occupied ? std::max(hi[k], z) : z;
-                }
-            occupied = occupied || (b->kind == 0 && b->ns);
-        }
-        if (occupied) {
-            int checked[6] = {1, 1, 1, 0, 0, 0};
-            bool valid = false;
-            hash_guard(B, checked, valid, lo, hi, S.reach);
         }
         // the fluid's grid (one empty cell when there is no fluid)
         const int none[6] = {0, 0, 0, 0, 0, 0};
         GridDev g = make_grid(n ? b1 : none, n ? b1 + 3 : none + 3);
-        DBuf<uint32_t> start, perm;
+        DBuf<uint32_t> start, perm, part, mkey;
         cell_sort(B, n, fci.p, g, start, perm, S.ref.p);
         g.start = start.p;
+        if (n) {
+            hash_partners(B, g, part, S.reach);
+            if (g.part) {  // merge keys: the reference index of each sorted member
+                mkey.reserve(n);
+                LAUNCH(k_gather<uint32_t>, n, B.s, n, perm.p, S.ref.p, mkey.p);
+                g.mkey = mkey.p;
+            }
+        }
         DBuf<BodyDev> bodies;
         DBuf<GridDev> sgrid;
         if (!bv.empty()) {

@@ -144,6 +144,45 @@ __global__ void k_hash_check(GridDev g, int reach, unsigned int *flag) {
             }
 }
 
+// The bucket partners (GridDev::part) of every cell of g's box widened by
+// g.pm: the cells of the box at most `span` away per axis whose keys equal
+// the cell's own (kNoCell where there are fewer than two). flags[0] is set
+// when any cell has a partner, flags[1] when one has more than two.
+__global__ void k_hash_partners(GridDev g, int span, uint32_t *part, unsigned int *flags) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t e0 = (size_t)g.dim[0] + 2 * g.pm, e1 = (size_t)g.dim[1] + 2 * g.pm,
+                 e2 = (size_t)g.dim[2] + 2 * g.pm;
+    if (t >= e0 * e1 * e2) return;
+    // the cell's coordinates relative to the box origin
+    const int gz = (int)(t % e2) - g.pm;
+    const int gy = (int)((t / e2) % e1) - g.pm;
+    const int gx = (int)(t / (e2 * e1)) - g.pm;
+    const unsigned long long h0 = hash3(gx + g.org[0], gy + g.org[1], gz + g.org[2]);
+    uint32_t found[2] = {kNoCell, kNoCell};
+    int nf = 0;
+    for (int a = -span; a <= span; a++) {
+        const int bx = gx + a;
+        if (bx < 0 || bx >= g.dim[0]) continue;
+        for (int b = -span; b <= span; b++) {
+            const int by = gy + b;
+            if (by < 0 || by >= g.dim[1]) continue;
+            for (int c = -span; c <= span; c++) {
+                const int bz = gz + c;
+                if (bz < 0 || bz >= g.dim[2] || (a == 0 && b == 0 && c == 0)) continue;
+                if (hash3(bx + g.org[0], by + g.org[1], bz + g.org[2]) != h0) continue;
+                if (nf < 2)
+                    found[nf] = ((uint32_t)bx * (uint32_t)g.dim[1] + (uint32_t)by) * (uint32_t)g.dim[2] +
+                                (uint32_t)bz;
+                nf++;
+            }
+        }
+    }
+    part[2 * t] = found[0];
+    part[2 * t + 1] = found[1];
+    if (nf > 0) atomicOr(&flags[0], 1u);
+    if (nf > 2) atomicOr(&flags[1], 1u);
+}
+
 // ------------------------------------------------------------------------
 // counting sort by (cell key, reference index)
 // ------------------------------------------------------------------------
@@ -228,6 +267,17 @@ __device__ __forceinline__ uint32_t stencil_walk(double px, double py, double pz
     const int cx = sat_i32(floor(px / cs)), cy = sat_i32(floor(py / cs)),
               cz = sat_i32(floor(pz / cs));
     uint32_t cnt = 0;
+    if (g.part) {  // shared hash buckets: the reference's bucket visits
+        bucket_walk(g, cx, cy, cz, reach, [&](uint32_t j) {
+            double ddx = px - sx[j], ddy = py - sy[j], ddz = pz - sz[j];
+            double d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+            if (d2 < dthr) {
+                emit(cnt, j);
+                cnt++;
+            }
+        });
+        return cnt;
+    }
     int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
     if (zlo < 0) zlo = 0;
     if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;
@@ -515,6 +565,10 @@ namespace sph {
 template <class Fn>
 __device__ __forceinline__ void stencil_walk(const GridDev &g, int cx, int cy, int cz, int reach,
                                              Fn &&fn) {
+    if (g.part) {  // shared hash buckets: the reference's bucket visits
+        bucket_walk(g, cx, cy, cz, reach, fn);
+        return;
+    }
     int zlo = cz - reach - g.org[2], zhi = cz + reach - g.org[2];
     if (zlo < 0) zlo = 0;
     if (zhi > g.dim[2] - 1) zhi = g.dim[2] - 1;

@@ -11,11 +11,92 @@ namespace sph {
 
 // uniform grid over occupied cells: cell (cx,cy,cz) -> key
 // ((cx-org.x)*dim.y + (cy-org.y))*dim.z + (cz-org.z); start has ncell+1 entries
+//
+// Shared hash buckets: the reference keeps its samples in a HashMap keyed by
+// the 64-bit spatial hash of the cell (spatial_hashing.rs:27-103), so two
+// cells with equal keys share one bucket, and a stencil holding both visits
+// that bucket twice (its samples in range are emitted twice, at each cell's
+// place) -- also when one of the two cells is empty or lies outside the
+// occupied box. part is null when no cell within two stencil reaches of the
+// box shares a key with a cell of the box; otherwise it covers the box
+// widened by pm cells on every side, and part[2 e + t] (t = 0, 1) are the box
+// keys of the cells of the box that share the bucket of the cell with widened
+// key e (kNoCell where there are fewer), and mkey[j - mbase] is the sample
+// index (the bucket's fill order) of the member at sorted position j.
+constexpr uint32_t kNoCell = 0xffffffffu;
 struct GridDev {
     int org[3];
     int dim[3];
     const uint32_t *start;
+    const uint32_t *part;
+    const uint32_t *mkey;
+    uint32_t mbase;
+    int pm;
 };
+
+// The reference's stencil visit over shared buckets: the (2 reach + 1)^3
+// cells in (dx, dy, dz) order, and at each cell every member of its bucket,
+// i.e. of the cell and of the cells of the box sharing its key, merged in
+// sample-index order. A bucket partner outside the query's stencil holds no
+// sample within range (it lies more than a stencil reach away), so only
+// partners inside the stencil are merged. fn(j) receives sorted positions as
+// stored in start.
+template <class Fn>
+__device__ void bucket_walk(const GridDev &g, int cx, int cy, int cz, int reach, Fn &&fn) {
+    const int ex0 = g.dim[0] + 2 * g.pm, ex1 = g.dim[1] + 2 * g.pm, ex2 = g.dim[2] + 2 * g.pm;
+    for (int dx = -reach; dx <= reach; dx++) {
+        const int gx = cx + dx - g.org[0];
+        if (gx < -g.pm || gx >= g.dim[0] + g.pm) continue;
+        for (int dy = -reach; dy <= reach; dy++) {
+            const int gy = cy + dy - g.org[1];
+            if (gy < -g.pm || gy >= g.dim[1] + g.pm) continue;
+            for (int dz = -reach; dz <= reach; dz++) {
+                const int gz = cz + dz - g.org[2];
+                if (gz < -g.pm || gz >= g.dim[2] + g.pm) continue;
+                uint32_t lo[3], hi[3];
+                int nl = 0;
+                if (gx >= 0 && gx < g.dim[0] && gy >= 0 && gy < g.dim[1] && gz >= 0 &&
+                    gz < g.dim[2]) {  // the cell's own members
+                    const uint32_t a =
+                        ((uint32_t)gx * (uint32_t)g.dim[1] + (uint32_t)gy) * (uint32_t)g.dim[2] +
+                        (uint32_t)gz;
+                    lo[0] = g.start[a];
+                    hi[0] = g.start[a + 1];
+                    nl = 1;
+                }
+                const size_t e = ((size_t)(gx + g.pm) * (size_t)ex1 + (size_t)(gy + g.pm)) * (size_t)ex2 +
+                                 (size_t)(gz + g.pm);
+                (void)ex0;
+                for (int t = 0; t < 2; t++) {
+                    const uint32_t b = g.part[2 * e + t];
+                    if (b == kNoCell) continue;
+                    const int bz = (int)(b % (uint32_t)g.dim[2]) + g.org[2];
+                    const int by = (int)((b / (uint32_t)g.dim[2]) % (uint32_t)g.dim[1]) + g.org[1];
+                    const int bx = (int)(b / ((uint32_t)g.dim[2] * (uint32_t)g.dim[1])) + g.org[0];
+                    if (abs(bx - cx) > reach || abs(by - cy) > reach || abs(bz - cz) > reach) continue;
+                    lo[nl] = g.start[b];
+                    hi[nl] = g.start[b + 1];
+                    nl++;
+                }
+                for (;;) {  // merge the member lists (each in sample-index order)
+                    int best = -1;
+                    uint32_t bk = 0;
+                    for (int t = 0; t < nl; t++)
+                        if (lo[t] < hi[t]) {
+                            const uint32_t k = g.mkey[lo[t] - g.mbase];
+                            if (best < 0 || k < bk) {
+                                best = t;
+                                bk = k;
+                            }
+                        }
+                    if (best < 0) break;
+                    fn(lo[best]);
+                    lo[best]++;
+                }
+            }
+        }
+    }
+}
 
 struct SolveConst {
     double dt;
@@ -33,6 +114,7 @@ __global__ void k_cells_aos(size_t n, double cs, const double *p, int *ci, int *
 __global__ void k_fold_bbox(const int *partials, int nparts, int *bbox);
 __global__ void k_init_bbox(int *bbox);
 __global__ void k_hash_check(GridDev g, int reach, unsigned int *flag);
+__global__ void k_hash_partners(GridDev g, int span, uint32_t *part, unsigned int *flags);
 __global__ void k_cell_count(size_t n, const int *ci, GridDev g, uint32_t *count);
 __global__ void k_cell_scatter(size_t n, const int *ci, GridDev g, const uint32_t *start,
                                uint32_t *cursor, uint32_t *perm);

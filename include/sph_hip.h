@@ -125,7 +125,9 @@ int sph_ctx_destroy(sph_ctx *ctx);
  * positions, the indices of the m samples closer than `range` (strict),
  * in the reference's order. offsets has n+1 entries; indices has room for
  * `cap` entries; *total receives the entry count (SPH_ERR_CAPACITY if it
- * exceeds cap). */
+ * exceeds cap). Where two cells of a query's stencil share a spatial-hash
+ * bucket (equal 64-bit keys), the bucket's samples in range are listed at
+ * each of the two cells, as the reference's HashMap lists them. */
 int sph_neighbors_find(sph_ctx *ctx, double cell_size, double range, const double *pos,
                        size_t n, const double *samples, size_t m, uint64_t *offsets,
                        uint64_t *indices, size_t cap, size_t *total);
@@ -139,7 +141,10 @@ int sph_neighbors_find(sph_ctx *ctx, double cell_size, double range, const doubl
  * buffers live in a static device array of 25 MiB per GPU, held from the
  * library's load until the process ends. Limits: n, m <= 262144 and an
  * occupied cell box of the samples of at most 2^20 cells, else
- * SPH_ERR_UNSUPPORTED before any launch that writes the outputs. Every call on
+ * SPH_ERR_UNSUPPORTED before any launch that writes the outputs; inputs whose
+ * stencils hold cells that share a spatial-hash bucket are refused with
+ * SPH_ERR_UNSUPPORTED too (the array holds no table of shared buckets, which
+ * sph_neighbors_find builds). Every call on
  * one GPU shares that array: calls that overlap on one GPU (several threads,
  * several contexts) race. The call returns after its work on the context's
  * stream has finished; the caller makes the inputs visible to that stream
