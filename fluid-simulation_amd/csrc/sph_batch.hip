@@ -758,16 +758,29 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
             if (b.kind == 0) {
                 for (uint32_t e = gm(b.soff)[i]; e < gm(b.soff)[i + 1]; e++) {
                     uint32_t s = gm(b.sidx)[e];
-                    double rx = px - gm(b.sx)[s], ry = py - gm(b.sy)[s], rz = pz - gm(b.sz)[s];
+                    const double sxk = gm(b.sx)[s], syk = gm(b.sy)[s], szk = gm(b.sz)[s];
+                    double rx = px - sxk, ry = py - syk, rz = pz - szk;
                     double wx, wy, wz;
                     kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
-                    // static samples have zero velocity
+                    // samples move only with a rigid body (zero velocity otherwise)
+                    const double vbx = b.dynamic ? gm(b.svx)[s] : 0.0,
+                                 vby = b.dynamic ? gm(b.svy)[s] : 0.0,
+                                 vbz = b.dynamic ? gm(b.svz)[s] : 0.0;
                     double cf = v.nu_b * 2. * (3. + 2.) * gm(b.svol)[s] *
-                                dot3(ux - 0.0, uy - 0.0, uz - 0.0, rx, ry, rz) /
+                                dot3(ux - vbx, uy - vby, uz - vbz, rx, ry, rz) /
                                 (dot3(rx, ry, rz, rx, ry, rz) + v.eps2);
-                    acx += cf * wx;
-                    acy += cf * wy;
-                    acz += cf * wz;
+                    const double a0 = cf * wx, a1 = cf * wy, a2 = cf * wz;
+                    if (b.dynamic) {  // the reaction onto the body at the sample
+                        double fx = -(mi * a0), fy = -(mi * a1), fz = -(mi * a2);
+                        double tx, ty, tz;
+                        cross3(sxk - gm(b.com)[0], syk - gm(b.com)[1], szk - gm(b.com)[2], fx, fy,
+                               fz, tx, ty, tz);
+                        f[0] += fx; f[1] += fy; f[2] += fz;
+                        f[3] += tx; f[4] += ty; f[5] += tz;
+                    }
+                    acx += a0;
+                    acy += a1;
+                    acz += a2;
                 }
             } else if (gm(b.flag)[i]) {
                 double vp[4][3];
@@ -862,12 +875,21 @@ __device__ __forceinline__ void pressure_accel_impl(const BatchDev &B, int targe
             if (b.kind == 0) {
                 for (uint32_t e = gm(b.soff)[i]; e < gm(b.soff)[i + 1]; e++) {
                     uint32_t s = gm(b.sidx)[e];
+                    const double sxk = gm(b.sx)[s], syk = gm(b.sy)[s], szk = gm(b.sz)[s];
                     double wx, wy, wz;
-                    kernel_grad_rh(K, rh, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
+                    kernel_grad_rh(K, rh, px - sxk, py - syk, pz - szk, wx, wy, wz);
                     double cf = 2. * gamma * Vi * gm(b.svol)[s] * pi;
-                    acx -= (cf * wx) / mi;
-                    acy -= (cf * wy) / mi;
-                    acz -= (cf * wz) / mi;
+                    const double fx = cf * wx, fy = cf * wy, fz = cf * wz;
+                    if (mirror && b.dynamic) {  // the mirrored force onto the body at the sample
+                        double tx, ty, tz;
+                        cross3(sxk - gm(b.com)[0], syk - gm(b.com)[1], szk - gm(b.com)[2], fx, fy,
+                               fz, tx, ty, tz);
+                        f[0] += fx; f[1] += fy; f[2] += fz;
+                        f[3] += tx; f[4] += ty; f[5] += tz;
+                    }
+                    acx -= fx / mi;
+                    acy -= fy / mi;
+                    acz -= fz / mi;
                 }
             } else if (gm(b.flag)[i]) {
                 double sxk = gm(b.bx)[i], syk = gm(b.by)[i], szk = gm(b.bz)[i];
@@ -979,7 +1001,11 @@ __global__ PAIR_PASS_BOUNDS void kb_source_diag(BatchDev B, int vp) {
                 double wx, wy, wz;
                 kernel_grad_rh(K, rh, px - gm(b.sx)[s], py - gm(b.sy)[s], pz - gm(b.sz)[s], wx, wy, wz);
                 double Vb = gm(b.svol)[s];
-                src -= dt * Vb * dot3(ux - 0.0, uy - 0.0, uz - 0.0, wx, wy, wz);
+                // samples move only with a rigid body (zero velocity otherwise)
+                const double vbx = b.dynamic ? gm(b.svx)[s] : 0.0,
+                             vby = b.dynamic ? gm(b.svy)[s] : 0.0,
+                             vbz = b.dynamic ? gm(b.svz)[s] : 0.0;
+                src -= dt * Vb * dot3(ux - vbx, uy - vby, uz - vbz, wx, wy, wz);
                 sbx += Vb * wx;
                 sby += Vb * wy;
                 sbz += Vb * wz;

@@ -231,6 +231,23 @@ struct Rigid {
         for (int k = 0; k < 3; k++) F[k] = T[k] = 0;
         derive();
     }
+    // local_to_world (rigid_body_motion.rs:104-107): q * p + c, with nalgebra's
+    // quaternion-vector product t = 2 (q_v x p), q_v x t + t w + p
+    void local_to_world(const double p[3], double o[3]) const {
+        const double t0 = 2.0 * (q[2] * p[2] - q[3] * p[1]), t1 = 2.0 * (q[3] * p[0] - q[1] * p[2]),
+                     t2 = 2.0 * (q[1] * p[1] - q[2] * p[0]);
+        const double c0 = q[2] * t2 - q[3] * t1, c1 = q[3] * t0 - q[1] * t2, c2 = q[1] * t1 - q[2] * t0;
+        o[0] = t0 * q[0] + c0 + p[0] + com[0];
+        o[1] = t1 * q[0] + c1 + p[1] + com[1];
+        o[2] = t2 * q[0] + c2 + p[2] + com[2];
+    }
+    // velocity_at_point (:132-137): v + omega x (p - c)
+    void velocity_at(const double p[3], double o[3]) const {
+        const double r0 = p[0] - com[0], r1 = p[1] - com[1], r2 = p[2] - com[2];
+        o[0] = v[0] + (w[1] * r2 - w[2] * r1);
+        o[1] = v[1] + (w[2] * r0 - w[0] * r2);
+        o[2] = v[2] + (w[0] * r1 - w[1] * r0);
+    }
     void add_force(const double f[3], const double loc[3]) {  // add_force (:143-146)
         for (int k = 0; k < 3; k++) F[k] += f[k];
         double r[3] = {loc[0] - com[0], loc[1] - com[1], loc[2] - com[2]};
@@ -336,8 +353,11 @@ struct Body {
     Rigid rb;
     // static samples
     size_t ns = 0;
-    std::vector<double> spos;            // caller order, [ns][3]
+    std::vector<double> spos;            // caller order, [ns][3]; body frame on a rigid body
     DBuf<double> sx, sy, sz, svol;       // sorted by (cell, index)
+    DBuf<double> svx, svy, svz;          // sorted sample velocities (on a rigid body only)
+    DBuf<double> svol_c;                 // pseudo volumes in caller order (on a rigid body only)
+    bool svol_set = false;               // svol_c holds the volumes (computed or copied)
     DBuf<uint32_t> sperm;                // sorted -> caller sample index
     DBuf<uint32_t> sgrid;                // cell start
     DBuf<uint32_t> spart;                // bucket partners of the sample cells (GridDev::part)
@@ -365,6 +385,9 @@ struct Body {
         b.sdf = sdf.d;
         b.vm = vm.d;
         b.sx = sx.p; b.sy = sy.p; b.sz = sz.p; b.svol = svol.p;
+        if (dynamic && kind == 0) {
+            b.svx = svx.p; b.svy = svy.p; b.svz = svz.p;
+        }
         b.soff = soff.p; b.sidx = sidx.p;
         b.flag = flag.p;
         b.bx = bx.p; b.by = by.p; b.bz = bz.p;
@@ -1100,20 +1123,39 @@ static void reserve_scene(Scene &c, size_t n) {
     c.tref.reserve(k);
 }
 
-// StaticSampleBoundary samples: sort once by (cell, index), build their grid and
-// pseudo volumes (BoundaryType::initialize, static_sample_boundary.rs:354-390)
-static void prepare_samples(Batch &B, Scene &c, Body &b) {
-    size_t m = b.ns;
-    DBuf<double> hx, hy, hz;
-    std::vector<double> X(m), Y(m), Z(m);
+// StaticSampleBoundary samples at the current pose, sorted by (cell, index),
+// with their grid: positions and velocities (update_positions_and_velocities,
+// static_sample_boundary.rs:393-412, for samples on a rigid body: world
+// positions from the body-frame ones and the rigid velocity there; static
+// samples keep theirs and zero velocity) and, once their pseudo volumes are
+// known, the volumes in sorted order.
+static void pose_samples(Batch &B, Scene &c, Body &b) {
+    const size_t m = b.ns;
+    DBuf<double> hx, hy, hz, hv[3];
+    std::vector<double> X(m), Y(m), Z(m), U[3];
+    const bool moving = b.dynamic;
+    if (moving)
+        for (auto &u : U) u.resize(m);
     for (size_t k = 0; k < m; k++) {
-        X[k] = b.spos[3 * k];
-        Y[k] = b.spos[3 * k + 1];
-        Z[k] = b.spos[3 * k + 2];
+        double w[3] = {b.spos[3 * k], b.spos[3 * k + 1], b.spos[3 * k + 2]};
+        if (moving) {
+            double p[3], u[3];
+            b.rb.local_to_world(w, p);
+            b.rb.velocity_at(p, u);
+            for (int q = 0; q < 3; q++) {
+                w[q] = p[q];
+                U[q][k] = u[q];
+            }
+        }
+        X[k] = w[0];
+        Y[k] = w[1];
+        Z[k] = w[2];
     }
     hx.upload(X.data(), m, B.s);
     hy.upload(Y.data(), m, B.s);
     hz.upload(Z.data(), m, B.s);
+    if (moving)
+        for (int q = 0; q < 3; q++) hv[q].upload(U[q].data(), m, B.s);
     B.sync();
     DBuf<int> sci;
     sci.reserve(3 * std::max<size_t>(m, 1));
@@ -1129,13 +1171,33 @@ static void prepare_samples(Batch &B, Scene &c, Body &b) {
     b.sg.start = b.sgrid.p;
     b.sg.mkey = b.sperm.p;  // members in caller sample order (the bucket order)
     if (m) hash_partners(B, b.sg, b.spart, c.reach);
-    b.sx.reserve(std::max<size_t>(m, 1));
-    b.sy.reserve(std::max<size_t>(m, 1));
-    b.sz.reserve(std::max<size_t>(m, 1));
-    b.svol.reserve(std::max<size_t>(m, 1));
+    const size_t m1 = std::max<size_t>(m, 1);
+    b.sx.reserve(m1);
+    b.sy.reserve(m1);
+    b.sz.reserve(m1);
+    b.svol.reserve(m1);
     LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, hx.p, b.sx.p);
     LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, hy.p, b.sy.p);
     LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, hz.p, b.sz.p);
+    if (moving) {
+        DBuf<double> *dst[3] = {&b.svx, &b.svy, &b.svz};
+        for (int q = 0; q < 3; q++) {
+            dst[q]->reserve(m1);
+            LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, hv[q].p, dst[q]->p);
+        }
+        if (b.svol_set) LAUNCH(k_gather<double>, m, B.s, m, b.sperm.p, b.svol_c.p, b.svol.p);
+    }
+    B.sync();  // the host staging buffers go out of scope
+}
+
+// BoundaryType::initialize (static_sample_boundary.rs:354-390): the samples at
+// their pose and their pseudo volumes from the boundary-boundary neighbors;
+// volumes of samples on a rigid body are kept in caller order, since the
+// samples are sorted again after every step of the body
+static void prepare_samples(Batch &B, Scene &c, Body &b) {
+    const size_t m = b.ns;
+    pose_samples(B, c, b);
+    if (b.dynamic && b.svol_set) return;  // volumes copied from the system it was cloned from
     DBuf<uint32_t> off, idx, scnt;
     off.reserve(m + 1);
     scnt.reserve(std::max<size_t>(m, 1));
@@ -1148,6 +1210,11 @@ static void prepare_samples(Batch &B, Scene &c, Body &b) {
            c.reach, c.dthr, b.sg, b.sx.p, b.sy.p, b.sz.p, off.p, (uint32_t *)nullptr, idx.p);
     LAUNCH(k_static_volume, m, B.s, m, c.K, c.P.boundary_rest_volume_weighting, b.sx.p, b.sy.p,
            b.sz.p, off.p, idx.p, b.svol.p);
+    if (b.dynamic) {
+        b.svol_c.reserve(std::max<size_t>(m, 1));
+        LAUNCH(k_scatter<double>, m, B.s, m, b.sperm.p, b.svol.p, b.svol_c.p);
+        b.svol_set = true;
+    }
     B.sync();
 }
 
@@ -2355,7 +2422,10 @@ static void step_batch(Batch &B) {
         }
         contact_step(B, *c, dt);  // B::step_forward_in_time: contact, then the bodies
         for (auto &b : c->bodies)
-            if (b->dynamic) b->rb.step(dt);
+            if (b->dynamic) {
+                b->rb.step(dt);
+                if (b->kind == 0) pose_samples(B, *c, *b);  // the samples follow the body
+            }
         c->steps++;
         if (c->P.cfl)
             c->time += dt;
@@ -2576,6 +2646,10 @@ int sph_ctx_clone(const sph_ctx *src, sph_ctx **out) {
             if (bp->kind == 1) {
                 b->sdf.copy_from(bp->sdf, cs);
                 b->vm.copy_from(bp->vm, cs);
+            }
+            if (bp->kind == 0 && bp->dynamic && bp->svol_set) {  // DynClone keeps the volumes
+                b->svol_c.copy_from(bp->svol_c, bp->ns, cs);
+                b->svol_set = true;
             }
             d.bodies.push_back(std::move(b));
         }
@@ -2824,13 +2898,17 @@ int sph_boundary_add_samples(sph_ctx *c, const double *pos, size_t m, const sph_
         check_solo(c);
         if (c->scene.bodies.size() >= (size_t)kMaxBodiesPerScene)
             throw Error(SPH_ERR_UNSUPPORTED, "too many boundaries");
-        if (rigid)
-            throw Error(SPH_ERR_UNSUPPORTED,
-                        "dynamic StaticSampleBoundary bodies are not built; use a volume map");
+        if (rigid && c->scene.P.solver == SPH_SOLVER_IISPHWOST)  // SUPPORTS_DYNAMIC_BOUNDARIES (setup/mod.rs:242-244)
+            throw Error(SPH_ERR_ARG, "IISPHwOST does not support dynamic boundaries");
+        if (rigid && !(rigid->mass != 0.0)) throw Error(SPH_ERR_ARG, "rigid body mass cannot be zero");
         std::unique_ptr<Body> b(new Body());
         b->kind = 0;
         b->ns = m;
         b->spos.assign(pos, pos + 3 * m);
+        if (rigid) {  // positions in the body frame (static_sample_boundary.rs:79-130)
+            b->dynamic = true;
+            b->rb.init(*rigid);
+        }
         c->scene.bodies.push_back(std::move(b));
         c->scene.boundaries_initialized = false;
         idx = (int)c->scene.bodies.size() - 1;
@@ -3069,6 +3147,9 @@ int sph_system_restore(sph_ctx *c, uint64_t time_steps_propagated, double curren
         if (!c || (nbodies && (!present || !bodies))) throw Error(SPH_ERR_ARG, "null argument");
         check_solo(c);
         Scene &S = c->scene;
+        // the system as System::new leaves it (sample pseudo volumes at the
+        // samples' initial pose), then the restored bodies
+        init_boundaries(c->own, S);
         // BoundaryHandling::restore_from_checkpoint skips every boundary when
         // the entry count differs (volume_map_boundary.rs:1504-1518), and a
         // boundary whose entry does not match its kind (:2099-2120)
@@ -3093,10 +3174,11 @@ int sph_system_restore(sph_ctx *c, uint64_t time_steps_propagated, double curren
                 memcpy(r.F, s + 13, 3 * sizeof(double));
                 memcpy(r.T, s + 16, 3 * sizeof(double));
                 r.derive();  // RigidBodyMotion::restore_from_checkpoint (rigid_body_motion.rs:214-222)
+                // samples on the body follow its restored pose (static_sample_boundary.rs:560-580)
+                if (body.kind == 0) pose_samples(c->own, S, body);
             }
         }
         // continue_from_checkpoint (sph/mod.rs:362-378): counters, then update()
-        init_boundaries(c->own, S);
         S.dt = S.P.cfl ? 0.0 : S.P.time_increment;
         S.time = current_time;
         S.steps = time_steps_propagated;

@@ -44,8 +44,10 @@ struct BodyDev {
     double ang_vel[3];
     LatticeDev sdf;
     LatticeDev vm;
-    // static samples, sorted by cell; per-fluid-particle CSR into them
+    // static samples, sorted by cell; per-fluid-particle CSR into them;
+    // sample velocities only for samples on a rigid body (null otherwise)
     const double *sx, *sy, *sz, *svol;
+    const double *svx, *svy, *svz;
     const uint32_t *soff, *sidx;
     // volume map: per fluid particle, the normal-mode sample (if flag)
     const uint8_t *flag;

@@ -233,6 +233,13 @@ __global__ void k_gather(size_t n, const uint32_t *perm, const T *src, T *dst) {
     if (i < n) dst[i] = src[perm[i]];
 }
 template __global__ void k_gather<double>(size_t, const uint32_t *, const double *, double *);
+
+template <class T>
+__global__ void k_scatter(size_t n, const uint32_t *perm, const T *src, T *dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[perm[i]] = src[i];
+}
+template __global__ void k_scatter<double>(size_t, const uint32_t *, const double *, double *);
 template __global__ void k_gather<uint32_t>(size_t, const uint32_t *, const uint32_t *,
                                              uint32_t *);
 template __global__ void k_gather<int>(size_t, const uint32_t *, const int *, int *);
@@ -627,7 +634,7 @@ __global__ void k_quantity(size_t nq, const double *qx, const double *qy, const 
     if (Q == kQSpeed || Q == kQVolume) {
         for (int bi = 0; bi < nb; bi++) {
             const BodyDev &b = bodies[bi];
-            if (b.kind == 0) {  // static samples (velocity zero)
+            if (b.kind == 0) {  // static samples (velocity zero unless on a rigid body)
                 stencil_walk(sgrid[bi], cx, cy, cz, C.reach, [&](uint32_t j) {
                     const double rx = px - b.sx[j], ry = py - b.sy[j], rz = pz - b.sz[j];
                     const double d2 = (rx * rx + ry * ry) + rz * rz;
@@ -635,10 +642,10 @@ __global__ void k_quantity(size_t nq, const double *qx, const double *qy, const 
                     const double w = K.wfac * kernel_w(sqrt(d2) / K.h);
                     if (Q == kQVolume) {
                         acc += b.svol[j] * w;
-                    } else {
-                        ax += (0.0 * b.svol[j]) * w;
-                        ay += (0.0 * b.svol[j]) * w;
-                        az += (0.0 * b.svol[j]) * w;
+                    } else {  // samples move only with a rigid body
+                        ax += ((b.dynamic ? b.svx[j] : 0.0) * b.svol[j]) * w;
+                        ay += ((b.dynamic ? b.svy[j] : 0.0) * b.svol[j]) * w;
+                        az += ((b.dynamic ? b.svz[j] : 0.0) * b.svol[j]) * w;
                     }
                 });
             } else {

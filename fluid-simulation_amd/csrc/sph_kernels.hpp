@@ -122,6 +122,8 @@ __global__ void k_cell_order(size_t ncell, const uint32_t *start, uint32_t *perm
                              const uint32_t *ref);
 template <class T>
 __global__ void k_gather(size_t n, const uint32_t *perm, const T *src, T *dst);
+template <class T>
+__global__ void k_scatter(size_t n, const uint32_t *perm, const T *src, T *dst);
 __global__ void k_gather_aos(size_t n, const uint32_t *perm, const double *p, double *x,
                              double *y, double *z);
 template <bool FILL>

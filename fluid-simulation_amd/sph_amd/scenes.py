@@ -613,6 +613,56 @@ def small_rigid_scene(side=(14, 14, 8), seed=2, dx=0.02, solver="IISPH") -> Scen
                   VolumeMap(s_l, v_l, rb, "cube", aabb_of(lo_l, hi_l))])
 
 
+def box_surface_samples(half, spacing):
+    """Samples on the surface of the cube [-half, half]^3 (body frame): on each
+    face a square grid of cell centers no wider than ``spacing`` apart."""
+    n = max(1, int(math.ceil(2.0 * half / spacing - 1e-9)))
+    t = -half + (np.arange(n) + 0.5) * (2.0 * half / n)
+    u, w = np.meshgrid(t, t, indexing="ij")
+    u, w = u.ravel(), w.ravel()
+    faces = []
+    for d in range(3):
+        a, b = [k for k in range(3) if k != d]
+        for sgn in (-1.0, 1.0):
+            f = np.zeros((u.size, 3))
+            f[:, d] = sgn * half
+            f[:, a] = u
+            f[:, b] = w
+            faces.append(f)
+    return np.ascontiguousarray(np.concatenate(faces), dtype=np.float64)
+
+
+def sample_body_scene(side=(12, 12, 6), seed=6, dx=0.02, solver="IISPH") -> Scene:
+    """Two-way coupling with a StaticSampleBoundary body
+    (static_sample_boundary.rs:79-130): a fluid layer in a box of two layers
+    of static samples, and a cube (edge 0.1 m, density 600) represented by
+    samples on its surface at 0.7 dx (the reference's sampling spacing),
+    tilted and spinning, released just above the layer."""
+    rng = np.random.default_rng(seed)
+    pos = lattice_block([0.0, 0.0, 0.0], side, dx, rng, 1e-4)
+    x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
+    h = 2 * dx
+    if solver == "SESPH":
+        p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, time_increment=2.5e-4,
+                   solver="SESPH", integrator="EulerCromer", stiffness=1000.0)
+    else:
+        p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
+                   max_time_increment=1e-3, cfl_number=0.4, solver=solver,
+                   integrator="EulerCromer")
+    hi = np.array([side[0] * dx, side[1] * dx, 0.4])
+    walls = box_shell_samples([0.0, 0.0, 0.0], hi, dx, 2)
+    size = 0.1
+    mass = 600.0 * size ** 3
+    rb = Rigid(mass=mass, inertia_body=np.eye(3) * (mass * size * size / 6.0),
+               com=np.array([0.5 * side[0] * dx + 0.011, 0.5 * side[1] * dx - 0.006,
+                             side[2] * dx + 0.5 * size + dx]),
+               q=np.array([math.cos(0.1), math.sin(0.1) * 0.6, 0.0, math.sin(0.1) * 0.8]),
+               lin_vel=np.array([0.0, 0.0, -0.3]), ang_vel=np.array([0.3, -0.2, 0.5]))
+    body = box_surface_samples(0.5 * size, 0.7 * dx)
+    return Scene(f"sample_body_{solver}", p, x, v, m, fid,
+                 [StaticSamples(walls), StaticSamples(body, rb)])
+
+
 def _rot_x(angle: float) -> np.ndarray:
     return np.array([math.cos(0.5 * angle), math.sin(0.5 * angle), 0.0, 0.0])
 

@@ -155,7 +155,9 @@ int sph_neighbors_find_noalloc(sph_ctx *ctx, double cell_size, double range, con
 
 /* ---- BoundaryHandling::add_static_boundary / add_dynamic_boundary ----------
  * (boundary_handling/mod.rs:44-60). StaticSampleBoundary: sample positions
- * (world frame when rigid == NULL, body frame otherwise).
+ * (world frame when rigid == NULL, body frame otherwise: the samples then
+ * follow the body's rigid motion, static_sample_boundary.rs:79-130; their
+ * pseudo volumes are those of the initial pose).
  * VolumeMapBoundary: signed-distance and volume-map lattices (body frame for
  * dynamic bodies). Return the boundary index (>= 0) or a negative status. */
 int sph_boundary_add_samples(sph_ctx *ctx, const double *pos, size_t m,

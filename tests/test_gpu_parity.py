@@ -350,14 +350,51 @@ def test_per_trait_calls_bit_identical(native, refcpu):
         assert np.array_equal(res["p"], st["p"]) and np.array_equal(res["a"], st["a"])
 
 
-def test_dynamic_static_sample_bodies_are_refused(native):
-    """Moving StaticSampleBoundary bodies (static_sample_boundary.rs:79-130) are
-    not built on the device: adding one fails with SPH_ERR_UNSUPPORTED and a
-    message naming the alternative, instead of simulating it as static."""
+@pytest.mark.parametrize("solver", ["IISPH", "SESPH"])
+def test_moving_static_sample_body(native, refcpu, solver):
+    """A StaticSampleBoundary body (static_sample_boundary.rs:79-130): samples
+    on a cube's surface follow its rigid motion (update_positions_and_velocities
+    after every body step), carry its velocity into the viscosity and source
+    terms, keep the pseudo volumes of their initial pose, and feed the
+    fluid's reactions back into the body. The body's force total is a sum the
+    oracle and the GPU group differently, so they agree to rounding: every step
+    compares the iteration counts, the step size, the body force and the
+    rigid state, every 40 steps the fluid. A clone carries on bit for bit."""
+    sc = scenes.sample_body_scene(solver=solver)
+    print(f"moving sample body: {solver}, {sc.n} particles", flush=True)
+    o, g = refcpu.OracleSystem(sc), native.System(sc)
+    worst = {"force": 0.0, "state": 0.0, "fluid": 0.0}
+    for k in range(1, 121):
+        o.step(1)
+        g.step_forward_in_time(1)
+        mo, mg = o.measure(), g.take_measurement()
+        assert mg["solver_iterations"] == mo["solver_iterations"], (k, mg, mo)
+        assert mg["time_step_size"] == pytest.approx(mo["time_step_size"], rel=1e-9), k
+        worst["force"] = max(worst["force"], rel_err(g.body_forces(1), o.body_forces(1)))
+        so, sg = o.body_state(1), g.body_state(1)
+        for f in ("com", "q", "lin_vel", "ang_mom"):
+            worst["state"] = max(worst["state"], rel_err(sg[f], so[f]))
+        if k % 40 == 0:
+            a, b = o.state(), g.state()
+            for f in ("x", "v", "a", "V", "p"):
+                worst["fluid"] = max(worst["fluid"], rel_err(b[f], a[f]))
+    print("  largest relative differences", worst, flush=True)
+    assert all(v < 1e-5 for v in worst.values()), worst
+    c = g.clone()
+    g.step_forward_in_time(5)
+    c.step_forward_in_time(5)
+    sg, scl = g.state(), c.state()
+    for f in ("x", "v", "a", "V", "p"):
+        assert np.array_equal(scl[f], sg[f]), f
+    assert np.array_equal(c.body_state(1)["com"], g.body_state(1)["com"])
+
+
+def test_moving_static_sample_body_refused_by_iisphwost(native):
+    """IISPHwOST does not support dynamic boundaries (SUPPORTS_DYNAMIC_BOUNDARIES,
+    setup/mod.rs:242-244)."""
     from sph_amd import _native as N
-    sc = scenes.small_iisph_box(side=6, jitter=2e-3)
-    rb = scenes.Rigid(mass=1.0, inertia_body=np.eye(3), com=np.array([0.3, 0.3, 0.3]))
-    sc.boundaries.append(scenes.StaticSamples(positions=np.zeros((4, 3)), rigid=rb))
-    with pytest.raises(N.SphError, match="dynamic StaticSampleBoundary") as e:
+    sc = scenes.sample_body_scene(solver="IISPH")
+    sc.params.solver, sc.params.integrator = "IISPHwOST", "TakePredicted"
+    with pytest.raises(N.SphError, match="IISPHwOST does not support dynamic boundaries"):
         native.System(sc)
-    assert e.value.status == -5
+

@@ -77,8 +77,14 @@ def test_resume_continues_the_run(native, tmp_path, case, fmt):
         assert float(np.max(np.abs(sb[k] - so[k]))) / den <= 1e-5, k
 
 
-def test_resume_restores_rigid_state(native):
-    sc = scenes.small_rigid_scene()
+@pytest.mark.parametrize("case", ["volume_map_body", "sample_body"])
+def test_resume_restores_rigid_state(native, case):
+    """A moving body resumes its rigid state; samples on a moving
+    StaticSampleBoundary body follow the restored pose
+    (static_sample_boundary.rs:560-580) with the volumes of their initial
+    pose."""
+    sc = scenes.small_rigid_scene() if case == "volume_map_body" else scenes.sample_body_scene()
+    print(f"resume rigid state: {case}", flush=True)
     a = native.System(sc)
     a.step_forward_in_time(4)
     cp = a.get_checkpoint()

@@ -525,3 +525,33 @@ def test_fixed_step_time_is_steps_times_dt():
     o.step(10)
     t = o.measure()["time"]
     assert t == 10 * 1e-3 and t == 0.01
+
+
+@pytest.mark.parametrize("com, lin_vel, ang_vel, fluid_at, pos_want, vel_want", [
+    # update_positions_and_velocities_transforms_body_frame_to_world
+    # (static_sample_boundary.rs:649-658): identity orientation, p + c
+    ((5.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (6.01, 0.0, 0.0),
+     (6.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
+    # update_positions_and_velocities_computes_rigid_body_velocity (:660-679):
+    # v + omega x (p - c) = (1, 0, 0) + (0, 0, 2) x (1, 0, 0) = (1, 2, 0)
+    ((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 0.0, 2.0), (1.01, 0.0, 0.0),
+     (1.0, 0.0, 0.0), (1.0, 2.0, 0.0)),
+])
+def test_moving_sample_pose_and_velocity(com, lin_vel, ang_vel, fluid_at, pos_want, vel_want):
+    """A StaticSampleBoundary sample on a rigid body, body-frame (1, 0, 0),
+    as a fluid particle within h of it sees it after System::new: the
+    reference's own known answers for its world position and velocity."""
+    dx = 0.02
+    p = scenes.Params(kernel_support_radius=2 * dx, rest_density_grid_spacing=dx,
+                      solver="SESPH", integrator="EulerCromer")
+    rb = scenes.Rigid(mass=1.0, inertia_body=np.eye(3), com=np.array(com),
+                      lin_vel=np.array(lin_vel), ang_vel=np.array(ang_vel))
+    x = np.array([fluid_at])
+    sc = scenes.Scene("moving_sample", p, x, np.zeros((1, 3)), np.array([dx ** 3 * 1000.0]),
+                      np.zeros(1, dtype=np.uint32),
+                      [scenes.StaticSamples(np.array([[1.0, 0.0, 0.0]]), rb)])
+    o = refcpu.OracleSystem(sc)
+    s = o.boundary_samples(0, 0)
+    assert s.shape == (1, 7)
+    assert tuple(s[0, :3]) == pos_want
+    assert tuple(s[0, 3:6]) == vel_want
