@@ -86,6 +86,15 @@ def test_world_scenes_leaving_the_solve_at_different_chunks(native):
         assert its[0] > its[1] + 2, its
 
 
+def test_world_with_moving_sample_body(native):
+    """A StaticSampleBoundary body that moves (its samples re-posed and
+    re-sorted every step) beside a volume-map scene: both bit-identical to
+    their solo runs."""
+    scs = [scenes.sample_body_scene(), scenes.small_iisph_box(side=9, seed=3, jitter=2e-3)]
+    print("world with a moving sample body", flush=True)
+    run_world_vs_solo(native, scs, 6)
+
+
 def test_world_iisphwost(native):
     scs = [scenes.small_iisph_box(side=9, seed=2, solver="IISPHwOST", jitter=2e-3),
            scenes.small_iisph_box(side=12, seed=3, solver="IISPHwOST", jitter=2e-3)]
