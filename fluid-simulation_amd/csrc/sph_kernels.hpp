@@ -43,7 +43,7 @@ struct GridDev {
 // stored in start.
 template <class Fn>
 __device__ void bucket_walk(const GridDev &g, int cx, int cy, int cz, int reach, Fn &&fn) {
-    const int ex0 = g.dim[0] + 2 * g.pm, ex1 = g.dim[1] + 2 * g.pm, ex2 = g.dim[2] + 2 * g.pm;
+    const int ex1 = g.dim[1] + 2 * g.pm, ex2 = g.dim[2] + 2 * g.pm;  // widened box extents
     for (int dx = -reach; dx <= reach; dx++) {
         const int gx = cx + dx - g.org[0];
         if (gx < -g.pm || gx >= g.dim[0] + g.pm) continue;
@@ -66,7 +66,6 @@ __device__ void bucket_walk(const GridDev &g, int cx, int cy, int cz, int reach,
                 }
                 const size_t e = ((size_t)(gx + g.pm) * (size_t)ex1 + (size_t)(gy + g.pm)) * (size_t)ex2 +
                                  (size_t)(gz + g.pm);
-                (void)ex0;
                 for (int t = 0; t < 2; t++) {
                     const uint32_t b = g.part[2 * e + t];
                     if (b == kNoCell) continue;
