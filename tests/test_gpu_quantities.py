@@ -56,6 +56,21 @@ def test_quantities_bit_identical(native, case):
                 assert np.count_nonzero(go) > 100, name
 
 
+def test_quantities_with_a_moving_sample_body(native):
+    """Samples on a moving StaticSampleBoundary body carry the body's
+    velocity into the Speed sums: queries right after System::new, where the
+    oracle and the device hold the same bits (body forces grouped differently
+    part the two at rounding level once the body steps)."""
+    sc = scenes.sample_body_scene()
+    o, g = refcpu.OracleSystem(sc), native.System(sc)
+    q = queries(np.concatenate([g.state()["x"], sc.boundaries[1].rigid.com[None, :]]), 4)
+    print(f"moving sample body: {len(q)} queries", flush=True)
+    for k, name in enumerate(NAMES):
+        go = g.get_quantity_at_positions(name, q)
+        oo = o.quantity_at_positions(k, q)
+        assert same_bits(go, oo), (name, int(np.sum(go != oo)))
+
+
 def test_queries_leave_state_and_trajectory_unchanged(native):
     sc = scenes.small_rigid_scene()
     a, b = native.System(sc), native.System(sc)
