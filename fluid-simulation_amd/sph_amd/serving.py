@@ -45,7 +45,7 @@ from . import scenes as S
 # without it). The world's launch buffers only grow, and the library frees them once its
 # residents need less than a quarter of them; servers report the bytes the
 # library really holds (device_bytes) beside these estimates.
-BYTES_PER_PARTICLE = 49 * 8 + 4 * 4 + 20 + 48 * 4 + 64   # + 64: the shared buffers' 1.25x growth
+BYTES_PER_PARTICLE = 47 * 8 + 4 * 4 + 20 + 47 * 4 + 64   # + 64: the shared buffers' 1.25x growth
 BYTES_PER_PARTICLE_BODY = 1 + 7 * 8
 BYTES_PER_SCENE = 1 << 20          # stream, pinned staging, small tables
 
