@@ -212,18 +212,37 @@ struct QP {  // {x, y, z, V} and the pressure
 // ------------------------------------------------------------------------
 // integration (integration_schemes/euler_cromer.rs:12-25)
 // ------------------------------------------------------------------------
+// explicit_euler.rs:12-25 (the position moves with the old velocity) and
+// verlet.rs:13-34 (x' = x + (x - x_prev) + dt^2 a, v = (x' - x) / dt, the slot
+// takes x); TakePredicted swaps slots on the host
 __global__ void kb_integrate_ec(BatchDev B) {
     PARTICLE_THREAD(B);
-    if (!live || v.integrator != SPH_INTEGRATOR_EULER_CROMER) return;
+    if (!live) return;
     const double dt = v.S.dt;
-    double nvx = gm(v.vx)[i] + dt * gm(v.ax)[i], nvy = gm(v.vy)[i] + dt * gm(v.ay)[i],
-           nvz = gm(v.vz)[i] + dt * gm(v.az)[i];
-    gm(v.vx)[i] = nvx;
-    gm(v.vy)[i] = nvy;
-    gm(v.vz)[i] = nvz;
-    gm(v.x)[i] = gm(v.x)[i] + dt * nvx;
-    gm(v.y)[i] = gm(v.y)[i] + dt * nvy;
-    gm(v.z)[i] = gm(v.z)[i] + dt * nvz;
+    double *X[3] = {v.x, v.y, v.z}, *U[3] = {v.vx, v.vy, v.vz};
+    const double *A[3] = {v.ax, v.ay, v.az};
+    if (v.integrator == SPH_INTEGRATOR_EULER_CROMER) {
+        for (int k = 0; k < 3; k++) {
+            const double nv = gm(U[k])[i] + dt * gm(A[k])[i];
+            gm(U[k])[i] = nv;
+            gm(X[k])[i] = gm(X[k])[i] + dt * nv;
+        }
+    } else if (v.integrator == SPH_INTEGRATOR_EXPLICIT_EULER) {
+        for (int k = 0; k < 3; k++) {
+            const double x0 = gm(X[k])[i], v0 = gm(U[k])[i];
+            gm(X[k])[i] = x0 + dt * v0;
+            gm(U[k])[i] = v0 + dt * gm(A[k])[i];
+        }
+    } else if (v.integrator == SPH_INTEGRATOR_VERLET) {
+        double *P[3] = {v.ix, v.iy, v.iz};
+        for (int k = 0; k < 3; k++) {
+            const double x0 = gm(X[k])[i];
+            const double nx = x0 + (x0 - gm(P[k])[i]) + (dt * dt) * gm(A[k])[i];
+            gm(U[k])[i] = (nx - x0) / dt;
+            gm(P[k])[i] = x0;
+            gm(X[k])[i] = nx;
+        }
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -385,6 +404,11 @@ __global__ void kb_permute(BatchDev B) {
     gm(v.tm)[i] = gm(v.m)[src];
     gm(v.tfid)[i] = gm(v.fid)[src];
     gm(v.tref)[i] = gm(v.ref)[src];
+    if (v.integrator == SPH_INTEGRATOR_VERLET) {  // the slot persists across steps
+        gm(v.sxp)[i] = gm(v.ix)[src];
+        gm(v.syp)[i] = gm(v.iy)[src];
+        gm(v.szp)[i] = gm(v.iz)[src];
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -523,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void kb_sweep_ell(BatchDev B, uint32_t cap,
         // a neighbor (d2 < dthr) in the order found: stored unless it is the
         // particle itself, and its volume term summed
         auto take = [&](uint32_t j, double d2) {
-            if (j != i) {  // the self pair is summed, not stored
+            if (j != i || v.keep_self) {  // the self pair is summed, not stored
                 if (cnt < kSweepRows) rows[w][cnt][l] = j;
                 else if (cnt < cap) gm(B.ell_j)[ELL_AT(e0, cnt)] = j;
                 cnt++;
@@ -830,6 +854,54 @@ __global__ PAIR_PASS_BOUNDS void kb_nonpressure(BatchDev B) {
 __global__ void kb_sesph_pressure(BatchDev B) {
     PARTICLE_THREAD(B);
     if (live) gm(v.p)[i] = v.stiffness * fmax(v.V0 / gm(v.V)[i] - 1., 0.);
+}
+
+// SESPHwSplitting (sesph_with_splitting.rs:31-81, 97-153): the density
+// predicted with v* over the fluid neighbors (the particle itself included:
+// the pair cache keeps it for this solver) and the normal boundary samples,
+// which mirror the particle's mass and enter with v*_i alone; then the state
+// equation on V = m / rho*
+__global__ void kb_sesph_split_density(BatchDev B) {
+    PARTICLE_THREAD(B);
+    if (!live) return;
+    const KernelConst K = v.K;
+    const double rh = refined_rcp(K.h);
+    const double dt = v.S.dt;
+    const rec4 qi = ld4(v.q4, i);
+    const double px = qi.x, py = qi.y, pz = qi.z, mi = gm(v.m)[i];
+    const double ux = gm(v.svx)[i], uy = gm(v.svy)[i], uz = gm(v.svz)[i];
+    double accu = 0.;
+    const uint32_t c = gm(B.cnt)[gi];
+    const size_t e0 = ELL_BASE(B, v, i);
+    for (uint32_t k = 0; k < c; k++) {
+        const uint32_t j = gm(B.ell_j)[ELL_AT(e0, k)];
+        const rec4 q = ldj4(v.q4, j), u = ldj4(v.sv4, j);  // {x, V}, {v*, m}
+        const double rx = px - q.x, ry = py - q.y, rz = pz - q.z;
+        double wx, wy, wz;
+        kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
+        accu += u.w * kernel_fn(K, rx, ry, rz) + dt * dot3(ux - u.x, uy - u.y, uz - u.z, wx, wy, wz);
+    }
+    for (uint32_t bi = 0; bi < v.nb; bi++) {
+        const BodyDev &b = B.bodies[v.body_base + bi];
+        if (b.kind == 0) {
+            for (uint32_t e = gm(b.soff)[i]; e < gm(b.soff)[i + 1]; e++) {
+                const uint32_t s = gm(b.sidx)[e];
+                const double rx = px - gm(b.sx)[s], ry = py - gm(b.sy)[s], rz = pz - gm(b.sz)[s];
+                double wx, wy, wz;
+                kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
+                accu += gm(b.svol)[s] * mi / v.V0 * kernel_fn(K, rx, ry, rz) +
+                        dt * dot3(ux, uy, uz, wx, wy, wz);
+            }
+        } else if (gm(b.flag)[i]) {
+            const double rx = px - gm(b.bx)[i], ry = py - gm(b.by)[i], rz = pz - gm(b.bz)[i];
+            double wx, wy, wz;
+            kernel_grad_rh(K, rh, rx, ry, rz, wx, wy, wz);
+            accu += gm(b.bvol)[i] * mi / v.V0 * kernel_fn(K, rx, ry, rz) +
+                    dt * dot3(ux, uy, uz, wx, wy, wz);
+        }
+    }
+    const double vol = mi / accu;
+    gm(v.p)[i] = v.stiffness * fmax(v.V0 / vol - 1., 0.);
 }
 
 // ------------------------------------------------------------------------

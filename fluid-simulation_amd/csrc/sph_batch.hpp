@@ -43,6 +43,8 @@ struct SceneView {
     double *sv4;  // {predicted velocity, m} per particle: gathered by the source pass
     double *v3;   // {velocity} per particle (24 bytes): gathered by the viscosity pass
     double *s_f, *a_ff, *svx, *svy, *svz, *sxp, *syp, *szp;
+    double *ix, *iy, *iz;  // the integrator's position slot (Verlet: x one step back)
+    int keep_self;         // the pair cache lists the particle itself (SESPHwSplitting's density)
     double *tx, *ty, *tz, *tvx, *tvy, *tvz, *tm;  // permutation targets
     uint32_t *fid, *ref, *tfid, *tref;
 };
@@ -113,6 +115,7 @@ __global__ void kb_pairs_volume(BatchDev B);
 template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_nonpressure(BatchDev B);
 __global__ void kb_sesph_pressure(BatchDev B);
+__global__ void kb_sesph_split_density(BatchDev B);
 template <bool CACHED, int UNROLL, int WAVES, int PIPE>
 __global__ void kb_pressure_accel(BatchDev B, int target_scratch, int overwrite, int mirror);
 __global__ void kb_pred_vel(BatchDev B, int to_pred);

@@ -519,6 +519,8 @@ struct Scene {
         v.q4 = q4.p; v.pa3 = pa3.p; v.sv4 = sv4.p; v.v3 = v3.p;
         v.svx = svx.p; v.svy = svy.p; v.svz = svz.p;
         v.sxp = sxp.p; v.syp = syp.p; v.szp = szp.p;
+        v.ix = ix.p; v.iy = iy.p; v.iz = iz.p;
+        v.keep_self = P.solver == SPH_SOLVER_SESPH_SPLITTING ? 1 : 0;
         v.tx = tx.p; v.ty = ty.p; v.tz = tz.p;
         v.tvx = tvx.p; v.tvy = tvy.p; v.tvz = tvz.p; v.tm = tm.p;
         v.fid = fid.p; v.ref = ref.p; v.tfid = tfid.p; v.tref = tref.p;
@@ -1372,8 +1374,19 @@ static void compact_below(Batch &B, Scene &c) {
     LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.vz.p, c.tvz.p);
     LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.m.p, c.tm.p);
     LAUNCH(k_gather<uint32_t>, nn, B.s, nn, sel.p, c.fid.p, c.tfid.p);
+    const bool slot = c.P.integrator == SPH_INTEGRATOR_VERLET;  // the slot persists
+    if (slot) {
+        LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.ix.p, c.sxp.p);
+        LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.iy.p, c.syp.p);
+        LAUNCH(k_gather<double>, nn, B.s, nn, sel.p, c.iz.p, c.szp.p);
+    }
     c.tref.upload(kref.data(), nn, B.s);
     B.sync();
+    if (slot) {
+        c.ix.swap(c.sxp);
+        c.iy.swap(c.syp);
+        c.iz.swap(c.szp);
+    }
     c.x.swap(c.tx);
     c.y.swap(c.ty);
     c.z.swap(c.tz);
@@ -1444,6 +1457,11 @@ static void sort_scenes(Batch &B) {
     PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_cell_order, B.hc_blk.back(), d);
     PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_permute, B.total_pblocks, d);
     for (Scene *c : B.sc) {
+        if (c->P.integrator == SPH_INTEGRATOR_VERLET) {  // kb_permute moved the slot there
+            c->ix.swap(c->sxp);
+            c->iy.swap(c->syp);
+            c->iz.swap(c->szp);
+        }
         c->x.swap(c->tx);
         c->y.swap(c->ty);
         c->z.swap(c->tz);
@@ -1796,6 +1814,10 @@ static void solve_pressure(Batch &B) {
     BatchDev d = batch_dev(B);
     if (solver == SPH_SOLVER_SESPH) {  // sesph.rs:31-67
         PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_sesph_pressure, B.total_pblocks, d);
+        pressure_accel(B, 0);
+    } else if (solver == SPH_SOLVER_SESPH_SPLITTING) {  // sesph_with_splitting.rs:31-81
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_pred_vel, B.total_pblocks, d, 0);
+        PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_sesph_split_density, B.total_pblocks, d);
         pressure_accel(B, 0);
     } else if (solver == SPH_SOLVER_IISPH) {  // iisph.rs:55-96
         PLAUNCH(B, SPH_PROF_KERNELS, 0, kb_pred_vel, B.total_pblocks, d, 0);
@@ -2436,6 +2458,11 @@ static void step_batch(Batch &B) {
     tune_end_step(B, B.total_particles);
 }
 
+// SESPH and SESPHwSplitting: a state equation, no iterations
+static bool state_equation(const sph_params &P) {
+    return P.solver == SPH_SOLVER_SESPH || P.solver == SPH_SOLVER_SESPH_SPLITTING;
+}
+
 // a context's scene alone in the context's own batch
 static Batch &solo(sph_ctx *c) {
     c->own.sc.assign(1, &c->scene);
@@ -2455,9 +2482,9 @@ static void fill_measurement(sph_ctx *c, sph_measurement *o) {
     o->density = S.avg_density;
     o->density_error = n ? 100. * (sums[1] / (double)n - 1.) : 0.0;
     o->kinetic_energy = n ? sums[2] / (double)n : 0.0;
-    o->solver_iterations = S.P.solver == SPH_SOLVER_SESPH ? 0 : S.last_iter;
+    o->solver_iterations = state_equation(S.P) ? 0 : S.last_iter;
     o->eqs1_iterations = S.P.solver == SPH_SOLVER_IISPHWOST ? S.eqs1_iter : 0;
-    o->predicted_density_error = S.P.solver == SPH_SOLVER_SESPH ? 0.0 : S.last_err;
+    o->predicted_density_error = state_equation(S.P) ? 0.0 : S.last_err;
     o->steps = S.steps;
     o->num_particles = n;
 }
@@ -2465,8 +2492,8 @@ static void fill_measurement(sph_ctx *c, sph_measurement *o) {
 static void validate_params(const sph_params &P) {
     if (!(P.kernel_support_radius > 0) || !(P.rest_density_grid_spacing > 0))
         throw Error(SPH_ERR_ARG, "kernel_support_radius and rest_density_grid_spacing must be > 0");
-    if (P.solver < 0 || P.solver > 2) throw Error(SPH_ERR_ARG, "unknown solver");
-    if (P.integrator < 0 || P.integrator > 1) throw Error(SPH_ERR_ARG, "unknown integrator");
+    if (P.solver < 0 || P.solver > 3) throw Error(SPH_ERR_ARG, "unknown solver");
+    if (P.integrator < 0 || P.integrator > 3) throw Error(SPH_ERR_ARG, "unknown integrator");
     // PressureSolver::MANAGES_OWN_INTEGRATION pairing (setup/mod.rs:236-238)
     if ((P.solver == SPH_SOLVER_IISPHWOST) != (P.integrator == SPH_INTEGRATOR_TAKE_PREDICTED))
         throw Error(SPH_ERR_ARG, "IISPHwOST must be paired with TakePredicted and vice versa");
@@ -3117,13 +3144,59 @@ int sph_body_state(sph_ctx *c, int bi, double out[22]) {
     });
 }
 
+// IntegrationScheme::POSITION_SLOTS / VELOCITY_SLOTS (integration_schemes/mod.rs:
+// 23-42): TakePredicted one of each (take_predicted.rs:9-10), Verlet one
+// position slot (verlet.rs:10-11), EulerCromer and ExplicitEuler none
+static void slot_counts(const sph_params &P, int &npos, int &nvel) {
+    npos = (P.integrator == SPH_INTEGRATOR_TAKE_PREDICTED || P.integrator == SPH_INTEGRATOR_VERLET) ? 1 : 0;
+    nvel = P.integrator == SPH_INTEGRATOR_TAKE_PREDICTED ? 1 : 0;
+}
+
+int sph_fluid_slot_counts(sph_ctx *c, int *position_slots, int *velocity_slots) {
+    return guarded([&] {
+        if (!c || !position_slots || !velocity_slots) throw Error(SPH_ERR_ARG, "null argument");
+        slot_counts(c->scene.P, *position_slots, *velocity_slots);
+    });
+}
+
+int sph_fluid_upload_slots(sph_ctx *c, const double *xp, const double *vp) {
+    return guarded([&] {
+        if (!c) throw Error(SPH_ERR_ARG, "null argument");
+        check_solo(c);
+        Scene &S = c->scene;
+        int npos, nvel;
+        slot_counts(S.P, npos, nvel);
+        const size_t n = S.n;
+        Batch &B = c->own;
+        // host arrays in the reference's order; the device keeps the particles
+        // sorted once initialized, so position i takes reference index ref[i]
+        auto put = [&](const double *h, DBuf<double> &X, DBuf<double> &Y, DBuf<double> &Z) {
+            std::vector<double> a(n), b(n), d(n);
+            for (size_t i = 0; i < n; i++) {
+                a[i] = h[3 * i];
+                b[i] = h[3 * i + 1];
+                d[i] = h[3 * i + 2];
+            }
+            DBuf<double> *dst[3] = {&X, &Y, &Z};
+            const std::vector<double> *src[3] = {&a, &b, &d};
+            for (int k = 0; k < 3; k++) {
+                DBuf<double> tmp;
+                tmp.upload(src[k]->data(), n, B.s);
+                LAUNCH(k_gather<double>, n, B.s, n, S.ref.p, tmp.p, dst[k]->p);
+                B.sync();
+            }
+        };
+        if (xp && npos) put(xp, S.ix, S.iy, S.iz);
+        if (vp && nvel) put(vp, S.ivx, S.ivy, S.ivz);
+    });
+}
+
 int sph_fluid_download_slots(sph_ctx *c, int *nslots, double *xp, double *vp) {
     return guarded([&] {
         if (!c || !nslots) throw Error(SPH_ERR_ARG, "null argument");
         Scene &S = c->scene;
-        // TakePredicted keeps one position and one velocity slot
-        // (take_predicted.rs:9-10); EulerCromer none (integration_schemes/mod.rs:25-28)
-        *nslots = S.P.integrator == SPH_INTEGRATOR_TAKE_PREDICTED ? 1 : 0;
+        int nvel;
+        slot_counts(S.P, *nslots, nvel);
         if (*nslots == 0) return;
         std::vector<double> A, B_, C_;
         if (xp) {
@@ -3132,7 +3205,7 @@ int sph_fluid_download_slots(sph_ctx *c, int *nslots, double *xp, double *vp) {
             download_field(c, S.iz, C_);
             split3(A, B_, C_, xp);
         }
-        if (vp) {
+        if (vp && nvel) {
             download_field(c, S.ivx, A);
             download_field(c, S.ivy, B_);
             download_field(c, S.ivz, C_);
@@ -3178,6 +3251,12 @@ int sph_system_restore(sph_ctx *c, uint64_t time_steps_propagated, double curren
                 if (body.kind == 0) pose_samples(c->own, S, body);
             }
         }
+        // both resume paths re-size the fluid's slots (sph/mod.rs:368-373,
+        // 404-409; Fluid::resize_slots, fluid.rs:112-129): the integrator's
+        // slots start at the origin and at zero again, and a checkpoint's
+        // saved slots are not read
+        DBuf<double> *slots[] = {&S.ix, &S.iy, &S.iz, &S.ivx, &S.ivy, &S.ivz};
+        for (auto *q : slots) LAUNCH(k_fill_f64, S.n, c->own.s, q->p, S.n, 0.0);
         // continue_from_checkpoint (sph/mod.rs:362-378): counters, then update()
         S.dt = S.P.cfl ? 0.0 : S.P.time_increment;
         S.time = current_time;
@@ -3346,9 +3425,9 @@ int sph_pressure_solve(sph_ctx *c, size_t n, const double *x, const double *v, c
         }
         if (info_out) {
             memset(info_out, 0, sizeof *info_out);
-            info_out->solver_iterations = S.P.solver == SPH_SOLVER_SESPH ? 0 : S.last_iter;
+            info_out->solver_iterations = state_equation(S.P) ? 0 : S.last_iter;
             info_out->eqs1_iterations = S.P.solver == SPH_SOLVER_IISPHWOST ? S.eqs1_iter : 0;
-            info_out->predicted_density_error = S.P.solver == SPH_SOLVER_SESPH ? 0.0 : S.last_err;
+            info_out->predicted_density_error = state_equation(S.P) ? 0.0 : S.last_err;
             info_out->time_step_size = dt;
             info_out->num_particles = n;
         }

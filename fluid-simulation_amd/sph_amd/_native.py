@@ -110,6 +110,8 @@ SIGNATURES = [
     ("sph_body_state", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("sph_body_forces", C.c_int, [C.c_void_p, C.c_int, _dp]),
     ("sph_fluid_download_slots", C.c_int, [C.c_void_p, C.POINTER(C.c_int), _dp, _dp]),
+    ("sph_fluid_slot_counts", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("sph_fluid_upload_slots", C.c_int, [C.c_void_p, _dp, _dp]),
     ("sph_system_restore", C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_size_t, _u8p,
                                      _dp]),
     ("sph_profile", C.c_int, [C.c_void_p, C.c_int]),

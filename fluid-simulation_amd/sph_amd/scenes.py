@@ -33,8 +33,8 @@ from typing import List, Optional
 
 import numpy as np
 
-SOLVERS = {"SESPH": 0, "IISPH": 1, "IISPHwOST": 2}
-INTEGRATORS = {"EulerCromer": 0, "TakePredicted": 1}
+SOLVERS = {"SESPH": 0, "IISPH": 1, "IISPHwOST": 2, "SESPHwSplitting": 3}
+INTEGRATORS = {"EulerCromer": 0, "TakePredicted": 1, "ExplicitEuler": 2, "Verlet": 3}
 
 # to_offset() of the 32 reference nodes (utilities/discretization.rs:262-300):
 # 8 corners, then the x-, y- and z-edge nodes

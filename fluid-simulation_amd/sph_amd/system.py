@@ -301,12 +301,13 @@ class System:
         from . import state_io
         st = self.state()
         n = len(st["m"])
-        ns = C.c_int()
+        npos, nvel = self.integrator_slot_counts()
         xp, vp = np.zeros((n, 3)), np.zeros((n, 3))
+        ns = C.c_int()
         N.check(N.lib().sph_fluid_download_slots(self._ctx.h, C.byref(ns), N.dptr(xp),
                                                  N.dptr(vp)))
-        pslots = [xp] if ns.value else []
-        vslots = [vp] if ns.value else []
+        pslots = [xp] if npos else []
+        vslots = [vp] if nvel else []
         m = self.take_measurement()
         states = []
         for b, desc in enumerate(self.scene.boundaries):
@@ -322,6 +323,23 @@ class System:
                                          pslots, vslots)
         return state_io.SystemCheckpoint(int(m["steps"]), float(m["time"]), fluid, states)
 
+    def integrator_slot_counts(self):
+        """``IntegrationScheme::POSITION_SLOTS`` / ``VELOCITY_SLOTS``
+        (integration_schemes/mod.rs:23-42) of this system's integrator."""
+        npos, nvel = C.c_int(), C.c_int()
+        N.check(N.lib().sph_fluid_slot_counts(self._ctx.h, C.byref(npos), C.byref(nvel)))
+        return int(npos.value), int(nvel.value)
+
+    def set_integrator_slots(self, positions=None, velocities=None):
+        """The integrator's position and velocity slots, [n][3] in the
+        reference's particle order (Verlet's position slot holds the position
+        one step back; Fluid::resize_slots starts it at the origin)."""
+        def arr(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)
+        xp, vp = arr(positions), arr(velocities)
+        N.check(N.lib().sph_fluid_upload_slots(self._ctx.h, N.dptr(xp) if xp is not None else None,
+                                               N.dptr(vp) if vp is not None else None))
+
     @classmethod
     def from_checkpoint(cls, scene: Scene, cp) -> "System":
         """``SPHSystem::continue_from_checkpoint``: a system of ``scene``'s
@@ -334,6 +352,8 @@ class System:
                                  m=np.asarray(f.mass, dtype=np.float64).ravel(),
                                  fluid_id=np.asarray(f.fluid_id, dtype=np.uint32).ravel())
         sim = cls(sc, init=False)
+        # the checkpoint's integrator slots are not read back: both of the
+        # reference's resume paths re-size them (sph/mod.rs:368-373, 404-409)
         nb = len(cp.dynamic_states)
         present = np.zeros(max(nb, 1), dtype=np.uint8)
         bodies = np.zeros((max(nb, 1), 19))

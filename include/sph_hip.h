@@ -43,8 +43,10 @@ typedef enum sph_status {
     SPH_ERR_NO_DEVICE = -6     /* no GPU visible */
 } sph_status;
 
-enum { SPH_SOLVER_SESPH = 0, SPH_SOLVER_IISPH = 1, SPH_SOLVER_IISPHWOST = 2 };
-enum { SPH_INTEGRATOR_EULER_CROMER = 0, SPH_INTEGRATOR_TAKE_PREDICTED = 1 };
+enum { SPH_SOLVER_SESPH = 0, SPH_SOLVER_IISPH = 1, SPH_SOLVER_IISPHWOST = 2,
+       SPH_SOLVER_SESPH_SPLITTING = 3 };
+enum { SPH_INTEGRATOR_EULER_CROMER = 0, SPH_INTEGRATOR_TAKE_PREDICTED = 1,
+       SPH_INTEGRATOR_EXPLICIT_EULER = 2, SPH_INTEGRATOR_VERLET = 3 };
 enum { SPH_GRAVITY_UNIFORM = 0, SPH_GRAVITY_RADIAL = 1 };
 
 /* [parameters] + [procedures] of the reference (sph/setup/input.rs:103-260,
@@ -215,14 +217,31 @@ int sph_body_forces(sph_ctx *ctx, int b, double out[6]);
 
 /* ---- checkpoints ------------------------------------------------------------------
  * The integrator's position and velocity slots of FluidCheckpoint
- * (fluid.rs:352-381): *nslots = 1 for TakePredicted (take_predicted.rs:9-10),
- * 0 for EulerCromer; with one slot, xp and vp (either may be NULL) receive
- * [n][3] in the reference's particle order. */
+ * (fluid.rs:352-381, IntegrationScheme::POSITION_SLOTS / VELOCITY_SLOTS,
+ * integration_schemes/mod.rs:23-42): one position and one velocity slot for
+ * TakePredicted (take_predicted.rs:9-10), one position slot for Verlet (the
+ * position one step back, verlet.rs:10-11), none for EulerCromer and
+ * ExplicitEuler. sph_fluid_slot_counts reports them; sph_fluid_download_slots
+ * sets *nslots to the position-slot count and fills xp and vp (either may be
+ * NULL; vp only where there is a velocity slot) with [n][3] in the
+ * reference's particle order. */
+int sph_fluid_slot_counts(sph_ctx *ctx, int *position_slots, int *velocity_slots);
 int sph_fluid_download_slots(sph_ctx *ctx, int *nslots, double *xp, double *vp);
+/* Sets the integrator's slots ([n][3] in the reference's particle order;
+ * either may be NULL; ignored where the scheme has no such slot). No
+ * counterpart in the reference, whose slots always start where
+ * Fluid::resize_slots leaves them (fluid.rs:112-129): Verlet's position slot
+ * at the origin, so its first step moves each particle to about twice its
+ * position. This lets a caller start Verlet from rest (the slot at the
+ * positions) instead. sph_system_restore resets the slots, as the
+ * reference's resume does. */
+int sph_fluid_upload_slots(sph_ctx *ctx, const double *xp, const double *vp);
 /* SPHSystem::continue_from_checkpoint (sph/mod.rs:362-378), called after
  * sph_fluid_upload of the checkpoint's fluid and in place of sph_system_init:
  * restores the step count and current time, each dynamic boundary's
- * RigidBodyMotion state (rigid_body_motion.rs:214-222) and runs update().
+ * RigidBodyMotion state (rigid_body_motion.rs:214-222), re-sizes the
+ * integrator's slots as Fluid::resize_slots does (origin and zero: the
+ * reference does not read a checkpoint's saved slots back) and runs update().
  * bodies holds 19 binary64 per boundary: center of mass[3], orientation
  * (w, i, j, k; normalized as UnitQuaternion::from_quaternion does,
  * rigid_body_motion.rs:273-289), linear velocity[3], angular momentum[3],

@@ -1108,6 +1108,43 @@ static void sesph_solve(ref_system *s) {
     add_pressure_acceleration(s, s->a, 0, 1);
 }
 
+/* SESPHwSplitting::solve_and_add_acceleration   (sesph_with_splitting.rs:31-81,
+ * calc_predicted_density :97-153): v* = v + dt a, the density predicted from
+ * the fluid's masses and v* (boundary samples mirror the particle's mass and
+ * enter with v*_i alone), the state equation on m / rho*, then the pressure
+ * acceleration as in SESPH */
+static void sesph_split_solve(ref_system *s) {
+    const double h = s->P.kernel_support_radius, dt = s->dt;
+    set_pred_vel(s, 0);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < s->n; i++) {
+        const double *xi = s->x + 3 * i, *vi = s->sv + 3 * i;
+        double accu = 0.;
+        for (uint64_t e = s->nl.offsets[i]; e < s->nl.offsets[i + 1]; e++) {
+            uint64_t j = s->nl.indices[e];
+            double r[3], gw[3], dv[3];
+            sub3(xi, s->x + 3 * j, r);
+            sub3(vi, s->sv + 3 * j, dv);
+            ref_kernel_gradient(r, h, gw);
+            accu += s->m[j] * ref_kernel_function(r, h) + dt * dot3(dv, gw);
+        }
+        for (int bi = 0; bi < s->nb; bi++) {
+            const ref_boundary *b = &s->b[bi];
+            for (uint64_t e = b->off[0][i]; e < b->off[0][i + 1]; e++) {
+                uint64_t id = b->ids[0][e];
+                double r[3], gw[3];
+                sub3(xi, b->pos + 3 * id, r);
+                ref_kernel_gradient(r, h, gw);
+                accu += b->vol[id] * s->m[i] / s->rest_volume * ref_kernel_function(r, h) +
+                        dt * dot3(vi, gw);
+            }
+        }
+        const double vol = s->m[i] / accu;
+        s->p[i] = s->P.stiffness * fmax(s->rest_volume / vol - 1., 0.);
+    }
+    add_pressure_acceleration(s, s->a, 0, 1);
+}
+
 /* IISPH source terms   (iisph.rs:117-172 vde, 175-251 vp) */
 static void set_source_term(ref_system *s, int vp) {
     double h = s->P.kernel_support_radius, dt = s->dt;
@@ -1287,6 +1324,8 @@ static void calc_acceleration(ref_system *s) {
     add_viscosity(s);
     if (s->P.solver == REF_SESPH)
         sesph_solve(s);
+    else if (s->P.solver == REF_SESPH_SPLITTING)
+        sesph_split_solve(s);
     else if (s->P.solver == REF_IISPH)
         iisph_solve(s);
     else
@@ -2016,20 +2055,39 @@ size_t ref_system_contact_cells(const ref_system *s, double *out, size_t cap) {
     return n;
 }
 
+/* IntegrationScheme::integrate (sph/integration_schemes) */
+static void integrate(ref_system *s, double dt) {
+    if (s->P.integrator == REF_EULER_CROMER) { /* euler_cromer.rs:12-25 */
+        for (size_t i = 0; i < 3 * s->n; i++) {
+            s->v[i] = s->v[i] + dt * s->a[i];
+            s->x[i] = s->x[i] + dt * s->v[i];
+        }
+    } else if (s->P.integrator == REF_EXPLICIT_EULER) { /* explicit_euler.rs:12-25 */
+        for (size_t i = 0; i < 3 * s->n; i++) {
+            const double x0 = s->x[i], v0 = s->v[i];
+            s->x[i] = x0 + dt * v0;
+            s->v[i] = v0 + dt * s->a[i];
+        }
+    } else if (s->P.integrator == REF_VERLET) { /* verlet.rs:13-34; slot = x(t - dt) */
+        for (size_t i = 0; i < 3 * s->n; i++) {
+            const double x0 = s->x[i];
+            const double nx = x0 + (x0 - s->ix[i]) + powi2(dt) * s->a[i];
+            s->v[i] = (nx - x0) / dt;
+            s->ix[i] = x0;
+            s->x[i] = nx;
+        }
+    } else { /* take_predicted.rs:14-27 */
+        double *t;
+        t = s->x; s->x = s->ix; s->ix = t;
+        t = s->v; s->v = s->iv; s->iv = t;
+    }
+}
+
 /* SPHSystem::step_forward_in_time   (sph/mod.rs:125-144) */
 int ref_system_step(ref_system *s, int nsteps) {
     for (int st = 0; st < nsteps; st++) {
         double dt = s->dt;
-        if (s->P.integrator == REF_EULER_CROMER) { /* euler_cromer.rs:12-25 */
-            for (size_t i = 0; i < 3 * s->n; i++) {
-                s->v[i] = s->v[i] + dt * s->a[i];
-                s->x[i] = s->x[i] + dt * s->v[i];
-            }
-        } else { /* take_predicted.rs:14-27 */
-            double *t;
-            t = s->x; s->x = s->ix; s->ix = t;
-            t = s->v; s->v = s->iv; s->iv = t;
-        }
+        integrate(s, dt);
         contact_step(s, dt); /* VolumeMapBoundary::step_forward_in_time (:1440-1453) */
         for (int i = 0; i < s->nb; i++) { /* B::step_forward_in_time */
             ref_boundary *b = &s->b[i];
@@ -2075,9 +2133,10 @@ void ref_system_measure(const ref_system *s, ref_measurement *o) {
     }
     o->density_error = s->n ? 100. * (se / (double)s->n - 1.) : 0.0;
     o->kinetic_energy = s->n ? ke / (double)s->n : 0.0;
-    o->solver_iterations = s->P.solver == REF_SESPH ? 0 : s->last_iter;
+    const int state_equation = s->P.solver == REF_SESPH || s->P.solver == REF_SESPH_SPLITTING;
+    o->solver_iterations = state_equation ? 0 : s->last_iter;
     o->eqs1_iterations = s->P.solver == REF_IISPHWOST ? s->eqs1_iter : 0;
-    o->predicted_density_error = s->P.solver == REF_SESPH ? 0.0 : s->last_err;
+    o->predicted_density_error = state_equation ? 0.0 : s->last_err;
     o->steps = s->steps;
     o->num_particles = s->n;
 }
@@ -2254,6 +2313,8 @@ int ref_system_call(ref_system *s, int fn) {
         case REF_FN_DIAGONAL: set_diagonal(s); return 0;
         case REF_FN_INITIALIZE: iisph_initialize(s); return 0;
         case REF_FN_RESOLVE: return (int)resolve_pressure(s, -1);
+        case REF_FN_SESPH_SPLITTING: sesph_split_solve(s); return 0;
+        case REF_FN_INTEGRATE: integrate(s, s->dt); return 0;
         default: return -1;
     }
 }
@@ -2274,6 +2335,7 @@ int ref_system_field(ref_system *s, int field, int write, double *buf) {
         case REF_F_DIAGONAL: f = s->a_ff; k = s->n; break;
         case REF_F_PRESSURE_ACC: f = s->pacc; k = 3 * s->n; break;
         case REF_F_TIME_STEP: f = &s->dt; k = 1; break;
+        case REF_F_SLOT_POS: f = s->ix; k = 3 * s->n; break;
         default: return -1;
     }
     if (write)

@@ -100,8 +100,8 @@ void ref_quat_rotate(const double q[4], const double v[3], double out[3]);
 void ref_quat_to_matrix(const double q[4], double R[9]);
 
 /* ---------------- system (sph/mod.rs) */
-enum { REF_SESPH = 0, REF_IISPH = 1, REF_IISPHWOST = 2 };
-enum { REF_EULER_CROMER = 0, REF_TAKE_PREDICTED = 1 };
+enum { REF_SESPH = 0, REF_IISPH = 1, REF_IISPHWOST = 2, REF_SESPH_SPLITTING = 3 };
+enum { REF_EULER_CROMER = 0, REF_TAKE_PREDICTED = 1, REF_EXPLICIT_EULER = 2, REF_VERLET = 3 };
 enum { REF_GRAVITY_UNIFORM = 0, REF_GRAVITY_RADIAL = 1 };
 
 typedef struct ref_params {
@@ -269,7 +269,8 @@ enum {
     REF_FN_NEIGHBORS = 0, REF_FN_VOLUME, REF_FN_RESET_ACCELERATION, REF_FN_GRAVITY,
     REF_FN_VISCOSITY, REF_FN_PRESSURE_ACCELERATION, REF_FN_PRESSURE_ACCELERATION_OVERWRITE,
     REF_FN_PRESSURE_ACCELERATION_SCRATCH, REF_FN_SESPH, REF_FN_PRED_VEL, REF_FN_SOURCE_VDE,
-    REF_FN_SOURCE_VP, REF_FN_DIAGONAL, REF_FN_INITIALIZE, REF_FN_RESOLVE
+    REF_FN_SOURCE_VP, REF_FN_DIAGONAL, REF_FN_INITIALIZE, REF_FN_RESOLVE,
+    REF_FN_SESPH_SPLITTING, REF_FN_INTEGRATE
 };
 int ref_system_call(ref_system *s, int fn);
 /* read (write = 0) or overwrite (write = 1) one field; returns its length in
@@ -278,7 +279,8 @@ int ref_system_call(ref_system *s, int fn);
  * time step (1) */
 enum {
     REF_F_X = 0, REF_F_VEL, REF_F_ACC, REF_F_MASS, REF_F_VOLUME, REF_F_PRESSURE, REF_F_PRED_VEL,
-    REF_F_PRED_POS, REF_F_SOURCE, REF_F_DIAGONAL, REF_F_PRESSURE_ACC, REF_F_TIME_STEP
+    REF_F_PRED_POS, REF_F_SOURCE, REF_F_DIAGONAL, REF_F_PRESSURE_ACC, REF_F_TIME_STEP,
+    REF_F_SLOT_POS /* the integrator's position slot (TakePredicted, Verlet) */
 };
 int ref_system_field(ref_system *s, int field, int write, double *buf);
 /* pseudo volumes of static-sample boundary b (read or overwrite) */

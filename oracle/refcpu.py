@@ -64,9 +64,9 @@ class RefRigid(C.Structure):
 FUNCTIONS = ["neighbors", "volume", "reset_acceleration", "gravity", "viscosity",
              "pressure_acceleration", "pressure_acceleration_overwrite",
              "pressure_acceleration_scratch", "sesph", "pred_vel", "source_vde", "source_vp",
-             "diagonal", "initialize", "resolve"]
+             "diagonal", "initialize", "resolve", "sesph_splitting", "integrate"]
 FIELDS = {"x": 3, "v": 3, "a": 3, "m": 1, "V": 1, "p": 1, "pred_v": 3, "pred_x": 3, "s": 1,
-          "a_ii": 1, "pacc": 3, "dt": 0}
+          "a_ii": 1, "pacc": 3, "dt": 0, "slot_x": 3}
 
 
 def build():
@@ -353,8 +353,8 @@ def contact_solve(bodies, cells, dt):
 
 
 def params_struct(P):
-    from_solver = {"SESPH": 0, "IISPH": 1, "IISPHwOST": 2}
-    from_integ = {"EulerCromer": 0, "TakePredicted": 1}
+    from_solver = {"SESPH": 0, "IISPH": 1, "IISPHwOST": 2, "SESPHwSplitting": 3}
+    from_integ = {"EulerCromer": 0, "TakePredicted": 1, "ExplicitEuler": 2, "Verlet": 3}
     s = RefParams()
     s.time_increment = P.time_increment
     s.cfl = int(bool(P.cfl))

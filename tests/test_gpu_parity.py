@@ -398,3 +398,63 @@ def test_moving_static_sample_body_refused_by_iisphwost(native):
     with pytest.raises(N.SphError, match="IISPHwOST does not support dynamic boundaries"):
         native.System(sc)
 
+
+
+# ------------------------------------------- other integrators and solvers
+def with_scheme(sc, solver=None, integrator=None):
+    if solver:
+        sc.params.solver = solver
+    if integrator:
+        sc.params.integrator = integrator
+    return sc
+
+
+SCHEMES = {
+    # SESPH + ExplicitEuler on the dam break's static samples
+    "sesph_explicit_euler": lambda: with_scheme(scenes.dam_break_sesph(side=8),
+                                                integrator="ExplicitEuler"),
+    # IISPH + Verlet from a position slot equal to the positions (at rest)
+    "iisph_verlet": lambda: with_scheme(scenes.small_iisph_box(side=10, jitter=2e-3),
+                                        integrator="Verlet"),
+    # Verlet from the slot Fluid::resize_slots leaves (the origin)
+    "sesph_verlet_origin_slot": lambda: with_scheme(scenes.dam_break_sesph(side=6),
+                                                    integrator="Verlet"),
+    # SESPHwSplitting on a volume-map box, on static samples, and with a
+    # two-way coupled body
+    "splitting_box": lambda: with_scheme(scenes.small_iisph_box(side=10, jitter=2e-3),
+                                         solver="SESPHwSplitting"),
+    "splitting_static": lambda: with_scheme(scenes.dam_break_sesph(side=8),
+                                            solver="SESPHwSplitting"),
+    "splitting_rigid": lambda: with_scheme(scenes.small_rigid_scene(), solver="SESPHwSplitting"),
+}
+
+
+@pytest.mark.parametrize("case", list(SCHEMES))
+def test_other_schemes_bit_identical(native, refcpu, case):
+    """ExplicitEuler and Verlet (integration_schemes/explicit_euler.rs,
+    verlet.rs) and SESPHwSplitting (pressure_solver/sesph_with_splitting.rs):
+    positions, velocities, accelerations, volumes and pressures equal the
+    oracle's bit for bit while no moving body couples (the body's force total
+    is grouped differently on the two sides), within 1e-9 otherwise."""
+    sc = SCHEMES[case]()
+    print(f"scheme {case}: {sc.n} particles", flush=True)
+    o, g = refcpu.OracleSystem(sc), native.System(sc)
+    if case == "iisph_verlet":
+        o.set_field("slot_x", sc.x)  # both systems in the reference's particle order
+        g.set_integrator_slots(sc.x)
+    exact = case != "splitting_rigid"
+    for k in range(3 if case == "sesph_verlet_origin_slot" else 8):
+        o.step(1)
+        g.step_forward_in_time(1)
+        so, sg = o.state(), g.state()
+        for f in ("x", "v", "a", "V", "p"):
+            if exact:
+                assert np.array_equal(sg[f], so[f]), (k, f)
+            else:
+                assert rel_err(sg[f], so[f]) <= 1e-9, (k, f)
+        mg, mo = g.take_measurement(), o.measure()
+        assert mg["solver_iterations"] == mo["solver_iterations"]
+        # the average (a sum over particles, grouped differently) minus 1
+        assert abs(mg["density_error"] - mo["density_error"]) <= 1e-12 + 1e-9 * abs(mo["density_error"])
+    if "verlet" in case:
+        assert g.integrator_slot_counts() == (1, 0)

@@ -5,7 +5,8 @@ A system saved after k steps and resumed from the saved file
 bit as the uninterrupted system when no body moves: the fluid is stored in the
 reference's particle order, which is the order the library sorts ties by, and
 update() recomputes everything else from it. The continued trajectory is also
-checked against the oracle stepping the uninterrupted run.
+checked against the oracle stepping the uninterrupted run. The integrator's
+slots are saved but not read back (the reference re-sizes them on resume).
 
 A moving body resumes differently, as in the reference: the saved orientation
 is re-normalized (UnitQuaternion::from_quaternion, rigid_body_motion.rs:
@@ -75,6 +76,46 @@ def test_resume_continues_the_run(native, tmp_path, case, fmt):
     for k in ("x", "v", "V", "p"):
         den = float(np.max(np.abs(so[k]))) or 1.0
         assert float(np.max(np.abs(sb[k] - so[k]))) / den <= 1e-5, k
+
+
+@pytest.mark.parametrize("fmt", ["ron", "bincode"])
+def test_resume_resets_the_verlet_slot(native, tmp_path, fmt):
+    """Verlet's position slot (the position one step back, verlet.rs:10-34)
+    is saved with the fluid (fluid.rs:352-381), but both of the reference's
+    resume paths re-size the slots before the first step (sph/mod.rs:368-373,
+    404-409), so the resumed run starts again from the origin slot. It then
+    follows the oracle started from the checkpoint's fluid bit for bit (under
+    SESPH: the origin slot throws every particle to about twice its position,
+    and IISPH's solve, as the reference's, then does not converge)."""
+    import dataclasses
+    sc = scenes.dam_break_sesph(side=6)
+    sc.params.integrator = "Verlet"
+    print(f"resume verlet {fmt}", flush=True)
+    a = native.System(sc)
+    a.set_integrator_slots(sc.x)
+    a.step_forward_in_time(3)
+    path = str(tmp_path / f"state.{fmt}")
+    state_io.save_checkpoint(a.get_checkpoint(), path, fmt)
+    cp = state_io.load_checkpoint(path)
+    assert len(cp.fluid.integrator_position_slots) == 1
+    assert len(cp.fluid.integrator_velocity_slots) == 0
+    saved = np.asarray(cp.fluid.integrator_position_slots[0], dtype=np.float64).reshape(-1, 3)
+    assert np.count_nonzero(saved) > 0
+    b = native.System.from_checkpoint(sc, cp)
+    xp = b.get_checkpoint().fluid.integrator_position_slots[0]
+    assert not np.any(np.asarray(xp, dtype=np.float64))
+    f = cp.fluid
+    o = refcpu.OracleSystem(dataclasses.replace(
+        sc, x=np.asarray(f.position, dtype=np.float64).reshape(-1, 3),
+        v=np.asarray(f.velocity, dtype=np.float64).reshape(-1, 3),
+        m=np.asarray(f.mass, dtype=np.float64).ravel(),
+        fluid_id=np.asarray(f.fluid_id, dtype=np.uint32).ravel()))
+    for k in range(2):
+        o.step(1)
+        b.step_forward_in_time(1)
+        so, sb = o.state(), b.state()
+        for f_ in ("x", "v", "a", "V", "p"):
+            assert np.array_equal(bits(sb[f_]), bits(so[f_])), (k, f_)
 
 
 @pytest.mark.parametrize("case", ["volume_map_body", "sample_body"])

@@ -555,3 +555,98 @@ def test_moving_sample_pose_and_velocity(com, lin_vel, ang_vel, fluid_at, pos_wa
     assert s.shape == (1, 7)
     assert tuple(s[0, :3]) == pos_want
     assert tuple(s[0, 3:6]) == vel_want
+
+
+# --------------------------------------------- other integrators and solvers
+def w_kernel(r, h):
+    q = math.sqrt(float(np.asarray(r) @ np.asarray(r))) / h
+    return ALPHA / h ** 3 * (max(1.0 - q, 0.0) ** 3 - 4.0 * max(0.5 - q, 0.0) ** 3)
+
+
+def integrate_one(integrator, x, v, a, dt, slot=None):
+    o = make([x], v=[v], solver="SESPH", integrator=integrator)
+    o.set_field("x", [x])
+    o.set_field("v", [v])
+    o.set_field("a", [a])
+    o.set_field("dt", dt)
+    if slot is not None:
+        o.set_field("slot_x", [slot])
+    o.call("integrate")
+    return o.field("x")[0], o.field("v")[0], o.field("slot_x")[0]
+
+
+def test_explicit_euler_known_answers():
+    """tests/sph/integration_schemes/explicit_euler.rs:84-135: x += dt v_old,
+    v += dt a (the position takes the velocity from the start of the step)."""
+    x0, v0, a0, dt = np.array([1.0, 2.0, 3.0]), np.array([0.5, -0.5, 1.0]), np.array([2.0, 0.0, -9.81]), 0.1
+    x, v, _ = integrate_one("ExplicitEuler", x0, v0, a0, dt)
+    assert np.allclose(x, x0 + dt * v0, rtol=0, atol=1e-15)
+    assert np.allclose(v, v0 + dt * a0, rtol=0, atol=1e-15)
+    xe, _, _ = integrate_one("EulerCromer", x0, v0, a0, dt)
+    assert not np.allclose(xe, x)  # Euler-Cromer moves with the new velocity
+
+
+def test_verlet_known_answers():
+    """tests/sph/integration_schemes/verlet.rs:106-184: x' = x + (x - x_prev) +
+    dt^2 a, v = (x' - x) / dt, and the slot keeps the pre-step position; a
+    particle at rest with x_prev = x and no acceleration stays put."""
+    xn, xp, a, dt = np.array([1.0, 0.0, 0.0]), np.array([0.5, 0.0, 0.0]), np.array([0.0, -9.81, 0.0]), 0.1
+    x, v, slot = integrate_one("Verlet", xn, np.zeros(3), a, dt, slot=xp)
+    exp = xn + (xn - xp) + dt * dt * a
+    assert np.allclose(x, exp, rtol=0, atol=1e-15)
+    assert np.allclose(v, (exp - xn) / dt, rtol=0, atol=1e-13)
+    assert np.array_equal(slot, xn)
+    x, v, _ = integrate_one("Verlet", xn, np.zeros(3), np.zeros(3), dt, slot=xn)
+    assert np.array_equal(x, xn) and np.array_equal(v, np.zeros(3))
+
+
+def test_sesph_splitting_compressed_pair():
+    """tests/sph/pressure_solver/sesph_with_splitting.rs:376-429: two
+    particles 0.3 apart (h = 1, m = 0.5, k = 500, dt = 0.05): rho*_i =
+    sum_j m_j W_ij (the self pair included), p = k max(V0 / V - 1, 0) with
+    V = m / rho*; with the particles approaching (v*_0 = +u e_x,
+    v*_1 = -u e_x) the density gains dt (v*_i - v*_j) . grad W_ij. The
+    spacing is 0.9 instead of the reference test's 0.3, whose rest volume
+    leaves both pressures at zero."""
+    h, dx, k, dt, m = 1.0, 0.9, 500.0, 0.05, 0.5
+    V0 = (dx * dx) * dx
+    x = np.array([[0.0, 0.0, 0.0], [0.3, 0.0, 0.0]])
+    for u in (0.0, 0.4):
+        v = np.array([[u, 0.0, 0.0], [-u, 0.0, 0.0]])
+        o = make(x, v=v, m=np.full(2, m), h=h, dx=dx, stiffness=k, solver="SESPHwSplitting")
+        o.set_field("v", v)
+        o.set_field("a", np.zeros((2, 3)))
+        o.set_field("dt", dt)
+        o.call("sesph_splitting")
+        p = o.field("p")
+        for i in range(2):
+            r = x[i] - x[1 - i]
+            rho = m * w_kernel(np.zeros(3), h) + m * w_kernel(r, h) + \
+                dt * float((v[i] - v[1 - i]) @ grad_w(r, h))
+            exp = k * max(V0 / (m / rho) - 1.0, 0.0)
+            assert abs(p[i] - exp) <= 1e-9 * max(exp, 1.0), (u, i, p[i], exp)
+        static = k * max(V0 / (m / (m * w_kernel(np.zeros(3), h) + m * w_kernel(x[1], h))) - 1, 0)
+        assert static > 0 and (p[0] > static if u > 0 else p[0] == static)
+
+
+def test_sesph_splitting_boundary_mirrors_the_particle_mass():
+    """sesph_with_splitting.rs:132-148 and tests/...:432-473: a boundary sample
+    of volume V_b adds V_b m_i / V0 W_ib + dt v*_i . grad W_ib."""
+    h, dx, k, dt, m, Vb = 1.0, 0.9, 500.0, 0.05, 0.5, 0.2
+    V0 = (dx * dx) * dx
+    xb = np.array([[0.2, 0.0, 0.0]])
+    u = np.array([[0.0, 0.0, 0.0]]), np.array([[-0.3, 0.0, 0.0]])
+    for v in u:
+        o = make([[0.0, 0.0, 0.0]], v=v, m=[m], h=h, dx=dx, stiffness=k, solver="SESPHwSplitting",
+                 boundaries=[scenes.StaticSamples(xb)])
+        o.sample_volumes(0, [Vb])
+        o.set_field("v", v)
+        o.set_field("a", np.zeros((1, 3)))
+        o.set_field("dt", dt)
+        o.call("neighbors")
+        o.call("sesph_splitting")
+        r = -xb[0]
+        rho = m * w_kernel(np.zeros(3), h) + (Vb * m / V0 * w_kernel(r, h) + dt * float(v[0] @ grad_w(r, h)))
+        exp = k * max(V0 / (m / rho) - 1.0, 0.0)
+        assert exp > 0
+        assert abs(o.field("p")[0] - exp) <= 1e-9 * max(exp, 1.0), (v, o.field("p")[0], exp)
