@@ -10,8 +10,9 @@ step's measured and predicted average density error within 1e-9 relative,
 and the full fluid state equal bit for bit at checkpoints.
 
 Workloads: the SESPH dam break (8,000 particles, static samples) for all
-1,000 steps of its configuration; the IISPH block drop (125,000 particles)
-for all 500 steps of its configuration; a pool at rest (17,640 particles
+1,000 steps of its configuration, and under SESPHwSplitting for 300 steps;
+the IISPH block drop (125,000 particles) for all 500 steps of its
+configuration; a pool at rest (17,640 particles
 filling a box wall to wall, 40 layers) for 300 steps under IISPH and under
 IISPHwOST, where every solve carries the column's weight (13 and 17 Jacobi
 iterations per step on average, up to 45); a dynamic cube dropped into a
@@ -77,6 +78,16 @@ def test_dam_break_all_1000_steps(native):
     sc = scenes.dam_break_sesph()
     print(f"long run: SESPH dam break, {sc.n} particles, 1000 steps", flush=True)
     lockstep(native, sc, 1000, {100, 250, 500, 750, 1000})
+
+
+def test_sesph_splitting_dam_break_300_steps(native):
+    """The dam break under SESPHwSplitting (sesph_with_splitting.rs) for 300
+    steps: every step's measurement, and the state bit for bit at
+    checkpoints."""
+    sc = scenes.dam_break_sesph()
+    sc.params.solver = "SESPHwSplitting"
+    print(f"long run: SESPHwSplitting dam break, {sc.n} particles, 300 steps", flush=True)
+    lockstep(native, sc, 300, {50, 150, 300})
 
 
 def test_block_drop_all_500_steps(native):

@@ -195,6 +195,31 @@ def test_compaction_below_threshold(native, refcpu):
     assert np.array_equal(g.state()["fluid_id"], o.state()["fluid_id"])
 
 
+def test_compaction_keeps_the_verlet_slot(native, refcpu):
+    """Fluid::swap moves the integrator's slots with their particle and
+    drop_inactive truncates them (fluid.rs:159-210): IISPH + Verlet, the slot
+    at the positions after System::new, particles of the lowest layer dropped
+    as they fall below disable_particles_below; state and slot bit-identical
+    to the oracle's."""
+    sc = with_scheme(scenes.small_iisph_box(side=8, jitter=2e-3), integrator="Verlet")
+    sc.params.disable_particles_below = float(np.sort(sc.x[:, 2])[40]) - 1e-6
+    print("compaction with Verlet's slot", flush=True)
+    o, g = refcpu.OracleSystem(sc), native.System(sc)
+    n0 = len(g)
+    o.set_field("slot_x", o.state()["x"])
+    g.set_integrator_slots(g.state()["x"])
+    for k in range(8):
+        o.step(1)
+        g.step_forward_in_time(1)
+        so, sg = o.state(), g.state()
+        for f in ("x", "v", "a", "V", "p", "fluid_id"):
+            assert np.array_equal(sg[f], so[f]), (k, f)
+        slot = np.asarray(g.get_checkpoint().fluid.integrator_position_slots[0],
+                          dtype=np.float64).reshape(-1, 3)
+        assert np.array_equal(slot, o.field("slot_x")), k
+    assert len(g) == len(o) < n0
+
+
 @pytest.mark.parametrize("solver", ["IISPH", "IISPHwOST", "SESPH"])
 def test_compaction_to_no_particles(native, refcpu, solver):
     """Every particle below disable_particles_below: System::new's update

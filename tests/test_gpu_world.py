@@ -106,6 +106,44 @@ def test_world_sesph_static_samples(native):
     run_world_vs_solo(native, scs, 6)
 
 
+def verlet_slot(sim):
+    return np.asarray(sim.get_checkpoint().fluid.integrator_position_slots[0],
+                      dtype=np.float64).reshape(-1, 3)
+
+
+def test_world_mixed_integrators(native):
+    """EulerCromer, ExplicitEuler and Verlet members (IISPH) in one world,
+    Verlet's slot set to the positions after System::new: every member and
+    Verlet's slot bit-identical to the solo runs."""
+    scs = [scenes.small_iisph_box(side=8, seed=1, jitter=2e-3),
+           scenes.small_iisph_box(side=9, seed=2, jitter=2e-3),
+           scenes.small_iisph_box(side=7, seed=3, jitter=2e-3)]
+    scs[1].params.integrator = "ExplicitEuler"
+    scs[2].params.integrator = "Verlet"
+    print("world with mixed integrators", flush=True)
+    solo = [native.System(sc) for sc in scs]
+    mates = [native.System(sc) for sc in scs]
+    for s in (solo[2], mates[2]):
+        s.set_integrator_slots(s.state()["x"])
+    w = native.World(mates)
+    for _ in range(6):
+        for s in solo:
+            s.step_forward_in_time(1)
+        w.step(1)
+        for a, b in zip(solo, mates):
+            assert_same(a, b)
+        assert np.array_equal(verlet_slot(solo[2]), verlet_slot(mates[2]))
+
+
+def test_world_sesph_splitting(native):
+    """SESPHwSplitting members on static samples and on a volume-map box."""
+    scs = [scenes.dam_break_sesph(side=8, seed=0), scenes.small_iisph_box(side=9, seed=2, jitter=2e-3)]
+    for sc in scs:
+        sc.params.solver = "SESPHwSplitting"
+    print("world of SESPHwSplitting members", flush=True)
+    run_world_vs_solo(native, scs, 6)
+
+
 def test_world_compaction(native):
     a = scenes.small_iisph_box(side=8, seed=1, jitter=2e-3)
     a.params.disable_particles_below = float(np.sort(a.x[:, 2])[40]) - 1e-6
