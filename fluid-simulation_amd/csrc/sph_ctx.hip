@@ -700,7 +700,9 @@ struct sph_world {
 };
 
 // continuous batching: FIFO admission under slots and a byte budget, admitted
-// systems resident in one world, finished systems queued for the caller
+// systems resident in one world per pressure solver (a world runs one solver's
+// sequence of launches), finished systems queued for the caller
+constexpr int kServerWorlds = SPH_SOLVER_SESPH_SPLITTING + 1;
 struct sph_server {
     struct Req {
         uint64_t ticket, bytes, steps, done;
@@ -719,7 +721,8 @@ struct sph_server {
         int status;
     };
     std::deque<Failed> failed;       // out of the world after a failed step
-    sph_world *world = nullptr;
+    sph_world *worlds[kServerWorlds] = {};  // by pressure solver, created on first use
+    std::vector<std::pair<int, int>> options;  // set on every world, also on later ones
 };
 
 namespace sph {
@@ -3514,13 +3517,23 @@ static void server_stats(const sph_server *s, sph_server_stats *o) {
     o->device_bytes = g_device_bytes.load();
 }
 
-// the server's world, created on first use
-static sph_world *server_world(sph_server *s) {
-    if (!s->world) {
-        int e = sph_world_create(&s->world);
+static void set_option(Batch &B, int option, int value);
+
+// the server's world for one pressure solver, created on first use with the
+// server's options
+static sph_world *server_world(sph_server *s, int solver) {
+    if (solver < 0 || solver >= kServerWorlds) throw Error(SPH_ERR_ARG, "unknown solver");
+    sph_world *&w = s->worlds[solver];
+    if (!w) {
+        int e = sph_world_create(&w);
         if (e) throw Error(e, sph_last_error());
+        for (const auto &o : s->options) set_option(w->batch, o.first, o.second);
     }
-    return s->world;
+    return w;
+}
+
+static sph_world *world_of(sph_server *s, const sph_ctx *c) {
+    return s->worlds[c->scene.P.solver];
 }
 
 int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out) {
@@ -3536,7 +3549,8 @@ int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out) {
 int sph_server_destroy(sph_server *s) {
     if (!s) return SPH_OK;
     return guarded([&] {
-        if (s->world) (void)sph_world_destroy(s->world);
+        for (sph_world *w : s->worlds)
+            if (w) (void)sph_world_destroy(w);
         delete s;
     });
 }
@@ -3575,7 +3589,7 @@ int sph_server_attach(sph_server *s, uint64_t ticket, sph_ctx *ctx) {
         auto it = std::find_if(s->admitted.begin(), s->admitted.end(),
                                [&](const sph_server::Req &r) { return r.ticket == ticket; });
         if (it == s->admitted.end()) throw Error(SPH_ERR_STATE, "ticket is not awaiting a system");
-        int e = sph_world_attach(server_world(s), ctx);
+        int e = sph_world_attach(server_world(s, ctx->scene.P.solver), ctx);
         if (e) throw Error(e, sph_last_error());
         sph_server::Req r = *it;
         r.ctx = ctx;
@@ -3584,30 +3598,43 @@ int sph_server_attach(sph_server *s, uint64_t ticket, sph_ctx *ctx) {
     });
 }
 
-// One world step of every resident request. A request whose step fails (its
-// pressure solve did not converge within its iteration cap) leaves the world
-// for the failed queue and the others count the step as done; an error that
-// is not confined to one request (a HIP error) fails every resident request.
+// One step of every resident request: one world step per pressure solver that
+// has resident requests. A request whose step fails (its pressure solve did
+// not converge within its iteration cap) leaves its world for the failed
+// queue and the others count the step as done; an error that is not confined
+// to one request (a HIP error) fails every request of that world, the other
+// worlds complete the step, and the error is returned.
 int sph_server_step(sph_server *s, sph_server_stats *out) {
     return guarded([&] {
         if (!s) throw Error(SPH_ERR_ARG, "null argument");
         if (!s->resident.empty()) {
-            int e = sph_world_step(s->world, 1);
-            bool confined = e == SPH_OK;
-            if (e) {
+            int err = SPH_OK;
+            std::string err_msg;
+            for (sph_world *w : s->worlds) {
+                if (!w || w->members.empty()) continue;
+                int e = sph_world_step(w, 1);
+                if (e == SPH_OK) continue;
                 std::string msg = sph_last_error();
-                for (const sph_server::Req &r : s->resident)
-                    if (r.ctx->scene.failed) confined = true;
-                if (!confined) {  // every resident request fails with the world's error
-                    (void)hipStreamSynchronize(s->world->batch.s);
-                    for (sph_server::Req &r : s->resident) {
-                        auto &m = s->world->members;
-                        m.erase(std::remove(m.begin(), m.end(), r.ctx), m.end());
-                        r.ctx->scene.in_world = false;
-                        s->failed.push_back({r, e});
+                bool confined = false;
+                for (sph_ctx *c : w->members)
+                    if (c->scene.failed) confined = true;
+                if (confined) continue;
+                // every request of this world fails with the world's error
+                (void)hipStreamSynchronize(w->batch.s);
+                std::vector<sph_server::Req> others;
+                for (sph_server::Req &r : s->resident) {
+                    if (world_of(s, r.ctx) != w) {
+                        others.push_back(r);
+                        continue;
                     }
-                    s->resident.clear();
-                    throw Error(e, msg);
+                    r.ctx->scene.in_world = false;
+                    s->failed.push_back({r, e});
+                }
+                w->members.clear();
+                s->resident.swap(others);
+                if (err == SPH_OK) {
+                    err = e;
+                    err_msg = msg;
                 }
             }
             s->world_steps++;
@@ -3618,7 +3645,7 @@ int sph_server_step(sph_server *s, sph_server_stats *out) {
                     keep.push_back(r);
                     continue;
                 }
-                e = sph_world_detach(s->world, r.ctx);
+                int e = sph_world_detach(world_of(s, r.ctx), r.ctx);
                 if (e) throw Error(e, sph_last_error());
                 if (f)
                     s->failed.push_back({r, f});
@@ -3626,13 +3653,17 @@ int sph_server_step(sph_server *s, sph_server_stats *out) {
                     s->finished.push_back(r);
             }
             s->resident.swap(keep);
-            // the world's launch buffers only grow; once the resident
-            // particles need less than a quarter of them (a large request
-            // left), free them, and the next step reserves what it needs
-            size_t n = 0;
-            for (sph_server::Req &r : s->resident) n += r.ctx->scene.n;
-            Batch &B = s->world->batch;
-            if (B.perm.cap > 4 * std::max<size_t>(n, (size_t)1 << 16)) B.release_scratch();
+            // a world's launch buffers only grow; once its resident particles
+            // need less than a quarter of them (a large request left), free
+            // them, and the next step reserves what it needs
+            for (sph_world *w : s->worlds) {
+                if (!w) continue;
+                size_t n = 0;
+                for (sph_ctx *c : w->members) n += c->scene.n;
+                Batch &B = w->batch;
+                if (B.perm.cap > 4 * std::max<size_t>(n, (size_t)1 << 16)) B.release_scratch();
+            }
+            if (err != SPH_OK) throw Error(err, err_msg);
         }
         if (out) server_stats(s, out);
     });
@@ -3722,6 +3753,18 @@ int sph_world_profile_read(sph_world *w, int kernel, uint64_t *launches, uint64_
     });
 }
 
+// the errors set_option raises, without a batch
+static void check_option(int option, int value) {
+    const bool shape = option == SPH_OPT_SHAPE ||
+                       (option >= SPH_OPT_SHAPE_VOLUME && option < SPH_OPT_SHAPE_VOLUME + kPairPasses);
+    if (option != SPH_OPT_CACHE_GRADIENTS && option != SPH_OPT_AUTOTUNE && option != SPH_OPT_GRAPHS &&
+        option != SPH_OPT_SWEEP && !shape)
+        throw Error(SPH_ERR_ARG, "unknown option");
+    if (option == SPH_OPT_AUTOTUNE && value < 0) throw Error(SPH_ERR_ARG, "steps per shape must be >= 0");
+    if (option == SPH_OPT_SWEEP && value < 0) throw Error(SPH_ERR_ARG, "sweep option must be >= 0");
+    if (shape && (value < 0 || value >= kPairVariants)) throw Error(SPH_ERR_ARG, "no such code shape");
+}
+
 static void set_option(Batch &B, int option, int value) {
     B.sync();
     if (option == SPH_OPT_CACHE_GRADIENTS) {
@@ -3806,7 +3849,10 @@ int sph_world_set_option(sph_world *w, int option, int value) {
 int sph_server_set_option(sph_server *s, int option, int value) {
     return guarded([&] {
         if (!s) throw Error(SPH_ERR_ARG, "null argument");
-        set_option(server_world(s)->batch, option, value);
+        check_option(option, value);
+        for (sph_world *w : s->worlds)
+            if (w) set_option(w->batch, option, value);
+        s->options.emplace_back(option, value);
     });
 }
 

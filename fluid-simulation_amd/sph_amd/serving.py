@@ -214,7 +214,8 @@ def _check_fits(req: Request, budget: int):
 class BatchedServer:
     """Continuous batching: the library's server (sph_server_*) admits requests
     in arrival order under slots and the HBM budget and steps the admitted
-    scenes together in one world; this loop feeds it arrivals, builds each
+    scenes together in one world per pressure solver; this loop feeds it
+    arrivals, builds each
     admitted request's System and collects finished ones."""
 
     def __init__(self, hbm_budget_bytes: int, max_slots: int = 64,

@@ -605,7 +605,7 @@ def build_volume_map(sdf, h: float, x_min, n, dx: float, gl_x, gl_w, system=None
 class Server:
     """Continuous batching in the library (sph_server_*): FIFO admission under
     slots and a device-memory budget, admitted systems stepped together in one
-    world, finished systems handed back. Admission and its bookkeeping touch
+    world per pressure solver, finished systems handed back. Admission and its bookkeeping touch
     no device; a request's System is built after it is admitted."""
 
     def __init__(self, budget_bytes: int, max_slots: int):
@@ -640,7 +640,7 @@ class Server:
         return out.as_dict()
 
     def set_option(self, option: int, value: int):
-        """An implementation option (N.OPT_*) of the server's world."""
+        """An implementation option (N.OPT_*) of the server's worlds."""
         N.check(N.lib().sph_server_set_option(self.h, int(option), int(value)))
 
     def poll(self):

@@ -340,17 +340,18 @@ int sph_world_profile_read(sph_world *world, int kernel, uint64_t *launches,
  * No counterpart in the reference, whose worker thread steps one system at a
  * time (worker_loop, sci-phi-backend/src/lib.rs:507). A server admits requests in submission
  * order while a slot and the HBM budget allow, keeps admitted systems resident
- * in one world and steps them together, and hands back each system once it has
- * taken its requested steps.
+ * in one world per pressure solver and steps them together, and hands back
+ * each system once it has taken its requested steps.
  * Protocol, per scheduling round: sph_server_submit for arrived requests
  * (a device-memory estimate and a step count; nothing is allocated);
  * sph_server_admit while it returns 1, building each admitted request's
  * context only then (so allocations follow admission) and handing it over
- * with sph_server_attach; sph_server_step (one world step, then finished
+ * with sph_server_attach; sph_server_step (one step of every world, then finished
  * systems leave the world); sph_server_poll while it returns 1, which returns
  * a finished context to the caller and releases its slot and budget.
  * Creating a server and the submit/admit bookkeeping touch no device; the
- * world is created at the first attach. One thread at a time. */
+ * worlds are created at the first attach of a system of their solver. One
+ * thread at a time. */
 typedef struct sph_server sph_server;
 typedef struct sph_server_stats {
     uint64_t queued;          /* submitted, not yet admitted */
@@ -363,7 +364,7 @@ typedef struct sph_server_stats {
     uint64_t device_bytes;    /* device memory the library holds (sph_device_bytes_held) */
 } sph_server_stats;
 int sph_server_create(uint64_t budget_bytes, int max_slots, sph_server **out);
-/* frees the world; contexts still attached are detached, not destroyed */
+/* frees the worlds; contexts still attached are detached, not destroyed */
 int sph_server_destroy(sph_server *server);
 /* SPH_ERR_ARG if device_bytes exceeds the whole budget or steps is 0 */
 int sph_server_submit(sph_server *server, uint64_t device_bytes, uint64_t steps,
@@ -371,18 +372,20 @@ int sph_server_submit(sph_server *server, uint64_t device_bytes, uint64_t steps,
 /* returns 1 and the oldest queued ticket if it fits (taking its slot and
  * bytes), 0 if the queue is empty or its head does not fit */
 int sph_server_admit(sph_server *server, uint64_t *ticket);
-/* ctx (after sph_system_init) serves an admitted ticket; it joins the world */
+/* ctx (after sph_system_init) serves an admitted ticket; it joins the world
+ * of its pressure solver */
 int sph_server_attach(sph_server *server, uint64_t ticket, sph_ctx *ctx);
 /* gives up an admitted ticket whose system could not be built (e.g. its
  * System::new update failed), releasing its slot and bytes */
 int sph_server_cancel(sph_server *server, uint64_t ticket);
-/* one world step if any system is resident; systems that have taken their
- * steps leave the world for the finished queue. A system whose step fails
- * (its pressure solve does not meet the termination test within its
- * SPH_OPT_MAX_ITERATIONS cap) leaves for the failed queue and the others
+/* one step of every world that holds a resident system; systems that have
+ * taken their steps leave their world for the finished queue. A system whose
+ * step fails (its pressure solve does not meet the termination test within
+ * its SPH_OPT_MAX_ITERATIONS cap) leaves for the failed queue and the others
  * complete the step (the call returns SPH_OK); an error not confined to one
- * system (a HIP error) moves every resident system to the failed queue and is
- * returned. *out may be NULL. */
+ * system (a HIP error) moves every system of that world to the failed queue,
+ * the other worlds complete the step, and the error is returned. *out may be
+ * NULL. */
 int sph_server_step(sph_server *server, sph_server_stats *out);
 /* returns 1 with the oldest failed ticket, its context (detached, now the
  * caller's, its state not valid) and the failure status, releasing its slot
@@ -393,7 +396,8 @@ int sph_server_poll_failed(sph_server *server, uint64_t *ticket, sph_ctx **ctx, 
  * the caller's), releasing its slot and bytes; 0 if none has finished */
 int sph_server_poll(sph_server *server, uint64_t *ticket, sph_ctx **ctx);
 int sph_server_read_stats(const sph_server *server, sph_server_stats *out);
-/* an implementation option (SPH_OPT_*) of the server's world */
+/* an implementation option (SPH_OPT_*) of the server's worlds, present and
+ * future */
 int sph_server_set_option(sph_server *server, int option, int value);
 /* ---- sensor queries -------------------------------------------------------------
  * SPHSystem::get_quantity_at_positions (sph/mod.rs:241-341): a scalar field

@@ -36,6 +36,32 @@ def test_batched_stream_matches_solo_runs():
     assert s["requests"] == 7 and s["p99_step_ms"] >= s["p50_step_ms"] > 0
 
 
+def test_mixed_solvers_stream_matches_solo_runs():
+    """Requests of the four pressure solvers (IISPH, IISPHwOST with
+    TakePredicted, SESPH with ExplicitEuler, SESPHwSplitting) in one stream:
+    the server keeps one world per solver and steps them all in each server
+    step; every request equals its solo run bit for bit."""
+    from sph_amd import scenes
+    kinds = [("IISPH", "EulerCromer"), ("IISPHwOST", "TakePredicted"),
+             ("SESPH", "ExplicitEuler"), ("SESPHwSplitting", "EulerCromer"),
+             ("IISPH", "EulerCromer"), ("SESPHwSplitting", "EulerCromer")]
+    stream = []
+    for k, (solver, integ) in enumerate(kinds):
+        sc = scenes.serving_scene(1500 + 700 * k, 1000.0, 0.01, "box" if k % 2 else "sphere", 40 + k)
+        sc.params.solver, sc.params.integrator = solver, integ
+        stream.append(serving.Request(k, 0.0, sc, 8 + k))
+    print(f"mixed-solver stream: {[r.scene.n for r in stream]} particles", flush=True)
+    rep = serving.BatchedServer(1 << 34, max_slots=8, keep_state=True).run(stream)
+    assert set(rep.results) == {r.rid for r in stream}
+    assert rep.peak_resident == len(stream)  # all admitted at once, four worlds
+    for req in stream:
+        st, meas = solo_result(req)
+        got = rep.results[req.rid]
+        for k in ("x", "v", "a", "V", "p"):
+            assert np.array_equal(got.state[k], st[k]), (req.rid, k)
+        assert got.measurement == meas, req.rid
+
+
 def test_sequential_baseline_matches_batched():
     stream = serving.make_stream(n_requests=3, rate=1000.0, n_min=800, n_max=3000, steps=6,
                                  seed=4)
