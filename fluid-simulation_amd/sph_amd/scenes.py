@@ -642,9 +642,9 @@ def sample_body_scene(side=(12, 12, 6), seed=6, dx=0.02, solver="IISPH") -> Scen
     pos = lattice_block([0.0, 0.0, 0.0], side, dx, rng, 1e-4)
     x, v, m, fid = _fluid([(pos, 1000.0, 0)], dx)
     h = 2 * dx
-    if solver == "SESPH":
+    if solver in ("SESPH", "SESPHwSplitting"):
         p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, time_increment=2.5e-4,
-                   solver="SESPH", integrator="EulerCromer", stiffness=1000.0)
+                   solver=solver, integrator="EulerCromer", stiffness=1000.0)
     else:
         p = Params(kernel_support_radius=h, rest_density_grid_spacing=dx, cfl=True,
                    max_time_increment=1e-3, cfl_number=0.4, solver=solver,

@@ -120,9 +120,9 @@ def test_volume_and_boundary_samples(native, refcpu):
 
 
 def test_nonpressure_and_pressure_solve(native, refcpu):
-    for solver in ("SESPH", "IISPH", "IISPHwOST"):
+    for solver in ("SESPH", "IISPH", "IISPHwOST", "SESPHwSplitting"):
         sc = scenes.small_iisph_box(side=10, solver=solver, jitter=2e-3)
-        if solver == "SESPH":
+        if solver in ("SESPH", "SESPHwSplitting"):
             sc.params.integrator = "EulerCromer"
         o = refcpu.OracleSystem(sc)
         o.step(3)
@@ -359,9 +359,9 @@ def test_headline_scene_bit_identical_at_full_size(native, refcpu, case):
 
 
 def test_per_trait_calls_bit_identical(native, refcpu):
-    for solver in ("SESPH", "IISPH", "IISPHwOST"):
+    for solver in ("SESPH", "IISPH", "IISPHwOST", "SESPHwSplitting"):
         sc = scenes.small_iisph_box(side=10, solver=solver, jitter=2e-3)
-        if solver == "SESPH":
+        if solver in ("SESPH", "SESPHwSplitting"):
             sc.params.integrator = "EulerCromer"
         o = refcpu.OracleSystem(sc)
         o.step(3)
@@ -375,7 +375,7 @@ def test_per_trait_calls_bit_identical(native, refcpu):
         assert np.array_equal(res["p"], st["p"]) and np.array_equal(res["a"], st["a"])
 
 
-@pytest.mark.parametrize("solver", ["IISPH", "SESPH"])
+@pytest.mark.parametrize("solver", ["IISPH", "SESPH", "SESPHwSplitting"])
 def test_moving_static_sample_body(native, refcpu, solver):
     """A StaticSampleBoundary body (static_sample_boundary.rs:79-130): samples
     on a cube's surface follow its rigid motion (update_positions_and_velocities
