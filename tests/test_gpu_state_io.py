@@ -37,10 +37,19 @@ def bits(a):
     return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
 
 
+def scheme(sc, solver=None, integrator=None):
+    sc.params.solver = solver or sc.params.solver
+    sc.params.integrator = integrator or sc.params.integrator
+    return sc
+
+
 SCENES = {
     "iisph_box": lambda: scenes.small_iisph_box(side=10, jitter=2e-3),
     "ost_box": lambda: scenes.small_iisph_box(side=10, solver="IISPHwOST", jitter=2e-3),
     "sesph": lambda: scenes.dam_break_sesph(side=8),
+    "splitting": lambda: scheme(scenes.dam_break_sesph(side=8), solver="SESPHwSplitting"),
+    "explicit_euler": lambda: scheme(scenes.small_iisph_box(side=10, jitter=2e-3),
+                                     integrator="ExplicitEuler"),
 }
 
 
