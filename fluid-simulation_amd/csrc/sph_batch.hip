@@ -75,6 +75,25 @@ __device__ __forceinline__ GLOBAL_AS T *gm(T *p) {
     return (GLOBAL_AS T *)p;
 }
 
+// a[q * stride] summed for q = q0, q0 + kBlock, ... < q1, added in that order
+// (one thread's share of a fixed-shape fold); eight loads are issued before
+// their additions, which keeps the order and so the bits
+__device__ __forceinline__ double strided_sum(const double *a_, uint32_t q0, uint32_t q1,
+                                              size_t stride) {
+    const GLOBAL_AS double *a = gm(a_);
+    double acc = 0.0;
+    uint32_t q = q0;
+    for (; q1 > 7u * kBlock && q < q1 - 7u * kBlock; q += 8u * kBlock) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = a[(size_t)(q + u * kBlock) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += v[u];
+    }
+    for (; q < q1; q += kBlock) acc += a[(size_t)q * stride];
+    return acc;
+}
+
 // packed per-particle records gathered by the neighbor loops (one record
 // instead of separate 8-byte loads): q4[i] = {x, y, z, V} (32 bytes),
 // pa3[i] = {pressure acceleration x, y, z} and v3[i] = {velocity x, y, z}
@@ -1215,10 +1234,7 @@ __global__ void kb_fold_sum(const double *partials, const uint32_t *blk, int ns,
                             double *out) {
     __shared__ double lds[kBlock];
     const int s = blockIdx.x / rows, r = blockIdx.x % rows;
-    double acc = 0.0;
-    for (uint32_t q = blk[s] + threadIdx.x; q < blk[s + 1]; q += kBlock)
-        acc += partials[(size_t)q * rows + r];
-    double v[1] = {acc};
+    double v[1] = {strided_sum(partials + r, blk[s] + threadIdx.x, blk[s + 1], (size_t)rows)};
     block_sum<1>(v, lds);
     if (threadIdx.x == 0) out[(size_t)s * rows + r] = v[0];
 }
@@ -1229,10 +1245,8 @@ __device__ __forceinline__ void fold_converge_impl(const BatchDev &B, int after,
     const int sk = blockIdx.x;
     if (DESC && sk >= B.ns) return;  // grid rounded up (uniform per workgroup)
     const int s = B.smap ? (int)gm(B.smap)[sk] : sk;
-    double acc = 0.0;  // the shape of kb_fold_sum with one row
-    for (uint32_t q = gm(B.pblk)[sk] + threadIdx.x; q < gm(B.pblk)[sk + 1]; q += kBlock)
-        acc += gm(B.partials)[q];
-    double r[1] = {acc};
+    // the shape of kb_fold_sum with one row
+    double r[1] = {strided_sum(B.partials, gm(B.pblk)[sk] + threadIdx.x, gm(B.pblk)[sk + 1], 1)};
     block_sum<1>(r, lds);
     if (threadIdx.x == 0) {
         const size_t slot = (size_t)k * B.ns_all + s;
