@@ -36,12 +36,19 @@ def queries(x, seed):
     return np.concatenate(q).astype(np.float32)
 
 
-@pytest.mark.parametrize("case", ["iisph_box", "ost_box", "rigid", "sesph"])
+def splitting_scene():
+    sc = scenes.dam_break_sesph(side=10)
+    sc.params.solver = "SESPHwSplitting"
+    return sc
+
+
+@pytest.mark.parametrize("case", ["iisph_box", "ost_box", "rigid", "sesph", "splitting"])
 def test_quantities_bit_identical(native, case):
     sc = {"iisph_box": lambda: scenes.small_iisph_box(side=12, jitter=2e-3),
           "ost_box": lambda: scenes.small_iisph_box(side=12, solver="IISPHwOST", jitter=2e-3),
           "rigid": scenes.small_rigid_scene,
-          "sesph": lambda: scenes.dam_break_sesph(side=10)}[case]()
+          "sesph": lambda: scenes.dam_break_sesph(side=10),
+          "splitting": splitting_scene}[case]()
     o, g = refcpu.OracleSystem(sc), native.System(sc)
     for step in range(3):
         o.step(2)
