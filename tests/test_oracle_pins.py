@@ -607,26 +607,32 @@ def test_sesph_splitting_compressed_pair():
     V = m / rho*; with the particles approaching (v*_0 = +u e_x,
     v*_1 = -u e_x) the density gains dt (v*_i - v*_j) . grad W_ij. The
     spacing is 0.9 instead of the reference test's 0.3, whose rest volume
-    leaves both pressures at zero."""
-    h, dx, k, dt, m = 1.0, 0.9, 500.0, 0.05, 0.5
+    leaves both pressures at zero. With unequal masses (0.5 and 0.8) each
+    neighbor term carries the neighbor's mass."""
+    h, dx, k, dt = 1.0, 0.9, 500.0, 0.05
     V0 = (dx * dx) * dx
     x = np.array([[0.0, 0.0, 0.0], [0.3, 0.0, 0.0]])
-    for u in (0.0, 0.4):
-        v = np.array([[u, 0.0, 0.0], [-u, 0.0, 0.0]])
-        o = make(x, v=v, m=np.full(2, m), h=h, dx=dx, stiffness=k, solver="SESPHwSplitting")
-        o.set_field("v", v)
-        o.set_field("a", np.zeros((2, 3)))
-        o.set_field("dt", dt)
-        o.call("sesph_splitting")
-        p = o.field("p")
-        for i in range(2):
-            r = x[i] - x[1 - i]
-            rho = m * w_kernel(np.zeros(3), h) + m * w_kernel(r, h) + \
-                dt * float((v[i] - v[1 - i]) @ grad_w(r, h))
-            exp = k * max(V0 / (m / rho) - 1.0, 0.0)
-            assert abs(p[i] - exp) <= 1e-9 * max(exp, 1.0), (u, i, p[i], exp)
-        static = k * max(V0 / (m / (m * w_kernel(np.zeros(3), h) + m * w_kernel(x[1], h))) - 1, 0)
-        assert static > 0 and (p[0] > static if u > 0 else p[0] == static)
+    for ms in ((0.5, 0.5), (0.5, 0.8)):
+        m = np.array(ms)
+        for u in (0.0, 0.4):
+            v = np.array([[u, 0.0, 0.0], [-u, 0.0, 0.0]])
+            o = make(x, v=v, m=m, h=h, dx=dx, stiffness=k, solver="SESPHwSplitting")
+            o.set_field("v", v)
+            o.set_field("a", np.zeros((2, 3)))
+            o.set_field("dt", dt)
+            o.call("sesph_splitting")
+            p = o.field("p")
+            for i in range(2):
+                r = x[i] - x[1 - i]
+                rho = m[i] * w_kernel(np.zeros(3), h) + m[1 - i] * w_kernel(r, h) + \
+                    dt * float((v[i] - v[1 - i]) @ grad_w(r, h))
+                exp = k * max(V0 / (m[i] / rho) - 1.0, 0.0)
+                assert abs(p[i] - exp) <= 1e-9 * max(exp, 1.0), (ms, u, i, p[i], exp)
+            if ms[0] == ms[1]:
+                mm = ms[0]
+                static = k * max(V0 / (mm / (mm * w_kernel(np.zeros(3), h) +
+                                             mm * w_kernel(x[1], h))) - 1, 0)
+                assert static > 0 and (p[0] > static if u > 0 else p[0] == static)
 
 
 def test_sesph_splitting_boundary_mirrors_the_particle_mass():
