@@ -11,6 +11,7 @@ over. Every call runs on the GPU through libsphhip.so.
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -187,7 +188,16 @@ class System:
         return self._ctx.h
 
     def step_forward_in_time(self, n: int = 1):
-        N.check(N.lib().sph_step(self._ctx.h, int(n), None))
+        """n x SPHSystem::step_forward_in_time (sph/mod.rs:125-144). The wall
+        clock of the last step is kept for recordings, as the reference keeps
+        time_step_wall_clock_time (sph/mod.rs:143)."""
+        n = int(n)
+        if n > 1:
+            N.check(N.lib().sph_step(self._ctx.h, n - 1, None))
+        t0 = time.perf_counter()
+        N.check(N.lib().sph_step(self._ctx.h, min(n, 1), None))
+        if n >= 1:
+            self._wall = time.perf_counter() - t0
 
     def __len__(self):
         n = C.c_size_t()
@@ -383,12 +393,18 @@ class System:
                 "solver_iterations": m["solver_iterations"],
                 "predicted_density_error": m["predicted_density_error"]}
         P = self.scene.params
-        meas.update(stiffness=P.stiffness, fluid_viscosity=P.fluid_viscosity,
+        # SolverMeasurementInfo: the state-equation solvers report their
+        # stiffness only, IISPH and IISPHwOST their target error and
+        # relaxation factor (sesph.rs:69-74, iisph.rs:98-106), the rest 0
+        iterative = P.solver in ("IISPH", "IISPHwOST")
+        meas.update(stiffness=0.0 if iterative else P.stiffness,
+                    fluid_viscosity=P.fluid_viscosity,
                     boundary_viscosity=P.boundary_viscosity,
                     rest_density_grid_spacing=P.rest_density_grid_spacing,
                     kernel_support_radius=P.kernel_support_radius,
-                    target_density_error=P.target_density_error,
-                    relaxation_factor=P.relaxation_factor)
+                    target_density_error=P.target_density_error if iterative else 0.0,
+                    relaxation_factor=P.relaxation_factor if iterative else 0.0,
+                    time_step_wall_clock_time=getattr(self, "_wall", 0.0))
         if fluid["kind"] == "Samples":
             st = self.state()
             col = fluid.get("coloring", {"kind": "Uniform"})

@@ -190,10 +190,31 @@ def test_recording_of_a_run(native, tmp_path):
     st = sim.state()
     m = sim.take_measurement()
     assert last["measurement"]["solver_iterations"] == m["solver_iterations"]
+    # SolverMeasurementInfo of IISPH (iisph.rs:98-106): no stiffness
+    mm = last["measurement"]
+    assert mm["stiffness"] == 0.0 and mm["time_step_wall_clock_time"] > 0.0
+    assert mm["target_density_error"] == sc.params.target_density_error
+    assert mm["relaxation_factor"] == sc.params.relaxation_factor
     assert steps[1]["fluid"]["positions"].shape == (sc.n, 3)
     dens = steps[1]["fluid"]["coloring"]["data"]
     assert dens.dtype == np.float32 and np.all(dens > 0)
     assert os.path.getsize(path) > 0 and st["x"].shape == (sc.n, 3)
+
+
+@pytest.mark.parametrize("solver", ["SESPH", "SESPHwSplitting"])
+def test_recorded_measurement_of_state_equation_solvers(native, solver):
+    """SolverMeasurementInfo of SESPH and SESPHwSplitting (sesph.rs:69-74,
+    sesph_with_splitting.rs:76-81): the stiffness, and zero iterations, target
+    error, relaxation factor and predicted error."""
+    sc = scenes.dam_break_sesph(side=6)
+    sc.params.solver = solver
+    sim = native.System(sc)
+    sim.step_forward_in_time(2)
+    m = sim.time_step_info()["measurement"]
+    assert m["stiffness"] == sc.params.stiffness > 0
+    for k in ("target_density_error", "relaxation_factor", "predicted_density_error",
+              "solver_iterations"):
+        assert m[k] == 0, k
 
 
 def test_fixed_step_time_is_offset_plus_steps_times_dt(native, tmp_path):
