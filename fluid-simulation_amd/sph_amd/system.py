@@ -303,6 +303,37 @@ class System:
         msg = N.lib().sph_last_error() if v.value else b""
         return int(v.value), (msg.decode() if msg else "")
 
+    # ---- the SPHSystem trait's accessors (sph/mod.rs:40-71, 107-238, 358-360) --
+    def time(self) -> float:
+        """``SPHSystem::time`` (sph/mod.rs:107-116)."""
+        return float(self.take_measurement()["time"])
+
+    def time_steps_propagated(self) -> int:
+        """``SPHSystem::time_steps_propagated`` (sph/mod.rs:118-120)."""
+        return int(self.take_measurement()["steps"])
+
+    def get_fluid_ids(self) -> np.ndarray:
+        """``SPHSystem::get_fluid_ids`` (sph/mod.rs:186-188)."""
+        return self.state()["fluid_id"]
+
+    def get_fluid_pos(self) -> np.ndarray:
+        """``SPHSystem::get_fluid_pos`` (sph/mod.rs:190-196): positions as f32."""
+        return self.state()["x"].astype(np.float32)
+
+    def get_fluid_checkpoint(self):
+        """``SPHSystem::get_fluid_checkpoint`` (sph/mod.rs:198-200)."""
+        return self.get_checkpoint().fluid
+
+    def get_boundary_checkpoint(self) -> list:
+        """``SPHSystem::get_boundary_checkpoint`` (sph/mod.rs:358-360): one
+        entry per boundary, the rigid state of a dynamic one, else None."""
+        return self.get_checkpoint().dynamic_states
+
+    def get_quantity_of_fluid_samples(self, quantity) -> np.ndarray:
+        """``SPHSystem::get_quantity_of_fluid_samples`` (sph/mod.rs:202-238)."""
+        q = N.QUANTITIES[quantity] if not isinstance(quantity, str) else quantity
+        return self.quantity_of_fluid_samples(q)
+
     # ---- checkpoints (sph/mod.rs:362-378, 884-915) -------------------------
     def get_checkpoint(self):
         """``SystemCheckpoint`` of the current state (fluid in the reference's

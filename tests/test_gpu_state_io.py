@@ -201,6 +201,27 @@ def test_recording_of_a_run(native, tmp_path):
     assert os.path.getsize(path) > 0 and st["x"].shape == (sc.n, 3)
 
 
+def test_trait_accessors(native):
+    """The SPHSystem trait's accessors (sph/mod.rs:40-71, 107-238): time, step
+    count, fluid ids and f32 positions in the reference's particle order, the
+    fluid and boundary checkpoints, per-particle quantities."""
+    sc = scenes.small_rigid_scene()
+    sim = native.System(sc)
+    sim.step_forward_in_time(2)
+    st, m = sim.state(), sim.take_measurement()
+    assert sim.time() == m["time"] > 0 and sim.time_steps_propagated() == 2
+    assert np.array_equal(sim.get_fluid_ids(), st["fluid_id"])
+    pos = sim.get_fluid_pos()
+    assert pos.dtype == np.float32 and np.array_equal(pos, st["x"].astype(np.float32))
+    f = sim.get_fluid_checkpoint()
+    assert np.array_equal(np.asarray(f.position).reshape(-1, 3), st["x"])
+    b = sim.get_boundary_checkpoint()
+    assert len(b) == len(sc.boundaries) and any(e is not None for e in b)
+    rho = sim.get_quantity_of_fluid_samples("Density")
+    assert np.array_equal(rho, (st["m"] / st["V"]).astype(np.float32))
+    assert np.array_equal(sim.get_quantity_of_fluid_samples(2), rho)
+
+
 @pytest.mark.parametrize("solver", ["SESPH", "SESPHwSplitting"])
 def test_recorded_measurement_of_state_equation_solvers(native, solver):
     """SolverMeasurementInfo of SESPH and SESPHwSplitting (sesph.rs:69-74,
