@@ -205,9 +205,27 @@ class System:
         return int(n.value)
 
     def take_measurement(self) -> dict:
+        """``SPHSystem::take_measurement`` (sph/mod.rs:147-184): the library's
+        sums and counters (also eqs1_iterations, steps and num_particles), and
+        the reference's Measurement fields. The parameters enter through the
+        solver's SolverMeasurementInfo: the state-equation solvers report their
+        stiffness only, IISPH and IISPHwOST their target error and relaxation
+        factor (sesph.rs:69-74, iisph.rs:98-106), the others 0. fluid_depth is
+        never set by the reference (sph/mod.rs:738-746). The step's wall clock
+        is left to recordings (time_step_info), so that two runs of one scene
+        give equal measurements."""
         m = N.SphMeasurement()
         N.check(N.lib().sph_measure(self._ctx.h, C.byref(m)))
-        return m.as_dict()
+        out = m.as_dict()
+        P = self.scene.params
+        iterative = P.solver in ("IISPH", "IISPHwOST")
+        out.update(stiffness=0.0 if iterative else P.stiffness,
+                   fluid_viscosity=P.fluid_viscosity, boundary_viscosity=P.boundary_viscosity,
+                   fluid_depth=0.0, rest_density_grid_spacing=P.rest_density_grid_spacing,
+                   kernel_support_radius=P.kernel_support_radius,
+                   target_density_error=P.target_density_error if iterative else 0.0,
+                   relaxation_factor=P.relaxation_factor if iterative else 0.0)
+        return out
 
     def state(self) -> dict:
         n = len(self)
@@ -419,23 +437,9 @@ class System:
         boundary = {"kind": "Samples", "positions": [m][3], "coloring": ...}."""
         fluid = fluid or {"kind": "Samples", "coloring": {"kind": "Uniform"}}
         m = self.take_measurement()
-        meas = {"time": m["time"], "density": m["density"], "density_error": m["density_error"],
-                "kinetic_energy": m["kinetic_energy"], "time_step_size": m["time_step_size"],
-                "solver_iterations": m["solver_iterations"],
-                "predicted_density_error": m["predicted_density_error"]}
-        P = self.scene.params
-        # SolverMeasurementInfo: the state-equation solvers report their
-        # stiffness only, IISPH and IISPHwOST their target error and
-        # relaxation factor (sesph.rs:69-74, iisph.rs:98-106), the rest 0
-        iterative = P.solver in ("IISPH", "IISPHwOST")
-        meas.update(stiffness=0.0 if iterative else P.stiffness,
-                    fluid_viscosity=P.fluid_viscosity,
-                    boundary_viscosity=P.boundary_viscosity,
-                    rest_density_grid_spacing=P.rest_density_grid_spacing,
-                    kernel_support_radius=P.kernel_support_radius,
-                    target_density_error=P.target_density_error if iterative else 0.0,
-                    relaxation_factor=P.relaxation_factor if iterative else 0.0,
-                    time_step_wall_clock_time=getattr(self, "_wall", 0.0))
+        from .state_io import MEASUREMENT_FIELDS
+        meas = {k: m[k] for k, _ in MEASUREMENT_FIELDS if k in m}
+        meas["time_step_wall_clock_time"] = getattr(self, "_wall", 0.0)
         if fluid["kind"] == "Samples":
             st = self.state()
             col = fluid.get("coloring", {"kind": "Uniform"})
