@@ -399,6 +399,19 @@ class System:
         N.check(N.lib().sph_fluid_upload_slots(self._ctx.h, N.dptr(xp) if xp is not None else None,
                                                N.dptr(vp) if vp is not None else None))
 
+    def continue_from_checkpoint(self, cp):
+        """``SPHSystem::continue_from_checkpoint`` (sph/mod.rs:362-378) on this
+        system: its scene's parameters and boundaries with the checkpoint's
+        fluid, counters and rigid-body states, after the update() that resuming
+        runs. The device context is built anew (as from_checkpoint does), so
+        implementation options set on the old one are not carried over; not
+        for a system attached to a world."""
+        other = System.from_checkpoint(self.scene, cp)
+        self._ctx, other._ctx = other._ctx, self._ctx
+        self.scene, self._keep = other.scene, other._keep
+        self.__dict__.pop("_wall", None)
+        other._ctx.close()
+
     @classmethod
     def from_checkpoint(cls, scene: Scene, cp) -> "System":
         """``SPHSystem::continue_from_checkpoint``: a system of ``scene``'s

@@ -217,6 +217,12 @@ def test_trait_accessors(native):
     assert np.array_equal(np.asarray(f.position).reshape(-1, 3), st["x"])
     b = sim.get_boundary_checkpoint()
     assert len(b) == len(sc.boundaries) and any(e is not None for e in b)
+    # continue_from_checkpoint on a live system: the saved state and counters
+    cp = sim.get_checkpoint()
+    sim.step_forward_in_time(3)
+    sim.continue_from_checkpoint(cp)
+    assert sim.time_steps_propagated() == 2
+    assert np.array_equal(sim.state()["x"], st["x"])
     rho = sim.get_quantity_of_fluid_samples("Density")
     assert np.array_equal(rho, (st["m"] / st["V"]).astype(np.float32))
     assert np.array_equal(sim.get_quantity_of_fluid_samples(2), rho)
