@@ -188,7 +188,8 @@ def serving_bench(args):
     t = time.perf_counter()
     stream = serving.make_stream(n_requests=args.requests, rate=args.rate,
                                  n_min=args.n_min, n_max=args.n_max, steps=args.serve_steps,
-                                 seed=0)
+                                 seed=0,
+                                 schemes=serving.MIXED_SCHEMES if args.mixed_solvers else None)
     log(f"[bench] serving stream: {len(stream)} requests, "
         f"{sum(r.scene.n for r in stream)} particles, built in {time.perf_counter() - t:.1f} s")
     free, total = system.device_memory()
@@ -225,7 +226,9 @@ def serving_bench(args):
         "data": "synthetic (seeded serving stream)",
         "config": {"workload": "serving_stream", "requests": args.requests,
                    "arrival_rate_per_s": args.rate, "particles": [args.n_min, args.n_max],
-                   "steps_per_request": args.serve_steps, "solver": "IISPH",
+                   "steps_per_request": args.serve_steps,
+                   "solver": ("IISPH, IISPHwOST, SESPH, SESPHwSplitting in turn"
+                              if args.mixed_solvers else "IISPH"),
                    "hbm_budget_bytes": budget, "max_slots": args.slots,
                    "world_graphs": not args.no_graphs,
                    "device_memory_bytes": total},
@@ -631,6 +634,9 @@ def main():
                     help="serving: skip the sequential baseline")
     ap.add_argument("--no-graphs", action="store_true",
                     help="serving: launch the world's Jacobi chunks directly")
+    ap.add_argument("--mixed-solvers", action="store_true",
+                    help="serving: requests cycle through the four pressure solvers "
+                         "(one world per solver)")
     args = ap.parse_args()
 
     if args.gpus != 1:

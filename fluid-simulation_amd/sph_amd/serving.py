@@ -150,10 +150,12 @@ def arrival_bound(stream: List[Request], solo: Report) -> dict:
 
 def make_stream(n_requests: int = 256, rate: float = 20.0, n_min: int = 4096,
                 n_max: int = 262144, steps: int = 200, seed: int = 0,
-                dx: float = 0.02) -> List[Request]:
+                dx: float = 0.02, schemes=None) -> List[Request]:
     """The seeded serving stream: Poisson arrivals at ``rate`` per second,
     particle counts log-uniform in [n_min, n_max], rest density U[800, 1500],
-    viscosity U[0.005, 0.05], box or sphere container, ``steps`` IISPH steps."""
+    viscosity U[0.005, 0.05], box or sphere container, ``steps`` IISPH steps.
+    ``schemes``, a list of (solver, integrator) pairs, gives request k the
+    pair k mod its length instead (the rest of the stream is unchanged)."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
     t -= t[0]
@@ -166,8 +168,15 @@ def make_stream(n_requests: int = 256, rate: float = 20.0, n_min: int = 4096,
     for k in range(n_requests):
         sc = S.serving_scene(int(ns[k]), float(rho[k]), float(nu[k]),
                              "box" if shape[k] == 0 else "sphere", int(seeds[k]), dx=dx)
+        if schemes:
+            sc.params.solver, sc.params.integrator = schemes[k % len(schemes)]
         out.append(Request(k, float(t[k]), sc, int(steps)))
     return out
+
+
+# the four pressure solvers, each with its integrator of the reference's configs
+MIXED_SCHEMES = [("IISPH", "EulerCromer"), ("IISPHwOST", "TakePredicted"),
+                 ("SESPH", "EulerCromer"), ("SESPHwSplitting", "EulerCromer")]
 
 
 def lattice_bytes(lat: S.Lattice) -> int:
