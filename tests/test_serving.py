@@ -29,6 +29,20 @@ def test_stream_is_seeded_and_in_range():
     assert kinds == {"box", "sphere"}
 
 
+def test_mixed_scheme_stream_changes_only_the_schemes():
+    a = serving.make_stream(n_requests=9, n_min=1000, n_max=4000, steps=3, seed=5)
+    b = serving.make_stream(n_requests=9, n_min=1000, n_max=4000, steps=3, seed=5,
+                            schemes=serving.MIXED_SCHEMES)
+    for k, (ra, rb) in enumerate(zip(a, b)):
+        assert ra.arrival == rb.arrival and ra.steps == rb.steps
+        assert np.array_equal(ra.scene.x, rb.scene.x)
+        assert ra.scene.params.solver == "IISPH"
+        solver, integrator = serving.MIXED_SCHEMES[k % 4]
+        assert (rb.scene.params.solver, rb.scene.params.integrator) == (solver, integrator)
+        # IISPHwOST pairs with TakePredicted only (setup/mod.rs:236-238)
+        assert (solver == "IISPHwOST") == (integrator == "TakePredicted")
+
+
 def test_stream_arrival_rate():
     # mean gap of the Poisson process is 1/rate (checked on a long seeded draw)
     rng = np.random.default_rng(0)
