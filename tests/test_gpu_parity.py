@@ -426,6 +426,11 @@ def test_moving_static_sample_body_refused_by_iisphwost(native):
 
 
 # ------------------------------------------- other integrators and solvers
+def compacting(sc):
+    sc.params.disable_particles_below = float(np.sort(sc.x[:, 2])[40]) - 1e-6
+    return sc
+
+
 def with_scheme(sc, solver=None, integrator=None):
     if solver:
         sc.params.solver = solver
@@ -451,6 +456,10 @@ SCHEMES = {
     "splitting_static": lambda: with_scheme(scenes.dam_break_sesph(side=8),
                                             solver="SESPHwSplitting"),
     "splitting_rigid": lambda: with_scheme(scenes.small_rigid_scene(), solver="SESPHwSplitting"),
+    # SESPHwSplitting (its pair cache keeps the self pair) while compaction
+    # drops the lowest layer
+    "splitting_compaction": lambda: compacting(with_scheme(
+        scenes.small_iisph_box(side=8, jitter=2e-3), solver="SESPHwSplitting")),
 }
 
 
